@@ -1,0 +1,307 @@
+"""ctypes binding of libgamesplat_hip.so (the C ABI declared in include/gs_api.h).
+
+The shared library is built in-tree by ``python -m gamesplat_amd.build``.  There
+is no fallback: if the library is missing or a HIP device is absent, calls
+fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libgamesplat_hip.so"
+
+GS_OK = 0
+GS_ERR_INVALID = -1
+GS_ERR_OOM = -2
+GS_ERR_HIP = -3
+GS_ERR_STATE = -4
+GS_ERR_LIMIT = -5
+
+GS_IMAGE_ALPHA_RGBA16F = 0
+GS_IMAGE_GPS_SAMPLE_RGBA32F = 1
+GS_IMAGE_GPS_DEPTH_R32F = 2
+GS_IMAGE_ACCUM_RGBA32F = 3
+
+GS_BUFFER_SORTED_KEYS = 0
+GS_BUFFER_SORTED_IDS = 1
+GS_BUFFER_VISIBLE_IDS = 2
+GS_BUFFER_VISIBLE_KEYS = 3
+GS_BUFFER_PROJECTED = 4
+GS_BUFFER_GPS_PACKED = 5
+
+GS_OPT_GPS_KEEP_CELLS = 0
+GS_OPT_TILE_SIZE = 1
+
+STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve")
+
+GAUSSIAN_FLOATS = 59  # GsGaussian: pos 0-2, scale 3-5, rot 6-9 (w,x,y,z), dc 10-12, opacity 13, rest 14-58
+
+
+class GsCameraUniforms(C.Structure):
+    _fields_ = [
+        ("view", C.c_float * 16),
+        ("proj", C.c_float * 16),
+        ("viewport", C.c_float * 2),
+        ("orthographic", C.c_float),
+        ("persp_focal", C.c_float),
+        ("ortho_focal", C.c_float),
+        ("clip_y_sign", C.c_float),
+        ("clip_z_01", C.c_float),
+        ("pad", C.c_float * 1),
+    ]
+
+    def as_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+    @classmethod
+    def from_array(cls, a) -> "GsCameraUniforms":
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.size == 40
+        return cls.from_buffer_copy(a.tobytes())
+
+
+class GsSplatEffectParams(C.Structure):
+    _fields_ = [("center_radius", C.c_float * 4), ("params", C.c_float * 4), ("color", C.c_float * 4)]
+
+
+class GsCullOptions(C.Structure):
+    _fields_ = [("sh_degree", C.c_int32), ("sh_lod_distance", C.c_float), ("collect_stats", C.c_int32)]
+
+
+class GsStats(C.Structure):
+    _fields_ = [
+        ("gaussian_count", C.c_uint32),
+        ("visible_count", C.c_uint32),
+        ("tile_entries", C.c_uint32),
+        ("gps_requested_work_items", C.c_uint32),
+        ("gps_work_budget_exhausted", C.c_uint32),
+        ("gps_dropped_work_items", C.c_uint32),
+        ("gps_max_work_items", C.c_uint32),
+        ("diagnostics", C.c_uint32 * 8),
+        ("diagnostics_valid", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "diagnostics"}
+        d["diagnostics"] = list(self.diagnostics)
+        return d
+
+
+EXPORTS = {
+    "gs_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "gs_destroy": (None, [C.c_void_p]),
+    "gs_last_error": (C.c_char_p, []),
+    "gs_device_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "gs_set_memory_budget": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "gs_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "gs_upload_gaussians": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "gs_set_gps_budget": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "gs_cull": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
+                          C.POINTER(GsCullOptions)]),
+    "gs_sort": (C.c_int, [C.c_void_p]),
+    "gs_composite_alpha": (C.c_int, [C.c_void_p, C.c_int]),
+    "gs_gps_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_float]),
+    "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "gs_render_alpha": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
+                                  C.POINTER(GsCullOptions), C.c_int]),
+    "gs_read_image": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "gs_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "gs_stats": (C.c_int, [C.c_void_p, C.POINTER(GsStats)]),
+    "gs_synchronize": (C.c_int, [C.c_void_p]),
+    "gs_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "gs_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    "gs_reset_stage_times": (C.c_int, [C.c_void_p]),
+    "gs_stream": (C.c_void_p, [C.c_void_p]),
+    "gs_generate_scene": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]),
+    "gs_camera_from_pose": (C.c_int, [C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float,
+                                      C.POINTER(GsCameraUniforms)]),
+    "gs_look_at_origin_pose": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+class GsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gs error {code}: {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Load libgamesplat_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gamesplat_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int) -> None:
+    if code != GS_OK:
+        raise GsError(code, load().gs_last_error().decode())
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def generate_scene(seed: int, count: int, clusters: int, opacity_logit_mean: float = 0.0) -> np.ndarray:
+    """Seeded synthetic scene, (count, 59) float32 GsGaussian rows."""
+    out = np.empty((count, GAUSSIAN_FLOATS), dtype=np.float32)
+    check(load().gs_generate_scene(seed, count, clusters, opacity_logit_mean, out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def look_at_origin_pose(position) -> np.ndarray:
+    pos = np.ascontiguousarray(position, dtype=np.float32)
+    out = np.empty(16, dtype=np.float32)
+    check(load().gs_look_at_origin_pose(_fptr(pos), _fptr(out)))
+    return out
+
+
+def camera_from_pose(pose, focal_over_height: float, width: int, height: int) -> GsCameraUniforms:
+    p = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
+    cam = GsCameraUniforms()
+    check(load().gs_camera_from_pose(_fptr(p), focal_over_height, float(width), float(height), C.byref(cam)))
+    return cam
+
+
+class Context:
+    """One gs_ctx: one HIP stream and the device memory of one renderer."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        self._lib = lib
+        h = C.c_void_p()
+        check(lib.gs_create(C.byref(h), device))
+        self._h = h
+        self.n = 0
+        self.width = 0
+        self.height = 0
+
+    def close(self):
+        if self._h:
+            self._lib.gs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_info(self) -> str:
+        buf = C.create_string_buffer(512)
+        check(self._lib.gs_device_info(self._h, buf, 512))
+        return buf.value.decode()
+
+    def set_memory_budget(self, nbytes: int):
+        check(self._lib.gs_set_memory_budget(self._h, nbytes))
+
+    def set_option(self, option: int, value: int):
+        check(self._lib.gs_set_option(self._h, option, value))
+
+    def upload(self, gaussians: np.ndarray):
+        g = np.ascontiguousarray(gaussians, dtype=np.float32)
+        assert g.ndim == 2 and g.shape[1] == GAUSSIAN_FLOATS
+        check(self._lib.gs_upload_gaussians(self._h, g.ctypes.data_as(C.c_void_p), g.shape[0]))
+        self.n = g.shape[0]
+
+    def set_gps_budget(self, items: int):
+        check(self._lib.gs_set_gps_budget(self._h, items))
+
+    def cull(self, cam: GsCameraUniforms, effect: GsSplatEffectParams | None = None, sh_degree: int = 3,
+             sh_lod_distance: float = 0.0, collect_stats: bool = False):
+        opts = GsCullOptions(sh_degree, sh_lod_distance, int(collect_stats))
+        check(self._lib.gs_cull(self._h, C.byref(cam), C.byref(effect) if effect is not None else None,
+                                C.byref(opts)))
+        self.width = int(cam.viewport[0])
+        self.height = int(cam.viewport[1])
+
+    def sort(self):
+        check(self._lib.gs_sort(self._h))
+
+    def composite(self, tile: int = 0):
+        check(self._lib.gs_composite_alpha(self._h, tile))
+
+    def render_alpha(self, cam: GsCameraUniforms, sh_degree: int = 3, tile: int = 0):
+        opts = GsCullOptions(sh_degree, 0.0, 0)
+        check(self._lib.gs_render_alpha(self._h, C.byref(cam), None, C.byref(opts), tile))
+        self.width = int(cam.viewport[0])
+        self.height = int(cam.viewport[1])
+
+    def gps_sample(self, frame_seed: int, supersample: int = 1, linear_color: bool = False,
+                   background: float = 0.0):
+        check(self._lib.gs_gps_sample(self._h, frame_seed, supersample, int(linear_color), background))
+
+    def accumulate(self, sample_count: int):
+        check(self._lib.gs_accumulate(self._h, sample_count))
+
+    def synchronize(self):
+        check(self._lib.gs_synchronize(self._h))
+
+    def read_image(self, which: int, shape=None) -> np.ndarray:
+        w, h = self.width, self.height
+        if which == GS_IMAGE_ALPHA_RGBA16F:
+            out = np.empty((h, w, 4), dtype=np.float16)
+        elif which == GS_IMAGE_GPS_DEPTH_R32F:
+            out = np.empty((h, w), dtype=np.float32)
+        else:
+            out = np.empty((h, w, 4), dtype=np.float32)
+        check(self._lib.gs_read_image(self._h, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def read_buffer(self, which: int, count: int | None = None, shape=None) -> np.ndarray:
+        if which in (GS_BUFFER_SORTED_KEYS, GS_BUFFER_SORTED_IDS):
+            out = np.empty(self.n, dtype=np.uint32)
+        elif which in (GS_BUFFER_VISIBLE_IDS, GS_BUFFER_VISIBLE_KEYS):
+            v = self.stats()["visible_count"] if count is None else count
+            out = np.empty(v, dtype=np.uint32)
+        elif which == GS_BUFFER_PROJECTED:
+            out = np.empty((self.n, 16), dtype=np.float32)
+        elif which == GS_BUFFER_GPS_PACKED:
+            out = np.empty(shape, dtype=np.uint64)
+        else:
+            raise ValueError(which)
+        check(self._lib.gs_read_buffer(self._h, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def stats(self) -> dict:
+        s = GsStats()
+        check(self._lib.gs_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def set_profiling(self, on: bool):
+        check(self._lib.gs_set_profiling(self._h, int(on)))
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * len(STAGES))()
+        cnt = (C.c_uint32 * len(STAGES))()
+        check(self._lib.gs_stage_times(self._h, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES)}
+
+    def reset_stage_times(self):
+        check(self._lib.gs_reset_stage_times(self._h))
+
+    def stream(self) -> int:
+        return self._lib.gs_stream(self._h) or 0

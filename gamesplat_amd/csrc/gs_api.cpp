@@ -1,0 +1,824 @@
+// gs_api.cpp - C ABI of libgamesplat_hip: context, device memory, stage
+// sequencing and readback.  The per-stage kernels live in gs_*.hip.
+//
+// Stage order of a frame mirrors renderer_draw_frame
+// (src/renderer.cpp:2139-2289): cull -> sort -> composite for sorted alpha,
+// cull -> GPS sample -> accumulate for the stochastic path.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gs_internal.h"
+
+using namespace gs;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int ok() {
+    g_error.clear();
+    return GS_OK;
+}
+
+#define GS_HIP(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(GS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                  \
+    } while (0)
+
+#define GS_CHECK(expr)             \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != GS_OK) return r_; \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+constexpr uint32_t kMaxGaussians = (1u << 30) - 1u;
+constexpr uint32_t kMaxWorkItems = 1u << 28;
+constexpr uint32_t kUploadChunk = 262144u;  // Gaussians per staging copy (about 59 MiB)
+
+struct TimedLaunch {
+    int stage;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct gs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    size_t budget = 0;
+    size_t held = 0;
+
+    uint32_t n = 0;
+    DevBuf planes, records, keys[3], ids[3], offsets, status, counters;
+    uint32_t status_words = 0;
+    uint32_t epoch = 1;
+
+    DevBuf dup_keys[3], dup_vals[3], ranges, image;
+    uint32_t dup_cap = 0, last_dup = 0;
+    int img_w = 0, img_h = 0;
+
+    uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
+    DevBuf work, prepared, item_base, cells, sample, depth, accum;
+    int gps_w = 0, gps_h = 0, gps_ss = 1, res_w = 0, res_h = 0, acc_w = 0, acc_h = 0;
+    bool cells_dirty = true;
+    int keep_cells = 0;
+    int default_tile = 16;
+
+    GsCameraUniforms cam{};
+    bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
+    bool stats_collected = false;
+    int sort_slot = 0;
+
+    bool profiling = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> free_events;
+    double stage_ms[GS_STAGE_COUNT] = {};
+    uint32_t stage_count[GS_STAGE_COUNT] = {};
+
+    DevCounters* dc() { return static_cast<DevCounters*>(counters.p); }
+};
+
+namespace {
+
+int set_device(gs_ctx* c) {
+    GS_HIP(hipSetDevice(c->device));
+    return GS_OK;
+}
+
+void release(gs_ctx* c, DevBuf& b) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->held -= b.bytes;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// Grow `b` to at least `bytes` (slack: extra fraction for buffers whose size
+// varies per frame).  Existing contents are not preserved.
+int ensure(gs_ctx* c, DevBuf& b, size_t bytes, const char* what, double slack = 0.0) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return GS_OK;
+    size_t want = bytes + (size_t)((double)bytes * slack);
+    want = (want + 255) & ~(size_t)255;
+    if (c->budget && c->held - b.bytes + want > c->budget)
+        return fail(GS_ERR_LIMIT, "%s needs %.1f MiB; memory budget %.1f MiB with %.1f MiB held", what,
+                    want / 1048576.0, c->budget / 1048576.0, c->held / 1048576.0);
+    release(c, b);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GS_ERR_OOM, "hipMalloc(%zu) for %s failed: %s", want, what, hipGetErrorString(e));
+    }
+    b.p = p;
+    b.bytes = want;
+    c->held += want;
+    return GS_OK;
+}
+
+template <typename T>
+T* ptr(DevBuf& b) {
+    return static_cast<T*>(b.p);
+}
+
+uint32_t next_epoch(gs_ctx* c, uint32_t launches) {
+    uint32_t e = c->epoch;
+    c->epoch += launches;
+    return e;
+}
+
+// Status words are tagged with 30-bit epochs; before the tag space wraps the
+// array is cleared so no stale word can carry a current epoch.
+int reserve_epochs(gs_ctx* c, uint32_t launches) {
+    if (c->epoch + launches >= 0x3FFFFFF0u) {
+        GS_HIP(launch_fill_u64(ptr<uint64_t>(c->status), 0ull, c->status_words, c->stream));
+        c->epoch = 1;
+    }
+    return GS_OK;
+}
+
+int ensure_status(gs_ctx* c, uint32_t words) {
+    if (words <= c->status_words) return GS_OK;
+    GS_CHECK(ensure(c, c->status, (size_t)words * 8, "look-back status"));
+    c->status_words = (uint32_t)(c->status.bytes / 8);
+    GS_HIP(launch_fill_u64(ptr<uint64_t>(c->status), 0ull, c->status_words, c->stream));
+    return GS_OK;
+}
+
+uint32_t sort_status_words(uint32_t elems) {
+    uint32_t tile = onesweep_tile_elems();
+    return ((elems + tile - 1) / tile) * 256u + 256u;
+}
+
+int timed_begin(gs_ctx* c, int stage, TimedLaunch* t) {
+    if (!c->profiling) return GS_OK;
+    t->stage = stage;
+    for (hipEvent_t* ev : {&t->start, &t->stop}) {
+        if (!c->free_events.empty()) {
+            *ev = c->free_events.back();
+            c->free_events.pop_back();
+        } else {
+            GS_HIP(hipEventCreate(ev));
+        }
+    }
+    GS_HIP(hipEventRecord(t->start, c->stream));
+    return GS_OK;
+}
+
+int timed_end(gs_ctx* c, TimedLaunch* t) {
+    if (!c->profiling) return GS_OK;
+    GS_HIP(hipEventRecord(t->stop, c->stream));
+    c->timed.push_back(*t);
+    return GS_OK;
+}
+
+int check_camera(const GsCameraUniforms* cam) {
+    if (!cam) return fail(GS_ERR_INVALID, "camera uniforms are NULL");
+    for (int k = 0; k < 2; ++k) {
+        float v = cam->viewport[k];
+        if (!(v >= 1.0f && v <= 32768.0f) || v != (float)(int)v)
+            return fail(GS_ERR_INVALID, "viewport must be integral pixel sizes in [1, 32768], got %g x %g",
+                        cam->viewport[0], cam->viewport[1]);
+    }
+    if (cam->clip_y_sign != 1.0f && cam->clip_y_sign != -1.0f)
+        return fail(GS_ERR_INVALID, "clip_y_sign must be +1 (GL) or -1 (WebGPU)");
+    return GS_OK;
+}
+
+int read_u32(gs_ctx* c, const uint32_t* dev, uint32_t* host) {
+    GS_HIP(hipMemcpyAsync(host, dev, 4, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return GS_OK;
+}
+
+int copy_image(gs_ctx* c, const void* dev, void* host, int w, int h, size_t px_bytes, size_t bytes) {
+    size_t need = (size_t)w * h * px_bytes;
+    if (bytes != need) return fail(GS_ERR_INVALID, "image needs %zu bytes, got %zu", need, bytes);
+    std::vector<uint8_t> tmp(need);
+    GS_HIP(hipMemcpyAsync(tmp.data(), dev, need, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    size_t row = (size_t)w * px_bytes;
+    bool flip = c->cam.clip_y_sign > 0.0f;  // GL framebuffer rows start at the bottom
+    for (int y = 0; y < h; ++y) {
+        int src = flip ? h - 1 - y : y;
+        memcpy(static_cast<uint8_t*>(host) + (size_t)y * row, tmp.data() + (size_t)src * row, row);
+    }
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gs_last_error(void) { return g_error.c_str(); }
+
+int gs_create(gs_ctx** out, int device) {
+    if (!out) return fail(GS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    GS_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(GS_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+    gs_ctx* c = new gs_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&c->props, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(GS_ERR_HIP, "device setup failed: %s", hipGetErrorString(e));
+    }
+    int r = ensure(c, c->counters, sizeof(DevCounters), "counters");
+    if (r == GS_OK) {
+        e = hipMemsetAsync(c->counters.p, 0, sizeof(DevCounters), c->stream);
+        if (e != hipSuccess) r = fail(GS_ERR_HIP, "counter init failed: %s", hipGetErrorString(e));
+    }
+    if (r != GS_OK) {
+        gs_destroy(c);
+        return r;
+    }
+    *out = c;
+    return ok();
+}
+
+void gs_destroy(gs_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->timed) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->ids[0], &c->ids[1],
+                      &c->ids[2], &c->offsets, &c->status, &c->counters, &c->dup_keys[0], &c->dup_keys[1],
+                      &c->dup_keys[2], &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges,
+                      &c->image, &c->work, &c->prepared, &c->item_base, &c->cells, &c->sample, &c->depth,
+                      &c->accum};
+    for (DevBuf* b : bufs) release(c, *b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int gs_device_info(gs_ctx* c, char* out, size_t out_bytes) {
+    if (!c || !out || !out_bytes) return fail(GS_ERR_INVALID, "bad arguments");
+    snprintf(out, out_bytes, "%s (%s): %d CUs, %.1f GiB HBM, %zu KiB LDS/block, clock %d MHz, mem clock %d MHz, bus %d bit",
+             c->props.name, c->props.gcnArchName, c->props.multiProcessorCount,
+             c->props.totalGlobalMem / 1073741824.0, c->props.sharedMemPerBlock / 1024, c->props.clockRate / 1000,
+             c->props.memoryClockRate / 1000, c->props.memoryBusWidth);
+    return ok();
+}
+
+int gs_set_memory_budget(gs_ctx* c, size_t bytes) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    c->budget = bytes;
+    return ok();
+}
+
+int gs_set_option(gs_ctx* c, int option, int value) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    switch (option) {
+        case GS_OPT_GPS_KEEP_CELLS:
+            c->keep_cells = value ? 1 : 0;
+            return ok();
+        case GS_OPT_TILE_SIZE:
+            if (value != 8 && value != 16 && value != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+            c->default_tile = value;
+            return ok();
+        default:
+            return fail(GS_ERR_INVALID, "unknown option %d", option);
+    }
+}
+
+int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
+    if (!c || (!g && count)) return fail(GS_ERR_INVALID, "bad arguments");
+    if (count == 0 || count > kMaxGaussians)
+        return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
+    GS_CHECK(set_device(c));
+    // Scene-resident bytes: planes 240 + record 48 + keys/ids 24 + offsets 4
+    // + GPS prepared 48 + item base 4 per Gaussian, plus the GPS work list.
+    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 48 + 4) + (size_t)c->gps_budget * 4;
+    size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
+                        c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
+                        c->offsets.bytes - c->prepared.bytes - c->item_base.bytes - c->work.bytes;
+    if (c->budget && held_other + required > c->budget)
+        return fail(GS_ERR_LIMIT, "scene of %u Gaussians needs %.1f MiB; memory budget is %.1f MiB", count,
+                    required / 1048576.0, c->budget / 1048576.0);
+
+    DevBuf planes;
+    {
+        size_t bytes = (size_t)count * kPlanes * 16;
+        if (c->budget && c->held + bytes > c->budget) {
+            // the previous scene cannot stay resident next to the new one
+            release(c, c->planes);
+            c->n = 0;
+            c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(GS_ERR_OOM, "hipMalloc(%zu) for Gaussian planes failed: %s", bytes, hipGetErrorString(e));
+        }
+        planes.p = p;
+        planes.bytes = bytes;
+        c->held += bytes;
+    }
+    DevBuf staging;
+    int r = ensure(c, staging, (size_t)std::min(count, kUploadChunk) * sizeof(GsGaussian), "upload staging");
+    if (r != GS_OK) {
+        release(c, planes);
+        return r;
+    }
+    for (uint32_t first = 0; first < count && r == GS_OK; first += kUploadChunk) {
+        uint32_t chunk = std::min(kUploadChunk, count - first);
+        hipError_t e = hipMemcpyAsync(staging.p, g + first, (size_t)chunk * sizeof(GsGaussian), hipMemcpyHostToDevice,
+                                      c->stream);
+        if (e == hipSuccess)
+            e = launch_transpose_upload(ptr<float>(staging), chunk, first, count, ptr<float4>(planes), c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = fail(GS_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+    }
+    release(c, staging);
+    if (r != GS_OK) {
+        release(c, planes);
+        return r;
+    }
+    release(c, c->planes);
+    c->planes = planes;
+    c->n = count;
+    GS_CHECK(ensure(c, c->records, (size_t)count * kRecordVec4 * 16, "projected splats"));
+    for (int k = 0; k < 3; ++k) {
+        GS_CHECK(ensure(c, c->keys[k], (size_t)count * 4, "sort keys"));
+        GS_CHECK(ensure(c, c->ids[k], (size_t)count * 4, "sort ids"));
+    }
+    GS_CHECK(ensure(c, c->offsets, (size_t)count * 4, "bin offsets"));
+    GS_CHECK(ensure_status(c, std::max(sort_status_words(count), (count + 255u) / 256u + 256u)));
+    c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
+    return ok();
+}
+
+int gs_set_gps_budget(gs_ctx* c, uint32_t max_work_items) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (max_work_items == 0 || max_work_items > kMaxWorkItems)
+        return fail(GS_ERR_LIMIT, "GPS work budget %u outside [1, %u]; keeping %u", max_work_items, kMaxWorkItems,
+                    c->gps_budget);
+    if (max_work_items == c->gps_budget) return ok();
+    size_t bytes = (size_t)max_work_items * 4;
+    if (c->budget && c->held - c->work.bytes + bytes > c->budget)
+        return fail(GS_ERR_LIMIT, "GPS work buffer of %.1f MiB exceeds the memory budget; keeping %u work items",
+                    bytes / 1048576.0, c->gps_budget);
+    if (c->work.p) {
+        // keep the old buffer until the new one exists (src/renderer.cpp:1083-1117)
+        GS_CHECK(set_device(c));
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(GS_ERR_OOM, "GPS work buffer allocation failed; keeping %u work items", c->gps_budget);
+        }
+        release(c, c->work);
+        c->work.p = p;
+        c->work.bytes = bytes;
+        c->held += bytes;
+    }
+    c->gps_budget = max_work_items;
+    return ok();
+}
+
+int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* effect, const GsCullOptions* opts) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->n) return fail(GS_ERR_STATE, "no Gaussians uploaded");
+    GS_CHECK(check_camera(cam));
+    int sh_degree = opts ? opts->sh_degree : 3;
+    if (sh_degree < 0 || sh_degree > 3) return fail(GS_ERR_INVALID, "sh_degree %d outside [0, 3]", sh_degree);
+    GS_CHECK(set_device(c));
+    GS_CHECK(reserve_epochs(c, 1));
+    c->cam = *cam;
+    CullLaunch a{};
+    a.planes = ptr<float4>(c->planes);
+    a.n = c->n;
+    a.cam = *cam;
+    if (effect) a.effect = *effect;
+    a.effect_active = effect && effect->params[3] > 0.5f;
+    a.sh_degree = sh_degree;
+    a.sh_lod_distance = opts ? opts->sh_lod_distance : 0.0f;
+    a.collect_stats = opts ? opts->collect_stats : 0;
+    a.records = ptr<float4>(c->records);
+    a.vis_keys = ptr<uint32_t>(c->keys[0]);
+    a.vis_ids = ptr<uint32_t>(c->ids[0]);
+    a.counters = c->dc();
+    a.status = ptr<uint64_t>(c->status);
+    a.epoch = next_epoch(c, 1);
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->cull), 0u, sizeof(CullCounters) / 4, c->stream));
+    GS_HIP(launch_cull(a, c->stream));
+    GS_CHECK(timed_end(c, &t));
+    c->have_cull = true;
+    c->have_sort = c->have_composite = false;
+    c->stats_collected = a.collect_stats != 0;
+    return ok();
+}
+
+int gs_sort(gs_ctx* c) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_cull) return fail(GS_ERR_STATE, "gs_sort needs a gs_cull first");
+    GS_CHECK(set_device(c));
+    GS_CHECK(reserve_epochs(c, 4));
+    SortLaunch s{};
+    for (int k = 0; k < 3; ++k) {
+        s.keys[k] = ptr<uint32_t>(c->keys[k]);
+        s.vals[k] = ptr<uint32_t>(c->ids[k]);
+    }
+    s.count = &c->dc()->cull.visible_count;
+    s.capacity = c->n;
+    s.hist = &c->dc()->depth_sort.hist[0][0];
+    s.tickets = c->dc()->depth_sort.ticket;
+    s.status = ptr<uint64_t>(c->status);
+    s.passes = 4;
+    s.bits = 8;
+    s.first_epoch = next_epoch(c, 4);
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->depth_sort), 0u, sizeof(SortCounters) / 4, c->stream));
+    GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->n, s.hist, 4, 8, c->stream));
+    int slot = 0;
+    GS_HIP(launch_onesweep(s, &slot, c->stream));
+    GS_CHECK(timed_end(c, &t));
+    c->sort_slot = slot;
+    c->have_sort = true;
+    c->have_composite = false;
+    return ok();
+}
+
+int gs_composite_alpha(gs_ctx* c, int tile) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_sort) return fail(GS_ERR_STATE, "gs_composite_alpha needs gs_cull and gs_sort first");
+    if (tile == 0) tile = c->default_tile;
+    if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+    GS_CHECK(set_device(c));
+    const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
+    const int tiles_x = (W + tile - 1) / tile, tiles_y = (H + tile - 1) / tile;
+    const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
+    GS_CHECK(ensure(c, c->image, (size_t)W * H * 8, "RGBA16F image"));
+    GS_CHECK(ensure(c, c->ranges, (size_t)ntiles * 8, "tile ranges"));
+    GS_CHECK(reserve_epochs(c, 1));
+
+    BinLaunch b{};
+    b.records = ptr<float4>(c->records);
+    b.sorted_ids = ptr<uint32_t>(c->ids[c->sort_slot]);
+    b.count = &c->dc()->cull.visible_count;
+    b.capacity = c->n;
+    b.width = W;
+    b.height = H;
+    b.tile = tile;
+    b.tiles_x = tiles_x;
+    b.tiles_y = tiles_y;
+    b.offsets = ptr<uint32_t>(c->offsets);
+    b.counters = c->dc();
+    b.status = ptr<uint64_t>(c->status);
+    b.epoch = next_epoch(c, 1);
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->bin), 0u, sizeof(BinCounters) / 4, c->stream));
+    GS_HIP(launch_bin_count(b, c->stream));
+    uint32_t dup = 0;
+    GS_CHECK(read_u32(c, &c->dc()->bin.dup_total, &dup));
+    if (dup > 0x7FFFFFFFu) return fail(GS_ERR_LIMIT, "%u tile entries exceed the 2^31 limit", dup);
+    for (int k = 0; k < 3; ++k) {
+        GS_CHECK(ensure(c, c->dup_keys[k], (size_t)dup * 4, "tile keys", 0.25));
+        GS_CHECK(ensure(c, c->dup_vals[k], (size_t)dup * 4, "tile values", 0.25));
+    }
+    GS_CHECK(ensure_status(c, sort_status_words(dup)));
+    b.dup_keys = ptr<uint32_t>(c->dup_keys[0]);
+    b.dup_vals = ptr<uint32_t>(c->dup_vals[0]);
+    GS_HIP(launch_bin_emit(b, c->stream));
+
+    int passes = 1;
+    while (passes < 4 && (ntiles - 1u) >> (8 * passes)) ++passes;
+    GS_CHECK(reserve_epochs(c, (uint32_t)passes));
+    SortLaunch s{};
+    for (int k = 0; k < 3; ++k) {
+        s.keys[k] = ptr<uint32_t>(c->dup_keys[k]);
+        s.vals[k] = ptr<uint32_t>(c->dup_vals[k]);
+    }
+    s.count = &c->dc()->bin.dup_total;
+    s.capacity = dup;
+    s.hist = &c->dc()->tile_sort.hist[0][0];
+    s.tickets = c->dc()->tile_sort.ticket;
+    s.status = ptr<uint64_t>(c->status);
+    s.passes = passes;
+    s.bits = 8;
+    s.first_epoch = next_epoch(c, (uint32_t)passes);
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->tile_sort), 0u, sizeof(SortCounters) / 4, c->stream));
+    int slot = 0;
+    if (dup > 0) {
+        GS_HIP(launch_radix_histogram(s.keys[0], s.count, dup, s.hist, passes, 8, c->stream));
+        GS_HIP(launch_onesweep(s, &slot, c->stream));
+    }
+    GS_HIP(launch_fill_u32(ptr<uint32_t>(c->ranges), 0u, (size_t)ntiles * 2, c->stream));
+    if (dup > 0) GS_HIP(launch_tile_ranges(s.keys[slot], dup, ptr<uint2>(c->ranges), c->stream));
+    GS_CHECK(timed_end(c, &t));
+
+    CompositeLaunch cl{};
+    cl.records = ptr<float4>(c->records);
+    cl.tile_vals = s.vals[slot];
+    cl.ranges = ptr<uint2>(c->ranges);
+    cl.width = W;
+    cl.height = H;
+    cl.tile = tile;
+    cl.tiles_x = tiles_x;
+    cl.tiles_y = tiles_y;
+    cl.image = ptr<uint16_t>(c->image);
+    TimedLaunch t2{};
+    GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
+    GS_HIP(launch_composite(cl, c->stream));
+    GS_CHECK(timed_end(c, &t2));
+    c->img_w = W;
+    c->img_h = H;
+    c->last_dup = dup;
+    c->have_composite = true;
+    return ok();
+}
+
+int gs_render_alpha(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* effect,
+                    const GsCullOptions* opts, int tile) {
+    GS_CHECK(gs_cull(c, cam, effect, opts));
+    GS_CHECK(gs_sort(c));
+    return gs_composite_alpha(c, tile);
+}
+
+int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int linear_color, float background) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_cull) return fail(GS_ERR_STATE, "gs_gps_sample needs a gs_cull first");
+    GS_CHECK(set_device(c));
+    uint32_t ss = supersample < 1u ? 1u : (supersample > 4u ? 4u : supersample);
+    const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
+    const int gw = W * (int)ss, gh = H * (int)ss;
+    if (gw != c->gps_w || gh != c->gps_h) c->cells_dirty = true;
+    GS_CHECK(ensure(c, c->cells, (size_t)gw * gh * 8, "GPS cells"));
+    GS_CHECK(ensure(c, c->work, (size_t)c->gps_budget * 4, "GPS work list"));
+    GS_CHECK(ensure(c, c->prepared, (size_t)c->n * 48, "GPS prepared splats"));
+    GS_CHECK(ensure(c, c->item_base, (size_t)c->n * 4, "GPS item offsets"));
+    GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "GPS sample"));
+    GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "GPS depth"));
+    GS_CHECK(reserve_epochs(c, 1));
+    if (c->cells_dirty) {
+        GS_HIP(launch_fill_u64(ptr<uint64_t>(c->cells), 0xFFFFFFFF00000000ull, (size_t)gw * gh, c->stream));
+        c->cells_dirty = false;
+    }
+    c->gps_w = gw;
+    c->gps_h = gh;
+    c->gps_ss = (int)ss;
+    c->res_w = W;
+    c->res_h = H;
+
+    GpsLaunch g{};
+    g.records = ptr<float4>(c->records);
+    g.vis_ids = ptr<uint32_t>(c->ids[0]);
+    g.vis_count = &c->dc()->cull.visible_count;
+    g.capacity = c->n;
+    g.max_work_items = c->gps_budget;
+    g.ss = (float)ss;
+    g.clip_z_01 = c->cam.clip_z_01;
+    g.linear_color = linear_color;
+    g.prepared = ptr<uint4>(c->prepared);
+    g.work = ptr<uint32_t>(c->work);
+    g.item_base = ptr<uint32_t>(c->item_base);
+    g.counters = c->dc();
+    g.status = ptr<uint64_t>(c->status);
+    g.epoch = next_epoch(c, 1);
+    g.cells = ptr<uint64_t>(c->cells);
+    g.gps_width = gw;
+    g.gps_height = gh;
+    g.frame_seed = frame_seed;
+    g.width = W;
+    g.height = H;
+    g.background = background;
+    g.sample = ptr<float4>(c->sample);
+    g.depth = ptr<float>(c->depth);
+
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_GPS_EXPAND, &t));
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->gps), 0u, sizeof(GpsCounters) / 4, c->stream));
+    GS_HIP(launch_gps_expand(g, c->stream));
+    GS_CHECK(timed_end(c, &t));
+    TimedLaunch t2{};
+    GS_CHECK(timed_begin(c, GS_STAGE_GPS_SPLAT, &t2));
+    GS_HIP(launch_gps_splat(g, c->stream));
+    GS_CHECK(timed_end(c, &t2));
+    TimedLaunch t3{};
+    GS_CHECK(timed_begin(c, GS_STAGE_GPS_RESOLVE, &t3));
+    g.clear_cells = c->keep_cells ? 0 : 1;
+    GS_HIP(launch_gps_resolve(g, c->stream));
+    if (c->keep_cells) c->cells_dirty = true;
+    GS_CHECK(timed_end(c, &t3));
+    c->have_gps = true;
+    return ok();
+}
+
+int gs_accumulate(gs_ctx* c, uint32_t sample_count) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_gps) return fail(GS_ERR_STATE, "gs_accumulate needs a gs_gps_sample first");
+    GS_CHECK(set_device(c));
+    if (c->acc_w != c->res_w || c->acc_h != c->res_h) {
+        if (sample_count > 1)
+            return fail(GS_ERR_STATE, "accumulation history is %dx%d, sample is %dx%d; reset with sample_count 1",
+                        c->acc_w, c->acc_h, c->res_w, c->res_h);
+    }
+    GS_CHECK(ensure(c, c->accum, (size_t)c->res_w * c->res_h * 16, "accumulation image"));
+    GS_HIP(launch_accumulate(ptr<float4>(c->sample), ptr<float4>(c->accum), (uint32_t)(c->res_w * c->res_h),
+                             (float)sample_count, c->stream));
+    c->acc_w = c->res_w;
+    c->acc_h = c->res_h;
+    c->have_accum = true;
+    return ok();
+}
+
+int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
+    if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
+    GS_CHECK(set_device(c));
+    switch (which) {
+        case GS_IMAGE_ALPHA_RGBA16F:
+            if (!c->have_composite) return fail(GS_ERR_STATE, "no composite yet");
+            GS_CHECK(copy_image(c, c->image.p, host, c->img_w, c->img_h, 8, bytes));
+            return ok();
+        case GS_IMAGE_GPS_SAMPLE_RGBA32F:
+            if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
+            GS_CHECK(copy_image(c, c->sample.p, host, c->res_w, c->res_h, 16, bytes));
+            return ok();
+        case GS_IMAGE_GPS_DEPTH_R32F:
+            if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
+            GS_CHECK(copy_image(c, c->depth.p, host, c->res_w, c->res_h, 4, bytes));
+            return ok();
+        case GS_IMAGE_ACCUM_RGBA32F:
+            if (!c->have_accum) return fail(GS_ERR_STATE, "no accumulation yet");
+            GS_CHECK(copy_image(c, c->accum.p, host, c->acc_w, c->acc_h, 16, bytes));
+            return ok();
+        default:
+            return fail(GS_ERR_INVALID, "unknown image %d", which);
+    }
+}
+
+int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
+    if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
+    GS_CHECK(set_device(c));
+    if (!c->have_cull) return fail(GS_ERR_STATE, "no cull yet");
+    uint32_t V = 0;
+    GS_CHECK(read_u32(c, &c->dc()->cull.visible_count, &V));
+    const size_t n = c->n;
+    switch (which) {
+        case GS_BUFFER_SORTED_KEYS:
+        case GS_BUFFER_SORTED_IDS: {
+            if (!c->have_sort) return fail(GS_ERR_STATE, "no sort yet");
+            if (bytes != n * 4) return fail(GS_ERR_INVALID, "sorted buffer needs %zu bytes", n * 4);
+            DevBuf& src = which == GS_BUFFER_SORTED_KEYS ? c->keys[c->sort_slot] : c->ids[c->sort_slot];
+            GS_HIP(hipMemcpyAsync(host, src.p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
+            GS_HIP(hipStreamSynchronize(c->stream));
+            uint32_t* h = static_cast<uint32_t*>(host);
+            for (size_t i = V; i < n; ++i) h[i] = 0xFFFFFFFFu;
+            return ok();
+        }
+        case GS_BUFFER_VISIBLE_IDS:
+        case GS_BUFFER_VISIBLE_KEYS: {
+            if (bytes < (size_t)V * 4) return fail(GS_ERR_INVALID, "visible buffer needs %zu bytes", (size_t)V * 4);
+            DevBuf& src = which == GS_BUFFER_VISIBLE_KEYS ? c->keys[0] : c->ids[0];
+            GS_HIP(hipMemcpyAsync(host, src.p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
+            GS_HIP(hipStreamSynchronize(c->stream));
+            return ok();
+        }
+        case GS_BUFFER_PROJECTED: {
+            if (bytes != n * 64) return fail(GS_ERR_INVALID, "projected buffer needs %zu bytes", n * 64);
+            std::vector<uint32_t> vis(V);
+            std::vector<float> rec(n * kRecordVec4 * 4);
+            if (V) GS_HIP(hipMemcpyAsync(vis.data(), c->ids[0].p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
+            GS_HIP(hipMemcpyAsync(rec.data(), c->records.p, rec.size() * 4, hipMemcpyDeviceToHost, c->stream));
+            GS_HIP(hipStreamSynchronize(c->stream));
+            float* out = static_cast<float*>(host);
+            for (size_t i = 0; i < n; ++i) {
+                float* o = out + i * 16;
+                memset(o, 0, 64);
+                o[8] = 1.0f; o[10] = 1.0f; o[11] = 1.0f;  // culled record (shaders/cull.glsl:222-227)
+            }
+            for (uint32_t k = 0; k < V; ++k) {
+                uint32_t id = vis[k];
+                const float* r = rec.data() + (size_t)id * 12;
+                float* o = out + (size_t)id * 16;
+                float a = r[4], b = r[5], cc = r[6];
+                float det = a * cc - b * b;
+                if (det < 1e-6f) det = 1e-6f;
+                o[0] = r[8]; o[1] = r[9]; o[2] = r[10]; o[3] = r[2];
+                o[4] = r[0]; o[5] = r[1]; o[6] = r[7]; o[7] = r[11];
+                o[8] = cc / det; o[9] = -b / det; o[10] = a / det; o[11] = r[3];
+                o[12] = a; o[13] = b; o[14] = cc; o[15] = det;
+            }
+            return ok();
+        }
+        case GS_BUFFER_GPS_PACKED: {
+            if (!c->have_gps || !c->keep_cells) return fail(GS_ERR_STATE, "needs a GPS sample taken with GS_OPT_GPS_KEEP_CELLS");
+            size_t need = (size_t)c->gps_w * c->gps_h * 8;
+            if (bytes != need) return fail(GS_ERR_INVALID, "packed buffer needs %zu bytes", need);
+            GS_HIP(hipMemcpyAsync(host, c->cells.p, need, hipMemcpyDeviceToHost, c->stream));
+            GS_HIP(hipStreamSynchronize(c->stream));
+            return ok();
+        }
+        default:
+            return fail(GS_ERR_INVALID, "unknown buffer %d", which);
+    }
+}
+
+int gs_stats(gs_ctx* c, GsStats* out) {
+    if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    GS_CHECK(set_device(c));
+    DevCounters h;
+    GS_HIP(hipMemcpyAsync(&h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    out->gaussian_count = c->n;
+    out->visible_count = c->have_cull ? h.cull.visible_count : 0;
+    out->tile_entries = c->have_composite ? c->last_dup : 0;
+    out->gps_max_work_items = c->gps_budget;
+    if (c->have_gps) {
+        out->gps_requested_work_items = h.gps.requested;
+        out->gps_work_budget_exhausted = h.gps.overflow;
+        out->gps_dropped_work_items = h.gps.requested > c->gps_budget ? h.gps.requested - c->gps_budget : 0;
+    }
+    if (c->have_cull && c->stats_collected) {
+        memcpy(out->diagnostics, h.cull.diag, sizeof(out->diagnostics));
+        out->diagnostics_valid = 1;
+    }
+    return ok();
+}
+
+int gs_synchronize(gs_ctx* c) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    GS_CHECK(set_device(c));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return ok();
+}
+
+int gs_set_profiling(gs_ctx* c, int enabled) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    c->profiling = enabled != 0;
+    return ok();
+}
+
+int gs_stage_times(gs_ctx* c, double* ms, uint32_t* counts) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    GS_CHECK(set_device(c));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    for (auto& t : c->timed) {
+        float e = 0.0f;
+        GS_HIP(hipEventElapsedTime(&e, t.start, t.stop));
+        c->stage_ms[t.stage] += e;
+        c->stage_count[t.stage] += 1;
+        c->free_events.push_back(t.start);
+        c->free_events.push_back(t.stop);
+    }
+    c->timed.clear();
+    for (int k = 0; k < GS_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = c->stage_ms[k];
+        if (counts) counts[k] = c->stage_count[k];
+    }
+    return ok();
+}
+
+int gs_reset_stage_times(gs_ctx* c) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    GS_CHECK(gs_stage_times(c, nullptr, nullptr));
+    for (int k = 0; k < GS_STAGE_COUNT; ++k) {
+        c->stage_ms[k] = 0.0;
+        c->stage_count[k] = 0;
+    }
+    return ok();
+}
+
+void* gs_stream(gs_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+// gs_composite.hip - tile binning and front-to-back sorted-alpha compositing.
+//
+// Replaces the instanced quad draw of the depth-sorted splats into the
+// RGBA16F target with ONE / ONE_MINUS_SRC_ALPHA blending
+// (shaders/splat.glsl:6-105, src/renderer.cpp:277-299,1687-1720,2534-2541).
+//
+// Coverage follows the rasteriser: a splat covers pixel (i, j) when its quad
+// centre +- radius, taken through the same clip-space round trip as the
+// vertex shader and snapped to 1/256 pixel, contains the pixel centre
+// (i + 0.5, j + 0.5) with left/bottom edges inclusive.  Inside the quad the
+// fragment test and alpha are the fragment shader's.
+//
+//   bin_count   per front-to-back position: number of tiles the quad touches,
+//               exclusive offsets by decoupled look-back;
+//   bin_emit    (tile id, splat id) pairs in front-to-back order;
+//   (onesweep)  stable sort by tile id keeps the depth order inside a tile;
+//   tile_ranges [begin, end) of every tile;
+//   composite   one block per tile, splat batches staged in LDS, all pixels
+//               of the tile walk the batch front to back and stop once their
+//               transmittance drops below 1e-4; the block stops when every
+//               pixel has.
+#include <hip/hip_fp16.h>
+
+#include "gs_internal.h"
+#include "gs_math.h"
+
+namespace gs {
+
+namespace {
+
+constexpr float kTransmittanceMin = 1e-4f;
+
+__device__ __forceinline__ int32_t snap256(float w) {
+    float s = w * 256.0f;
+    if (s > 1073741824.0f) return 1073741824;
+    if (s < -1073741824.0f) return -1073741824;
+    return (int32_t)__builtin_rintf(s);
+}
+
+__device__ __forceinline__ int32_t ceil_div256(int32_t a) { return -((-a) >> 8); }
+
+// Quad edges in 24.8 fixed point window coordinates (splat_vs corners
+// through ndc = 2 pos / viewport - 1 and back).
+__device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float ry, float vw, float vh) {
+    float x0 = cx + -1.0f * rx, x1 = cx + 1.0f * rx;
+    float y0 = cy + -1.0f * ry, y1 = cy + 1.0f * ry;
+    float hw = 0.5f * vw, hh = 0.5f * vh;
+    int4 q;
+    q.x = snap256((2.0f * x0 / vw - 1.0f + 1.0f) * hw);
+    q.y = snap256((2.0f * x1 / vw - 1.0f + 1.0f) * hw);
+    q.z = snap256((2.0f * y0 / vh - 1.0f + 1.0f) * hh);
+    q.w = snap256((2.0f * y1 / vh - 1.0f + 1.0f) * hh);
+    return q;
+}
+
+struct TileRect { int x0, x1, y0, y1; };
+
+// Tiles touched by the covered pixels of splat g (empty when fully discarded).
+__device__ __forceinline__ TileRect splat_tiles(const float4* __restrict__ records, uint32_t g,
+                                                const BinLaunch& a) {
+    TileRect t = {0, 0, 0, 0};
+    float4 r0 = records[(size_t)g * kRecordVec4 + 0];
+    float4 r1 = records[(size_t)g * kRecordVec4 + 1];
+    float4 r2 = records[(size_t)g * kRecordVec4 + 2];
+    // every fragment fails power >= log((1/255)/max(o,1e-8)) when that log is > 0
+    if ((1.0f / 255.0f) / maxf(r0.z, 1e-8f) > 1.0f) return t;
+    int4 q = quad_fixed(r0.x, r0.y, r1.w, r2.w, (float)a.width, (float)a.height);
+    int i0 = max(ceil_div256(q.x - 128), 0), i1 = min(ceil_div256(q.y - 128), a.width);
+    int j0 = max(ceil_div256(q.z - 128), 0), j1 = min(ceil_div256(q.w - 128), a.height);
+    if (i0 >= i1 || j0 >= j1) return t;
+    t.x0 = i0 / a.tile;
+    t.x1 = (i1 - 1) / a.tile + 1;
+    t.y0 = j0 / a.tile;
+    t.y1 = (j1 - 1) / a.tile + 1;
+    return t;
+}
+
+__global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
+    __shared__ uint32_t s_ticket, s_prefix;
+    __shared__ uint32_t s_scan[4];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_ticket = atomicAdd(&a.counters->bin.ticket, 1u);
+    __syncthreads();
+    const uint32_t block = s_ticket;
+    const uint32_t V = *a.count;
+    if (block * 256u >= V) return;
+    const uint32_t q = block * 256u + tid;
+    uint32_t cnt = 0u;
+    if (q < V) {
+        uint32_t g = a.sorted_ids[V - 1u - q];
+        TileRect t = splat_tiles(a.records, g, a);
+        cnt = (uint32_t)((t.x1 - t.x0) * (t.y1 - t.y0));
+    }
+    uint32_t total;
+    uint32_t local = block_exclusive_sum<4>(cnt, s_scan, &total);
+    if (tid == 0) {
+        s_prefix = lookback(a.status, block, 1u, a.epoch, total);
+        if (total) atomicAdd(&a.counters->bin.dup_total, total);
+    }
+    __syncthreads();
+    if (q < V) a.offsets[q] = s_prefix + local;
+}
+
+__global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
+    const uint32_t V = *a.count;
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= V) return;
+    uint32_t g = a.sorted_ids[V - 1u - q];
+    TileRect t = splat_tiles(a.records, g, a);
+    uint32_t off = a.offsets[q];
+    for (int ty = t.y0; ty < t.y1; ++ty)
+        for (int tx = t.x0; tx < t.x1; ++tx) {
+            a.dup_keys[off] = (uint32_t)(ty * a.tiles_x + tx);
+            a.dup_vals[off] = g;
+            ++off;
+        }
+}
+
+__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t count, uint2* ranges) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) ranges[k].x = i;
+    if (i == count - 1 || keys[i + 1] != k) ranges[k].y = i + 1;
+}
+
+template <int T>
+struct TileShape {
+    static constexpr int kPixels = T * T;
+    static constexpr int kThreads = kPixels < 256 ? kPixels : 256;
+    static constexpr int kPerThread = kPixels / kThreads;
+    static constexpr int kBatch = kThreads;
+};
+
+template <int T>
+__global__ void __launch_bounds__(TileShape<T>::kThreads) composite_kernel(CompositeLaunch a) {
+    using S = TileShape<T>;
+    constexpr int NT = S::kThreads, PPT = S::kPerThread, BATCH = S::kBatch;
+    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
+    __shared__ float4 s_opa[BATCH];   // k2, opacity, cutoff, -
+    __shared__ float4 s_col[BATCH];   // r, g, b, -
+    __shared__ int4 s_quad[BATCH];    // x0, x1, y0, y1 (24.8)
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
+    const uint2 range = a.ranges[tile];
+    const float vw = (float)a.width, vh = (float)a.height;
+
+    int px[PPT], py[PPT];
+    int32_t xf[PPT], yf[PPT];
+    float fx[PPT], fy[PPT];
+    float cr[PPT], cg[PPT], cb[PPT], tr[PPT];
+    bool live[PPT];
+    bool any_live = false;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        int p = (int)tid + k * NT;
+        px[k] = tx * T + p % T;
+        py[k] = ty * T + p / T;
+        xf[k] = px[k] * 256 + 128;
+        yf[k] = py[k] * 256 + 128;
+        fx[k] = (float)px[k] + 0.5f;
+        fy[k] = (float)py[k] + 0.5f;
+        cr[k] = cg[k] = cb[k] = 0.0f;
+        tr[k] = 1.0f;
+        live[k] = px[k] < a.width && py[k] < a.height;
+        any_live |= live[k];
+    }
+
+    for (uint32_t base = range.x; base < range.y; base += BATCH) {
+        if (!__syncthreads_or(any_live)) break;
+        uint32_t e = base + tid;
+        if (e < range.y) {
+            uint32_t g = a.tile_vals[e];
+            float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
+            float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
+            float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
+            float det = r1.x * r1.z - r1.y * r1.y;
+            if (det < 1e-6f) det = 1e-6f;
+            float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+            float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
+            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
+            s_opa[tid] = make_float4(k2, r0.z, cutoff, 0.0f);
+            s_col[tid] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+            s_quad[tid] = quad_fixed(r0.x, r0.y, r1.w, r2.w, vw, vh);
+        }
+        __syncthreads();
+        const int cnt = (int)min((uint32_t)BATCH, range.y - base);
+        if (any_live) {
+            for (int j = 0; j < cnt; ++j) {
+                const float4 geo = s_geo[j];
+                const float4 opa = s_opa[j];
+                const int4 q = s_quad[j];
+                float4 col;
+                bool loaded = false;
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    if (!live[k]) continue;
+                    if (xf[k] < q.x || xf[k] >= q.y || yf[k] < q.z || yf[k] >= q.w) continue;
+                    float dx = fx[k] - geo.x;
+                    float dy = fy[k] - geo.y;
+                    float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + opa.x * dy * dy);
+                    if (power > 0.0f || power < opa.z) continue;
+                    if (!loaded) { col = s_col[j]; loaded = true; }
+                    float alpha = minf(opa.y * __expf(power), 0.99f);
+                    float w = alpha * tr[k];
+                    cr[k] += col.x * w;
+                    cg[k] += col.y * w;
+                    cb[k] += col.z * w;
+                    tr[k] *= (1.0f - alpha);
+                    if (tr[k] < kTransmittanceMin) live[k] = false;
+                }
+            }
+            any_live = false;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) any_live |= live[k];
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        if (px[k] >= a.width || py[k] >= a.height) continue;
+        size_t o = ((size_t)py[k] * a.width + px[k]) * 4;
+        ushort4 h;
+        h.x = __half_as_ushort(__float2half_rn(cr[k]));
+        h.y = __half_as_ushort(__float2half_rn(cg[k]));
+        h.z = __half_as_ushort(__float2half_rn(cb[k]));
+        h.w = __half_as_ushort(__float2half_rn(1.0f - tr[k]));
+        *reinterpret_cast<ushort4*>(a.image + o) = h;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bin_count(const BinLaunch& a, hipStream_t s) {
+    uint32_t blocks = (a.capacity + 255u) / 256u;
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(bin_count_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_emit(const BinLaunch& a, hipStream_t s) {
+    uint32_t blocks = (a.capacity + 255u) / 256u;
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(bin_emit_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges(const uint32_t* keys, uint32_t count, uint2* ranges, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((count + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s) {
+    uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
+    if (!ntiles) return hipSuccess;
+    switch (a.tile) {
+        case 8:
+            hipLaunchKernelGGL(composite_kernel<8>, dim3(ntiles), dim3(TileShape<8>::kThreads), 0, s, a);
+            break;
+        case 16:
+            hipLaunchKernelGGL(composite_kernel<16>, dim3(ntiles), dim3(TileShape<16>::kThreads), 0, s, a);
+            break;
+        case 32:
+            hipLaunchKernelGGL(composite_kernel<32>, dim3(ntiles), dim3(TileShape<32>::kThreads), 0, s, a);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,409 @@
+// gs_cull.hip - per-Gaussian cull, EWA projection and SH colour.
+//
+// Replaces cull_reset_cs and cull_cs (shaders/cull.glsl:166-474).  One
+// thread per Gaussian, 256-thread blocks.  Each block takes a ticket so that
+// blocks run their decoupled look-back in issue order; the visible ids and
+// keys of the whole launch come out compacted in index order (the order a
+// stable sort over the reference's sentinel-padded arrays would keep).
+//
+// Float policy: the library is built with -ffp-contract=off and every
+// expression below keeps the operand order of the GLSL source, so each
+// value is the same IEEE binary32 result the CPU restatement computes.
+#include "gs_internal.h"
+#include "gs_math.h"
+
+namespace gs {
+
+namespace {
+
+struct M3 { float m[3][3]; };  // m[column][row], as GLSL
+
+__device__ __forceinline__ M3 mul3(const M3& a, const M3& b) {
+    M3 o;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            o.m[c][r] = (a.m[0][r] * b.m[c][0] + a.m[1][r] * b.m[c][1]) + a.m[2][r] * b.m[c][2];
+    return o;
+}
+
+__device__ __forceinline__ M3 transpose3(const M3& a) {
+    M3 o;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) o.m[c][r] = a.m[r][c];
+    return o;
+}
+
+__device__ __forceinline__ float hash11(float p) { return fractf(sinf_(p * 127.1f) * 43758.5453123f); }
+
+constexpr float kShC0 = 0.28209479177387814f;
+constexpr float kShC1 = 0.4886025119029199f;
+constexpr float kShC2_0 = 1.0925484305920792f;
+constexpr float kShC2_1 = -1.0925484305920792f;
+constexpr float kShC2_2 = 0.31539156525252005f;
+constexpr float kShC2_3 = -1.0925484305920792f;
+constexpr float kShC2_4 = 0.5462742152960396f;
+constexpr float kShC3_0 = -0.5900435899266435f;
+constexpr float kShC3_1 = 2.890611442640554f;
+constexpr float kShC3_2 = -0.4570457994644658f;
+constexpr float kShC3_3 = 0.3731763325901154f;
+constexpr float kShC3_4 = -0.4570457994644658f;
+constexpr float kShC3_5 = 1.445305721320277f;
+constexpr float kShC3_6 = -0.5900435899266435f;
+
+__global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
+    __shared__ uint32_t s_ticket;
+    __shared__ uint32_t s_scan[4];
+    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_diag[8];
+
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_ticket = atomicAdd(&a.counters->cull.ticket, 1u);
+    if (tid < 8) s_diag[tid] = 0u;
+    __syncthreads();
+    const uint32_t block = s_ticket;
+    const uint32_t idx = block * 256u + tid;
+    const uint32_t n = a.n;
+    const GsCameraUniforms& cam = a.cam;
+    const float* V = cam.view;
+    const float* P = cam.proj;
+
+    bool visible = false;
+    uint32_t key = 0u;
+    if (idx < n) {
+        float4 p0 = a.planes[idx];
+        float position[3] = {p0.x, p0.y, p0.z};
+        float opacity = p0.w;
+        float effect_wave = 0.0f;
+        float4 p1, p2;
+        float scale[3];
+        bool loaded = false;
+
+        if (a.effect_active) {
+            // shockwave (cull.glsl:183-207)
+            const GsSplatEffectParams& fx = a.effect;
+            p1 = a.planes[(size_t)1 * n + idx];
+            p2 = a.planes[(size_t)2 * n + idx];
+            loaded = true;
+            scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
+            float duration = maxf(fx.params[1], 0.001f);
+            float progress = clampf(fx.params[0] / duration, 0.0f, 1.0f);
+            float scene_radius = maxf(fx.center_radius[3], 0.001f);
+            float to_splat[3], jitter[3];
+            for (int k = 0; k < 3; ++k) to_splat[k] = position[k] - fx.center_radius[k];
+            float jscale = scene_radius * 0.015f;
+            float fidx = (float)idx;
+            for (int k = 0; k < 3; ++k) jitter[k] = (hash11(fidx + (float)(k + 1)) * 2.0f - 1.0f) * jscale;
+            float dist = __builtin_sqrtf((to_splat[0] * to_splat[0] + to_splat[1] * to_splat[1]) + to_splat[2] * to_splat[2]);
+            float wave_radius = progress * scene_radius * 1.15f;
+            float band_width = scene_radius * 0.085f;
+            float band = (dist - wave_radius) / band_width;
+            float wave = expf_(-band * band);
+            float attack = smoothstepf(0.0f, 0.08f, progress);
+            float release = 1.0f - smoothstepf(0.82f, 1.0f, progress);
+            effect_wave = wave * attack * release;
+            float d[3];
+            d[0] = (to_splat[0] + jitter[0]) + 0.0f;
+            d[1] = (to_splat[1] + jitter[1]) + scene_radius * 0.01f;
+            d[2] = (to_splat[2] + jitter[2]) + 0.0f;
+            float inv = 1.0f / __builtin_sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            for (int k = 0; k < 3; ++k) d[k] = d[k] * inv;
+            float signed_ripple = sinf_(progress * 18.849556f + dist * 8.0f / scene_radius);
+            float push = effect_wave * fx.params[2] * scene_radius;
+            for (int k = 0; k < 3; ++k) position[k] = position[k] + d[k] * push;
+            float ripple = effect_wave * signed_ripple * 0.012f * scene_radius;
+            for (int k = 0; k < 3; ++k) position[k] = position[k] + d[k] * ripple;
+            float sgrow = 1.0f + effect_wave * 0.75f;
+            for (int k = 0; k < 3; ++k) scale[k] = scale[k] * sgrow;
+            opacity = opacity * (1.0f - effect_wave * 0.20f);
+        }
+
+        // p_view (cull.glsl:209-210)
+        float pv[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            pv[r] = ((V[0 * 4 + r] * position[0] + V[1 * 4 + r] * position[1]) + V[2 * 4 + r] * position[2]) + V[3 * 4 + r] * 1.0f;
+
+        // visibility (cull.glsl:212-220)
+        visible = pv[2] <= -0.2f;
+        if (visible) {
+            float inv_z = -1.0f / pv[2];
+            float o = cam.orthographic;
+            float ndc_x = (P[0] * pv[0]) * inv_z * (1.0f - o) + P[0] * pv[0] * o;
+            float ndc_y = (P[5] * pv[1]) * inv_z * (1.0f - o) + P[5] * pv[1] * o;
+            visible = __builtin_fabsf(ndc_x) <= 1.3f && __builtin_fabsf(ndc_y) <= 1.3f;
+        }
+
+        if (visible) {
+            key = 0xFFFFFFFFu - fbits(maxf(-pv[2], 0.0f));
+            if (!loaded) {
+                p1 = a.planes[(size_t)1 * n + idx];
+                p2 = a.planes[(size_t)2 * n + idx];
+                scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
+            }
+            float4 p3 = a.planes[(size_t)3 * n + idx];
+            float qw = p1.x, qx = p1.y, qy = p1.z, qz = p1.w;
+            float dc[3] = {p2.w, p3.x, p3.y};
+
+            // 3D covariance (cull.glsl:234-254)
+            M3 R;
+            R.m[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+            R.m[0][1] = 2.0f * (qx * qy + qw * qz);
+            R.m[0][2] = 2.0f * (qx * qz - qw * qy);
+            R.m[1][0] = 2.0f * (qx * qy - qw * qz);
+            R.m[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+            R.m[1][2] = 2.0f * (qy * qz + qw * qx);
+            R.m[2][0] = 2.0f * (qx * qz + qw * qy);
+            R.m[2][1] = 2.0f * (qy * qz - qw * qx);
+            R.m[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+            M3 S;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) S.m[c][r] = 0.0f;
+            S.m[0][0] = scale[0]; S.m[1][1] = scale[1]; S.m[2][2] = scale[2];
+            M3 M = mul3(R, S);
+            M3 cov3d = mul3(M, transpose3(M));
+
+            // EWA projection (cull.glsl:256-277)
+            float z = pv[2], x = pv[0], y = pv[1];
+            float fx_p = cam.persp_focal, fy_p = cam.persp_focal;
+            float fx_o = cam.ortho_focal, fy_o = cam.ortho_focal;
+            float ortho = cam.orthographic;
+            float J00 = mixf(fx_p / z, -fx_o, ortho);
+            float J11 = mixf(fy_p / z, -fy_o, ortho);
+            float J02 = mixf(-fx_p * x / (z * z), 0.0f, ortho);
+            float J12 = mixf(-fy_p * y / (z * z), 0.0f, ortho);
+            M3 W;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) W.m[c][r] = V[c * 4 + r];
+            M3 Wt = transpose3(W);
+            M3 WcW = mul3(mul3(W, cov3d), Wt);
+            float ca = J00 * J00 * WcW.m[0][0] + 2.0f * J00 * J02 * WcW.m[0][2] + J02 * J02 * WcW.m[2][2];
+            float cb = J00 * J11 * WcW.m[0][1] + J00 * J12 * WcW.m[0][2] + J02 * J11 * WcW.m[1][2] + J02 * J12 * WcW.m[2][2];
+            float cc = J11 * J11 * WcW.m[1][1] + 2.0f * J11 * J12 * WcW.m[1][2] + J12 * J12 * WcW.m[2][2];
+            ca += 0.3f;
+            cc += 0.3f;
+            float det = ca * cc - cb * cb;
+            if (det < 1e-6f) det = 1e-6f;
+
+            // raster centre (cull.glsl:279-291)
+            float pcx = fx_p * x / z + cam.viewport[0] * 0.5f;
+            float pcy = cam.viewport[1] * 0.5f + fy_p * y / z;
+            float ocx = -fx_o * x + cam.viewport[0] * 0.5f;
+            float ocy = cam.viewport[1] * 0.5f - fy_o * y;
+            float cx = mixf(pcx, ocx, ortho);
+            float cy = mixf(pcy, ocy, ortho);
+            float y_flip = (1.0f - cam.clip_y_sign) * 0.5f;
+            float rcy = mixf(cy, cam.viewport[1] - cy, y_flip);
+            float radius_x = __builtin_ceilf(3.0f * __builtin_sqrtf(ca));
+            float radius_y = __builtin_ceilf(3.0f * __builtin_sqrtf(cc));
+
+            // diagnostics (cull.glsl:296-318)
+            if (a.collect_stats) {
+                float quad_area = maxf(0.0f, 4.0f * radius_x * radius_y);
+                uint32_t qpx = f2u_sat(minf(quad_area, 4294967295.0f));
+                float gps_points = 6.28318530718f * __builtin_sqrtf(maxf(det, 0.0f)) * dilog(clampf(opacity, 0.0f, 1.0f));
+                uint32_t gpc = f2u_sat(minf(__builtin_ceilf(gps_points), 4294967295.0f));
+                atomicAdd(&s_diag[0], (qpx + 1023u) >> 10u);
+                atomicMax(&s_diag[1], qpx);
+                if (qpx > 1024u) atomicAdd(&s_diag[2], 1u);
+                if (qpx > 16384u) atomicAdd(&s_diag[3], 1u);
+                atomicAdd(&s_diag[4], (gpc + 1023u) >> 10u);
+                atomicMax(&s_diag[5], gpc);
+                if (gpc > 1024u) atomicAdd(&s_diag[6], 1u);
+                if (gpc > 16384u) atomicAdd(&s_diag[7], 1u);
+            }
+
+            // ndc depth (cull.glsl:320-321)
+            float ndc_z = (P[2 * 4 + 2] * z + P[3 * 4 + 2]) / (-z);
+            ndc_z = mixf(ndc_z, ndc_z * 0.5f + 0.5f, cam.clip_z_01);
+
+            // view direction (cull.glsl:323-324)
+            float cpos[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                cpos[r] = -((Wt.m[0][r] * V[12] + Wt.m[1][r] * V[13]) + Wt.m[2][r] * V[14]);
+            float dv[3] = {position[0] - cpos[0], position[1] - cpos[1], position[2] - cpos[2]};
+            float dinv = 1.0f / __builtin_sqrtf((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]);
+            float dx = dv[0] * dinv, dy = dv[1] * dinv, dz = dv[2] * dinv;
+
+            // SH (cull.glsl:348-416); only the planes the degree needs are read
+            int eff = a.sh_degree;
+            if (a.sh_lod_distance > 0.0f && (-z) > a.sh_lod_distance) eff = 0;
+            const int planes_needed = eff <= 0 ? 0 : (eff == 1 ? 2 : (eff == 2 ? 6 : 11));
+            float sh[46];
+            sh[0] = p3.z; sh[1] = p3.w;
+#pragma unroll
+            for (int j = 0; j < 11; ++j) {
+                float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (j < planes_needed) t = a.planes[(size_t)(4 + j) * n + idx];
+                sh[2 + 4 * j] = t.x;
+                sh[3 + 4 * j] = t.y;
+                sh[4 + 4 * j] = t.z;
+                sh[5 + 4 * j] = t.w;
+            }
+            float color[3];
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                float res = kShC0 * dc[ch];
+                if (eff >= 1) {
+                    res += -kShC1 * dy * sh[0 * 3 + ch];
+                    res += kShC1 * dz * sh[1 * 3 + ch];
+                    res += -kShC1 * dx * sh[2 * 3 + ch];
+                    if (eff >= 2) {
+                        float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                        float xy = dx * dy, yz = dy * dz, xz = dz * dx;
+                        res += kShC2_0 * xy * sh[3 * 3 + ch];
+                        res += kShC2_1 * yz * sh[4 * 3 + ch];
+                        res += kShC2_2 * (2.0f * zz - xx - yy) * sh[5 * 3 + ch];
+                        res += kShC2_3 * xz * sh[6 * 3 + ch];
+                        res += kShC2_4 * (xx - yy) * sh[7 * 3 + ch];
+                        if (eff >= 3) {
+                            res += kShC3_0 * dy * (3.0f * xx - yy) * sh[8 * 3 + ch];
+                            res += kShC3_1 * xy * dz * sh[9 * 3 + ch];
+                            res += kShC3_2 * dy * (4.0f * zz - xx - yy) * sh[10 * 3 + ch];
+                            res += kShC3_3 * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[11 * 3 + ch];
+                            res += kShC3_4 * dx * (4.0f * zz - xx - yy) * sh[12 * 3 + ch];
+                            res += kShC3_5 * dz * (xx - yy) * sh[13 * 3 + ch];
+                            res += kShC3_6 * dx * (xx - 3.0f * yy) * sh[14 * 3 + ch];
+                        }
+                    }
+                }
+                res += 0.5f;
+                float col = maxf(res, 0.0f) * 1.0f;
+                float ecol = a.effect_active ? a.effect.color[ch] : 0.0f;
+                float ea = a.effect_active ? a.effect.color[3] : 0.0f;
+                col += ecol * (effect_wave * ea);
+                color[ch] = col;
+            }
+
+            float4* rec = a.records + (size_t)idx * kRecordVec4;
+            rec[0] = make_float4(cx, rcy, opacity, ndc_z);
+            rec[1] = make_float4(ca, cb * cam.clip_y_sign, cc, radius_x);
+            rec[2] = make_float4(color[0], color[1], color[2], radius_y);
+        }
+    }
+
+    // order-preserving compaction of the visible (key, id) pairs
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    uint64_t bal = __ballot(visible);
+    uint32_t rank = (uint32_t)__popcll(bal & lanemask_lt());
+    if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint32_t c = s_scan[w];
+        if ((uint32_t)w < wave) before += c;
+        total += c;
+    }
+    if (tid == 0) {
+        s_prefix = lookback(a.status, block, 1u, a.epoch, total);
+        if (total) atomicAdd(&a.counters->cull.visible_count, total);
+    }
+    if (a.collect_stats && tid < 8) {
+        uint32_t v = s_diag[tid];
+        if (tid == 1 || tid == 5) {
+            if (v) atomicMax(&a.counters->cull.diag[tid], v);
+        } else if (v) {
+            atomicAdd(&a.counters->cull.diag[tid], v);
+        }
+    }
+    __syncthreads();
+    if (visible) {
+        uint32_t pos = s_prefix + before + rank;
+        a.vis_keys[pos] = key;
+        a.vis_ids[pos] = idx;
+    }
+}
+
+__global__ void clear_counters_kernel(uint32_t* c, uint32_t words) {
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) c[i] = 0u;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = v;
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = v;
+}
+
+// AoS GsGaussian (59 floats) -> 15 float4 planes.  128 Gaussians per block
+// are staged through LDS so both the read and the plane writes coalesce.
+constexpr int kUploadBlock = 128;
+constexpr int kGaussianFloats = 59;
+
+__global__ void __launch_bounds__(kUploadBlock) transpose_upload_kernel(
+        const float* aos, uint32_t count, uint32_t first, uint32_t n, float4* planes) {
+    __shared__ float s[kUploadBlock * kGaussianFloats];
+    const uint32_t base = blockIdx.x * kUploadBlock;
+    const uint32_t here = min((uint32_t)kUploadBlock, count - base);
+    const float* src = aos + (size_t)base * kGaussianFloats;
+    for (uint32_t i = threadIdx.x; i < here * kGaussianFloats; i += kUploadBlock) s[i] = src[i];
+    __syncthreads();
+    if (threadIdx.x >= here) return;
+    const float* g = s + threadIdx.x * kGaussianFloats;
+    // GsGaussian: position 0-2, scale 3-5, rotation 6-9, color 10-12, opacity 13, rest 14-58
+    const float* rest = g + 14;
+    size_t i = (size_t)first + base + threadIdx.x;
+    planes[0 * (size_t)n + i] = make_float4(g[0], g[1], g[2], g[13]);
+    planes[1 * (size_t)n + i] = make_float4(g[6], g[7], g[8], g[9]);
+    planes[2 * (size_t)n + i] = make_float4(g[3], g[4], g[5], g[10]);
+    planes[3 * (size_t)n + i] = make_float4(g[11], g[12], rest[0], rest[1]);
+    for (int j = 0; j < 10; ++j)
+        planes[(size_t)(4 + j) * n + i] = make_float4(rest[2 + 4 * j], rest[3 + 4 * j], rest[4 + 4 * j], rest[5 + 4 * j]);
+    planes[14 * (size_t)n + i] = make_float4(rest[42], rest[43], rest[44], 0.0f);
+}
+
+}  // namespace
+
+hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    uint32_t blocks = (a.n + 255u) / 256u;
+    hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_counters(DevCounters* c, hipStream_t s) {
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<uint32_t*>(c),
+                       (uint32_t)(sizeof(DevCounters) / 4));
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
+                                   float4* planes, hipStream_t s) {
+    if (!count) return hipSuccess;
+    uint32_t blocks = (count + kUploadBlock - 1) / kUploadBlock;
+    hipLaunchKernelGGL(transpose_upload_kernel, dim3(blocks), dim3(kUploadBlock), 0, s, aos, count, first, n,
+                       planes);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,254 @@
+/*
+ * gs_api.h - C ABI of the MI355X (gfx950) Gaussian-splatting frame pipeline.
+ *
+ * The library replaces the GL compute/raster work that gamesplat's renderer
+ * issues from src/renderer.cpp through sokol_gfx:
+ *   cull + project + SH        (shaders/cull.glsl:166-474, src/renderer.cpp:1778-1871)
+ *   radix depth sort           (shaders/sort.glsl:4-164,   src/renderer.cpp:1873-1962)
+ *   sorted-alpha composite     (shaders/splat.glsl:6-105,  src/renderer.cpp:1687-1720,2516-2541)
+ *   GPS clear/expand/splat/resolve (shaders/gps.glsl:6-386, src/renderer.cpp:1964-2118)
+ *   running-mean accumulation  (shaders/accum.glsl:19-45,  src/renderer.cpp:2242-2268)
+ *
+ * Conventions
+ *   - Every entry point returns GS_OK (0) or a negative GS_ERR_* code and
+ *     records a message readable through gs_last_error() (thread-local).
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - One context owns one HIP stream and all device memory it uses; calls
+ *     on one context are ordered on that stream.  Calls that return data to
+ *     the host synchronise the stream; launches do not.
+ *   - Images returned to the host are top row first, like the reference
+ *     harness' linear_capture (tests/gsanim_render.cpp:144-165).
+ *   - No HIP or torch type appears in this header.
+ */
+#ifndef GS_API_H
+#define GS_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_API_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    GS_OK = 0,
+    GS_ERR_INVALID = -1,  /* bad argument or unsupported option            */
+    GS_ERR_OOM = -2,      /* device or host allocation failed              */
+    GS_ERR_HIP = -3,      /* a HIP runtime call failed                      */
+    GS_ERR_STATE = -4,    /* call out of order (e.g. sort before cull)     */
+    GS_ERR_LIMIT = -5     /* request exceeds the memory budget or a limit  */
+};
+
+/* ---- data contracts (byte-compatible with the reference types) --------- */
+
+/* Same layout as `Gaussian` (src/gaussian.h:9-16): activated scale and
+ * opacity, unit quaternion (w,x,y,z), raw DC and 45 SH-rest floats stored as
+ * RGB triples per coefficient. 59 floats = 236 bytes. */
+typedef struct GsGaussian {
+    float position[3];
+    float scale[3];
+    float rotation[4];
+    float color[3];
+    float opacity;
+    float sh_rest[45];
+} GsGaussian;
+
+/* Same layout as `CameraUniforms` (src/camera.h:23-38): column-major view and
+ * projection, scene-target viewport in pixels. */
+typedef struct GsCameraUniforms {
+    float view[16];
+    float proj[16];
+    float viewport[2];
+    float orthographic;
+    float persp_focal;
+    float ortho_focal;
+    float clip_y_sign;
+    float clip_z_01;
+    float pad[1];
+} GsCameraUniforms;
+
+/* Same layout as `SplatEffectParams` (src/renderer.h:52-56). */
+typedef struct GsSplatEffectParams {
+    float center_radius[4];
+    float params[4];
+    float color[4];
+} GsSplatEffectParams;
+
+/* Per-frame cull options: the CullUBO fields the renderer fills from its own
+ * state (src/renderer.cpp:1847-1849). */
+typedef struct GsCullOptions {
+    int32_t sh_degree;        /* 0..3                                         */
+    float sh_lod_distance;    /* <= 0 disables the distance LOD               */
+    int32_t collect_stats;    /* non-zero: fill the 8 splat diagnostics       */
+} GsCullOptions;
+
+/* Counters, read back with gs_stats (src/renderer.h:28-38,104-111). */
+typedef struct GsStats {
+    uint32_t gaussian_count;
+    uint32_t visible_count;
+    uint32_t tile_entries;            /* duplicated (tile, splat) pairs of the last composite */
+    uint32_t gps_requested_work_items;
+    uint32_t gps_work_budget_exhausted;
+    uint32_t gps_dropped_work_items;  /* max(0, requested - budget)           */
+    uint32_t gps_max_work_items;
+    uint32_t diagnostics[8];          /* SplatDiagnostics order               */
+    uint32_t diagnostics_valid;
+} GsStats;
+
+/* Images that gs_read_image can return. */
+enum {
+    GS_IMAGE_ALPHA_RGBA16F = 0,   /* sorted-alpha composite, 8 B/pixel (half RGBA) */
+    GS_IMAGE_GPS_SAMPLE_RGBA32F = 1, /* GPS resolved sample, premultiplied rgb + coverage */
+    GS_IMAGE_GPS_DEPTH_R32F = 2,  /* GPS resolved depth                         */
+    GS_IMAGE_ACCUM_RGBA32F = 3    /* running mean of GPS samples               */
+};
+
+/* Buffers that gs_read_buffer can return (element counts in elements). */
+enum {
+    GS_BUFFER_SORTED_KEYS = 0,    /* u32 x gaussian_count: sorted depth keys, culled = 0xFFFFFFFF */
+    GS_BUFFER_SORTED_IDS = 1,     /* u32 x gaussian_count: sorted Gaussian ids, culled = 0xFFFFFFFF */
+    GS_BUFFER_VISIBLE_IDS = 2,    /* u32 x visible_count: visible ids in index order */
+    GS_BUFFER_VISIBLE_KEYS = 3,   /* u32 x visible_count: their depth keys          */
+    GS_BUFFER_PROJECTED = 4,      /* 16 floats x gaussian_count, the reference's
+                                     CullProjectedSplatData layout (shaders/cull.glsl:12-17);
+                                     culled rows as the reference writes them    */
+    GS_BUFFER_GPS_PACKED = 5      /* u64 x (ss*W)*(ss*H): depth key << 32 | payload */
+};
+
+/* Pipeline stages timed by gs_stage_times when profiling is on. */
+enum {
+    GS_STAGE_CULL = 0,
+    GS_STAGE_SORT = 1,
+    GS_STAGE_BIN = 2,
+    GS_STAGE_COMPOSITE = 3,
+    GS_STAGE_GPS_EXPAND = 4,
+    GS_STAGE_GPS_SPLAT = 5,
+    GS_STAGE_GPS_RESOLVE = 6,
+    GS_STAGE_COUNT = 7
+};
+
+typedef struct gs_ctx gs_ctx;
+
+/* ---- lifetime ----------------------------------------------------------- */
+
+/* Replaces renderer_init's GPU-side setup (src/renderer.cpp:588-684).
+ * device: HIP device ordinal. */
+int gs_create(gs_ctx** out, int device);
+/* Replaces renderer_destroy (src/renderer.h:266). NULL is accepted. */
+void gs_destroy(gs_ctx* ctx);
+/* Message of the last failing call on this thread ("" when none). */
+const char* gs_last_error(void);
+/* Human-readable device description (name, CUs, HBM, LDS). */
+int gs_device_info(gs_ctx* ctx, char* out, size_t out_bytes);
+
+/* Cap on device memory this context may hold (0 = no cap).  Mirrors
+ * gpu_budget_bytes / renderer_gpu_buffer_limits (src/renderer.cpp:709-734). */
+int gs_set_memory_budget(gs_ctx* ctx, size_t bytes);
+
+/* ---- upload -------------------------------------------------------------- */
+
+/* Replaces renderer_upload_gaussians (src/renderer.cpp:736-1074): copies the
+ * host AoS Gaussians to HBM and repacks them into 15 float4 attribute planes.
+ * Refuses (GS_ERR_LIMIT) a scene that exceeds the memory budget and keeps the
+ * previous scene in that case. */
+int gs_upload_gaussians(gs_ctx* ctx, const GsGaussian* gaussians, uint32_t count);
+
+/* Replaces renderer_set_gps_work_budget (src/renderer.cpp:1076-1123):
+ * max_work_items in [1, 2^28]; refuses 0 or an over-budget value and keeps
+ * the previous budget. */
+int gs_set_gps_budget(gs_ctx* ctx, uint32_t max_work_items);
+
+/* ---- per-frame stages ---------------------------------------------------- */
+
+/* Replaces cull_reset_cs + cull_cs (shaders/cull.glsl:166-474) as dispatched
+ * by renderer_cull_gaussians_gpu (src/renderer.cpp:1778-1871).  Produces the
+ * projected splats, the order-preserving list of visible ids with their
+ * far-to-near depth keys, and (optionally) the 8 diagnostics.  effect may be
+ * NULL. */
+int gs_cull(gs_ctx* ctx, const GsCameraUniforms* cam,
+            const GsSplatEffectParams* effect, const GsCullOptions* opts);
+
+/* Replaces the 4-pass radix sort (shaders/sort.glsl:4-164,
+ * src/renderer.cpp:1873-1962): stable ascending sort of the visible (key, id)
+ * pairs; culled entries sort last as in the reference. */
+int gs_sort(gs_ctx* ctx);
+
+/* Replaces the sorted instanced splat draw into the RGBA16F target
+ * (shaders/splat.glsl:6-105, src/renderer.cpp:292-299,1687-1720,2516-2541):
+ * tile binning of the sorted list and front-to-back compositing, black
+ * (0,0,0,0) clear.  tile_size: 8, 16 or 32 (0 = tuned default). */
+int gs_composite_alpha(gs_ctx* ctx, int tile_size);
+
+/* Replaces renderer_draw_gps_sample (src/renderer.cpp:1964-2118): clear,
+ * expand under the work budget, point splat with a 64-bit depth|payload
+ * atomicMin, resolve to RGBA32F + R32F.  frame_seed as stochastic_frame_seed,
+ * supersample 1..4, linear_color as GSAnim v3 assets, background as the
+ * resolve clear value. */
+int gs_gps_sample(gs_ctx* ctx, uint32_t frame_seed, uint32_t supersample,
+                  int linear_color, float background);
+
+/* Replaces accum_fs (shaders/accum.glsl:19-45): accum = h + (s - h) / n with
+ * n = sample_count (1 = first sample, history ignored). */
+int gs_accumulate(gs_ctx* ctx, uint32_t sample_count);
+
+/* Whole sorted-alpha frame: gs_cull + gs_sort + gs_composite_alpha. */
+int gs_render_alpha(gs_ctx* ctx, const GsCameraUniforms* cam,
+                    const GsSplatEffectParams* effect, const GsCullOptions* opts,
+                    int tile_size);
+
+/* Options (gs_set_option). */
+enum {
+    /* 1: the GPS resolve leaves the depth|payload cells in place so they can
+     * be read with GS_BUFFER_GPS_PACKED; the next sample clears them first.
+     * 0 (default): the resolve clears the cells it reads. */
+    GS_OPT_GPS_KEEP_CELLS = 0,
+    /* Default tile size of gs_composite_alpha when 0 is passed (8, 16, 32). */
+    GS_OPT_TILE_SIZE = 1
+};
+int gs_set_option(gs_ctx* ctx, int option, int value);
+
+/* ---- readback ------------------------------------------------------------ */
+
+int gs_read_image(gs_ctx* ctx, int which, void* host, size_t bytes);
+int gs_read_buffer(gs_ctx* ctx, int which, void* host, size_t bytes);
+int gs_stats(gs_ctx* ctx, GsStats* out);
+int gs_synchronize(gs_ctx* ctx);
+
+/* ---- measurement --------------------------------------------------------- */
+
+/* Enable per-stage HIP-event timing (the counterpart of the reference's
+ * PROFILE_GPU zones, src/renderer.cpp:1890-1937,1984-2088). */
+int gs_set_profiling(gs_ctx* ctx, int enabled);
+/* Sum of stage times (ms) and launch counts since the last reset; ms and
+ * counts have GS_STAGE_COUNT entries.  Synchronises the stream. */
+int gs_stage_times(gs_ctx* ctx, double* ms, uint32_t* counts);
+int gs_reset_stage_times(gs_ctx* ctx);
+/* Opaque hipStream_t of the context, for event timing by the caller. */
+void* gs_stream(gs_ctx* ctx);
+
+/* ---- synthetic scenes and cameras (host only) --------------------------- */
+
+/* Seeded synthetic scene as the reference's PLY loader would activate it
+ * (src/gaussian_loader_ply.cpp:143-203).  Positions from `clusters` isotropic
+ * normal clusters (std 1.5, centres uniform in [-8,8]^3); log-scale
+ * N(-4.5, 0.8); quaternion N(0,1)^4 normalised; opacity logit
+ * N(opacity_mean, 2); DC N(0, 0.5); rest coefficient of band l N(0, 0.1/(l+1)). */
+int gs_generate_scene(uint64_t seed, uint32_t count, uint32_t clusters,
+                      float opacity_logit_mean, GsGaussian* out);
+
+/* Camera uniforms for a baked reference view (src/main.cpp:165-193): a
+ * row-major camera_to_world pose, focal/height, target size. */
+int gs_camera_from_pose(const float camera_to_world[16], float focal_over_height,
+                        float width, float height, GsCameraUniforms* out);
+/* Pose looking at the origin from `position` with world up +Z (Blender
+ * convention of the baked views); row-major camera_to_world. */
+int gs_look_at_origin_pose(const float position[3], float camera_to_world[16]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GS_API_H */

@@ -1,0 +1,180 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: integer results (visible set, depth keys, sorted order, GPS cells)
+bit-exact; projected splats bit-exact (same IEEE operations, deterministic
+log/exp/sin/cos); the RGBA16F composite within 1/1024 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from gamesplat_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL_RGBA16F = 1.0 / 1024.0
+
+
+def _cull_pair(ctx, orc, g, cam, **kw):
+    ctx.upload(g)
+    ctx.cull(cam, **kw)
+    ok, oi, op, ov, od = orc.cull(g, cam, sh_degree=kw.get("sh_degree", 3),
+                                  sh_lod_distance=kw.get("sh_lod_distance", 0.0),
+                                  collect_stats=kw.get("collect_stats", False), effect=kw.get("effect"))
+    return ok, oi, op, ov, od
+
+
+def test_cull_bit_exact(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    print(f"case: cull c1 n={len(g)}", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam, collect_stats=True)
+    st = ctx.stats()
+    assert st["visible_count"] == ov
+    vis_ids = ctx.read_buffer(abi.GS_BUFFER_VISIBLE_IDS)
+    vis_keys = ctx.read_buffer(abi.GS_BUFFER_VISIBLE_KEYS)
+    exp_ids = np.nonzero(oi != 0xFFFFFFFF)[0].astype(np.uint32)
+    np.testing.assert_array_equal(vis_ids, exp_ids)
+    np.testing.assert_array_equal(vis_keys, ok[exp_ids])
+    proj = ctx.read_buffer(abi.GS_BUFFER_PROJECTED)
+    np.testing.assert_array_equal(proj.view(np.uint32), op.view(np.uint32))
+    assert st["diagnostics_valid"] == 1
+    np.testing.assert_array_equal(np.array(st["diagnostics"], np.uint32), od)
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2])
+def test_cull_sh_degrees(ctx, orc, c1_scene, deg):
+    g, cfg, cam = c1_scene
+    print(f"case: cull sh_degree={deg}", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g[:20000], cam, sh_degree=deg)
+    proj = ctx.read_buffer(abi.GS_BUFFER_PROJECTED)
+    np.testing.assert_array_equal(proj.view(np.uint32), op.view(np.uint32))
+
+
+def test_cull_lod_and_effect(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    print("case: cull sh_lod + shockwave effect", flush=True)
+    fx = abi.GsSplatEffectParams((0.5, -0.2, 0.3, 9.0), (0.6, 1.5, 0.12, 1.0), (1.0, 0.55, 0.18, 0.4))
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g[:30000], cam, sh_lod_distance=15.0, effect=fx)
+    assert ctx.stats()["visible_count"] == ov
+    proj = ctx.read_buffer(abi.GS_BUFFER_PROJECTED)
+    np.testing.assert_array_equal(proj.view(np.uint32), op.view(np.uint32))
+
+
+def test_sort_bit_exact(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    print("case: sort c1", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    ctx.sort()
+    sk, si = orc.radix_sort(ok, oi)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_KEYS), sk)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si)
+
+
+def test_sort_ties_and_sizes(ctx, orc, gs):
+    # many Gaussians at identical depth: ties must resolve by index
+    print("case: sort ties", flush=True)
+    for n in (1, 255, 3840, 3841, 50_000):
+        g = gs.generate_scene(7, n, 4, 0.0)
+        g[:, 0:3] = np.round(g[:, 0:3])  # coarse positions -> many equal depths
+        cam = scenes.random_view_camera(11, 640, 480)
+        ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+        ctx.sort()
+        sk, si = orc.radix_sort(ok, oi)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si, err_msg=f"n={n}")
+
+
+def test_all_culled(ctx, orc, gs):
+    print("case: all culled", flush=True)
+    g = gs.generate_scene(3, 1000, 2, 0.0)
+    g[:, 0:3] += 1000.0  # far outside the frustum
+    cam = scenes.random_view_camera(5, 320, 240)
+    ctx.upload(g)
+    ctx.cull(cam)
+    assert ctx.stats()["visible_count"] == 0
+    ctx.sort()
+    assert (ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS) == 0xFFFFFFFF).all()
+    ctx.composite()
+    img = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F)
+    assert not img.any()
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_composite_vs_fp64(ctx, orc, c1_scene, tile):
+    g, cfg, cam = c1_scene
+    print(f"case: composite c1 tile={tile}", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    ctx.sort()
+    ctx.composite(tile)
+    img = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+    sk, si = orc.radix_sort(ok, oi)
+    ref = orc.composite(op, si, cfg["width"], cfg["height"], mode=0)
+    err = np.abs(img - ref)
+    print(f"max abs err vs fp64 {err.max():.3e}, mean {err.mean():.3e}")
+    assert err.max() <= TOL_RGBA16F
+    st = ctx.stats()
+    assert st["tile_entries"] > 0
+
+
+def test_gps_bit_exact(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    W, H = cfg["width"], cfg["height"]
+    print("case: gps c1 ss=1 keep cells", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 1)
+    try:
+        budget = 1 << 22
+        ctx.set_gps_budget(budget)
+        ctx.gps_sample(frame_seed=1, supersample=1)
+        cells = ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * W,))
+        prepared, work, requested, overflow = orc.gps_expand(op, oi, budget, 1.0, 0.0)
+        ref = orc.gps_splat(prepared, work, min(requested, budget), W, H, 1, 1.0)
+        st = ctx.stats()
+        assert st["gps_requested_work_items"] == requested
+        assert st["gps_work_budget_exhausted"] == overflow
+        np.testing.assert_array_equal(cells, ref)
+        color, depth = orc.gps_resolve(ref, W, H, 1, 0.0)
+        sample = ctx.read_image(abi.GS_IMAGE_GPS_SAMPLE_RGBA32F)
+        np.testing.assert_array_equal(sample, color[::-1])
+        np.testing.assert_array_equal(ctx.read_image(abi.GS_IMAGE_GPS_DEPTH_R32F), depth[::-1])
+    finally:
+        ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 0)
+        ctx.set_gps_budget(16 * 1024 * 1024)
+
+
+def test_gps_budget_overflow(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    W, H = cfg["width"], cfg["height"]
+    print("case: gps tiny budget overflow", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    budget = 1000
+    ctx.set_gps_budget(budget)
+    ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 1)
+    try:
+        ctx.gps_sample(frame_seed=3, supersample=2)
+        prepared, work, requested, overflow = orc.gps_expand(op, oi, budget, 2.0, 0.0)
+        st = ctx.stats()
+        assert overflow == 1 and st["gps_work_budget_exhausted"] == 1
+        assert st["gps_requested_work_items"] == requested
+        assert st["gps_dropped_work_items"] == requested - budget
+        cells = ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * 2 * W * 2,))
+        ref = orc.gps_splat(prepared, work, budget, W * 2, H * 2, 3, 2.0)
+        np.testing.assert_array_equal(cells, ref)
+    finally:
+        ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 0)
+        ctx.set_gps_budget(16 * 1024 * 1024)
+
+
+def test_gps_accumulate(ctx, orc, c1_scene):
+    g, cfg, cam = c1_scene
+    W, H = cfg["width"], cfg["height"]
+    print("case: gps accumulate 8 frames", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    prepared, work, requested, overflow = orc.gps_expand(op, oi, 16 << 20, 1.0, 0.0)
+    hist = None
+    for f in range(1, 9):
+        ctx.gps_sample(frame_seed=f, supersample=1)
+        ctx.accumulate(f)
+        cells = orc.gps_splat(prepared, work, min(requested, 16 << 20), W, H, f, 1.0)
+        color, _ = orc.gps_resolve(cells, W, H, 1, 0.0)
+        hist = orc.accumulate(color if hist is None else hist, color, f)
+    acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
+    np.testing.assert_array_equal(acc, hist[::-1])
