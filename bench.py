@@ -1,0 +1,195 @@
+"""Frame benchmark of the MI355X splat pipeline.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1] [--no-cpu-baseline]
+
+One step = one sorted-alpha frame (cull + SH-3 + depth sort + tile composite
+into RGBA16F) of a resident synthetic scene at the workload's size, inputs
+already in HBM.  Default workload: 6M Gaussians, SH degree 3, 1920x1080.
+Prints one JSON line.  Extra fields report the stochastic (GPS, 1 spp, 16 Mi
+work budget) frame rate on the same scene and per-stage times from HIP events
+on the context's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stochastic", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(g, cam, width, height, threads):
+    """The oracle's restatement of the reference pipeline on host cores: one
+    full frame (cull + SH-3, LSD radix sort, fp16-per-blend composite)."""
+    from oracle import oracle as orc
+
+    t0 = time.perf_counter()
+    keys, ids, proj, vis, _ = orc.cull(g, cam, threads=threads)
+    t1 = time.perf_counter()
+    sk, si = orc.radix_sort(keys, ids)
+    t2 = time.perf_counter()
+    orc.composite(proj, si, width, height, mode=2, threads=threads)
+    t3 = time.perf_counter()
+    return t3 - t0, {"cull_s": t1 - t0, "sort_s": t2 - t1, "composite_s": t3 - t2}
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(screen-space tiles / Gaussian shards), --gpus must be 1"}))
+        return 2
+
+    import numpy as np
+
+    import gamesplat_amd as gs
+    from gamesplat_amd import scenes
+
+    g, cfg = scenes.scene_config(args.workload)
+    W, H, N = cfg["width"], cfg["height"], cfg["count"]
+    cam = scenes.random_view_camera(cfg["seed"], W, H)
+    ctx = gs.Context(0)
+    ctx.upload(g)
+    print(f"# {ctx.device_info()}", file=sys.stderr, flush=True)
+
+    # sorted-alpha frames
+    for _ in range(args.warmup):
+        ctx.render_alpha(cam, tile=args.tile)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.render_alpha(cam, tile=args.tile)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1000.0 / args.steps
+    st = ctx.stats()
+    V, D = st["visible_count"], st["tile_entries"]
+
+    # per-stage HIP-event times (separate pass: events add launch gaps)
+    ctx.set_profiling(True)
+    ctx.reset_stage_times()
+    for _ in range(args.steps):
+        ctx.render_alpha(cam, tile=args.tile)
+    stages = ctx.stage_times()
+    ctx.set_profiling(False)
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
+    print(f"# stages (ms/frame): {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}",
+          file=sys.stderr, flush=True)
+
+    # stochastic frames: cull + GPS sample (1 spp) + running mean
+    stoch = None
+    if not args.no_stochastic:
+        ctx.set_gps_budget(16 * 1024 * 1024)
+        for f in range(1, args.warmup + 1):
+            ctx.cull(cam)
+            ctx.gps_sample(f, 1)
+            ctx.accumulate(f)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, args.steps + 1):
+            ctx.cull(cam)
+            ctx.gps_sample(f, 1)
+            ctx.accumulate(f)
+        ctx.synchronize()
+        sms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        gst = ctx.stats()
+        ctx.set_profiling(True)
+        ctx.reset_stage_times()
+        for f in range(1, args.steps + 1):
+            ctx.cull(cam)
+            ctx.gps_sample(f, 1)
+            ctx.accumulate(f)
+        sstages = ctx.stage_times()
+        ctx.set_profiling(False)
+        stoch = {
+            "frames_per_s": round(1000.0 / sms, 2),
+            "ms_per_frame": round(sms, 4),
+            "requested_work_items": gst["gps_requested_work_items"],
+            "dropped_work_items": gst["gps_dropped_work_items"],
+            "budget": gst["gps_max_work_items"],
+            "stage_ms": {k: round(v[0] / v[1], 4) for k, v in sstages.items() if v[1]},
+        }
+
+    # roofline of the dominant kernel stage (algorithmic bytes per launch)
+    cull_bytes = 12 * N + 224 * V + 56 * V + 8 * (N - V)
+    sort_bytes = 68 * V
+    comp_bytes = 56 * D + 8 * W * H
+    cands = {
+        "cull": (stage_ms["cull"], cull_bytes),
+        "sort": (stage_ms["sort"], sort_bytes),
+        "composite": (stage_ms["composite"], comp_bytes),
+    }
+    dom = max(cands, key=lambda k: cands[k][0])
+    dms, dbytes = cands[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "algorithmic_bytes": dbytes,
+        "ms": round(dms, 4),
+        "others": {k: {"ms": round(v[0], 4), "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1) if v[0] else 0.0}
+                   for k, v in cands.items()},
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        secs, parts = cpu_baseline(g, cam, W, H, threads)
+        cpu = {"value": round(1.0 / secs, 5), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"one full frame of the same workload ({N} Gaussians, {W}x{H}): oracle cull+SH-3, "
+                         f"LSD radix sort, fp16-per-blend composite; reference CPU path (Mesa llvmpipe) "
+                         f"unavailable", "parts_s": {k: round(v, 3) for k, v in parts.items()}}
+
+    out = {
+        "metric": "1080p frames/s, 6M Gaussians: sorted-alpha vs denoised-stochastic at PSNR>=40dB",
+        "value": round(1000.0 / ms, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"sorted-alpha frame, {N} Gaussians SH-3, {W}x{H}, RGBA16F target",
+                   "gaussians": N, "width": W, "height": H, "visible": V, "tile_entries": D,
+                   "seed": cfg["seed"], "parallelism": "single GPU"},
+        "gaussians_per_s": round(N * 1000.0 / ms, 1),
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
+        "stochastic": stoch,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -436,8 +436,8 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     a.status = ptr<uint64_t>(c->status);
     a.epoch = next_epoch(c, 1);
     TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->cull), 0u, sizeof(CullCounters) / 4, c->stream));
+    GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_cull(a, c->stream));
     GS_CHECK(timed_end(c, &t));
     c->have_cull = true;
@@ -465,8 +465,8 @@ int gs_sort(gs_ctx* c) {
     s.bits = 8;
     s.first_epoch = next_epoch(c, 4);
     TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->depth_sort), 0u, sizeof(SortCounters) / 4, c->stream));
+    GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
     GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->n, s.hist, 4, 8, c->stream));
     int slot = 0;
     GS_HIP(launch_onesweep(s, &slot, c->stream));
@@ -505,8 +505,8 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     b.status = ptr<uint64_t>(c->status);
     b.epoch = next_epoch(c, 1);
     TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->bin), 0u, sizeof(BinCounters) / 4, c->stream));
+    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
     GS_HIP(launch_bin_count(b, c->stream));
     uint32_t dup = 0;
     GS_CHECK(read_u32(c, &c->dc()->bin.dup_total, &dup));
@@ -625,8 +625,8 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     g.depth = ptr<float>(c->depth);
 
     TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_GPS_EXPAND, &t));
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->gps), 0u, sizeof(GpsCounters) / 4, c->stream));
+    GS_CHECK(timed_begin(c, GS_STAGE_GPS_EXPAND, &t));
     GS_HIP(launch_gps_expand(g, c->stream));
     GS_CHECK(timed_end(c, &t));
     TimedLaunch t2{};

@@ -1,0 +1,112 @@
+"""Pins the oracle's sorted-alpha composite (shaders/splat.glsl, ONE /
+ONE_MINUS_SRC_ALPHA into RGBA16F) with closed forms: one splat, the `over`
+operator for two, back-to-front == front-to-back, the coverage rule against a
+brute-force pixel-centre test, and fp16 storage against numpy float16."""
+import numpy as np
+import pytest
+
+W, H = 64, 48
+
+
+def _rec(cx, cy, a, b, c, color, opacity, ndc_z=0.5):
+    det = max(np.float32(a) * np.float32(c) - np.float32(b) * np.float32(b), np.float32(1e-6))
+    r = np.zeros(16, np.float32)
+    r[0:3] = color
+    r[3] = opacity
+    r[4:6] = (cx, cy)
+    r[6] = np.ceil(3 * np.sqrt(np.float32(a)))
+    r[7] = np.ceil(3 * np.sqrt(np.float32(c)))
+    r[8:11] = (np.float32(c) / det, -np.float32(b) / det, np.float32(a) / det)
+    r[11] = ndc_z
+    r[12:16] = (a, b, c, det)
+    return r
+
+
+def _pixel_value(rec, i, j):
+    """Closed form of one fragment: alpha = min(o exp(-q/2), 0.99) inside the
+    quad and above the 1/255 cutoff, window pixel (i, j)."""
+    cx, cy = rec[4], rec[5]
+    d = np.array([i + 0.5 - cx, j + 0.5 - cy], np.float64)
+    con = np.array([[rec[8], rec[9]], [rec[9], rec[10]]], np.float64)
+    power = -0.5 * d @ con @ d
+    inside = abs(d[0]) < rec[6] and abs(d[1]) < rec[7]
+    if not inside or power > 0 or rec[3] * np.exp(power) < 1 / 255:
+        return 0.0
+    return min(rec[3] * np.exp(power), 0.99)
+
+
+def test_single_splat_closed_form(orc):
+    rec = _rec(30.3, 20.7, 9.0, 2.0, 6.0, (0.8, 0.4, 0.2), 0.9)
+    img = orc.composite(rec[None], np.array([0], np.uint32), W, H, 1.0, mode=0)
+    exp = np.zeros((H, W))
+    for j in range(H):
+        for i in range(W):
+            exp[H - 1 - j, i] = _pixel_value(rec, i, j)  # top row first
+    np.testing.assert_allclose(img[..., 3], exp, atol=1e-6)
+    np.testing.assert_allclose(img[..., 0], 0.8 * exp, atol=1e-6)
+    np.testing.assert_allclose(img[..., 2], 0.2 * exp, atol=1e-6)
+
+
+def test_two_splats_over_operator(orc):
+    far = _rec(30.0, 24.0, 16.0, 0.0, 16.0, (1.0, 0.0, 0.0), 0.7)
+    near = _rec(33.0, 22.0, 9.0, -1.0, 12.0, (0.0, 0.0, 1.0), 0.6)
+    proj = np.stack([far, near])
+    img = orc.composite(proj, np.array([0, 1], np.uint32), W, H, 1.0, mode=0)  # sorted far first
+    for (i, j) in [(30, 24), (33, 22), (31, 23), (36, 20), (25, 27)]:
+        af, an = _pixel_value(far, i, j), _pixel_value(near, i, j)
+        px = img[H - 1 - j, i]
+        assert abs(px[3] - (an + (1 - an) * af)) < 1e-6
+        assert abs(px[0] - (1 - an) * af * 1.0) < 1e-6
+        assert abs(px[2] - an * 1.0) < 1e-6
+
+
+def test_back_to_front_equals_front_to_back(orc, c1_scene):
+    g, cfg, cam = c1_scene
+    keys, ids, proj, vis, _ = orc.cull(g[:30000], cam)
+    sk, si = orc.radix_sort(keys, ids)
+    w, h = cfg["width"], cfg["height"]
+    a = orc.composite(proj, si, w, h, 1.0, mode=0)
+    b = orc.composite(proj, si, w, h, 1.0, mode=1)
+    assert np.abs(a - b).max() < 1e-12
+    assert a[..., 3].max() > 0.5
+
+
+def test_fp16_blend_emulation(orc):
+    rng = np.random.default_rng(8)
+    recs = [_rec(20 + rng.uniform(-3, 3), 20 + rng.uniform(-3, 3), 30.0, 0.0, 30.0,
+                 tuple(rng.uniform(0, 1.5, 3)), rng.uniform(0.1, 0.9)) for _ in range(12)]
+    proj = np.stack(recs)
+    img = orc.composite(proj, np.arange(12, dtype=np.uint32), W, H, 1.0, mode=2)
+    # every stored value is a float16 and the centre pixel follows the per-step rounding
+    assert np.array_equal(img, img.astype(np.float16).astype(np.float64))
+    dst = np.zeros(4, np.float16)
+    for r in recs:
+        a = np.float32(min(np.float32(r[3]) * np.float32(np.exp(np.float32(_power(r, 20, 20)))), np.float32(0.99)))
+        if _pixel_value(r, 20, 20) == 0:
+            continue
+        src = np.array([r[0] * a, r[1] * a, r[2] * a, a], np.float32)
+        dst = (src + dst.astype(np.float32) * (np.float32(1) - a)).astype(np.float16)
+    np.testing.assert_allclose(img[H - 1 - 20, 20], dst.astype(np.float64), atol=2 ** -10)
+
+
+def _power(r, i, j):
+    d = np.array([i + 0.5 - r[4], j + 0.5 - r[5]], np.float64)
+    con = np.array([[r[8], r[9]], [r[9], r[10]]], np.float64)
+    return -0.5 * d @ con @ d
+
+
+def test_quad_coverage_rule(orc):
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        cx, cy = rng.uniform(-5, W + 5), rng.uniform(-5, H + 5)
+        rx, ry = float(rng.integers(1, 12)), float(rng.integers(1, 12))
+        rec = np.zeros(16, np.float32)
+        rec[4:8] = (cx, cy, rx, ry)
+        q = orc.quad_fixed(rec, float(W), float(H))
+        for i in range(-2, W + 2):
+            exp = (np.float32(cx) - np.float32(rx)) <= i + 0.5 < (np.float32(cx) + np.float32(rx))
+            got = q[0] <= 256 * i + 128 < q[1]
+            # the rule equals the exact half-open interval except within 1/512 px of an edge
+            edge = min(abs(i + 0.5 - (cx - rx)), abs(i + 0.5 - (cx + rx)))
+            if edge > 1 / 256:
+                assert got == exp
