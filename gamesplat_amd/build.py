@@ -22,6 +22,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libgamesplat_hip.so"
+HEADLESS = PKG / "gs_headless"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GS_OFFLOAD_ARCH", "gfx950")
 
@@ -58,8 +59,15 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # headless renderer driver (C++ host code over the C ABI)
+    rsrc = sorted((PKG / "renderer").glob("*.cpp"))
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-o", str(HEADLESS), *map(str, rsrc), "-L", str(PKG),
+           "-lgamesplat_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"headless driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {HEADLESS}")
     return LIB
 
 

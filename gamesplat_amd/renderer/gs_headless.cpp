@@ -1,0 +1,245 @@
+// gs_headless.cpp - headless frame driver and capture harness.
+//
+// Reproduces the call sequence of app_init / app_frame (src/main.cpp:276-573,
+// 1060-1977) for the splat path and the measurement protocol of the
+// reference harness (tests/gsanim_render.cpp:73-188): key=value flags,
+// warm-up then timed interval, per-frame GPU time, a RESULT JSON line and a
+// raw float32 RGBA linear capture (top row first).
+//
+// Flags (defaults in brackets):
+//   ply=<path>                 3DGS binary PLY; otherwise a synthetic scene:
+//   gaussians=[100000] clusters=[16] seed=[0x5EED0001] opacity_mean=[0]
+//   width=[1280] height=[720] render_mode=[alpha|gps]
+//   view_seed=[seed] radius=[15] orbit_deg=[0]   camera on a seeded direction
+//   gps_ss=[1] gps_budget_m=[16] gps_accumulation=[1] sh_degree=[3]
+//   gpu_budget_mb=[0] diagnostics=[0]
+//   warmup=[3] frames=[20]     untimed then timed frame counts
+//   linear_capture=<path>      raw f32 RGBA of the last frame
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "gs_renderer.h"
+
+namespace {
+
+std::map<std::string, std::string> g_args;
+
+const char* arg(const char* k, const char* dflt) {
+    auto it = g_args.find(k);
+    return it == g_args.end() ? dflt : it->second.c_str();
+}
+double argf(const char* k, double dflt) {
+    auto it = g_args.find(k);
+    return it == g_args.end() ? dflt : strtod(it->second.c_str(), nullptr);
+}
+long long argi(const char* k, long long dflt) {
+    auto it = g_args.find(k);
+    return it == g_args.end() ? dflt : strtoll(it->second.c_str(), nullptr, 0);
+}
+
+// Binary little-endian 3DGS PLY with float properties, activated as
+// src/gaussian_loader_ply.cpp:143-203 does.
+bool load_ply(const char* path, std::vector<Gaussian>& out) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    char line[512];
+    uint32_t count = 0;
+    std::vector<std::string> props;
+    bool binary_le = false;
+    while (fgets(line, sizeof(line), f)) {
+        if (!strncmp(line, "format binary_little_endian", 27)) binary_le = true;
+        if (!strncmp(line, "element vertex ", 15)) count = (uint32_t)strtoul(line + 15, nullptr, 10);
+        if (!strncmp(line, "property float ", 15)) {
+            std::string n(line + 15);
+            while (!n.empty() && (n.back() == '\n' || n.back() == '\r' || n.back() == ' ')) n.pop_back();
+            props.push_back(n);
+        } else if (!strncmp(line, "property ", 9)) {
+            fprintf(stderr, "unsupported non-float property: %s", line);
+            fclose(f);
+            return false;
+        }
+        if (!strncmp(line, "end_header", 10)) break;
+    }
+    if (!binary_le || !count || props.empty()) {
+        fclose(f);
+        return false;
+    }
+    auto find = [&](const std::string& n) {
+        for (size_t i = 0; i < props.size(); ++i)
+            if (props[i] == n) return (int)i;
+        return -1;
+    };
+    int ox = find("x"), oy = find("y"), oz = find("z");
+    if (ox < 0 || oy < 0 || oz < 0) {
+        fclose(f);
+        return false;
+    }
+    int os[3] = {find("scale_0"), find("scale_1"), find("scale_2")};
+    int orot[4] = {find("rot_0"), find("rot_1"), find("rot_2"), find("rot_3")};
+    int oop = find("opacity");
+    int odc[3] = {find("f_dc_0"), find("f_dc_1"), find("f_dc_2")};
+    int orest[45];
+    for (int k = 0; k < 45; ++k) orest[k] = find("f_rest_" + std::to_string(k));
+    size_t stride = props.size();
+    out.resize(count);
+    std::vector<float> row(stride * 4096);
+    for (uint32_t i = 0; i < count; i += 4096) {
+        uint32_t b = std::min(4096u, count - i);
+        if (fread(row.data(), stride * 4, b, f) != b) {
+            fclose(f);
+            return false;
+        }
+        for (uint32_t k = 0; k < b; ++k) {
+            const float* v = row.data() + (size_t)k * stride;
+            Gaussian& g = out[i + k];
+            g.position[0] = v[ox]; g.position[1] = v[oy]; g.position[2] = v[oz];
+            for (int a = 0; a < 3; ++a) g.scale[a] = os[0] >= 0 ? expf(v[os[a]]) : 0.01f;
+            if (orot[0] >= 0) {
+                float rw = v[orot[0]], rx = v[orot[1]], ry = v[orot[2]], rz = v[orot[3]];
+                float len = sqrtf(rw * rw + rx * rx + ry * ry + rz * rz);
+                if (len > 1e-8f) { rw /= len; rx /= len; ry /= len; rz /= len; }
+                g.rotation[0] = rw; g.rotation[1] = rx; g.rotation[2] = ry; g.rotation[3] = rz;
+            } else {
+                g.rotation[0] = 1; g.rotation[1] = g.rotation[2] = g.rotation[3] = 0;
+            }
+            g.opacity = oop >= 0 ? 1.0f / (1.0f + expf(-v[oop])) : 1.0f;
+            for (int a = 0; a < 3; ++a) g.color[a] = odc[0] >= 0 ? v[odc[a]] : 0.0f;
+            for (int j = 0; j < 15; ++j)
+                for (int c = 0; c < 3; ++c) {
+                    int o = orest[c * 15 + j];
+                    g.sh_rest[j * 3 + c] = o >= 0 ? v[o] : 0.0f;
+                }
+        }
+    }
+    fclose(f);
+    return true;
+}
+
+double percentile(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    return v[(size_t)((v.size() - 1) * p)];
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    for (int i = 1; i < argc; ++i) {
+        const char* eq = strchr(argv[i], '=');
+        if (!eq) {
+            fprintf(stderr, "flags are key=value, got %s\n", argv[i]);
+            return 2;
+        }
+        g_args[std::string(argv[i], eq - argv[i])] = eq + 1;
+    }
+    auto t_start = std::chrono::steady_clock::now();
+    const int width = (int)argi("width", 1280), height = (int)argi("height", 720);
+
+    std::vector<Gaussian> gaussians;
+    if (g_args.count("ply")) {
+        if (!load_ply(arg("ply", ""), gaussians)) {
+            fprintf(stderr, "could not load %s\n", arg("ply", ""));
+            return 1;
+        }
+    } else {
+        uint32_t n = (uint32_t)argi("gaussians", 100000);
+        gaussians.resize(n);
+        if (gs_generate_scene((uint64_t)argi("seed", 0x5EED0001), n, (uint32_t)argi("clusters", 16),
+                              (float)argf("opacity_mean", 0.0), reinterpret_cast<GsGaussian*>(gaussians.data())) != GS_OK)
+            return 1;
+    }
+    GaussianScene scene{};
+    scene.gaussians = gaussians.data();
+    scene.gaussian_count = (uint32_t)gaussians.size();
+
+    Renderer r;
+    if (!renderer_init(&r)) return 1;
+    const std::string mode = arg("render_mode", "alpha");
+    r.splat_render_mode = mode == "gps" ? SplatRenderMode::GaussianPointSplatting : SplatRenderMode::AlphaBlendSorted;
+    r.black_background = true;
+    r.sh_degree = (int)argi("sh_degree", 3);
+    r.gpu_budget_bytes = (size_t)argi("gpu_budget_mb", 0) * 1048576u;
+    r.gps_supersample_factor = (uint32_t)argi("gps_ss", 1);
+    r.gps_max_work_items = (uint32_t)(argf("gps_budget_m", 16.0) * 1048576.0);
+    r.stochastic_accumulation_enabled = argi("gps_accumulation", 1) != 0;
+    r.splat_diagnostics_enabled = argi("diagnostics", 0) != 0;
+    if (!renderer_upload_gaussians(&r, &scene)) return 1;
+    double startup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+
+    // seeded camera direction (splitmix64 -> normal via Box-Muller), radius
+    uint64_t s = (uint64_t)argi("view_seed", argi("seed", 0x5EED0001));
+    auto next = [&]() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    double d[3];
+    for (int k = 0; k < 3; k += 2) {
+        double u1 = 1.0 - next(), u2 = next();
+        double rr = sqrt(-2.0 * log(u1));
+        d[k] = rr * cos(6.283185307179586 * u2);
+        if (k + 1 < 3) d[k + 1] = rr * sin(6.283185307179586 * u2);
+    }
+    double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double radius = argf("radius", 15.0), orbit = argf("orbit_deg", 0.0);
+
+    const int warmup = (int)argi("warmup", 3), frames = (int)argi("frames", 20);
+    hipStream_t stream = (hipStream_t)gs_stream(r.ctx);
+    std::vector<hipEvent_t> ev(2 * frames);
+    for (auto& e : ev) hipEventCreate(&e);
+    std::vector<double> wall;
+    CameraUniforms cam;
+    for (int f = 0; f < warmup + frames; ++f) {
+        double a = orbit * M_PI / 180.0 * f, c = cos(a), sn = sin(a);
+        float pos[3] = {(float)((c * d[0] - sn * d[1]) / dn * radius), (float)((sn * d[0] + c * d[1]) / dn * radius),
+                        (float)(d[2] / dn * radius)};
+        float pose[16];
+        gs_look_at_origin_pose(pos, pose);
+        gs_camera_from_pose(pose, 0.8660254f, (float)width, (float)height, reinterpret_cast<GsCameraUniforms*>(&cam));
+        bool timed = f >= warmup;
+        auto t0 = std::chrono::steady_clock::now();
+        if (timed) hipEventRecord(ev[2 * (f - warmup)], stream);
+        renderer_draw_frame(&r, &scene, &cam, nullptr, nullptr);
+        if (timed) hipEventRecord(ev[2 * (f - warmup) + 1], stream);
+        if (r.last_frame_status != GS_OK) return 1;
+        if (timed) wall.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    gs_synchronize(r.ctx);
+    std::vector<double> gpu;
+    for (int f = 0; f < frames; ++f) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, ev[2 * f], ev[2 * f + 1]);
+        gpu.push_back(ms);
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+
+    const char* cap = arg("linear_capture", "");
+    if (cap[0]) {
+        std::vector<float> px((size_t)width * height * 4);
+        if (!renderer_read_linear(&r, px.data(), px.size())) return 1;
+        FILE* f = fopen(cap, "wb");
+        if (!f || fwrite(px.data(), 4, px.size(), f) != px.size()) return 1;
+        fclose(f);
+    }
+    GsStats st;
+    gs_stats(r.ctx, &st);
+    printf("RESULT {\"frames\":%zu,\"wall_median_ms\":%.6f,\"wall_p95_ms\":%.6f,\"gpu_median_ms\":%.6f,"
+           "\"gpu_p95_ms\":%.6f,\"startup_ms\":%.3f,\"gaussians\":%u,\"visible\":%u,\"tile_entries\":%u,"
+           "\"gps_requested\":%u,\"gps_exhausted\":%u}\n",
+           wall.size(), percentile(wall, .5), percentile(wall, .95), percentile(gpu, .5), percentile(gpu, .95),
+           startup_ms, st.gaussian_count, st.visible_count, st.tile_entries, st.gps_requested_work_items,
+           st.gps_work_budget_exhausted);
+    renderer_destroy(&r);
+    return 0;
+}

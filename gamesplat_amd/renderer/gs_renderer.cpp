@@ -1,0 +1,244 @@
+// gs_renderer.cpp - renderer.h splat path over the gs_* C ABI (headless).
+//
+// Pass order and state rules follow renderer_draw_frame
+// (src/renderer.cpp:2139-2289): cull every frame; sorted alpha sorts and
+// composites into the RGBA16F appearance image; GPS resets accumulation on
+// any CameraUniforms byte change, increments the frame seed before each
+// sample, reads the work statistics at most twice per second and keeps a
+// running mean when accumulation is enabled.
+#include "gs_renderer.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+namespace {
+
+void log_error(const char* what, int status) {
+    fprintf(stderr, "[renderer] %s failed (%d): %s\n", what, status, gs_last_error());
+}
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+float half_to_float(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t exp = (h >> 10) & 0x1fu, man = h & 0x3ffu;
+    uint32_t bits;
+    if (exp == 0) {
+        if (man == 0) {
+            bits = sign;
+        } else {
+            float v = (float)man * (1.0f / 16777216.0f);
+            return sign ? -v : v;
+        }
+    } else if (exp == 31) {
+        bits = sign | 0x7f800000u | (man << 13);
+    } else {
+        bits = sign | ((exp + 112u) << 23) | (man << 13);
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+double g_last_stats_time = 0.0;
+
+}  // namespace
+
+bool renderer_init(Renderer* r) {
+    memset(r, 0, sizeof(*r));
+    // defaults of renderer_init (src/renderer.cpp:588-598)
+    r->splat_render_mode = SplatRenderMode::GaussianPointSplatting;
+    r->stochastic_samples_per_frame = 1;
+    r->stochastic_taa_current_samples = 4;
+    r->gps_supersample_factor = 2;
+    r->gps_max_work_items = 16u * 1024u * 1024u;
+    r->stochastic_accumulation_enabled = true;
+    r->stochastic_taa_enabled = true;
+    r->sh_degree = 3;
+    r->sh_lod_distance = 0.0f;
+    int st = gs_create(&r->ctx, 0);
+    if (st != GS_OK) {
+        log_error("gs_create", st);
+        r->ctx = nullptr;
+        return false;
+    }
+    r->gps_capabilities = {true, true, true, true, true};
+    r->gps_gpu.max_work_items = r->gps_max_work_items;
+    r->gps_gpu.supersample_factor = r->gps_supersample_factor;
+    return true;
+}
+
+bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene) {
+    if (!r->ctx || !scene) return false;
+    gs_set_memory_budget(r->ctx, r->gpu_budget_bytes);
+    int st = gs_set_gps_budget(r->ctx, r->gps_max_work_items ? r->gps_max_work_items : 1u);
+    if (st == GS_OK)
+        st = gs_upload_gaussians(r->ctx, reinterpret_cast<const GsGaussian*>(scene->gaussians), scene->gaussian_count);
+    if (st != GS_OK) {
+        log_error("upload", st);
+        return false;
+    }
+    r->gaussian_count = scene->gaussian_count;
+    r->sort_capacity = scene->gaussian_count;
+    renderer_reset_stochastic_accumulation(r);
+    return true;
+}
+
+bool renderer_set_gps_work_budget(Renderer* r, uint32_t max_work_items) {
+    r->gps_work_budget_resize_failed = false;
+    if (max_work_items == r->gps_gpu.max_work_items) return true;
+    int st = r->ctx ? gs_set_gps_budget(r->ctx, max_work_items) : GS_ERR_STATE;
+    if (st != GS_OK) {
+        log_error("GPS work budget resize", st);
+        r->gps_work_budget_resize_failed = true;
+        return false;
+    }
+    r->gps_gpu.max_work_items = max_work_items;
+    r->gps_max_work_items = max_work_items;
+    r->gps_gpu.work_stats_valid = false;
+    r->gps_gpu.work_stats_elapsed = 0.0f;
+    renderer_reset_stochastic_accumulation(r);
+    return true;
+}
+
+void renderer_reset_stochastic_accumulation(Renderer* r) {
+    r->stochastic_sample_count = 0;
+    r->stochastic_accum_write_index = 0;
+    r->stochastic_prev_cam_valid = false;
+}
+
+void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms* cam, const OverlayParams*,
+                         const NodeRenderParams*, const SplatEffectParams* splat_effect, float, const CameraUniforms*) {
+    r->last_frame_status = GS_OK;
+    if (!r->ctx || !scene || !cam) {
+        r->last_frame_status = GS_ERR_STATE;
+        return;
+    }
+    if (scene->animation) {
+        fprintf(stderr, "[renderer] GSAnim appearance tracks are not supported by this build\n");
+        r->last_frame_status = GS_ERR_INVALID;
+        return;
+    }
+    const bool gps = r->splat_render_mode == SplatRenderMode::GaussianPointSplatting;
+    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) {
+        fprintf(stderr, "[renderer] StochasticSplats mode is not supported by this build\n");
+        r->last_frame_status = GS_ERR_INVALID;
+        return;
+    }
+    if (scene->gaussian_count == 0 || r->gaussian_count == 0) {
+        scene->visible_count = 0;
+        return;
+    }
+    const GsCameraUniforms* gcam = reinterpret_cast<const GsCameraUniforms*>(cam);
+    GsCullOptions opts = {r->sh_degree, r->sh_lod_distance, r->splat_diagnostics_enabled ? 1 : 0};
+    int st = gs_cull(r->ctx, gcam, reinterpret_cast<const GsSplatEffectParams*>(splat_effect), &opts);
+    if (st != GS_OK) {
+        log_error("cull", st);
+        r->last_frame_status = st;
+        return;
+    }
+    r->splat_diagnostics.valid = false;
+    if (r->splat_diagnostics_enabled) {
+        GsStats s;
+        if (gs_stats(r->ctx, &s) == GS_OK && s.diagnostics_valid) {
+            memcpy(&r->splat_diagnostics, s.diagnostics, sizeof(s.diagnostics));
+            r->splat_diagnostics.valid = true;
+        }
+    }
+    // no indirect draw count in the reference either (src/renderer.cpp:2177,2187)
+    scene->visible_count = scene->gaussian_count;
+
+    if (!gps) {
+        st = gs_sort(r->ctx);
+        if (st == GS_OK) st = gs_composite_alpha(r->ctx, 0);
+        if (st != GS_OK) {
+            log_error("sorted-alpha frame", st);
+            r->last_frame_status = st;
+            return;
+        }
+        r->appearance_width = (int)cam->viewport[0];
+        r->appearance_height = (int)cam->viewport[1];
+        return;
+    }
+
+    if (r->stochastic_prev_cam_valid && memcmp(&r->stochastic_prev_cam, cam, sizeof(*cam)) != 0)
+        renderer_reset_stochastic_accumulation(r);
+    memcpy(&r->stochastic_prev_cam, cam, sizeof(*cam));
+    r->stochastic_prev_cam_valid = true;
+    r->stochastic_frame_seed++;
+
+    uint32_t ss = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
+    const float bg = r->black_background ? 0.0f : 0.1f;
+    st = gs_gps_sample(r->ctx, r->stochastic_frame_seed, ss, 0, bg);
+    if (st != GS_OK) {
+        log_error("GPS sample", st);
+        r->last_frame_status = st;
+        return;
+    }
+    r->gps_gpu.width = (uint32_t)cam->viewport[0];
+    r->gps_gpu.height = (uint32_t)cam->viewport[1];
+    r->gps_gpu.supersample_factor = ss;
+    r->stochastic_width = r->gps_gpu.width;
+    r->stochastic_height = r->gps_gpu.height;
+    double t = now_s();
+    if (!r->gps_gpu.work_stats_valid || t - g_last_stats_time >= 0.5) {
+        GsStats s;
+        if (gs_stats(r->ctx, &s) == GS_OK) {
+            r->gps_gpu.requested_work_items = s.gps_requested_work_items;
+            r->gps_gpu.work_budget_exhausted = s.gps_work_budget_exhausted != 0;
+            r->gps_gpu.work_stats_valid = true;
+            r->gps_gpu.work_stats_elapsed = 0.0f;
+            g_last_stats_time = t;
+        }
+    }
+    if (!r->stochastic_accumulation_enabled) {
+        r->stochastic_sample_count = 0;
+        return;
+    }
+    uint32_t next = r->stochastic_sample_count + 1u;
+    st = gs_accumulate(r->ctx, next);
+    if (st != GS_OK) {
+        log_error("accumulate", st);
+        r->last_frame_status = st;
+        return;
+    }
+    r->stochastic_sample_count = next;
+    r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
+}
+
+bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
+    if (!r->ctx || !out) return false;
+    if (r->splat_render_mode == SplatRenderMode::AlphaBlendSorted) {
+        size_t px = (size_t)r->appearance_width * r->appearance_height;
+        if (floats != px * 4) return false;
+        std::vector<uint16_t> h(px * 4);
+        int st = gs_read_image(r->ctx, GS_IMAGE_ALPHA_RGBA16F, h.data(), h.size() * 2);
+        if (st != GS_OK) {
+            log_error("read appearance image", st);
+            return false;
+        }
+        for (size_t i = 0; i < h.size(); ++i) out[i] = half_to_float(h[i]);
+        return true;
+    }
+    size_t px = (size_t)r->gps_gpu.width * r->gps_gpu.height;
+    if (floats != px * 4) return false;
+    int which = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
+    int st = gs_read_image(r->ctx, which, out, floats * 4);
+    if (st != GS_OK) {
+        log_error("read stochastic image", st);
+        return false;
+    }
+    return true;
+}
+
+void renderer_destroy(Renderer* r) {
+    if (r->ctx) gs_destroy(r->ctx);
+    r->ctx = nullptr;
+}

@@ -1,0 +1,153 @@
+// gs_renderer.h - the splat path of gamesplat's renderer.h (src/renderer.h:249-266)
+// implemented over the gs_* C ABI, for headless use.
+//
+// Function names, argument meaning and return conventions are the
+// reference's; the Renderer struct keeps the fields src/main.cpp and the
+// harness read or write (src/main.cpp:341-470,1344-1470,
+// tests/gsanim_render.cpp:46,97-113,148-156,186).  sokol handles
+// (pipelines, images, views) are replaced by one gs_ctx; mesh, overlay and
+// ImGui passes are not part of this build.
+//
+// When compiled inside the reference tree, define GS_RENDERER_USE_REFERENCE_TYPES
+// and include the reference's gaussian.h / camera.h first: the data types
+// below are then taken from there (they are byte-identical).
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/gs_api.h"
+
+#ifndef GS_RENDERER_USE_REFERENCE_TYPES
+#define GAUSSIAN_SH_REST_FLOATS 45
+struct Gaussian {
+    float position[3];
+    float scale[3];
+    float rotation[4];  // w, x, y, z
+    float color[3];
+    float opacity;
+    float sh_rest[GAUSSIAN_SH_REST_FLOATS];
+};
+struct GSAnim;
+struct GaussianScene {
+    GSAnim* animation;
+    Gaussian* gaussians;
+    uint32_t gaussian_count;
+    uint32_t* visible_indices;
+    float* visible_depths;
+    uint32_t visible_count;
+    uint32_t* sorted_indices;
+    uint32_t* scratch_indices;
+    uint32_t* scratch_keys;
+    uint32_t* scratch_keys2;
+};
+struct CameraUniforms {
+    float view[16];
+    float proj[16];
+    float viewport[2];
+    float orthographic;
+    float persp_focal;
+    float ortho_focal;
+    float clip_y_sign;
+    float clip_z_01;
+    float pad[1];
+};
+struct SplatEffectParams {
+    float center_radius[4];
+    float params[4];
+    float color[4];
+};
+struct OverlayParams;
+struct NodeRenderParams;
+#endif
+
+static_assert(sizeof(Gaussian) == sizeof(GsGaussian), "Gaussian layout");
+static_assert(sizeof(CameraUniforms) == sizeof(GsCameraUniforms), "CameraUniforms layout");
+static_assert(sizeof(SplatEffectParams) == sizeof(GsSplatEffectParams), "SplatEffectParams layout");
+
+struct SplatDiagnostics {
+    uint32_t total_quad_kpix;
+    uint32_t max_quad_px;
+    uint32_t splats_over_1k_px;
+    uint32_t splats_over_16k_px;
+    uint32_t total_gps_kpoints;
+    uint32_t max_gps_points;
+    uint32_t gps_splats_over_1k_points;
+    uint32_t gps_splats_over_16k_points;
+    bool valid;
+};
+
+struct GpsCapabilities {
+    bool compute;
+    bool storage_images;
+    bool shader_int64;
+    bool atomic_min_64;
+    bool supported;
+};
+
+enum class SplatRenderMode {
+    AlphaBlendSorted = 0,
+    StochasticSplats = 1,
+    GaussianPointSplatting = 2,
+};
+
+struct GaussianPointSplatGpu {
+    uint32_t width;
+    uint32_t height;
+    uint32_t supersample_factor;
+    uint32_t max_work_items;
+    uint32_t requested_work_items;
+    float work_stats_elapsed;
+    bool work_stats_valid;
+    bool work_budget_exhausted;
+};
+
+struct Renderer {
+    SplatRenderMode splat_render_mode;
+    gs_ctx* ctx;                        // replaces every sokol resource of the splat path
+    int appearance_width, appearance_height;
+    float exposure_ev;
+    bool first_layer_only;
+    double animation_time;
+    bool black_background;
+    size_t gpu_budget_bytes;
+    bool gps_work_budget_resize_failed;
+    uint32_t sort_capacity;
+    uint32_t gaussian_count;
+    bool splat_diagnostics_enabled;
+    SplatDiagnostics splat_diagnostics;
+    GpsCapabilities gps_capabilities;
+    int sh_degree;
+    float sh_lod_distance;
+    GaussianPointSplatGpu gps_gpu;
+    uint32_t stochastic_width;
+    uint32_t stochastic_height;
+    uint32_t stochastic_sample_count;
+    uint32_t stochastic_samples_per_frame;
+    uint32_t stochastic_taa_current_samples;
+    uint32_t gps_supersample_factor;
+    uint32_t gps_max_work_items;
+    uint32_t stochastic_frame_seed;
+    uint32_t stochastic_accum_write_index;
+    CameraUniforms stochastic_prev_cam;
+    bool stochastic_prev_cam_valid;
+    bool stochastic_accumulation_enabled;
+    bool stochastic_taa_enabled;
+    int last_frame_status;              // gs status of the last renderer_draw_frame (0 = ok)
+};
+
+bool renderer_init(Renderer* r);
+bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene);
+bool renderer_set_gps_work_budget(Renderer* r, uint32_t max_work_items);
+void renderer_reset_stochastic_accumulation(Renderer* r);
+void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms* cam,
+                         const OverlayParams* overlay, const NodeRenderParams* nodes,
+                         const SplatEffectParams* splat_effect = NULL, float wireframe_occlusion = 1.0f,
+                         const CameraUniforms* map_cam = NULL);
+void renderer_destroy(Renderer* r);
+
+// Headless readback of what the reference harness captures with
+// linear_capture (tests/gsanim_render.cpp:144-165): raw float32 RGBA, top row
+// first, of the appearance image (sorted alpha) or the accumulation / GPS
+// sample image (stochastic modes).  out holds width*height*4 floats.
+bool renderer_read_linear(Renderer* r, float* out, size_t floats);
