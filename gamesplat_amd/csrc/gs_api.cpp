@@ -88,6 +88,7 @@ struct gs_ctx {
     bool cells_dirty = true;
     int keep_cells = 0;
     int default_tile = 16;
+    int quadrant_cull = 1;
 
     GsCameraUniforms cam{};
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
@@ -309,6 +310,9 @@ int gs_set_option(gs_ctx* c, int option, int value) {
         case GS_OPT_TILE_SIZE:
             if (value != 8 && value != 16 && value != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
             c->default_tile = value;
+            return ok();
+        case GS_OPT_QUADRANT_CULL:
+            c->quadrant_cull = value ? 1 : 0;
             return ok();
         default:
             return fail(GS_ERR_INVALID, "unknown option %d", option);
@@ -556,6 +560,7 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     cl.tiles_x = tiles_x;
     cl.tiles_y = tiles_y;
     cl.image = ptr<uint16_t>(c->image);
+    cl.quadrant_cull = c->quadrant_cull;
     TimedLaunch t2{};
     GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
     GS_HIP(launch_composite(cl, c->stream));

@@ -232,6 +232,99 @@ __global__ void __launch_bounds__(TileShape<T>::kThreads) composite_kernel(Compo
     }
 }
 
+// 16x16 tiles, one block of four wave64s, each wave owning one 8x8 quadrant.
+// Every splat of an LDS batch is tested against the quadrant's pixel range
+// with one ballot per 64 splats; the wave then walks only the overlapping
+// splats (scalar bit iteration), and stops as soon as all 64 of its pixels
+// are saturated.  Coverage and fragment arithmetic are those of
+// composite_kernel.
+__global__ void __launch_bounds__(256) composite_quad_kernel(CompositeLaunch a) {
+    constexpr int BATCH = 256;
+    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
+    __shared__ float4 s_opa[BATCH];   // k2, opacity, cutoff, -
+    __shared__ float4 s_col[BATCH];   // r, g, b, -
+    __shared__ int4 s_rect[BATCH];    // covered pixels [i0, i1) x [j0, j1)
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
+    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
+    const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
+    const uint2 range = a.ranges[tile];
+    const float vw = (float)a.width, vh = (float)a.height;
+
+    const int qx0 = tx * 16 + (int)(wave & 1u) * 8, qy0 = ty * 16 + (int)(wave >> 1) * 8;
+    const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = 1.0f;
+    bool live = px < a.width && py < a.height;
+
+    for (uint32_t base = range.x; base < range.y; base += BATCH) {
+        if (!__syncthreads_or(live)) break;
+        uint32_t e = base + tid;
+        if (e < range.y) {
+            uint32_t g = a.tile_vals[e];
+            float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
+            float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
+            float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
+            float det = r1.x * r1.z - r1.y * r1.y;
+            if (det < 1e-6f) det = 1e-6f;
+            float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+            float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
+            int4 q = quad_fixed(r0.x, r0.y, r1.w, r2.w, vw, vh);
+            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
+            s_opa[tid] = make_float4(k2, r0.z, cutoff, 0.0f);
+            s_col[tid] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+            s_rect[tid] = make_int4(ceil_div256(q.x - 128), ceil_div256(q.y - 128), ceil_div256(q.z - 128),
+                                    ceil_div256(q.w - 128));
+        }
+        __syncthreads();
+        const uint32_t cnt = min((uint32_t)BATCH, range.y - base);
+        if (__ballot(live)) {
+            for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
+                // which of these 64 splats touch this wave's quadrant
+                uint32_t j = chunk + lane;
+                bool hit = false;
+                if (j < cnt) {
+                    int4 r = s_rect[j];
+                    hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
+                }
+                uint64_t mask = __ballot(hit);
+                while (mask) {
+                    const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const int4 r = s_rect[k];
+                    if (!live || px < r.x || px >= r.y || py < r.z || py >= r.w) continue;
+                    const float4 geo = s_geo[k];
+                    const float4 opa = s_opa[k];
+                    float dx = fx - geo.x;
+                    float dy = fy - geo.y;
+                    float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + opa.x * dy * dy);
+                    if (power > 0.0f || power < opa.z) continue;
+                    const float4 col = s_col[k];
+                    float alpha = minf(opa.y * __expf(power), 0.99f);
+                    float w = alpha * tr;
+                    cr += col.x * w;
+                    cg += col.y * w;
+                    cb += col.z * w;
+                    tr *= (1.0f - alpha);
+                    if (tr < kTransmittanceMin) live = false;
+                }
+                if (!__ballot(live)) break;
+            }
+        }
+    }
+
+    if (px < a.width && py < a.height) {
+        size_t o = ((size_t)py * a.width + px) * 4;
+        ushort4 h;
+        h.x = __half_as_ushort(__float2half_rn(cr));
+        h.y = __half_as_ushort(__float2half_rn(cg));
+        h.z = __half_as_ushort(__float2half_rn(cb));
+        h.w = __half_as_ushort(__float2half_rn(1.0f - tr));
+        *reinterpret_cast<ushort4*>(a.image + o) = h;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_bin_count(const BinLaunch& a, hipStream_t s) {
@@ -262,7 +355,10 @@ hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s) {
             hipLaunchKernelGGL(composite_kernel<8>, dim3(ntiles), dim3(TileShape<8>::kThreads), 0, s, a);
             break;
         case 16:
-            hipLaunchKernelGGL(composite_kernel<16>, dim3(ntiles), dim3(TileShape<16>::kThreads), 0, s, a);
+            if (a.quadrant_cull)
+                hipLaunchKernelGGL(composite_quad_kernel, dim3(ntiles), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL(composite_kernel<16>, dim3(ntiles), dim3(TileShape<16>::kThreads), 0, s, a);
             break;
         case 32:
             hipLaunchKernelGGL(composite_kernel<32>, dim3(ntiles), dim3(TileShape<32>::kThreads), 0, s, a);

@@ -224,6 +224,7 @@ struct CompositeLaunch {
     const uint2* ranges;
     int width, height, tile, tiles_x, tiles_y;
     uint16_t* image;         // RGBA16F, framebuffer row order
+    int quadrant_cull;       // 16x16 tiles: per-wave 8x8 quadrant culling kernel
 };
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s);
 

@@ -206,7 +206,10 @@ enum {
      * 0 (default): the resolve clears the cells it reads. */
     GS_OPT_GPS_KEEP_CELLS = 0,
     /* Default tile size of gs_composite_alpha when 0 is passed (8, 16, 32). */
-    GS_OPT_TILE_SIZE = 1
+    GS_OPT_TILE_SIZE = 1,
+    /* 16x16 tiles: 1 (default) = per-wave 8x8 quadrant culling kernel,
+     * 0 = one pixel per thread over every splat of the tile. */
+    GS_OPT_QUADRANT_CULL = 2
 };
 int gs_set_option(gs_ctx* ctx, int option, int value);
 

@@ -1,0 +1,102 @@
+"""C ABI behaviour on the GPU: error paths forced from the host (memory budget
+refusals, budget resize that keeps the old budget, calls out of order),
+repeatability, and the headless renderer shim's frame sequence."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gamesplat_amd as gs
+from gamesplat_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_out_of_order_calls_fail_with_state_error(gs):
+    with gs.Context(0) as c:
+        with pytest.raises(abi.GsError) as e:
+            c.sort()
+        assert e.value.code in (abi.GS_ERR_STATE,)
+        c.upload(gs.generate_scene(1, 1000, 2))
+        with pytest.raises(abi.GsError) as e:
+            c.composite()
+        assert e.value.code == abi.GS_ERR_STATE
+        with pytest.raises(abi.GsError) as e:
+            c.accumulate(1)
+        assert e.value.code == abi.GS_ERR_STATE
+
+
+def test_memory_budget_refuses_upload_and_keeps_scene(gs):
+    print("case: budget refusal", flush=True)
+    g = gs.generate_scene(2, 50_000, 4)
+    cam = scenes.random_view_camera(2, 320, 240)
+    with gs.Context(0) as c:
+        c.upload(g)
+        c.render_alpha(cam)
+        ref = c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F)
+        c.set_memory_budget(64 << 20)
+        with pytest.raises(abi.GsError) as e:
+            c.upload(gs.generate_scene(3, 2_000_000, 4))
+        assert e.value.code == abi.GS_ERR_LIMIT
+        c.set_memory_budget(0)
+        c.render_alpha(cam)  # the previous scene is still resident
+        np.testing.assert_array_equal(c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F), ref)
+
+
+def test_gps_budget_resize_refusals_keep_budget(gs):
+    with gs.Context(0) as c:
+        c.upload(gs.generate_scene(4, 10_000, 4))
+        c.set_gps_budget(1 << 20)
+        for bad in (0, (1 << 28) + 1):
+            with pytest.raises(abi.GsError) as e:
+                c.set_gps_budget(bad)
+            assert e.value.code == abi.GS_ERR_LIMIT
+        assert c.stats()["gps_max_work_items"] == 1 << 20
+
+
+def test_invalid_camera_and_options(gs):
+    with gs.Context(0) as c:
+        c.upload(gs.generate_scene(5, 1000, 2))
+        cam = scenes.random_view_camera(5, 320, 240)
+        cam.viewport[0] = 320.5
+        with pytest.raises(abi.GsError) as e:
+            c.cull(cam)
+        assert e.value.code == abi.GS_ERR_INVALID
+        cam = scenes.random_view_camera(5, 320, 240)
+        with pytest.raises(abi.GsError):
+            c.cull(cam, sh_degree=4)
+        c.cull(cam)
+        c.sort()
+        with pytest.raises(abi.GsError):
+            c.composite(12)
+
+
+def test_frames_are_deterministic(gs, c1_scene):
+    g, cfg, cam = c1_scene
+    print("case: determinism", flush=True)
+    with gs.Context(0) as c:
+        c.upload(g)
+        out = []
+        for _ in range(2):
+            c.render_alpha(cam)
+            out.append(c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).copy())
+            c.gps_sample(5, 2)
+            out.append(c.read_image(abi.GS_IMAGE_GPS_SAMPLE_RGBA32F).copy())
+        np.testing.assert_array_equal(out[0], out[2])
+        np.testing.assert_array_equal(out[1], out[3])
+
+
+def test_headless_driver_runs(tmp_path):
+    exe = ROOT / "gamesplat_amd" / "gs_headless"
+    for mode in ("alpha", "gps"):
+        cap = tmp_path / f"{mode}.f32"
+        r = subprocess.run([str(exe), "gaussians=20000", "width=320", "height=240", f"render_mode={mode}",
+                            "warmup=1", "frames=4", f"linear_capture={cap}"], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        line = [l for l in r.stdout.splitlines() if l.startswith("RESULT ")]
+        assert line and '"frames":4' in line[0]
+        img = np.fromfile(cap, np.float32).reshape(240, 320, 4)
+        assert np.isfinite(img).all() and img[..., 3].max() > 0

@@ -85,7 +85,8 @@ def test_sort_ties_and_sizes(ctx, orc, gs):
 def test_all_culled(ctx, orc, gs):
     print("case: all culled", flush=True)
     g = gs.generate_scene(3, 1000, 2, 0.0)
-    g[:, 0:3] += 1000.0  # far outside the frustum
+    # the cull has no far plane: put every splat behind the camera instead
+    g[:, 0:3] = g[:, 0:3] * 0.01 + scenes._direction(5).astype(np.float32) * 45.0
     cam = scenes.random_view_camera(5, 320, 240)
     ctx.upload(g)
     ctx.cull(cam)
@@ -97,13 +98,17 @@ def test_all_culled(ctx, orc, gs):
     assert not img.any()
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
-def test_composite_vs_fp64(ctx, orc, c1_scene, tile):
+@pytest.mark.parametrize("tile,quad", [(8, 1), (16, 1), (16, 0), (32, 1)])
+def test_composite_vs_fp64(ctx, orc, c1_scene, tile, quad):
     g, cfg, cam = c1_scene
-    print(f"case: composite c1 tile={tile}", flush=True)
+    print(f"case: composite c1 tile={tile} quadrant_cull={quad}", flush=True)
     ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
     ctx.sort()
-    ctx.composite(tile)
+    ctx.set_option(abi.GS_OPT_QUADRANT_CULL, quad)
+    try:
+        ctx.composite(tile)
+    finally:
+        ctx.set_option(abi.GS_OPT_QUADRANT_CULL, 1)
     img = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
     sk, si = orc.radix_sort(ok, oi)
     ref = orc.composite(op, si, cfg["width"], cfg["height"], mode=0)
