@@ -74,16 +74,19 @@ struct gs_ctx {
     size_t held = 0;
 
     uint32_t n = 0;
-    DevBuf planes, records, keys[3], ids[3], offsets, status, counters;
-    uint32_t status_words = 0;
-    uint32_t epoch = 1;
+    DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, table,
+        counters;
 
     DevBuf dup_keys[3], dup_vals[3], ranges, image;
-    uint32_t dup_cap = 0, last_dup = 0;
+    uint32_t dup_cap = 0, last_dup = 0, overflow_reruns = 0;
+    uint32_t* dup_host = nullptr;     // pinned copy of the requested pair count
+    hipEvent_t dup_event = nullptr;
+    bool dup_pending = false;
+    int composite_tile = 16;
     int img_w = 0, img_h = 0;
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
-    DevBuf work, prepared, item_base, cells, sample, depth, accum;
+    DevBuf work, prepared, item_count, item_base, cells, sample, depth, accum;
     int gps_w = 0, gps_h = 0, gps_ss = 1, res_w = 0, res_h = 0, acc_w = 0, acc_h = 0;
     bool cells_dirty = true;
     int keep_cells = 0;
@@ -148,33 +151,8 @@ T* ptr(DevBuf& b) {
     return static_cast<T*>(b.p);
 }
 
-uint32_t next_epoch(gs_ctx* c, uint32_t launches) {
-    uint32_t e = c->epoch;
-    c->epoch += launches;
-    return e;
-}
-
-// Status words are tagged with 30-bit epochs; before the tag space wraps the
-// array is cleared so no stale word can carry a current epoch.
-int reserve_epochs(gs_ctx* c, uint32_t launches) {
-    if (c->epoch + launches >= 0x3FFFFFF0u) {
-        GS_HIP(launch_fill_u64(ptr<uint64_t>(c->status), 0ull, c->status_words, c->stream));
-        c->epoch = 1;
-    }
-    return GS_OK;
-}
-
-int ensure_status(gs_ctx* c, uint32_t words) {
-    if (words <= c->status_words) return GS_OK;
-    GS_CHECK(ensure(c, c->status, (size_t)words * 8, "look-back status"));
-    c->status_words = (uint32_t)(c->status.bytes / 8);
-    GS_HIP(launch_fill_u64(ptr<uint64_t>(c->status), 0ull, c->status_words, c->stream));
-    return GS_OK;
-}
-
-uint32_t sort_status_words(uint32_t elems) {
-    uint32_t tile = onesweep_tile_elems();
-    return ((elems + tile - 1) / tile) * 256u + 256u;
+int ensure_table(gs_ctx* c, uint32_t elems) {
+    return ensure(c, c->table, (size_t)sort_table_words(elems) * 4, "radix tile table");
 }
 
 int timed_begin(gs_ctx* c, int stage, TimedLaunch* t) {
@@ -257,7 +235,9 @@ int gs_create(gs_ctx** out, int device) {
     int r = ensure(c, c->counters, sizeof(DevCounters), "counters");
     if (r == GS_OK) {
         e = hipMemsetAsync(c->counters.p, 0, sizeof(DevCounters), c->stream);
-        if (e != hipSuccess) r = fail(GS_ERR_HIP, "counter init failed: %s", hipGetErrorString(e));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&c->dup_host, 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->dup_event, hipEventDisableTiming);
+        if (e != hipSuccess) r = fail(GS_ERR_HIP, "context init failed: %s", hipGetErrorString(e));
     }
     if (r != GS_OK) {
         gs_destroy(c);
@@ -277,11 +257,13 @@ void gs_destroy(gs_ctx* c) {
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->ids[0], &c->ids[1],
-                      &c->ids[2], &c->offsets, &c->status, &c->counters, &c->dup_keys[0], &c->dup_keys[1],
-                      &c->dup_keys[2], &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges,
-                      &c->image, &c->work, &c->prepared, &c->item_base, &c->cells, &c->sample, &c->depth,
-                      &c->accum};
+                      &c->ids[2], &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
+                      &c->rects, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
+                      &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
+                      &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum};
     for (DevBuf* b : bufs) release(c, *b);
+    if (c->dup_event) (void)hipEventDestroy(c->dup_event);
+    if (c->dup_host) (void)hipHostFree(c->dup_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -324,12 +306,14 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     if (count == 0 || count > kMaxGaussians)
         return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
     GS_CHECK(set_device(c));
-    // Scene-resident bytes: planes 240 + record 48 + keys/ids 24 + offsets 4
-    // + GPS prepared 48 + item base 4 per Gaussian, plus the GPS work list.
-    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 48 + 4) + (size_t)c->gps_budget * 4;
+    // Scene-resident bytes per Gaussian: planes 240 + record 48 + keys/ids 24
+    // + dense key 4 + bin rectangle 8 + GPS prepared 48 + item count/base 8,
+    // plus the GPS work list.
+    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 8 + 48 + 8) + (size_t)c->gps_budget * 4;
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
                         c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
-                        c->offsets.bytes - c->prepared.bytes - c->item_base.bytes - c->work.bytes;
+                        c->dense_keys.bytes - c->rects.bytes - c->prepared.bytes - c->item_count.bytes -
+                        c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
         return fail(GS_ERR_LIMIT, "scene of %u Gaussians needs %.1f MiB; memory budget is %.1f MiB", count,
                     required / 1048576.0, c->budget / 1048576.0);
@@ -381,8 +365,12 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
         GS_CHECK(ensure(c, c->keys[k], (size_t)count * 4, "sort keys"));
         GS_CHECK(ensure(c, c->ids[k], (size_t)count * 4, "sort ids"));
     }
-    GS_CHECK(ensure(c, c->offsets, (size_t)count * 4, "bin offsets"));
-    GS_CHECK(ensure_status(c, std::max(sort_status_words(count), (count + 255u) / 256u + 256u)));
+    const size_t blocks = (count + 255u) / 256u;
+    GS_CHECK(ensure(c, c->dense_keys, (size_t)count * 4, "dense keys"));
+    GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
+    GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
+    GS_CHECK(ensure(c, c->rects, (size_t)count * 8, "bin rectangles"));
+    GS_CHECK(ensure_table(c, count));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
     return ok();
 }
@@ -422,7 +410,9 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     int sh_degree = opts ? opts->sh_degree : 3;
     if (sh_degree < 0 || sh_degree > 3) return fail(GS_ERR_INVALID, "sh_degree %d outside [0, 3]", sh_degree);
     GS_CHECK(set_device(c));
-    GS_CHECK(reserve_epochs(c, 1));
+    const uint32_t blocks = (c->n + 255u) / 256u;
+    const int collect = opts ? opts->collect_stats : 0;
+    if (collect) GS_CHECK(ensure(c, c->block_diag, (size_t)blocks * 32, "block diagnostics"));
     c->cam = *cam;
     CullLaunch a{};
     a.planes = ptr<float4>(c->planes);
@@ -432,17 +422,18 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     a.effect_active = effect && effect->params[3] > 0.5f;
     a.sh_degree = sh_degree;
     a.sh_lod_distance = opts ? opts->sh_lod_distance : 0.0f;
-    a.collect_stats = opts ? opts->collect_stats : 0;
+    a.collect_stats = collect;
     a.records = ptr<float4>(c->records);
-    a.vis_keys = ptr<uint32_t>(c->keys[0]);
-    a.vis_ids = ptr<uint32_t>(c->ids[0]);
-    a.counters = c->dc();
-    a.status = ptr<uint64_t>(c->status);
-    a.epoch = next_epoch(c, 1);
+    a.dense_keys = ptr<uint32_t>(c->dense_keys);
+    a.block_counts = ptr<uint32_t>(c->block_counts);
+    a.block_diag = collect ? ptr<uint32_t>(c->block_diag) : nullptr;
     TimedLaunch t{};
-    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->cull), 0u, sizeof(CullCounters) / 4, c->stream));
     GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_cull(a, c->stream));
+    GS_HIP(launch_scan_counts(ptr<uint32_t>(c->block_counts), blocks, ptr<uint32_t>(c->block_offsets),
+                              &c->dc()->cull.visible_count, a.block_diag, c->dc()->cull.diag, c->stream));
+    GS_HIP(launch_compact(ptr<uint32_t>(c->dense_keys), c->n, ptr<uint32_t>(c->block_offsets),
+                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), c->stream));
     GS_CHECK(timed_end(c, &t));
     c->have_cull = true;
     c->have_sort = c->have_composite = false;
@@ -454,7 +445,6 @@ int gs_sort(gs_ctx* c) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
     if (!c->have_cull) return fail(GS_ERR_STATE, "gs_sort needs a gs_cull first");
     GS_CHECK(set_device(c));
-    GS_CHECK(reserve_epochs(c, 4));
     SortLaunch s{};
     for (int k = 0; k < 3; ++k) {
         s.keys[k] = ptr<uint32_t>(c->keys[k]);
@@ -463,17 +453,15 @@ int gs_sort(gs_ctx* c) {
     s.count = &c->dc()->cull.visible_count;
     s.capacity = c->n;
     s.hist = &c->dc()->depth_sort.hist[0][0];
-    s.tickets = c->dc()->depth_sort.ticket;
-    s.status = ptr<uint64_t>(c->status);
+    s.table = ptr<uint32_t>(c->table);
     s.passes = 4;
     s.bits = 8;
-    s.first_epoch = next_epoch(c, 4);
     TimedLaunch t{};
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->depth_sort), 0u, sizeof(SortCounters) / 4, c->stream));
     GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
     GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->n, s.hist, 4, 8, c->stream));
     int slot = 0;
-    GS_HIP(launch_onesweep(s, &slot, c->stream));
+    GS_HIP(launch_radix_sort(s, &slot, c->stream));
     GS_CHECK(timed_end(c, &t));
     c->sort_slot = slot;
     c->have_sort = true;
@@ -481,18 +469,33 @@ int gs_sort(gs_ctx* c) {
     return ok();
 }
 
-int gs_composite_alpha(gs_ctx* c, int tile) {
-    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
-    if (!c->have_sort) return fail(GS_ERR_STATE, "gs_composite_alpha needs gs_cull and gs_sort first");
-    if (tile == 0) tile = c->default_tile;
-    if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
-    GS_CHECK(set_device(c));
+// (tile id, splat id) buffers are sized by capacity so a frame needs no host
+// round trip.  The requested pair count is copied back asynchronously; before
+// any composite result is observed (image readback, stats, synchronize) the
+// count is checked, and a frame that overflowed is re-binned with grown
+// buffers.  A new frame also grows the buffers as soon as the previous
+// frame's count is known.
+static int grow_tile_buffers(gs_ctx* c, uint32_t entries) {
+    if (entries > 0x7FFFFFFFu) return fail(GS_ERR_LIMIT, "%u tile entries exceed the 2^31 limit", entries);
+    if (entries <= c->dup_cap) return GS_OK;
+    for (int k = 0; k < 3; ++k) {
+        GS_CHECK(ensure(c, c->dup_keys[k], (size_t)entries * 4, "tile keys"));
+        GS_CHECK(ensure(c, c->dup_vals[k], (size_t)entries * 4, "tile values"));
+    }
+    GS_CHECK(ensure_table(c, std::max(entries, c->n)));
+    c->dup_cap = entries;
+    return GS_OK;
+}
+
+static int enqueue_composite(gs_ctx* c, int tile) {
     const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
     const int tiles_x = (W + tile - 1) / tile, tiles_y = (H + tile - 1) / tile;
     const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
     GS_CHECK(ensure(c, c->image, (size_t)W * H * 8, "RGBA16F image"));
     GS_CHECK(ensure(c, c->ranges, (size_t)ntiles * 8, "tile ranges"));
-    GS_CHECK(reserve_epochs(c, 1));
+    if (!c->dup_cap) GS_CHECK(grow_tile_buffers(c, std::max(3u * c->n, 65536u)));
+    int passes = 1;
+    while (passes < 4 && (ntiles - 1u) >> (8 * passes)) ++passes;
 
     BinLaunch b{};
     b.records = ptr<float4>(c->records);
@@ -504,50 +507,36 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     b.tile = tile;
     b.tiles_x = tiles_x;
     b.tiles_y = tiles_y;
-    b.offsets = ptr<uint32_t>(c->offsets);
     b.counters = c->dc();
-    b.status = ptr<uint64_t>(c->status);
-    b.epoch = next_epoch(c, 1);
-    TimedLaunch t{};
-    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->bin), 0u, sizeof(BinCounters) / 4, c->stream));
-    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
-    GS_HIP(launch_bin_count(b, c->stream));
-    uint32_t dup = 0;
-    GS_CHECK(read_u32(c, &c->dc()->bin.dup_total, &dup));
-    if (dup > 0x7FFFFFFFu) return fail(GS_ERR_LIMIT, "%u tile entries exceed the 2^31 limit", dup);
-    for (int k = 0; k < 3; ++k) {
-        GS_CHECK(ensure(c, c->dup_keys[k], (size_t)dup * 4, "tile keys", 0.25));
-        GS_CHECK(ensure(c, c->dup_vals[k], (size_t)dup * 4, "tile values", 0.25));
-    }
-    GS_CHECK(ensure_status(c, sort_status_words(dup)));
+    b.rects = ptr<uint2>(c->rects);
+    b.block_counts = ptr<uint32_t>(c->block_counts);
+    b.block_offsets = ptr<uint32_t>(c->block_offsets);
     b.dup_keys = ptr<uint32_t>(c->dup_keys[0]);
     b.dup_vals = ptr<uint32_t>(c->dup_vals[0]);
-    GS_HIP(launch_bin_emit(b, c->stream));
+    b.dup_capacity = c->dup_cap;
 
-    int passes = 1;
-    while (passes < 4 && (ntiles - 1u) >> (8 * passes)) ++passes;
-    GS_CHECK(reserve_epochs(c, (uint32_t)passes));
     SortLaunch s{};
     for (int k = 0; k < 3; ++k) {
         s.keys[k] = ptr<uint32_t>(c->dup_keys[k]);
         s.vals[k] = ptr<uint32_t>(c->dup_vals[k]);
     }
-    s.count = &c->dc()->bin.dup_total;
-    s.capacity = dup;
+    s.count = &c->dc()->bin.dup_count;
+    s.capacity = c->dup_cap;
     s.hist = &c->dc()->tile_sort.hist[0][0];
-    s.tickets = c->dc()->tile_sort.ticket;
-    s.status = ptr<uint64_t>(c->status);
+    s.table = ptr<uint32_t>(c->table);
     s.passes = passes;
     s.bits = 8;
-    s.first_epoch = next_epoch(c, (uint32_t)passes);
+
+    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->bin), 0u, sizeof(BinCounters) / 4, c->stream));
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->tile_sort), 0u, sizeof(SortCounters) / 4, c->stream));
-    int slot = 0;
-    if (dup > 0) {
-        GS_HIP(launch_radix_histogram(s.keys[0], s.count, dup, s.hist, passes, 8, c->stream));
-        GS_HIP(launch_onesweep(s, &slot, c->stream));
-    }
     GS_HIP(launch_fill_u32(ptr<uint32_t>(c->ranges), 0u, (size_t)ntiles * 2, c->stream));
-    if (dup > 0) GS_HIP(launch_tile_ranges(s.keys[slot], dup, ptr<uint2>(c->ranges), c->stream));
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
+    GS_HIP(launch_bin(b, c->stream));
+    GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->dup_cap, s.hist, passes, 8, c->stream));
+    int slot = 0;
+    GS_HIP(launch_radix_sort(s, &slot, c->stream));
+    GS_HIP(launch_tile_ranges(s.keys[slot], s.count, c->dup_cap, ptr<uint2>(c->ranges), c->stream));
     GS_CHECK(timed_end(c, &t));
 
     CompositeLaunch cl{};
@@ -565,9 +554,48 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
     GS_HIP(launch_composite(cl, c->stream));
     GS_CHECK(timed_end(c, &t2));
+    GS_HIP(hipMemcpyAsync(c->dup_host, &c->dc()->bin.dup_total, 4, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipEventRecord(c->dup_event, c->stream));
+    c->dup_pending = true;
+    c->composite_tile = tile;
     c->img_w = W;
     c->img_h = H;
-    c->last_dup = dup;
+    return GS_OK;
+}
+
+// Waits for the pending pair count (or only polls when wait is false); grows
+// the buffers on overflow and, when the current composite overflowed and
+// rerun is set, composites again until every pair fits.
+static int settle_composite(gs_ctx* c, bool wait, bool rerun) {
+    while (c->dup_pending) {
+        if (wait) {
+            GS_HIP(hipEventSynchronize(c->dup_event));
+        } else {
+            hipError_t q = hipEventQuery(c->dup_event);
+            if (q == hipErrorNotReady) return GS_OK;
+            GS_HIP(q);
+        }
+        c->dup_pending = false;
+        const uint32_t total = *c->dup_host;
+        c->last_dup = total;
+        if (total <= c->dup_cap) return GS_OK;
+        GS_CHECK(grow_tile_buffers(c, total + total / 4u));
+        if (!(rerun && c->have_composite)) return GS_OK;
+        c->overflow_reruns++;
+        GS_CHECK(enqueue_composite(c, c->composite_tile));
+    }
+    return GS_OK;
+}
+
+int gs_composite_alpha(gs_ctx* c, int tile) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_sort) return fail(GS_ERR_STATE, "gs_composite_alpha needs gs_cull and gs_sort first");
+    if (tile == 0) tile = c->default_tile;
+    if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, false, false));
+    c->have_composite = false;
+    GS_CHECK(enqueue_composite(c, tile));
     c->have_composite = true;
     return ok();
 }
@@ -590,10 +618,10 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     GS_CHECK(ensure(c, c->cells, (size_t)gw * gh * 8, "GPS cells"));
     GS_CHECK(ensure(c, c->work, (size_t)c->gps_budget * 4, "GPS work list"));
     GS_CHECK(ensure(c, c->prepared, (size_t)c->n * 48, "GPS prepared splats"));
+    GS_CHECK(ensure(c, c->item_count, (size_t)c->n * 4, "GPS item counts"));
     GS_CHECK(ensure(c, c->item_base, (size_t)c->n * 4, "GPS item offsets"));
     GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "GPS sample"));
     GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "GPS depth"));
-    GS_CHECK(reserve_epochs(c, 1));
     if (c->cells_dirty) {
         GS_HIP(launch_fill_u64(ptr<uint64_t>(c->cells), 0xFFFFFFFF00000000ull, (size_t)gw * gh, c->stream));
         c->cells_dirty = false;
@@ -615,10 +643,11 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     g.linear_color = linear_color;
     g.prepared = ptr<uint4>(c->prepared);
     g.work = ptr<uint32_t>(c->work);
+    g.item_count = ptr<uint32_t>(c->item_count);
     g.item_base = ptr<uint32_t>(c->item_base);
+    g.block_counts = ptr<uint32_t>(c->block_counts);
+    g.block_offsets = ptr<uint32_t>(c->block_offsets);
     g.counters = c->dc();
-    g.status = ptr<uint64_t>(c->status);
-    g.epoch = next_epoch(c, 1);
     g.cells = ptr<uint64_t>(c->cells);
     g.gps_width = gw;
     g.gps_height = gh;
@@ -672,6 +701,7 @@ int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
     switch (which) {
         case GS_IMAGE_ALPHA_RGBA16F:
             if (!c->have_composite) return fail(GS_ERR_STATE, "no composite yet");
+            GS_CHECK(settle_composite(c, true, true));
             GS_CHECK(copy_image(c, c->image.p, host, c->img_w, c->img_h, 8, bytes));
             return ok();
         case GS_IMAGE_GPS_SAMPLE_RGBA32F:
@@ -761,6 +791,7 @@ int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
 int gs_stats(gs_ctx* c, GsStats* out) {
     if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");
     GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, true, true));
     DevCounters h;
     GS_HIP(hipMemcpyAsync(&h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
@@ -771,7 +802,9 @@ int gs_stats(gs_ctx* c, GsStats* out) {
     out->gps_max_work_items = c->gps_budget;
     if (c->have_gps) {
         out->gps_requested_work_items = h.gps.requested;
-        out->gps_work_budget_exhausted = h.gps.overflow;
+        // reservations are contiguous from 0: some reservation crossed the
+        // budget (the reference's overflow flag) exactly when the total did
+        out->gps_work_budget_exhausted = h.gps.requested > c->gps_budget ? 1u : 0u;
         out->gps_dropped_work_items = h.gps.requested > c->gps_budget ? h.gps.requested - c->gps_budget : 0;
     }
     if (c->have_cull && c->stats_collected) {
@@ -784,6 +817,7 @@ int gs_stats(gs_ctx* c, GsStats* out) {
 int gs_synchronize(gs_ctx* c) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
     GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, true, true));
     GS_HIP(hipStreamSynchronize(c->stream));
     return ok();
 }

@@ -75,49 +75,67 @@ __device__ __forceinline__ TileRect splat_tiles(const float4* __restrict__ recor
     return t;
 }
 
+// Pass 1 over the depth-sorted splats, nearest first: tile rectangle and
+// pair count per splat, pair count per block.
 __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
-    __shared__ uint32_t s_ticket, s_prefix;
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_ticket = atomicAdd(&a.counters->bin.ticket, 1u);
-    __syncthreads();
-    const uint32_t block = s_ticket;
     const uint32_t V = *a.count;
-    if (block * 256u >= V) return;
-    const uint32_t q = block * 256u + tid;
+    const uint32_t q = blockIdx.x * 256u + tid;
     uint32_t cnt = 0u;
     if (q < V) {
-        uint32_t g = a.sorted_ids[V - 1u - q];
-        TileRect t = splat_tiles(a.records, g, a);
+        TileRect t = splat_tiles(a.records, a.sorted_ids[V - 1u - q], a);
         cnt = (uint32_t)((t.x1 - t.x0) * (t.y1 - t.y0));
+        a.rects[q] = make_uint2((uint32_t)t.x0 | ((uint32_t)t.x1 << 16), (uint32_t)t.y0 | ((uint32_t)t.y1 << 16));
     }
     uint32_t total;
-    uint32_t local = block_exclusive_sum<4>(cnt, s_scan, &total);
-    if (tid == 0) {
-        s_prefix = lookback(a.status, block, 1u, a.epoch, total);
-        if (total) atomicAdd(&a.counters->bin.dup_total, total);
-    }
-    __syncthreads();
-    if (q < V) a.offsets[q] = s_prefix + local;
+    (void)block_exclusive_sum<4>(cnt, s_scan, &total);
+    if (tid == 0) a.block_counts[blockIdx.x] = total;
 }
 
+// Pass 2 (after the block counts are scanned): each block writes the
+// (tile id, splat id) pairs of its 256 splats as one contiguous, coalesced
+// range.  Pairs past the capacity are not written.
 __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
+    __shared__ uint32_t s_scan[4];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_id[256];
+    __shared__ uint2 s_rect[256];
+    const uint32_t tid = threadIdx.x;
     const uint32_t V = *a.count;
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= V) return;
-    uint32_t g = a.sorted_ids[V - 1u - q];
-    TileRect t = splat_tiles(a.records, g, a);
-    uint32_t off = a.offsets[q];
-    for (int ty = t.y0; ty < t.y1; ++ty)
-        for (int tx = t.x0; tx < t.x1; ++tx) {
-            a.dup_keys[off] = (uint32_t)(ty * a.tiles_x + tx);
-            a.dup_vals[off] = g;
-            ++off;
-        }
+    if (blockIdx.x * 256u >= V) return;
+    const uint32_t q = blockIdx.x * 256u + tid;
+    uint32_t cnt = 0u;
+    if (q < V) {
+        uint2 r = a.rects[q];
+        cnt = ((r.x >> 16) - (r.x & 0xFFFFu)) * ((r.y >> 16) - (r.y & 0xFFFFu));
+        s_rect[tid] = r;
+        s_id[tid] = a.sorted_ids[V - 1u - q];
+    }
+    uint32_t total;
+    s_start[tid] = block_exclusive_sum<4>(cnt, s_scan, &total);
+    __syncthreads();
+    const uint32_t base = a.block_offsets[blockIdx.x];
+    const uint32_t room = base < a.dup_capacity ? a.dup_capacity - base : 0u;
+    const uint32_t emit = min(total, room);
+    for (uint32_t i = tid; i < emit; i += 256u) {
+        const uint32_t o = owner_of(s_start, 256u, i);
+        const uint2 r = s_rect[o];
+        const uint32_t x0 = r.x & 0xFFFFu, w = (r.x >> 16) - x0, y0 = r.y & 0xFFFFu;
+        const uint32_t k = i - s_start[o];
+        a.dup_keys[base + i] = (y0 + k / w) * (uint32_t)a.tiles_x + x0 + k % w;
+        a.dup_vals[base + i] = s_id[o];
+    }
 }
 
-__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t count, uint2* ranges) {
+// Entry count the tile sort works on: min(total, capacity).
+__global__ void clamp_count_kernel(const uint32_t* total, uint32_t capacity, uint32_t* clamped) {
+    if (threadIdx.x == 0) *clamped = min(*total, capacity);
+}
+
+__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint32_t* count_ptr, uint2* ranges) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = *count_ptr;
     if (i >= count) return;
     uint32_t k = keys[i];
     if (i == 0 || keys[i - 1] != k) ranges[k].x = i;
@@ -327,23 +345,23 @@ __global__ void __launch_bounds__(256) composite_quad_kernel(CompositeLaunch a) 
 
 }  // namespace
 
-hipError_t launch_bin_count(const BinLaunch& a, hipStream_t s) {
-    uint32_t blocks = (a.capacity + 255u) / 256u;
+hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
+    const uint32_t blocks = (a.capacity + 255u) / 256u;
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(bin_count_kernel, dim3(blocks), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_bin_emit(const BinLaunch& a, hipStream_t s) {
-    uint32_t blocks = (a.capacity + 255u) / 256u;
-    if (!blocks) return hipSuccess;
+    hipError_t e = launch_scan_counts(a.block_counts, blocks, a.block_offsets, &a.counters->bin.dup_total, nullptr,
+                                      nullptr, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bin_emit_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(clamp_count_kernel, dim3(1), dim3(64), 0, s, &a.counters->bin.dup_total, a.dup_capacity,
+                       &a.counters->bin.dup_count);
     return hipGetLastError();
 }
 
-hipError_t launch_tile_ranges(const uint32_t* keys, uint32_t count, uint2* ranges, hipStream_t s) {
-    if (!count) return hipSuccess;
-    hipLaunchKernelGGL(tile_ranges_kernel, dim3((count + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint32_t capacity, uint2* ranges,
+                              hipStream_t s) {
+    if (!capacity) return hipSuccess;
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((capacity + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
     return hipGetLastError();
 }
 

@@ -1,10 +1,12 @@
 // gs_cull.hip - per-Gaussian cull, EWA projection and SH colour.
 //
 // Replaces cull_reset_cs and cull_cs (shaders/cull.glsl:166-474).  One
-// thread per Gaussian, 256-thread blocks.  Each block takes a ticket so that
-// blocks run their decoupled look-back in issue order; the visible ids and
-// keys of the whole launch come out compacted in index order (the order a
-// stable sort over the reference's sentinel-padded arrays would keep).
+// thread per Gaussian, 256-thread blocks.  The cull writes the projected
+// splat, a dense depth key (0xFFFFFFFF when culled, as the reset leaves it)
+// and a per-block visible count; after a scan of those counts
+// compact_kernel moves the visible (key, id) pairs to one list in index
+// order, the order a stable sort over the reference's sentinel-padded
+// arrays keeps.
 //
 // Float policy: the library is built with -ffp-contract=off and every
 // expression below keeps the operand order of the GLSL source, so each
@@ -55,16 +57,13 @@ constexpr float kShC3_5 = 1.445305721320277f;
 constexpr float kShC3_6 = -0.5900435899266435f;
 
 __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
-    __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_scan[4];
-    __shared__ uint32_t s_prefix;
     __shared__ uint32_t s_diag[8];
 
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_ticket = atomicAdd(&a.counters->cull.ticket, 1u);
     if (tid < 8) s_diag[tid] = 0u;
     __syncthreads();
-    const uint32_t block = s_ticket;
+    const uint32_t block = blockIdx.x;
     const uint32_t idx = block * 256u + tid;
     const uint32_t n = a.n;
     const GsCameraUniforms& cam = a.cam;
@@ -290,41 +289,39 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
         }
     }
 
-    // order-preserving compaction of the visible (key, id) pairs
+    // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
+    // block's visible count; compaction runs after the block counts are scanned
     const uint32_t lane = tid & 63u, wave = tid >> 6;
+    if (idx < n) a.dense_keys[idx] = visible ? key : 0xFFFFFFFFu;
     uint64_t bal = __ballot(visible);
-    uint32_t rank = (uint32_t)__popcll(bal & lanemask_lt());
     if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
     __syncthreads();
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        uint32_t c = s_scan[w];
-        if ((uint32_t)w < wave) before += c;
-        total += c;
-    }
-    if (tid == 0) {
-        s_prefix = lookback(a.status, block, 1u, a.epoch, total);
-        if (total) atomicAdd(&a.counters->cull.visible_count, total);
-    }
-    if (a.collect_stats && tid < 8) {
-        uint32_t v = s_diag[tid];
-        if (tid == 1 || tid == 5) {
-            if (v) atomicMax(&a.counters->cull.diag[tid], v);
-        } else if (v) {
-            atomicAdd(&a.counters->cull.diag[tid], v);
-        }
-    }
-    __syncthreads();
-    if (visible) {
-        uint32_t pos = s_prefix + before + rank;
-        a.vis_keys[pos] = key;
-        a.vis_ids[pos] = idx;
-    }
+    if (tid == 0) a.block_counts[block] = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
+    if (a.collect_stats && tid < 8) a.block_diag[(size_t)block * 8 + tid] = s_diag[tid];
 }
 
-__global__ void clear_counters_kernel(uint32_t* c, uint32_t words) {
-    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) c[i] = 0u;
+// Order-preserving compaction: block b moves the visible entries of its 256
+// Gaussians to offsets[b] + rank.
+__global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict__ dense_keys, uint32_t n,
+                                                      const uint32_t* __restrict__ offsets, uint32_t* vis_keys,
+                                                      uint32_t* vis_ids) {
+    __shared__ uint32_t s_scan[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t idx = blockIdx.x * 256u + tid;
+    const uint32_t key = idx < n ? dense_keys[idx] : 0xFFFFFFFFu;
+    const bool visible = key != 0xFFFFFFFFu;
+    const uint64_t bal = __ballot(visible);
+    if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        if ((uint32_t)w < wave) before += s_scan[w];
+    if (visible) {
+        const uint32_t pos = offsets[blockIdx.x] + before + (uint32_t)__popcll(bal & lanemask_lt());
+        vis_keys[pos] = key;
+        vis_ids[pos] = idx;
+    }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
@@ -375,9 +372,11 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_clear_counters(DevCounters* c, hipStream_t s) {
-    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<uint32_t*>(c),
-                       (uint32_t)(sizeof(DevCounters) / 4));
+hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
+                          uint32_t* vis_ids, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, dense_keys, n, offsets, vis_keys,
+                       vis_ids);
     return hipGetLastError();
 }
 

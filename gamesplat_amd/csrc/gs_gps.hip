@@ -6,11 +6,12 @@
 //
 // Work-budget semantics are the reference's: each visible splat asks for
 // floor(E/ss^2) work items plus one with probability frac(E/ss^2) (frame
-// invariant hash), items are reserved from a global budget, a reservation
-// that does not fit raises the overflow flag and keeps only what fits.  The
-// reservation order is the visible-index order (a decoupled look-back scan
-// instead of an atomicAdd race), so which items are dropped on overflow is
-// deterministic.  Each work item stores only its owner (4 bytes instead of
+// invariant hash), items are reserved from a global budget, and a
+// reservation that does not fit keeps only what fits.  Reservations are
+// contiguous from 0, so the reference's overflow flag is exactly
+// "requested > budget".  The reservation order is the visible-index order
+// (count, scan, reserve instead of an atomicAdd race), so which items are
+// dropped on overflow is deterministic.  Each work item stores only its owner (4 bytes instead of
 // the reference's 8-byte (splat, item) pair); the item index is recovered
 // from the owner's base offset.
 //
@@ -29,30 +30,25 @@ namespace {
 
 constexpr uint64_t kEmptyCell = 0xFFFFFFFF00000000ull;
 
-__global__ void __launch_bounds__(256) gps_expand_kernel(GpsLaunch a) {
-    __shared__ uint32_t s_ticket, s_prefix;
+// Pass 1: work-item count (gps.glsl:127-142) and prepared record
+// (gps.glsl:152-169) per visible splat, item count per block.  The prepared
+// record is written for every splat with items; the reference writes it only
+// for reservations that start inside the budget, and no other record is read.
+__global__ void __launch_bounds__(256) gps_count_kernel(GpsLaunch a) {
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_ticket = atomicAdd(&a.counters->gps.ticket, 1u);
-    __syncthreads();
-    const uint32_t block = s_ticket;
     const uint32_t V = *a.vis_count;
-    if (block * 256u >= V) return;
-    const uint32_t v = block * 256u + tid;
+    const uint32_t v = blockIdx.x * 256u + tid;
     const float ss = maxf(a.ss, 1.0f);
-
-    uint32_t count = 0u, splat_id = 0u;
-    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
-    float cov_a = 0.f, cov_b = 0.f, cov_c = 0.f;
+    uint32_t count = 0u;
     if (v < V) {
-        splat_id = a.vis_ids[v];
-        r0 = a.records[(size_t)splat_id * kRecordVec4 + 0];
-        r1 = a.records[(size_t)splat_id * kRecordVec4 + 1];
-        r2 = a.records[(size_t)splat_id * kRecordVec4 + 2];
-        // second dilation and expected point count (gps.glsl:127-142)
-        cov_a = r1.x + 0.3f;
-        cov_b = r1.y;
-        cov_c = r1.z + 0.3f;
+        const uint32_t splat_id = a.vis_ids[v];
+        const float4 r0 = a.records[(size_t)splat_id * kRecordVec4 + 0];
+        const float4 r1 = a.records[(size_t)splat_id * kRecordVec4 + 1];
+        // second dilation and expected point count
+        const float cov_a = r1.x + 0.3f;
+        const float cov_b = r1.y;
+        const float cov_c = r1.z + 0.3f;
         float det = cov_a * cov_c - cov_b * cov_b;
         float expected_points = ss * ss * 6.28318530718f * __builtin_sqrtf(maxf(det, 0.0f)) *
             dilog(clampf(r0.z, 0.0f, 1.0f));
@@ -61,42 +57,54 @@ __global__ void __launch_bounds__(256) gps_expand_kernel(GpsLaunch a) {
         float fl = clampf(__builtin_floorf(scaled), 0.0f, 4294967295.0f);
         count = f2u_sat(fl);
         if (hash01(splat_id + 0x9e3779b9u) < fractf(scaled) && count < 0xFFFFFFFFu) count += 1u;
+        a.item_count[v] = count;
+        if (count) {
+            const float4 r2 = a.records[(size_t)splat_id * kRecordVec4 + 2];
+            float l00 = __builtin_sqrtf(maxf(cov_a, 1.0e-5f));
+            float l10 = cov_b / l00;
+            float l11 = __builtin_sqrtf(maxf(cov_c - l10 * l10, 1.0e-5f));
+            float alpha = clampf(r0.z, 1.0e-6f, 1.0f);
+            float dilog_alpha = dilog(alpha);
+            float ndc_depth = r0.w;
+            float z = mixf(ndc_depth * 0.5f + 0.5f, ndc_depth, a.clip_z_01);
+            uint32_t depth_key = f2u_sat(clampf(z, 0.0f, 1.0f) * 4294967294.0f);
+            uint32_t rr = (uint32_t)__builtin_rintf(clampf(r2.x, 0.0f, 1.0f) * 255.0f);
+            uint32_t gg = (uint32_t)__builtin_rintf(clampf(r2.y, 0.0f, 1.0f) * 255.0f);
+            uint32_t bb = (uint32_t)__builtin_rintf(clampf(r2.z, 0.0f, 1.0f) * 255.0f);
+            uint32_t packed = rr | (gg << 8) | (bb << 16) | 0xFF000000u;
+            if (a.linear_color) packed = splat_id;
+            uint4* pr = a.prepared + (size_t)v * 3;
+            pr[0] = make_uint4(fbits(r0.x), fbits(r0.y), fbits(l00), fbits(l10));
+            pr[1] = make_uint4(fbits(l11), fbits(alpha), fbits(dilog_alpha), 0u);
+            pr[2] = make_uint4(depth_key, packed, splat_id, 0u);
+        }
     }
-
     uint32_t total;
-    uint32_t local = block_exclusive_sum<4>(count, s_scan, &total);
-    if (tid == 0) {
-        s_prefix = lookback(a.status, block, 1u, a.epoch, total);
-        if (total) atomicAdd(&a.counters->gps.requested, total);
-    }
-    __syncthreads();
-    if (count == 0u) return;
-    const uint32_t base = s_prefix + local;
-    const uint32_t capacity = a.max_work_items;
-    if (base >= capacity || count > capacity - base) atomicOr(&a.counters->gps.overflow, 1u);
-    if (base >= capacity) return;
-    const uint32_t write_count = min(count, capacity - base);
+    (void)block_exclusive_sum<4>(count, s_scan, &total);
+    if (tid == 0) a.block_counts[blockIdx.x] = total;
+}
 
-    // prepared record (gps.glsl:152-169)
-    float l00 = __builtin_sqrtf(maxf(cov_a, 1.0e-5f));
-    float l10 = cov_b / l00;
-    float l11 = __builtin_sqrtf(maxf(cov_c - l10 * l10, 1.0e-5f));
-    float alpha = clampf(r0.z, 1.0e-6f, 1.0f);
-    float dilog_alpha = dilog(alpha);
-    float ndc_depth = r0.w;
-    float z = mixf(ndc_depth * 0.5f + 0.5f, ndc_depth, a.clip_z_01);
-    uint32_t depth_key = f2u_sat(clampf(z, 0.0f, 1.0f) * 4294967294.0f);
-    uint32_t rr = (uint32_t)__builtin_rintf(clampf(r2.x, 0.0f, 1.0f) * 255.0f);
-    uint32_t gg = (uint32_t)__builtin_rintf(clampf(r2.y, 0.0f, 1.0f) * 255.0f);
-    uint32_t bb = (uint32_t)__builtin_rintf(clampf(r2.z, 0.0f, 1.0f) * 255.0f);
-    uint32_t packed = rr | (gg << 8) | (bb << 16) | 0xFF000000u;
-    if (a.linear_color) packed = splat_id;
-    uint4* pr = a.prepared + (size_t)v * 3;
-    pr[0] = make_uint4(fbits(r0.x), fbits(r0.y), fbits(l00), fbits(l10));
-    pr[1] = make_uint4(fbits(l11), fbits(alpha), fbits(dilog_alpha), 0u);
-    pr[2] = make_uint4(depth_key, packed, splat_id, 0u);
-    a.item_base[v] = base;
-    for (uint32_t i = 0; i < write_count; ++i) a.work[base + i] = v;
+// Pass 2 (after the block counts are scanned): reservation offsets in
+// visible-index order; the block's items form one contiguous range of the
+// work list, written together (owner position per item) up to the budget.
+__global__ void __launch_bounds__(256) gps_reserve_kernel(GpsLaunch a) {
+    __shared__ uint32_t s_scan[4];
+    __shared__ uint32_t s_start[256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t V = *a.vis_count;
+    if (blockIdx.x * 256u >= V) return;
+    const uint32_t v = blockIdx.x * 256u + tid;
+    const uint32_t count = v < V ? a.item_count[v] : 0u;
+    uint32_t total;
+    const uint32_t local = block_exclusive_sum<4>(count, s_scan, &total);
+    s_start[tid] = local;
+    __syncthreads();
+    const uint32_t capacity = a.max_work_items;
+    const uint32_t block_base = a.block_offsets[blockIdx.x];
+    const uint32_t room = block_base < capacity ? capacity - block_base : 0u;
+    const uint32_t emit = min(total, room);
+    for (uint32_t i = tid; i < emit; i += 256u) a.work[block_base + i] = blockIdx.x * 256u + owner_of(s_start, 256u, i);
+    if (count && block_base + local < capacity) a.item_base[v] = block_base + local;
 }
 
 __global__ void __launch_bounds__(256) gps_splat_kernel(GpsLaunch a) {
@@ -197,9 +205,13 @@ __global__ void accumulate_kernel(const float4* __restrict__ sample, float4* acc
 }  // namespace
 
 hipError_t launch_gps_expand(const GpsLaunch& a, hipStream_t s) {
-    uint32_t blocks = (a.capacity + 255u) / 256u;
+    const uint32_t blocks = (a.capacity + 255u) / 256u;
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(gps_expand_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(gps_count_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipError_t e = launch_scan_counts(a.block_counts, blocks, a.block_offsets, &a.counters->gps.requested, nullptr,
+                                      nullptr, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(gps_reserve_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

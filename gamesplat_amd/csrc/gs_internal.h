@@ -32,25 +32,23 @@ constexpr int kRecordVec4 = 3;
 
 // Device counters.  Each stage clears its own group with one fill launch
 // before it runs, so a stage can be re-run without re-running earlier ones.
+// Totals are written by the scan kernels, never accumulated with atomics.
 struct CullCounters {
     uint32_t visible_count;
-    uint32_t ticket;
     uint32_t diag[8];
+    uint32_t pad[7];
 };
 struct SortCounters {
-    uint32_t ticket[8];
     uint32_t hist[8][256];
 };
 struct BinCounters {
-    uint32_t ticket;
-    uint32_t dup_total;
+    uint32_t dup_total;   // (tile, splat) pairs requested
+    uint32_t dup_count;   // min(dup_total, capacity): pairs written and sorted
     uint32_t pad[6];
 };
 struct GpsCounters {
-    uint32_t ticket;
     uint32_t requested;
-    uint32_t overflow;
-    uint32_t pad[5];
+    uint32_t pad[7];
 };
 struct DevCounters {
     CullCounters cull;
@@ -59,56 +57,6 @@ struct DevCounters {
     SortCounters tile_sort;
     GpsCounters gps;
 };
-
-// Decoupled look-back status words: [63:62] flag, [61:32] epoch, [31:0] value.
-// A word whose epoch differs from the launch's epoch reads as "not yet
-// published", so status arrays never need clearing between launches.
-constexpr uint64_t kFlagAggregate = 1ull;
-constexpr uint64_t kFlagInclusive = 2ull;
-
-__device__ __forceinline__ uint64_t make_status(uint64_t flag, uint32_t epoch, uint32_t value) {
-    return (flag << 62) | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | value;
-}
-
-__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint64_t load_status(uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Exclusive prefix of `aggregate` over all earlier tiles of a look-back chain
-// with stride `stride` (status[tile * stride]).  Publishes this tile's
-// aggregate first and its inclusive prefix last.  Values add modulo 2^32.
-__device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t tile, uint32_t stride,
-                                             uint32_t epoch, uint32_t aggregate) {
-    uint64_t* mine = status + (size_t)tile * stride;
-    if (tile == 0) {
-        store_status(mine, make_status(kFlagInclusive, epoch, aggregate));
-        return 0u;
-    }
-    store_status(mine, make_status(kFlagAggregate, epoch, aggregate));
-    uint32_t prefix = 0u;
-    int64_t j = (int64_t)tile - 1;
-    const uint32_t want = epoch & 0x3FFFFFFFu;
-    while (j >= 0) {
-        uint64_t s = load_status(status + (size_t)j * stride);
-        uint32_t e = (uint32_t)(s >> 32) & 0x3FFFFFFFu;
-        uint64_t flag = s >> 62;
-        if (e != want || flag == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        prefix += (uint32_t)s;
-        if (flag == kFlagInclusive) break;
-        --j;
-    }
-    store_status(mine, make_status(kFlagInclusive, epoch, prefix + aggregate));
-    return prefix;
-}
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t l = __lane_id();
@@ -127,7 +75,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
 }
 
 // Exclusive prefix over a block of WAVES wave64s; returns the block total
-// through *total.  `scratch` holds WAVES words of LDS.
+// through *total.  `scratch` holds WAVES words of LDS.  All threads call it.
 template <int WAVES>
 __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* scratch, uint32_t* total) {
     uint32_t inc = wave_inclusive_sum(v);
@@ -146,6 +94,18 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* sc
     return before + inc - v;
 }
 
+// Index of the last entry of the non-decreasing LDS array start[0..n) that is
+// <= i (threads with nothing to emit share the next thread's start, so this
+// is the owner of output i).
+__device__ __forceinline__ uint32_t owner_of(const uint32_t* start, uint32_t n, uint32_t i) {
+    uint32_t lo = 0u, hi = n - 1u;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1u) >> 1;
+        if (start[mid] <= i) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
 // Map a linear block index so that each of the 8 XCDs of an MI355X works on
 // one contiguous range of work items (blocks are dealt to XCDs round-robin).
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
@@ -156,8 +116,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
 }
 
 // ---------------------------------------------------------------------------
-// Kernel launch interfaces (one .hip file each).
+// Kernel launch interfaces.
 
+// gs_scan.hip: exclusive scan of nb block counts into offsets; *total gets
+// the sum.  block_diag (nb x 8, may be NULL) is reduced into diag_out.
+hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
+                              const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s);
+
+// gs_cull.hip
 struct CullLaunch {
     const float4* planes;
     uint32_t n;
@@ -168,55 +134,57 @@ struct CullLaunch {
     float sh_lod_distance;
     int collect_stats;
     float4* records;
-    uint32_t* vis_keys;
-    uint32_t* vis_ids;
-    DevCounters* counters;
-    uint64_t* status;
-    uint32_t epoch;
+    uint32_t* dense_keys;    // n: depth key or 0xFFFFFFFF when culled
+    uint32_t* block_counts;  // visible Gaussians per 256-block
+    uint32_t* block_diag;    // 8 diagnostics per block (collect_stats)
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
-hipError_t launch_clear_counters(DevCounters* c, hipStream_t s);
+hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
+                          uint32_t* vis_ids, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
                                    float4* planes, hipStream_t s);
 
+// gs_sort.hip
 struct SortLaunch {
     uint32_t* keys[3];       // keys[0] input; passes ping-pong keys[1] <-> keys[2]
     uint32_t* vals[3];
     const uint32_t* count;   // device element count
     uint32_t capacity;       // grid bound
-    uint32_t* hist;          // [passes][256]
-    uint32_t* tickets;       // [passes]
-    uint64_t* status;        // [tiles * 256]
-    uint32_t first_epoch;
+    uint32_t* hist;          // [passes][256], from launch_radix_histogram
+    uint32_t* table;         // sort_table_words(capacity) words
     int passes;
     int bits;                // digit width (8 or 4)
 };
 // Histograms of all digit passes.  hist must be zeroed.
 hipError_t launch_radix_histogram(const uint32_t* keys, const uint32_t* count, uint32_t capacity,
                                   uint32_t* hist, int passes, int bits, hipStream_t s);
-// Stable LSD onesweep.  Returns the index (1 or 2) of the buffer holding the
+// Stable LSD radix sort.  Returns the index (1 or 2) of the buffer holding the
 // result through *result_slot.
-hipError_t launch_onesweep(const SortLaunch& a, int* result_slot, hipStream_t s);
-uint32_t onesweep_tile_elems();
+hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s);
+uint32_t sort_table_words(uint32_t capacity);
 
+// gs_composite.hip
 struct BinLaunch {
     const float4* records;
     const uint32_t* sorted_ids;
     const uint32_t* count;   // visible count (device)
-    uint32_t capacity;
+    uint32_t capacity;       // Gaussian capacity (grid bound)
     int width, height, tile, tiles_x, tiles_y;
-    uint32_t* offsets;       // per front-to-back position
+    uint2* rects;            // per front-to-back position: tile rectangle
+    uint32_t* block_counts;  // pairs per 256-block
+    uint32_t* block_offsets;
     DevCounters* counters;
-    uint64_t* status;
-    uint32_t epoch;
     uint32_t* dup_keys;
     uint32_t* dup_vals;
+    uint32_t dup_capacity;   // pairs that fit in dup_keys / dup_vals
 };
-hipError_t launch_bin_count(const BinLaunch& a, hipStream_t s);
-hipError_t launch_bin_emit(const BinLaunch& a, hipStream_t s);
-hipError_t launch_tile_ranges(const uint32_t* keys, uint32_t count, uint2* ranges, hipStream_t s);
+// Counts and emits (tile id, splat id) pairs; leaves the requested total in
+// counters->bin.dup_total and min(total, capacity) in counters->bin.dup_count.
+hipError_t launch_bin(const BinLaunch& a, hipStream_t s);
+hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint32_t capacity, uint2* ranges,
+                              hipStream_t s);
 
 struct CompositeLaunch {
     const float4* records;
@@ -228,6 +196,7 @@ struct CompositeLaunch {
 };
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s);
 
+// gs_gps.hip
 struct GpsLaunch {
     const float4* records;
     const uint32_t* vis_ids;
@@ -239,10 +208,11 @@ struct GpsLaunch {
     int linear_color;
     uint4* prepared;             // 3 x uint4 per visible position
     uint32_t* work;              // owner position per work item
+    uint32_t* item_count;        // per visible position
     uint32_t* item_base;         // per visible position
+    uint32_t* block_counts;
+    uint32_t* block_offsets;
     DevCounters* counters;
-    uint64_t* status;
-    uint32_t epoch;
     // splat
     uint64_t* cells;
     int gps_width, gps_height;

@@ -1,21 +1,21 @@
-// gs_sort.hip - stable LSD onesweep radix sort of 32-bit keys with 32-bit
-// values for gfx950.
+// gs_sort.hip - stable LSD radix sort of 32-bit keys with 32-bit values for
+// gfx950.
 //
 // Replaces radix_hist_cs / radix_prefix_cs / radix_scatter_cs
 // (shaders/sort.glsl:4-164) and their 12 dispatches
-// (src/renderer.cpp:1873-1962).  The reference recomputes a histogram and
-// runs a one-workgroup serial prefix per pass; here one up-front kernel
-// histograms every digit at once and each pass is a single kernel:
-//   1. a block takes the next tile (ticket order) and loads 15 keys per
-//      lane, wave-contiguous so each load is a coalesced 256-byte access;
-//   2. keys are ranked within each wave64 by matching peers with BITS
-//      ballots (stable: lane order, then item order, then wave order);
-//   3. per-digit tile counts are published and the tile's global digit
-//      offsets come from a decoupled look-back over earlier tiles;
-//   4. keys and values are reordered by digit in LDS and written out, so
-//      runs of equal digits leave the CU as contiguous stores.
-// The element count lives in device memory, so no host round trip is needed
-// between the cull and the sort; blocks past the count exit at once.
+// (src/renderer.cpp:1873-1962).  Differences from the reference:
+//   - one up-front kernel histograms every digit of every pass at once;
+//   - tiles are 3840 keys (256 lanes x 15), not 256, so per-tile tables are
+//     15x smaller;
+//   - the per-pass prefix is a parallel column scan (one block per digit)
+//     instead of one workgroup looping over all groups twice;
+//   - ranking inside a tile uses wave64 ballots (peer matching over the digit
+//     bits), and keys are reordered by digit in LDS before the scatter, so
+//     equal digits leave the CU as contiguous runs.
+// Each pass is upsweep (tile digit counts) -> column scan -> downsweep.  No
+// kernel waits on another block or takes a global ticket (device-scope
+// atomics on one address serialize across XCDs on MI355X).  The element
+// count lives in device memory; blocks past it exit at once.
 #include "gs_internal.h"
 
 namespace gs {
@@ -55,14 +55,67 @@ struct PassArgs {
     uint32_t* vals_out;
     const uint32_t* count;
     const uint32_t* hist;   // this pass, RADIX bins
-    uint32_t* ticket;
-    uint64_t* status;       // [tiles][RADIX]
-    uint32_t epoch;
+    uint32_t* table;        // [RADIX][tiles]: tile digit counts, then global offsets
+    uint32_t tiles;         // table stride (grid size)
     int shift;
 };
 
+// Digit counts of one tile (same tile layout as the downsweep).
 template <int BITS>
-__global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
+__global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr uint32_t MASK = RADIX - 1;
+    __shared__ uint32_t s_hist[kWaves][RADIX];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t count = *a.count;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    if (base >= count) return;
+    for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t wbase = base + wave * (64u * kItems);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        uint32_t g = wbase + (uint32_t)i * 64u + lane;
+        if (g < count) atomicAdd(&s_hist[wave][(a.keys_in[g] >> a.shift) & MASK], 1u);
+    }
+    __syncthreads();
+    if (tid < (uint32_t)RADIX) {
+        uint32_t c = 0u;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) c += s_hist[w][tid];
+        a.table[(size_t)tid * a.tiles + tile] = c;
+    }
+}
+
+// One block per digit: exclusive scan of the digit's column over the tiles
+// in use, offset by the number of keys with a smaller digit.
+template <int BITS>
+__global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_scan[kWaves];
+    const uint32_t tid = threadIdx.x, d = blockIdx.x;
+    const uint32_t count = *a.count;
+    const uint32_t used = (count + kTile - 1) / kTile;
+    static_assert(RADIX <= kThreads, "one thread per digit");
+    uint32_t below;  // keys with a smaller digit
+    (void)block_exclusive_sum<kWaves>(tid < d ? a.hist[tid] : 0u, s_scan, &below);
+    const uint32_t per = (used + kThreads - 1) / kThreads;
+    const uint32_t t0 = min(used, tid * per), t1 = min(used, t0 + per);
+    uint32_t* col = a.table + (size_t)d * a.tiles;
+    uint32_t sum = 0u;
+    for (uint32_t t = t0; t < t1; ++t) sum += col[t];
+    uint32_t all;
+    uint32_t run = below + block_exclusive_sum<kWaves>(sum, s_scan, &all);
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t c = col[t];
+        col[t] = run;
+        run += c;
+    }
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
     __shared__ uint32_t s_keys[kTile];
@@ -71,21 +124,19 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
     __shared__ uint32_t s_local[RADIX];
     __shared__ uint32_t s_dest[RADIX];
     __shared__ uint32_t s_scan[kWaves];
-    __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     const uint32_t count = *a.count;
-    if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
-    for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_whist[0][0])[i] = 0u;
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     if (base >= count) return;
+    for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_whist[0][0])[i] = 0u;
+    __syncthreads();
     const uint32_t valid = min((uint32_t)kTile, count - base);
     const int shift = a.shift;
 
-    // 1. load
+    // 1. load, wave-contiguous (each load is a coalesced 256-byte access)
     const uint32_t wbase = base + wave * (64u * kItems);
     uint32_t keys[kItems], vals[kItems], ranks[kItems];
 #pragma unroll
@@ -96,7 +147,8 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
         vals[i] = in ? a.vals_in[g] : 0u;
     }
 
-    // 2. rank within the wave
+    // 2. stable rank within the wave: peers = lanes with the same digit,
+    //    found with BITS ballots; the lowest peer updates the wave's counter
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
@@ -121,7 +173,7 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
     }
     __syncthreads();
 
-    // 3. digit totals, local offsets, look-back for global offsets
+    // 3. per-digit offsets: within the tile (over waves, then digits) and global
     uint32_t bin_total = 0u;
     if (tid < (uint32_t)RADIX) {
         uint32_t sum = 0u;
@@ -135,13 +187,9 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
     }
     uint32_t total;
     uint32_t local = block_exclusive_sum<kWaves>(bin_total, s_scan, &total);
-    uint32_t h = tid < (uint32_t)RADIX ? a.hist[tid] : 0u;
-    uint32_t gstart = block_exclusive_sum<kWaves>(h, s_scan, &total);
     if (tid < (uint32_t)RADIX) {
         s_local[tid] = local;
-        uint32_t agg = bin_total - (tid == MASK ? (uint32_t)kTile - valid : 0u);
-        uint32_t prefix = lookback(a.status + tid, tile, RADIX, a.epoch, agg);
-        s_dest[tid] = gstart + prefix - local;
+        s_dest[tid] = a.table[(size_t)tid * a.tiles + tile] - local;
     }
     __syncthreads();
 
@@ -155,6 +203,8 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
         s_vals[p] = vals[i];
     }
     __syncthreads();
+    // invalid lanes carry the last digit and rank after every valid key, so
+    // the valid keys occupy local positions [0, valid)
     for (uint32_t p = tid; p < valid; p += kThreads) {
         uint32_t k = s_keys[p];
         uint32_t d = (k >> shift) & MASK;
@@ -164,9 +214,40 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(PassArgs a) {
     }
 }
 
+template <int BITS>
+hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
+    uint32_t tiles = (a.capacity + kTile - 1) / kTile;
+    if (tiles < 1) tiles = 1;
+    int src = 0, dst = 1;
+    for (int p = 0; p < a.passes; ++p) {
+        PassArgs pa;
+        pa.keys_in = a.keys[src];
+        pa.vals_in = a.vals[src];
+        pa.keys_out = a.keys[dst];
+        pa.vals_out = a.vals[dst];
+        pa.count = a.count;
+        pa.hist = a.hist + p * 256;
+        pa.table = a.table;
+        pa.tiles = tiles;
+        pa.shift = p * BITS;
+        hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL(downsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        src = dst;
+        dst = dst == 1 ? 2 : 1;
+    }
+    *result_slot = src;
+    return hipSuccess;
+}
+
 }  // namespace
 
-uint32_t onesweep_tile_elems() { return (uint32_t)kTile; }
+uint32_t sort_table_words(uint32_t capacity) {
+    uint32_t tiles = (capacity + kTile - 1) / kTile;
+    return (tiles < 1 ? 1 : tiles) * 256u;
+}
 
 hipError_t launch_radix_histogram(const uint32_t* keys, const uint32_t* count, uint32_t capacity,
                                   uint32_t* hist, int passes, int bits, hipStream_t s) {
@@ -185,36 +266,10 @@ hipError_t launch_radix_histogram(const uint32_t* keys, const uint32_t* count, u
     return hipGetLastError();
 }
 
-hipError_t launch_onesweep(const SortLaunch& a, int* result_slot, hipStream_t s) {
-    uint32_t tiles = (a.capacity + kTile - 1) / kTile;
-    if (tiles < 1) tiles = 1;
-    int src = 0, dst = 1;
-    for (int p = 0; p < a.passes; ++p) {
-        PassArgs pa;
-        pa.keys_in = a.keys[src];
-        pa.vals_in = a.vals[src];
-        pa.keys_out = a.keys[dst];
-        pa.vals_out = a.vals[dst];
-        pa.count = a.count;
-        pa.hist = a.hist + p * 256;
-        pa.ticket = a.tickets + p;
-        pa.status = a.status;
-        pa.epoch = a.first_epoch + (uint32_t)p;
-        pa.shift = p * a.bits;
-        if (a.bits == 8) {
-            hipLaunchKernelGGL(onesweep_kernel<8>, dim3(tiles), dim3(kThreads), 0, s, pa);
-        } else if (a.bits == 4) {
-            hipLaunchKernelGGL(onesweep_kernel<4>, dim3(tiles), dim3(kThreads), 0, s, pa);
-        } else {
-            return hipErrorInvalidValue;
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        src = dst;
-        dst = dst == 1 ? 2 : 1;
-    }
-    *result_slot = src;
-    return hipSuccess;
+hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {
+    if (a.bits == 8) return run_passes<8>(a, result_slot, s);
+    if (a.bits == 4) return run_passes<4>(a, result_slot, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace gs

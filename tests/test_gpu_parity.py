@@ -119,6 +119,24 @@ def test_composite_vs_fp64(ctx, orc, c1_scene, tile, quad):
     assert st["tile_entries"] > 0
 
 
+def test_composite_tile_overflow_rerun(orc, gs):
+    # large splats: far more (tile, splat) pairs than the initial capacity
+    print("case: composite tile-pair overflow and re-run", flush=True)
+    g = gs.generate_scene(13, 3000, 4, 0.0)
+    g[:, 3:6] *= 40.0
+    cam = scenes.random_view_camera(13, 640, 480)
+    with gs.Context(0) as c:
+        c.upload(g)
+        c.render_alpha(cam, tile=8)
+        img = c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+        st = c.stats()
+        assert st["tile_entries"] > 3 * len(g)
+    keys, ids, proj, vis, _ = orc.cull(g, cam)
+    sk, si = orc.radix_sort(keys, ids)
+    ref = orc.composite(proj, si, 640, 480, mode=0)
+    assert np.abs(img - ref).max() <= TOL_RGBA16F
+
+
 def test_gps_bit_exact(ctx, orc, c1_scene):
     g, cfg, cam = c1_scene
     W, H = cfg["width"], cfg["height"]
