@@ -23,6 +23,25 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# Stage whose HIP-event bracket holds exactly one launch of this kernel.
+STAGE_KERNEL = {"cull": "cull_kernel", "sort": "downsweep_kernel<8>", "composite": "composite_quad_kernel"}
+PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
+
+
+def pmc_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed counter profile of
+    the default workload (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    if not PMC_PROFILE.exists():
+        return None
+    for line in PMC_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == kernel:
+            try:
+                return int(round((float(cells[1]) + float(cells[2])) * 1e6))
+            except ValueError:
+                return None
+    return None
+
 
 def _parse():
     ap = argparse.ArgumentParser()
@@ -34,7 +53,27 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stochastic", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (calibrates HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes: int) -> None:
+    """Plain device-to-device copies of a known size through the HIP runtime,
+    so that FETCH_SIZE / WRITE_SIZE of a profiled run can be scaled."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
 def cpu_baseline(g, cam, width, height, threads):
@@ -54,6 +93,9 @@ def cpu_baseline(g, cam, width, height, threads):
 
 def main():
     args = _parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
                                    "(screen-space tiles / Gaussian shards), --gpus must be 1"}))
@@ -148,7 +190,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": (pmc_traffic(STAGE_KERNEL[dom])
+                    if args.workload == "c3" and args.tile in (0, 16) else None),
+        "traffic_source": f"{PMC_PROFILE.relative_to(ROOT)} ({STAGE_KERNEL[dom]})",
         "algorithmic_bytes": dbytes,
         "ms": round(dms, 4),
         "others": {k: {"ms": round(v[0], 4), "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1) if v[0] else 0.0}

@@ -74,7 +74,8 @@ struct gs_ctx {
     size_t held = 0;
 
     uint32_t n = 0;
-    DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, table,
+    DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
+        table,
         counters;
 
     DevBuf dup_keys[3], dup_vals[3], ranges, image;
@@ -258,7 +259,7 @@ void gs_destroy(gs_ctx* c) {
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->ids[0], &c->ids[1],
                       &c->ids[2], &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
-                      &c->rects, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
+                      &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum};
     for (DevBuf* b : bufs) release(c, *b);
@@ -307,12 +308,12 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
         return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
     GS_CHECK(set_device(c));
     // Scene-resident bytes per Gaussian: planes 240 + record 48 + keys/ids 24
-    // + dense key 4 + bin rectangle 8 + GPS prepared 48 + item count/base 8,
+    // + dense key 4 + rectangles 16 + GPS prepared 48 + item count/base 8,
     // plus the GPS work list.
-    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 8 + 48 + 8) + (size_t)c->gps_budget * 4;
+    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 16 + 48 + 8) + (size_t)c->gps_budget * 4;
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
                         c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
-                        c->dense_keys.bytes - c->rects.bytes - c->prepared.bytes - c->item_count.bytes -
+                        c->dense_keys.bytes - c->rects.bytes - c->rect_px.bytes - c->prepared.bytes - c->item_count.bytes -
                         c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
         return fail(GS_ERR_LIMIT, "scene of %u Gaussians needs %.1f MiB; memory budget is %.1f MiB", count,
@@ -370,6 +371,7 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
     GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
     GS_CHECK(ensure(c, c->rects, (size_t)count * 8, "bin rectangles"));
+    GS_CHECK(ensure(c, c->rect_px, (size_t)count * 8, "pixel rectangles"));
     GS_CHECK(ensure_table(c, count));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
     return ok();
@@ -427,6 +429,7 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     a.dense_keys = ptr<uint32_t>(c->dense_keys);
     a.block_counts = ptr<uint32_t>(c->block_counts);
     a.block_diag = collect ? ptr<uint32_t>(c->block_diag) : nullptr;
+    a.rect_px = ptr<uint2>(c->rect_px);
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_cull(a, c->stream));
@@ -452,14 +455,12 @@ int gs_sort(gs_ctx* c) {
     }
     s.count = &c->dc()->cull.visible_count;
     s.capacity = c->n;
-    s.hist = &c->dc()->depth_sort.hist[0][0];
+    s.scratch = c->dc()->depth_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
     s.passes = 4;
     s.bits = 8;
     TimedLaunch t{};
-    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->depth_sort), 0u, sizeof(SortCounters) / 4, c->stream));
     GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
-    GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->n, s.hist, 4, 8, c->stream));
     int slot = 0;
     GS_HIP(launch_radix_sort(s, &slot, c->stream));
     GS_CHECK(timed_end(c, &t));
@@ -508,6 +509,7 @@ static int enqueue_composite(gs_ctx* c, int tile) {
     b.tiles_x = tiles_x;
     b.tiles_y = tiles_y;
     b.counters = c->dc();
+    b.rect_px = ptr<uint2>(c->rect_px);
     b.rects = ptr<uint2>(c->rects);
     b.block_counts = ptr<uint32_t>(c->block_counts);
     b.block_offsets = ptr<uint32_t>(c->block_offsets);
@@ -522,18 +524,15 @@ static int enqueue_composite(gs_ctx* c, int tile) {
     }
     s.count = &c->dc()->bin.dup_count;
     s.capacity = c->dup_cap;
-    s.hist = &c->dc()->tile_sort.hist[0][0];
+    s.scratch = c->dc()->tile_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
     s.passes = passes;
     s.bits = 8;
 
-    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->bin), 0u, sizeof(BinCounters) / 4, c->stream));
-    GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->tile_sort), 0u, sizeof(SortCounters) / 4, c->stream));
     GS_HIP(launch_fill_u32(ptr<uint32_t>(c->ranges), 0u, (size_t)ntiles * 2, c->stream));
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
     GS_HIP(launch_bin(b, c->stream));
-    GS_HIP(launch_radix_histogram(s.keys[0], s.count, c->dup_cap, s.hist, passes, 8, c->stream));
     int slot = 0;
     GS_HIP(launch_radix_sort(s, &slot, c->stream));
     GS_HIP(launch_tile_ranges(s.keys[slot], s.count, c->dup_cap, ptr<uint2>(c->ranges), c->stream));

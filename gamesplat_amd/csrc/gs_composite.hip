@@ -10,10 +10,12 @@
 // (i + 0.5, j + 0.5) with left/bottom edges inclusive.  Inside the quad the
 // fragment test and alpha are the fragment shader's.
 //
-//   bin_count   per front-to-back position: number of tiles the quad touches,
-//               exclusive offsets by decoupled look-back;
+//   bin_count   per front-to-back position: tiles touched by the splat's
+//               covered pixels (from the pixel rectangle the cull wrote),
+//               pair count per block;
+//   (scan)      block offsets;
 //   bin_emit    (tile id, splat id) pairs in front-to-back order;
-//   (onesweep)  stable sort by tile id keeps the depth order inside a tile;
+//   (sort)      stable LSD sort by tile id keeps the depth order in a tile;
 //   tile_ranges [begin, end) of every tile;
 //   composite   one block per tile, splat batches staged in LDS, all pixels
 //               of the tile walk the batch front to back and stop once their
@@ -23,57 +25,13 @@
 
 #include "gs_internal.h"
 #include "gs_math.h"
+#include "gs_raster.h"
 
 namespace gs {
 
 namespace {
 
 constexpr float kTransmittanceMin = 1e-4f;
-
-__device__ __forceinline__ int32_t snap256(float w) {
-    float s = w * 256.0f;
-    if (s > 1073741824.0f) return 1073741824;
-    if (s < -1073741824.0f) return -1073741824;
-    return (int32_t)__builtin_rintf(s);
-}
-
-__device__ __forceinline__ int32_t ceil_div256(int32_t a) { return -((-a) >> 8); }
-
-// Quad edges in 24.8 fixed point window coordinates (splat_vs corners
-// through ndc = 2 pos / viewport - 1 and back).
-__device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float ry, float vw, float vh) {
-    float x0 = cx + -1.0f * rx, x1 = cx + 1.0f * rx;
-    float y0 = cy + -1.0f * ry, y1 = cy + 1.0f * ry;
-    float hw = 0.5f * vw, hh = 0.5f * vh;
-    int4 q;
-    q.x = snap256((2.0f * x0 / vw - 1.0f + 1.0f) * hw);
-    q.y = snap256((2.0f * x1 / vw - 1.0f + 1.0f) * hw);
-    q.z = snap256((2.0f * y0 / vh - 1.0f + 1.0f) * hh);
-    q.w = snap256((2.0f * y1 / vh - 1.0f + 1.0f) * hh);
-    return q;
-}
-
-struct TileRect { int x0, x1, y0, y1; };
-
-// Tiles touched by the covered pixels of splat g (empty when fully discarded).
-__device__ __forceinline__ TileRect splat_tiles(const float4* __restrict__ records, uint32_t g,
-                                                const BinLaunch& a) {
-    TileRect t = {0, 0, 0, 0};
-    float4 r0 = records[(size_t)g * kRecordVec4 + 0];
-    float4 r1 = records[(size_t)g * kRecordVec4 + 1];
-    float4 r2 = records[(size_t)g * kRecordVec4 + 2];
-    // every fragment fails power >= log((1/255)/max(o,1e-8)) when that log is > 0
-    if ((1.0f / 255.0f) / maxf(r0.z, 1e-8f) > 1.0f) return t;
-    int4 q = quad_fixed(r0.x, r0.y, r1.w, r2.w, (float)a.width, (float)a.height);
-    int i0 = max(ceil_div256(q.x - 128), 0), i1 = min(ceil_div256(q.y - 128), a.width);
-    int j0 = max(ceil_div256(q.z - 128), 0), j1 = min(ceil_div256(q.w - 128), a.height);
-    if (i0 >= i1 || j0 >= j1) return t;
-    t.x0 = i0 / a.tile;
-    t.x1 = (i1 - 1) / a.tile + 1;
-    t.y0 = j0 / a.tile;
-    t.y1 = (j1 - 1) / a.tile + 1;
-    return t;
-}
 
 // Pass 1 over the depth-sorted splats, nearest first: tile rectangle and
 // pair count per splat, pair count per block.
@@ -84,9 +42,16 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
     const uint32_t q = blockIdx.x * 256u + tid;
     uint32_t cnt = 0u;
     if (q < V) {
-        TileRect t = splat_tiles(a.records, a.sorted_ids[V - 1u - q], a);
-        cnt = (uint32_t)((t.x1 - t.x0) * (t.y1 - t.y0));
-        a.rects[q] = make_uint2((uint32_t)t.x0 | ((uint32_t)t.x1 << 16), (uint32_t)t.y0 | ((uint32_t)t.y1 << 16));
+        const uint2 px = a.rect_px[a.sorted_ids[V - 1u - q]];
+        uint2 t = make_uint2(0u, 0u);
+        if (px.x) {
+            const uint32_t T = (uint32_t)a.tile;
+            const uint32_t x0 = (px.x & 0xFFFFu) / T, x1 = ((px.x >> 16) - 1u) / T + 1u;
+            const uint32_t y0 = (px.y & 0xFFFFu) / T, y1 = ((px.y >> 16) - 1u) / T + 1u;
+            cnt = (x1 - x0) * (y1 - y0);
+            t = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
+        }
+        a.rects[q] = t;
     }
     uint32_t total;
     (void)block_exclusive_sum<4>(cnt, s_scan, &total);

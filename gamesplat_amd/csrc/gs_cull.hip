@@ -13,6 +13,7 @@
 // value is the same IEEE binary32 result the CPU restatement computes.
 #include "gs_internal.h"
 #include "gs_math.h"
+#include "gs_raster.h"
 
 namespace gs {
 
@@ -286,6 +287,8 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             rec[0] = make_float4(cx, rcy, opacity, ndc_z);
             rec[1] = make_float4(ca, cb * cam.clip_y_sign, cc, radius_x);
             rec[2] = make_float4(color[0], color[1], color[2], radius_y);
+            a.rect_px[idx] = pixel_rect(cx, rcy, radius_x, radius_y, opacity, (int)cam.viewport[0],
+                                        (int)cam.viewport[1]);
         }
     }
 

@@ -39,7 +39,7 @@ struct CullCounters {
     uint32_t pad[7];
 };
 struct SortCounters {
-    uint32_t hist[8][256];
+    uint32_t scratch[512];   // per pass: digit column totals, digit bases
 };
 struct BinCounters {
     uint32_t dup_total;   // (tile, splat) pairs requested
@@ -137,6 +137,7 @@ struct CullLaunch {
     uint32_t* dense_keys;    // n: depth key or 0xFFFFFFFF when culled
     uint32_t* block_counts;  // visible Gaussians per 256-block
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats)
+    uint2* rect_px;          // n: covered pixel rectangle of each visible splat
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
@@ -152,14 +153,11 @@ struct SortLaunch {
     uint32_t* vals[3];
     const uint32_t* count;   // device element count
     uint32_t capacity;       // grid bound
-    uint32_t* hist;          // [passes][256], from launch_radix_histogram
+    uint32_t* scratch;       // 512 words
     uint32_t* table;         // sort_table_words(capacity) words
     int passes;
     int bits;                // digit width (8 or 4)
 };
-// Histograms of all digit passes.  hist must be zeroed.
-hipError_t launch_radix_histogram(const uint32_t* keys, const uint32_t* count, uint32_t capacity,
-                                  uint32_t* hist, int passes, int bits, hipStream_t s);
 // Stable LSD radix sort.  Returns the index (1 or 2) of the buffer holding the
 // result through *result_slot.
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s);
@@ -172,6 +170,7 @@ struct BinLaunch {
     const uint32_t* count;   // visible count (device)
     uint32_t capacity;       // Gaussian capacity (grid bound)
     int width, height, tile, tiles_x, tiles_y;
+    const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
     uint2* rects;            // per front-to-back position: tile rectangle
     uint32_t* block_counts;  // pairs per 256-block
     uint32_t* block_offsets;

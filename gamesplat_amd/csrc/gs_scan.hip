@@ -5,9 +5,8 @@
 // The kernels that produce counts never wait on each other and never take a
 // global ticket: on MI355X device-scope atomics on one address serialize
 // across the XCDs, which made a per-block ticket or total counter cost
-// ~25 ns per block.  One 1024-thread block scans the counts instead: each
-// thread sums a contiguous run, the 1024 run sums are scanned in LDS, and
-// each thread writes the offsets of its run.  The total goes to *total.
+// ~25 ns per block.  One 1024-thread block scans the counts instead.  The
+// total goes to *total.
 #include "gs_internal.h"
 
 namespace gs {
@@ -15,6 +14,7 @@ namespace gs {
 namespace {
 
 constexpr int kScanThreads = 1024;
+constexpr int kRun = 32;  // consecutive counts per thread per round
 
 __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_t* __restrict__ counts,
                                                                    uint32_t nb, uint32_t* offsets,
@@ -24,18 +24,29 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
     __shared__ uint32_t s_scan[kScanThreads / 64];
     __shared__ uint32_t s_diag[8];
     const uint32_t tid = threadIdx.x;
-    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = min(nb, tid * per), b1 = min(nb, b0 + per);
-    uint32_t sum = 0u;
-    for (uint32_t i = b0; i < b1; ++i) sum += counts[i];
-    uint32_t all;
-    uint32_t run = block_exclusive_sum<kScanThreads / 64>(sum, s_scan, &all);
-    for (uint32_t i = b0; i < b1; ++i) {
-        uint32_t c = counts[i];
-        offsets[i] = run;
-        run += c;
+    // Rounds of 1024 runs of kRun consecutive counts: every thread issues the
+    // loads of its run together (one memory latency per round), sums it, one
+    // block scan gives the run offsets, and the run is written back.
+    uint32_t carry = 0u;
+    for (uint32_t first = 0; first < nb; first += kRun * kScanThreads) {
+        const uint32_t r0 = first + tid * kRun;
+        uint32_t v[kRun];
+        uint32_t sum = 0u;
+#pragma unroll
+        for (int k = 0; k < kRun; ++k) {
+            v[k] = r0 + k < nb ? counts[r0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t chunk;
+        uint32_t run = carry + block_exclusive_sum<kScanThreads / 64>(sum, s_scan, &chunk);
+#pragma unroll
+        for (int k = 0; k < kRun; ++k) {
+            if (r0 + k < nb) offsets[r0 + k] = run;
+            run += v[k];
+        }
+        carry += chunk;
     }
-    if (tid == 0) *total = all;
+    if (tid == 0) *total = carry;
     if (block_diag) {
         // SplatDiagnostics: sums except the two maxima (fields 1 and 5)
         if (tid < 8) s_diag[tid] = 0u;

@@ -3,17 +3,16 @@
 //
 // Replaces radix_hist_cs / radix_prefix_cs / radix_scatter_cs
 // (shaders/sort.glsl:4-164) and their 12 dispatches
-// (src/renderer.cpp:1873-1962).  Differences from the reference:
-//   - one up-front kernel histograms every digit of every pass at once;
-//   - tiles are 3840 keys (256 lanes x 15), not 256, so per-tile tables are
-//     15x smaller;
-//   - the per-pass prefix is a parallel column scan (one block per digit)
-//     instead of one workgroup looping over all groups twice;
-//   - ranking inside a tile uses wave64 ballots (peer matching over the digit
-//     bits), and keys are reordered by digit in LDS before the scatter, so
-//     equal digits leave the CU as contiguous runs.
-// Each pass is upsweep (tile digit counts) -> column scan -> downsweep.  No
-// kernel waits on another block or takes a global ticket (device-scope
+// (src/renderer.cpp:1873-1962).  Each pass is
+//   upsweep    digit counts of every 3840-key tile (256 lanes x 15 keys);
+//   colscan    one block per digit: exclusive scan of that digit's column over
+//              the tiles, plus the column total (the reference's single
+//              workgroup loops over every group twice instead);
+//   digitbase  exclusive scan of the column totals;
+//   downsweep  stable ranking inside the tile with wave64 ballots (peers =
+//              lanes whose digit matches on every bit), reorder by digit in
+//              LDS, then contiguous runs of equal digits are written out.
+// No kernel waits on another block or takes a global ticket (device-scope
 // atomics on one address serialize across XCDs on MI355X).  The element
 // count lives in device memory; blocks past it exit at once.
 #include "gs_internal.h"
@@ -27,40 +26,19 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTile = kThreads * kItems;
 
-template <int BITS>
-__global__ void __launch_bounds__(kThreads) radix_histogram_kernel(const uint32_t* __restrict__ keys,
-                                                                   const uint32_t* count_ptr,
-                                                                   uint32_t* hist, int passes) {
-    constexpr int RADIX = 1 << BITS;
-    __shared__ uint32_t s_hist[8 * RADIX];
-    const uint32_t count = *count_ptr;
-    for (int i = threadIdx.x; i < passes * RADIX; i += kThreads) s_hist[i] = 0u;
-    __syncthreads();
-    const uint32_t stride = gridDim.x * kThreads;
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < count; i += stride) {
-        uint32_t k = keys[i];
-        for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < passes * RADIX; i += kThreads) {
-        uint32_t v = s_hist[i];
-        if (v) atomicAdd(&hist[(i / RADIX) * 256 + (i % RADIX)], v);
-    }
-}
-
 struct PassArgs {
     const uint32_t* keys_in;
     const uint32_t* vals_in;
     uint32_t* keys_out;
     uint32_t* vals_out;
     const uint32_t* count;
-    const uint32_t* hist;   // this pass, RADIX bins
-    uint32_t* table;        // [RADIX][tiles]: tile digit counts, then global offsets
+    uint32_t* table;        // [RADIX][tiles]: tile digit counts, then column offsets
     uint32_t tiles;         // table stride (grid size)
+    uint32_t* col_total;    // [RADIX]
+    uint32_t* digit_base;   // [RADIX]
     int shift;
 };
 
-// Digit counts of one tile (same tile layout as the downsweep).
 template <int BITS>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
@@ -74,10 +52,16 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t wbase = base + wave * (64u * kItems);
+    uint32_t keys[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        if (g < count) atomicAdd(&s_hist[wave][(a.keys_in[g] >> a.shift) & MASK], 1u);
+        keys[i] = g < count ? a.keys_in[g] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        uint32_t g = wbase + (uint32_t)i * 64u + lane;
+        if (g < count) atomicAdd(&s_hist[wave][(keys[i] >> a.shift) & MASK], 1u);
     }
     __syncthreads();
     if (tid < (uint32_t)RADIX) {
@@ -88,30 +72,47 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     }
 }
 
-// One block per digit: exclusive scan of the digit's column over the tiles
-// in use, offset by the number of keys with a smaller digit.
+// One block per digit: exclusive scan of the digit's column over the tiles in
+// use (all loads of a thread's run issued before use), column total to
+// col_total[digit].
 template <int BITS>
 __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
-    constexpr int RADIX = 1 << BITS;
+    constexpr int kRun = 8;
     __shared__ uint32_t s_scan[kWaves];
     const uint32_t tid = threadIdx.x, d = blockIdx.x;
     const uint32_t count = *a.count;
     const uint32_t used = (count + kTile - 1) / kTile;
-    static_assert(RADIX <= kThreads, "one thread per digit");
-    uint32_t below;  // keys with a smaller digit
-    (void)block_exclusive_sum<kWaves>(tid < d ? a.hist[tid] : 0u, s_scan, &below);
-    const uint32_t per = (used + kThreads - 1) / kThreads;
-    const uint32_t t0 = min(used, tid * per), t1 = min(used, t0 + per);
     uint32_t* col = a.table + (size_t)d * a.tiles;
-    uint32_t sum = 0u;
-    for (uint32_t t = t0; t < t1; ++t) sum += col[t];
-    uint32_t all;
-    uint32_t run = below + block_exclusive_sum<kWaves>(sum, s_scan, &all);
-    for (uint32_t t = t0; t < t1; ++t) {
-        uint32_t c = col[t];
-        col[t] = run;
-        run += c;
+    uint32_t carry = 0u;
+    for (uint32_t first = 0; first < used; first += kThreads * kRun) {
+        const uint32_t t0 = first + tid * kRun;
+        uint32_t v[kRun];
+        uint32_t sum = 0u;
+#pragma unroll
+        for (int k = 0; k < kRun; ++k) {
+            v[k] = t0 + k < used ? col[t0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t chunk;
+        uint32_t run = carry + block_exclusive_sum<kWaves>(sum, s_scan, &chunk);
+#pragma unroll
+        for (int k = 0; k < kRun; ++k) {
+            if (t0 + k < used) col[t0 + k] = run;
+            run += v[k];
+        }
+        carry += chunk;
     }
+    if (tid == 0) a.col_total[d] = carry;
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(kThreads) digitbase_kernel(PassArgs a) {
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_scan[kWaves];
+    const uint32_t tid = threadIdx.x;
+    uint32_t total;
+    uint32_t base = block_exclusive_sum<kWaves>(tid < (uint32_t)RADIX ? a.col_total[tid] : 0u, s_scan, &total);
+    if (tid < (uint32_t)RADIX) a.digit_base[tid] = base;
 }
 
 template <int BITS>
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     uint32_t local = block_exclusive_sum<kWaves>(bin_total, s_scan, &total);
     if (tid < (uint32_t)RADIX) {
         s_local[tid] = local;
-        s_dest[tid] = a.table[(size_t)tid * a.tiles + tile] - local;
+        s_dest[tid] = a.digit_base[tid] + a.table[(size_t)tid * a.tiles + tile] - local;
     }
     __syncthreads();
 
@@ -226,12 +227,14 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.keys_out = a.keys[dst];
         pa.vals_out = a.vals[dst];
         pa.count = a.count;
-        pa.hist = a.hist + p * 256;
         pa.table = a.table;
         pa.tiles = tiles;
+        pa.col_total = a.scratch;
+        pa.digit_base = a.scratch + 256;
         pa.shift = p * BITS;
         hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL(digitbase_kernel<BITS>, dim3(1), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(downsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -249,24 +252,8 @@ uint32_t sort_table_words(uint32_t capacity) {
     return (tiles < 1 ? 1 : tiles) * 256u;
 }
 
-hipError_t launch_radix_histogram(const uint32_t* keys, const uint32_t* count, uint32_t capacity,
-                                  uint32_t* hist, int passes, int bits, hipStream_t s) {
-    if (passes < 1 || passes > 8) return hipErrorInvalidValue;
-    uint32_t blocks = (capacity + 16383u) / 16384u;
-    if (blocks < 1) blocks = 1;
-    if (blocks > 2048) blocks = 2048;
-    if (bits == 8) {
-        if (passes > 4) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(radix_histogram_kernel<8>, dim3(blocks), dim3(kThreads), 0, s, keys, count, hist, passes);
-    } else if (bits == 4) {
-        hipLaunchKernelGGL(radix_histogram_kernel<4>, dim3(blocks), dim3(kThreads), 0, s, keys, count, hist, passes);
-    } else {
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {
+    if (a.passes < 1 || a.passes * a.bits > 32) return hipErrorInvalidValue;
     if (a.bits == 8) return run_passes<8>(a, result_slot, s);
     if (a.bits == 4) return run_passes<4>(a, result_slot, s);
     return hipErrorInvalidValue;
