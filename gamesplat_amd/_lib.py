@@ -35,6 +35,7 @@ GS_BUFFER_GPS_PACKED = 5
 GS_OPT_GPS_KEEP_CELLS = 0
 GS_OPT_TILE_SIZE = 1
 GS_OPT_QUADRANT_CULL = 2
+GS_OPT_TIGHT_RECTS = 3
 
 STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve")
 

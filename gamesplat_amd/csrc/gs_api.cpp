@@ -93,6 +93,8 @@ struct gs_ctx {
     int keep_cells = 0;
     int default_tile = 16;
     int quadrant_cull = 1;
+    int tight_rects = 1;
+    int cull_tight = 1;      // tight_rects as the last cull used it
 
     GsCameraUniforms cam{};
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
@@ -297,6 +299,9 @@ int gs_set_option(gs_ctx* c, int option, int value) {
         case GS_OPT_QUADRANT_CULL:
             c->quadrant_cull = value ? 1 : 0;
             return ok();
+        case GS_OPT_TIGHT_RECTS:
+            c->tight_rects = value ? 1 : 0;
+            return ok();
         default:
             return fail(GS_ERR_INVALID, "unknown option %d", option);
     }
@@ -430,6 +435,8 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     a.block_counts = ptr<uint32_t>(c->block_counts);
     a.block_diag = collect ? ptr<uint32_t>(c->block_diag) : nullptr;
     a.rect_px = ptr<uint2>(c->rect_px);
+    a.tight_rects = c->tight_rects;
+    c->cull_tight = c->tight_rects;
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_cull(a, c->stream));
@@ -549,6 +556,7 @@ static int enqueue_composite(gs_ctx* c, int tile) {
     cl.tiles_y = tiles_y;
     cl.image = ptr<uint16_t>(c->image);
     cl.quadrant_cull = c->quadrant_cull;
+    cl.tight_rects = c->cull_tight;
     TimedLaunch t2{};
     GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
     GS_HIP(launch_composite(cl, c->stream));

@@ -107,6 +107,63 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
     if (i == count - 1 || keys[i + 1] != k) ranges[k].y = i + 1;
 }
 
+// One splat as the compositors keep it in LDS (4 x 16 bytes).
+struct StagedSplats {
+    int4* rect;     // covered pixels [i0, i1) x [j0, j1)
+    float4* geo;    // cx, cy, -k0/2, -k1
+    float4* par;    // -k2/2, opacity, cutoff, -
+    float4* col;    // r, g, b, -
+};
+
+// Stage splat g into slot t: conic from one reciprocal of the clamped
+// determinant, fragment cutoff log((1/255) / max(o, 1e-8)), pixel rectangle.
+__device__ __forceinline__ void stage_splat(const CompositeLaunch& a, uint32_t g, uint32_t t, StagedSplats s) {
+    const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
+    const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
+    const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
+    float det = r1.x * r1.z - r1.y * r1.y;
+    if (det < 1e-6f) det = 1e-6f;
+    const float inv = 1.0f / det;
+    const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
+    const float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
+    s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, a.width, a.height,
+                                 a.tight_rects != 0);
+    s.geo[t] = make_float4(r0.x, r0.y, -0.5f * k0, -k1);
+    s.par[t] = make_float4(-0.5f * k2, r0.z, cutoff, 0.0f);
+    s.col[t] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+}
+
+// One front-to-back blend step of pixel (px, py), branch-free: coverage (the
+// pixel rectangle), the fragment test (power in [cutoff, 0]) and the
+// saturation test select the alpha or 0, and a zero alpha leaves colour and
+// transmittance bit-for-bit unchanged.  The quadratic form is evaluated with
+// fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy.
+__device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, int4 r, float4 geo, float4 par,
+                                           float4 col, float& cr, float& cg, float& cb, float& tr) {
+    const float dx = fx - geo.x, dy = fy - geo.y;
+    const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, (par.x * dy) * dy);
+    float alpha = minf(par.y * __expf(power), 0.99f);
+    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power <= 0.0f) &
+                      (power >= par.z) & (tr >= kTransmittanceMin);
+    alpha = take ? alpha : 0.0f;
+    const float w = alpha * tr;
+    cr = fmaf(col.x, w, cr);
+    cg = fmaf(col.y, w, cg);
+    cb = fmaf(col.z, w, cb);
+    tr = fmaf(-alpha, tr, tr);
+}
+
+__device__ __forceinline__ void store_pixel(const CompositeLaunch& a, int px, int py, float cr, float cg, float cb,
+                                            float tr) {
+    size_t o = ((size_t)py * a.width + px) * 4;
+    ushort4 h;
+    h.x = __half_as_ushort(__float2half_rn(cr));
+    h.y = __half_as_ushort(__float2half_rn(cg));
+    h.z = __half_as_ushort(__float2half_rn(cb));
+    h.w = __half_as_ushort(__float2half_rn(1.0f - tr));
+    *reinterpret_cast<ushort4*>(a.image + o) = h;
+}
+
 template <int T>
 struct TileShape {
     static constexpr int kPixels = T * T;
@@ -115,197 +172,114 @@ struct TileShape {
     static constexpr int kBatch = kThreads;
 };
 
+// T x T tiles, one block per tile, PPT pixels per thread; every pixel walks
+// every staged splat of its tile.
 template <int T>
 __global__ void __launch_bounds__(TileShape<T>::kThreads) composite_kernel(CompositeLaunch a) {
     using S = TileShape<T>;
     constexpr int NT = S::kThreads, PPT = S::kPerThread, BATCH = S::kBatch;
-    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
-    __shared__ float4 s_opa[BATCH];   // k2, opacity, cutoff, -
-    __shared__ float4 s_col[BATCH];   // r, g, b, -
-    __shared__ int4 s_quad[BATCH];    // x0, x1, y0, y1 (24.8)
+    __shared__ int4 s_rect[BATCH];
+    __shared__ float4 s_geo[BATCH];
+    __shared__ float4 s_par[BATCH];
+    __shared__ float4 s_col[BATCH];
+    const StagedSplats st{s_rect, s_geo, s_par, s_col};
 
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
     const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
     const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
     const uint2 range = a.ranges[tile];
-    const float vw = (float)a.width, vh = (float)a.height;
 
     int px[PPT], py[PPT];
-    int32_t xf[PPT], yf[PPT];
     float fx[PPT], fy[PPT];
     float cr[PPT], cg[PPT], cb[PPT], tr[PPT];
-    bool live[PPT];
-    bool any_live = false;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         int p = (int)tid + k * NT;
         px[k] = tx * T + p % T;
         py[k] = ty * T + p / T;
-        xf[k] = px[k] * 256 + 128;
-        yf[k] = py[k] * 256 + 128;
         fx[k] = (float)px[k] + 0.5f;
         fy[k] = (float)py[k] + 0.5f;
         cr[k] = cg[k] = cb[k] = 0.0f;
-        tr[k] = 1.0f;
-        live[k] = px[k] < a.width && py[k] < a.height;
-        any_live |= live[k];
+        // pixels outside the image start saturated and are never written
+        tr[k] = px[k] < a.width && py[k] < a.height ? 1.0f : 0.0f;
     }
 
     for (uint32_t base = range.x; base < range.y; base += BATCH) {
-        if (!__syncthreads_or(any_live)) break;
+        bool live = false;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) live |= tr[k] >= kTransmittanceMin;
+        if (!__syncthreads_or(live)) break;
         uint32_t e = base + tid;
-        if (e < range.y) {
-            uint32_t g = a.tile_vals[e];
-            float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
-            float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
-            float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
-            float det = r1.x * r1.z - r1.y * r1.y;
-            if (det < 1e-6f) det = 1e-6f;
-            float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
-            float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
-            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
-            s_opa[tid] = make_float4(k2, r0.z, cutoff, 0.0f);
-            s_col[tid] = make_float4(r2.x, r2.y, r2.z, 0.0f);
-            s_quad[tid] = quad_fixed(r0.x, r0.y, r1.w, r2.w, vw, vh);
-        }
+        if (e < range.y) stage_splat(a, a.tile_vals[e], tid, st);
         __syncthreads();
         const int cnt = (int)min((uint32_t)BATCH, range.y - base);
-        if (any_live) {
-            for (int j = 0; j < cnt; ++j) {
-                const float4 geo = s_geo[j];
-                const float4 opa = s_opa[j];
-                const int4 q = s_quad[j];
-                float4 col;
-                bool loaded = false;
+        if (!__ballot(live)) continue;
+        for (int j = 0; j < cnt; ++j) {
+            const int4 r = s_rect[j];
+            const float4 geo = s_geo[j], par = s_par[j], col = s_col[j];
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    if (!live[k]) continue;
-                    if (xf[k] < q.x || xf[k] >= q.y || yf[k] < q.z || yf[k] >= q.w) continue;
-                    float dx = fx[k] - geo.x;
-                    float dy = fy[k] - geo.y;
-                    float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + opa.x * dy * dy);
-                    if (power > 0.0f || power < opa.z) continue;
-                    if (!loaded) { col = s_col[j]; loaded = true; }
-                    float alpha = minf(opa.y * __expf(power), 0.99f);
-                    float w = alpha * tr[k];
-                    cr[k] += col.x * w;
-                    cg[k] += col.y * w;
-                    cb[k] += col.z * w;
-                    tr[k] *= (1.0f - alpha);
-                    if (tr[k] < kTransmittanceMin) live[k] = false;
-                }
-            }
-            any_live = false;
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) any_live |= live[k];
+            for (int k = 0; k < PPT; ++k) blend_step(px[k], py[k], fx[k], fy[k], r, geo, par, col, cr[k], cg[k], cb[k], tr[k]);
         }
     }
 
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        if (px[k] >= a.width || py[k] >= a.height) continue;
-        size_t o = ((size_t)py[k] * a.width + px[k]) * 4;
-        ushort4 h;
-        h.x = __half_as_ushort(__float2half_rn(cr[k]));
-        h.y = __half_as_ushort(__float2half_rn(cg[k]));
-        h.z = __half_as_ushort(__float2half_rn(cb[k]));
-        h.w = __half_as_ushort(__float2half_rn(1.0f - tr[k]));
-        *reinterpret_cast<ushort4*>(a.image + o) = h;
-    }
+    for (int k = 0; k < PPT; ++k)
+        if (px[k] < a.width && py[k] < a.height) store_pixel(a, px[k], py[k], cr[k], cg[k], cb[k], tr[k]);
 }
 
 // 16x16 tiles, one block of four wave64s, each wave owning one 8x8 quadrant.
 // Every splat of an LDS batch is tested against the quadrant's pixel range
 // with one ballot per 64 splats; the wave then walks only the overlapping
-// splats (scalar bit iteration), and stops as soon as all 64 of its pixels
-// are saturated.  Coverage and fragment arithmetic are those of
-// composite_kernel.
+// splats (scalar bit iteration) with one LDS wait per splat, and stops as
+// soon as all 64 of its pixels are saturated.
 __global__ void __launch_bounds__(256) composite_quad_kernel(CompositeLaunch a) {
     constexpr int BATCH = 256;
-    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
-    __shared__ float4 s_opa[BATCH];   // k2, opacity, cutoff, -
-    __shared__ float4 s_col[BATCH];   // r, g, b, -
-    __shared__ int4 s_rect[BATCH];    // covered pixels [i0, i1) x [j0, j1)
+    __shared__ int4 s_rect[BATCH];
+    __shared__ float4 s_geo[BATCH];
+    __shared__ float4 s_par[BATCH];
+    __shared__ float4 s_col[BATCH];
+    const StagedSplats st{s_rect, s_geo, s_par, s_col};
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
     const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
     const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
     const uint2 range = a.ranges[tile];
-    const float vw = (float)a.width, vh = (float)a.height;
 
     const int qx0 = tx * 16 + (int)(wave & 1u) * 8, qy0 = ty * 16 + (int)(wave >> 1) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-    float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = 1.0f;
-    bool live = px < a.width && py < a.height;
+    const bool valid = px < a.width && py < a.height;
+    // pixels outside the image start saturated and are never written
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? 1.0f : 0.0f;
 
     for (uint32_t base = range.x; base < range.y; base += BATCH) {
-        if (!__syncthreads_or(live)) break;
+        if (!__syncthreads_or(tr >= kTransmittanceMin)) break;
         uint32_t e = base + tid;
-        if (e < range.y) {
-            uint32_t g = a.tile_vals[e];
-            float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
-            float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
-            float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
-            float det = r1.x * r1.z - r1.y * r1.y;
-            if (det < 1e-6f) det = 1e-6f;
-            float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
-            float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
-            int4 q = quad_fixed(r0.x, r0.y, r1.w, r2.w, vw, vh);
-            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
-            s_opa[tid] = make_float4(k2, r0.z, cutoff, 0.0f);
-            s_col[tid] = make_float4(r2.x, r2.y, r2.z, 0.0f);
-            s_rect[tid] = make_int4(ceil_div256(q.x - 128), ceil_div256(q.y - 128), ceil_div256(q.z - 128),
-                                    ceil_div256(q.w - 128));
-        }
+        if (e < range.y) stage_splat(a, a.tile_vals[e], tid, st);
         __syncthreads();
         const uint32_t cnt = min((uint32_t)BATCH, range.y - base);
-        if (__ballot(live)) {
-            for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
-                // which of these 64 splats touch this wave's quadrant
-                uint32_t j = chunk + lane;
-                bool hit = false;
-                if (j < cnt) {
-                    int4 r = s_rect[j];
-                    hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
-                }
-                uint64_t mask = __ballot(hit);
-                while (mask) {
-                    const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
-                    mask &= mask - 1ull;
-                    const int4 r = s_rect[k];
-                    if (!live || px < r.x || px >= r.y || py < r.z || py >= r.w) continue;
-                    const float4 geo = s_geo[k];
-                    const float4 opa = s_opa[k];
-                    float dx = fx - geo.x;
-                    float dy = fy - geo.y;
-                    float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + opa.x * dy * dy);
-                    if (power > 0.0f || power < opa.z) continue;
-                    const float4 col = s_col[k];
-                    float alpha = minf(opa.y * __expf(power), 0.99f);
-                    float w = alpha * tr;
-                    cr += col.x * w;
-                    cg += col.y * w;
-                    cb += col.z * w;
-                    tr *= (1.0f - alpha);
-                    if (tr < kTransmittanceMin) live = false;
-                }
-                if (!__ballot(live)) break;
+        if (!__ballot(tr >= kTransmittanceMin)) continue;
+        for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
+            // which of these 64 splats touch this wave's quadrant
+            uint32_t j = chunk + lane;
+            bool hit = false;
+            if (j < cnt) {
+                int4 r = s_rect[j];
+                hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
             }
+            uint64_t mask = __ballot(hit);
+            while (mask) {
+                const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                blend_step(px, py, fx, fy, s_rect[k], s_geo[k], s_par[k], s_col[k], cr, cg, cb, tr);
+            }
+            if (!__ballot(tr >= kTransmittanceMin)) break;
         }
     }
 
-    if (px < a.width && py < a.height) {
-        size_t o = ((size_t)py * a.width + px) * 4;
-        ushort4 h;
-        h.x = __half_as_ushort(__float2half_rn(cr));
-        h.y = __half_as_ushort(__float2half_rn(cg));
-        h.z = __half_as_ushort(__float2half_rn(cb));
-        h.w = __half_as_ushort(__float2half_rn(1.0f - tr));
-        *reinterpret_cast<ushort4*>(a.image + o) = h;
-    }
+    if (valid) store_pixel(a, px, py, cr, cg, cb, tr);
 }
 
 }  // namespace

@@ -287,8 +287,8 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             rec[0] = make_float4(cx, rcy, opacity, ndc_z);
             rec[1] = make_float4(ca, cb * cam.clip_y_sign, cc, radius_x);
             rec[2] = make_float4(color[0], color[1], color[2], radius_y);
-            a.rect_px[idx] = pixel_rect(cx, rcy, radius_x, radius_y, opacity, (int)cam.viewport[0],
-                                        (int)cam.viewport[1]);
+            a.rect_px[idx] = pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
+                                        (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0);
         }
     }
 

@@ -138,6 +138,7 @@ struct CullLaunch {
     uint32_t* block_counts;  // visible Gaussians per 256-block
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats)
     uint2* rect_px;          // n: covered pixel rectangle of each visible splat
+    int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
@@ -192,6 +193,7 @@ struct CompositeLaunch {
     int width, height, tile, tiles_x, tiles_y;
     uint16_t* image;         // RGBA16F, framebuffer row order
     int quadrant_cull;       // 16x16 tiles: per-wave 8x8 quadrant culling kernel
+    int tight_rects;         // per-splat pixel rectangle cut to the opacity-aware ellipse box
 };
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s);
 

@@ -38,18 +38,52 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
     return q;
 }
 
-// Covered pixels [i0, i1) x [j0, j1) inside the image, packed
-// (i0 | i1 << 16, j0 | j1 << 16); all zero when no fragment can survive:
-// the fragment test power >= log((1/255) / max(o, 1e-8)) fails everywhere
-// when that log is positive, since power <= 0.
-__device__ __forceinline__ uint2 pixel_rect(float cx, float cy, float rx, float ry, float opacity, int width,
-                                            int height) {
-    if ((1.0f / 255.0f) / maxf(opacity, 1e-8f) > 1.0f) return make_uint2(0u, 0u);
+// Pixels [i0, i1) x [j0, j1) inside the image whose centre the quad covers,
+// optionally cut down to those where the fragment test can pass.
+//
+// The fragment test keeps power = -q(d)/2 >= cutoff = log((1/255) / o), i.e.
+// q(d) <= c = -2 cutoff, with q(d) = d^T K d and K the conic.  When the
+// determinant clamp of the conic is not hit, K = S^-1 for the covariance
+// S = [[sa, sb], [sb, sc]], and {q <= c} is an ellipse whose axis-aligned box
+// has half extents sqrt(c sa) and sqrt(c sc).  With rho the correlation,
+// 1 - rho^2 = det / (sa sc); the float evaluation of power differs from the
+// exact form by at most ~10 eps q / (1 - |rho|)^2 <= 40 eps q / (1 - rho^2)^2,
+// so the box is grown by that relative margin (plus 0.3% and an absolute
+// 1e-5 on c, and 1e-3 px) and is only used for 1 - rho^2 >= 0.01.  Every
+// pixel that can pass the fragment test stays inside, so composites with and
+// without the cut are identical; it only shortens the tile lists.
+__device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx,
+                                                 float ry, float opacity, int width, int height, bool tight) {
     int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height);
     int i0 = max(ceil_div256(q.x - 128), 0), i1 = min(ceil_div256(q.y - 128), width);
     int j0 = max(ceil_div256(q.z - 128), 0), j1 = min(ceil_div256(q.w - 128), height);
-    if (i0 >= i1 || j0 >= j1) return make_uint2(0u, 0u);
-    return make_uint2((uint32_t)i0 | ((uint32_t)i1 << 16), (uint32_t)j0 | ((uint32_t)j1 << 16));
+    if (tight) {
+        const float det = sa * sc - sb * sb;
+        const float r2 = det / (sa * sc);
+        if (det >= 1e-6f && r2 >= 0.01f) {
+            const float c = maxf(-2.0f * __logf((1.0f / 255.0f) / maxf(opacity, 1e-8f)), 0.0f);
+            const float cg = c * (1.003f + 4.8e-6f / (r2 * r2)) + 1e-5f;
+            const float ex = __builtin_sqrtf(cg * sa) + 1e-3f;
+            const float ey = __builtin_sqrtf(cg * sc) + 1e-3f;
+            // pixel i can pass only if |i + 0.5 - cx| <= ex
+            i0 = max(i0, (int)__builtin_ceilf(clampf(cx - ex - 0.5f, -2.0f, 65536.0f)));
+            i1 = min(i1, (int)__builtin_floorf(clampf(cx + ex - 0.5f, -2.0f, 65536.0f)) + 1);
+            j0 = max(j0, (int)__builtin_ceilf(clampf(cy - ey - 0.5f, -2.0f, 65536.0f)));
+            j1 = min(j1, (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1);
+        }
+    }
+    return make_int4(i0, i1, j0, j1);
+}
+
+// The rectangle above packed (i0 | i1 << 16, j0 | j1 << 16); all zero when
+// no fragment can survive: the fragment test power >= log((1/255) / max(o,
+// 1e-8)) fails everywhere when that log is positive, since power <= 0.
+__device__ __forceinline__ uint2 pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx, float ry,
+                                            float opacity, int width, int height, bool tight) {
+    if ((1.0f / 255.0f) / maxf(opacity, 1e-8f) > 1.0f) return make_uint2(0u, 0u);
+    int4 r = splat_pixel_rect(cx, cy, sa, sb, sc, rx, ry, opacity, width, height, tight);
+    if (r.x >= r.y || r.z >= r.w) return make_uint2(0u, 0u);
+    return make_uint2((uint32_t)r.x | ((uint32_t)r.y << 16), (uint32_t)r.z | ((uint32_t)r.w << 16));
 }
 
 }  // namespace gs

@@ -209,7 +209,13 @@ enum {
     GS_OPT_TILE_SIZE = 1,
     /* 16x16 tiles: 1 (default) = per-wave 8x8 quadrant culling kernel,
      * 0 = one pixel per thread over every splat of the tile. */
-    GS_OPT_QUADRANT_CULL = 2
+    GS_OPT_QUADRANT_CULL = 2,
+    /* 1 (default): each splat's pixel rectangle (binning and compositing) is
+     * cut to the box of the ellipse where its fragment test can pass, with a
+     * margin that keeps every passing pixel inside; images are identical to
+     * 0 = the full quad rectangle, with fewer (tile, splat) pairs.  Takes
+     * effect from the next gs_cull. */
+    GS_OPT_TIGHT_RECTS = 3
 };
 int gs_set_option(gs_ctx* ctx, int option, int value);
 

@@ -119,6 +119,47 @@ def test_composite_vs_fp64(ctx, orc, c1_scene, tile, quad):
     assert st["tile_entries"] > 0
 
 
+def _needle_scene(gs, seed=21, n=20000):
+    """Elongated, rotated splats with opacities down to just above 1/255:
+    the cases where the opacity-aware rectangle cut has the least slack."""
+    g = gs.generate_scene(seed, n, 8, 0.0)
+    rng = np.random.default_rng(seed)
+    g[:, 3] *= rng.uniform(1.0, 40.0, n).astype(np.float32)
+    g[:, 4] *= rng.uniform(0.5, 4.0, n).astype(np.float32)
+    g[:, 13] = rng.choice(np.array([0.0040, 0.0045, 0.01, 0.05, 0.3, 0.9, 0.999], np.float32), n)
+    return g
+
+
+@pytest.mark.parametrize("scene", ["c1", "needles"])
+def test_tight_rects_identical_image(ctx, gs, c1_scene, scene):
+    """Cutting each splat's pixel rectangle to the box of its alpha >= 1/255
+    ellipse must not change a single pixel, only the (tile, splat) pairs."""
+    g, cfg, cam = c1_scene
+    if scene == "needles":
+        g = _needle_scene(gs)
+    ctx.upload(g)
+    fewer = False
+    try:
+        for tile, quad in [(16, 1), (16, 0), (8, 1), (32, 1)]:
+            print(f"case: tight rects scene={scene} tile={tile} quadrant_cull={quad}", flush=True)
+            ctx.set_option(abi.GS_OPT_QUADRANT_CULL, quad)
+            out = {}
+            for tight in (0, 1):
+                ctx.set_option(abi.GS_OPT_TIGHT_RECTS, tight)
+                ctx.render_alpha(cam, tile=tile)
+                out[tight] = (ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy(),
+                              ctx.stats()["tile_entries"])
+            print(f"tile entries full {out[0][1]} tight {out[1][1]}", flush=True)
+            assert out[1][1] <= out[0][1]
+            fewer |= out[1][1] < out[0][1]
+            np.testing.assert_array_equal(out[1][0], out[0][0])
+            assert out[0][0].any()
+    finally:
+        ctx.set_option(abi.GS_OPT_QUADRANT_CULL, 1)
+        ctx.set_option(abi.GS_OPT_TIGHT_RECTS, 1)
+    assert fewer
+
+
 def test_composite_tile_overflow_rerun(orc, gs):
     # large splats: far more (tile, splat) pairs than the initial capacity
     print("case: composite tile-pair overflow and re-run", flush=True)
