@@ -6,8 +6,10 @@ One step = one sorted-alpha frame (cull + SH-3 + depth sort + tile composite
 into RGBA16F) of a resident synthetic scene at the workload's size, inputs
 already in HBM.  Default workload: 6M Gaussians, SH degree 3, 1920x1080.
 Prints one JSON line.  Extra fields report the stochastic (GPS, 1 spp, 16 Mi
-work budget) frame rate on the same scene and per-stage times from HIP events
-on the context's stream.
+work budget) frame rate on the same scene, per-stage times from HIP events on
+the context's stream, and the batched multi-view workload (3M Gaussians, 64
+cameras of mixed sizes) as separate, batched, graph-replayed and autotuned
+frames.
 """
 from __future__ import annotations
 
@@ -52,6 +54,8 @@ def _parse():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stochastic", action="store_true")
+    ap.add_argument("--no-multiview", action="store_true")
+    ap.add_argument("--multiview-steps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (calibrates HBM byte counters) and exit")
@@ -89,6 +93,60 @@ def cpu_baseline(g, cam, width, height, threads):
     orc.composite(proj, si, width, height, mode=2, threads=threads)
     t3 = time.perf_counter()
     return t3 - t0, {"cull_s": t1 - t0, "sort_s": t2 - t1, "composite_s": t3 - t2}
+
+
+def multiview(steps: int, warmup: int) -> dict:
+    """Batched multi-view: one resident 3M-Gaussian SH-3 scene, 64 cameras of
+    mixed sizes (16 each of 256x256, 640x480, 1280x720, 1920x1080) at radius
+    10-20.  Timed: 64 separate single-view frames, one batched frame (direct
+    launches and hipGraph replay), and the batched frame in the configuration
+    the autotuner picks."""
+    import gamesplat_amd as gs
+    from gamesplat_amd import scenes
+
+    g, cfg = scenes.scene_config("c5")
+    N = cfg["count"]
+    cams = scenes.multiview_cameras(cfg["seed"])
+    V = len(cams)
+    out = {"workload": f"{N} Gaussians SH-3, {V} views (16x256x256, 16x640x480, 16x1280x720, 16x1920x1080)"}
+    with gs.Context(0) as ctx:
+        ctx.upload(g)
+        del g
+
+        def timed(fn, k):
+            for _ in range(warmup):
+                fn()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+            ctx.synchronize()
+            return (time.perf_counter() - t0) / k
+
+        def separate():
+            for cam in cams:
+                ctx.render_alpha(cam)
+
+        def batched():
+            ctx.render_views(cams)
+
+        sep = timed(separate, max(1, steps // 2))
+        bat = timed(batched, steps)
+        ctx.set_option(gs.abi.GS_OPT_GRAPH, 1)
+        gra = timed(batched, steps)
+        tune = ctx.autotune(cams, frames=2)
+        tuned = timed(batched, steps)
+        fi = ctx.frame_info()
+        st = ctx.stats()
+    for name, sec in (("separate", sep), ("batched", bat), ("batched_graph", gra), ("batched_tuned", tuned)):
+        out[name] = {"ms_per_64_views": round(sec * 1e3, 3), "views_per_s": round(V / sec, 1),
+                     "gaussians_per_s": round(V * N / sec, 1)}
+    out["speedup_tuned_vs_separate"] = round(sep / tuned, 3)
+    out["autotune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}
+    out["frame_info"] = fi
+    out["visible_total"] = st["visible_count"]
+    out["tile_entries_total"] = st["tile_entries"]
+    return out
 
 
 def main():
@@ -199,6 +257,12 @@ def main():
                    for k, v in cands.items()},
     }
 
+    mv = None
+    if not args.no_multiview:
+        ctx.close()
+        mv = multiview(args.multiview_steps, 1)
+        print(f"# multiview: {json.dumps(mv)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -229,6 +293,7 @@ def main():
         "stochastic": stoch,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "multiview": mv,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

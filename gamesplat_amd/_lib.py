@@ -36,6 +36,10 @@ GS_OPT_GPS_KEEP_CELLS = 0
 GS_OPT_TILE_SIZE = 1
 GS_OPT_QUADRANT_CULL = 2
 GS_OPT_TIGHT_RECTS = 3
+GS_OPT_RADIX_BITS = 4
+GS_OPT_BLOCKS_PER_CU = 5
+GS_OPT_GRAPH = 6
+MAX_VIEWS = 64
 
 STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve")
 
@@ -92,6 +96,31 @@ class GsStats(C.Structure):
         return d
 
 
+class GsFrameInfo(C.Structure):
+    _fields_ = [
+        ("views", C.c_uint32),
+        ("total_tiles", C.c_uint32),
+        ("tile_size", C.c_int32),
+        ("radix_bits", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("graph", C.c_int32),
+        ("graph_captures", C.c_uint32),
+        ("graph_replays", C.c_uint32),
+        ("overflow_reruns", C.c_uint32),
+    ]
+
+
+class GsTuneResult(C.Structure):
+    _fields_ = [
+        ("tile_size", C.c_int32),
+        ("radix_bits", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("configs", C.c_int32),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+    ]
+
+
 EXPORTS = {
     "gs_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
     "gs_destroy": (None, [C.c_void_p]),
@@ -109,6 +138,12 @@ EXPORTS = {
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gs_render_alpha": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
                                   C.POINTER(GsCullOptions), C.c_int]),
+    "gs_render_views": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.c_uint32,
+                                  C.POINTER(GsSplatEffectParams), C.POINTER(GsCullOptions), C.c_int]),
+    "gs_read_view_image": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t]),
+    "gs_frame_info": (C.c_int, [C.c_void_p, C.POINTER(GsFrameInfo)]),
+    "gs_autotune": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.c_uint32, C.POINTER(GsCullOptions),
+                              C.c_int, C.POINTER(GsTuneResult)]),
     "gs_read_image": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "gs_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "gs_stats": (C.c_int, [C.c_void_p, C.POINTER(GsStats)]),
@@ -239,6 +274,7 @@ class Context:
                                 C.byref(opts)))
         self.width = int(cam.viewport[0])
         self.height = int(cam.viewport[1])
+        self._views = [(self.width, self.height)]
 
     def sort(self):
         check(self._lib.gs_sort(self._h))
@@ -251,6 +287,34 @@ class Context:
         check(self._lib.gs_render_alpha(self._h, C.byref(cam), None, C.byref(opts), tile))
         self.width = int(cam.viewport[0])
         self.height = int(cam.viewport[1])
+        self._views = [(self.width, self.height)]
+
+    def render_views(self, cams, sh_degree: int = 3, tile: int = 0):
+        """One batched sorted-alpha frame over up to 64 cameras."""
+        arr = (GsCameraUniforms * len(cams))(*cams)
+        opts = GsCullOptions(sh_degree, 0.0, 0)
+        check(self._lib.gs_render_views(self._h, arr, len(cams), None, C.byref(opts), tile))
+        self._views = [(int(c.viewport[0]), int(c.viewport[1])) for c in cams]
+        self.width, self.height = self._views[0]
+
+    def read_view_image(self, view: int) -> np.ndarray:
+        views = getattr(self, "_views", [])
+        w, h = views[view] if 0 <= view < len(views) else (0, 0)  # the library reports a bad view
+        out = np.empty((h, w, 4), dtype=np.float16)
+        check(self._lib.gs_read_view_image(self._h, view, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def frame_info(self) -> dict:
+        fi = GsFrameInfo()
+        check(self._lib.gs_frame_info(self._h, C.byref(fi)))
+        return {k: getattr(fi, k) for k, _ in fi._fields_}
+
+    def autotune(self, cams, frames: int = 3, sh_degree: int = 3) -> dict:
+        arr = (GsCameraUniforms * len(cams))(*cams)
+        opts = GsCullOptions(sh_degree, 0.0, 0)
+        res = GsTuneResult()
+        check(self._lib.gs_autotune(self._h, arr, len(cams), C.byref(opts), frames, C.byref(res)))
+        return {k: getattr(res, k) for k, _ in res._fields_}
 
     def gps_sample(self, frame_seed: int, supersample: int = 1, linear_color: bool = False,
                    background: float = 0.0):

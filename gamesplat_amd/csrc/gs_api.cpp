@@ -64,6 +64,26 @@ struct TimedLaunch {
     hipEvent_t start, stop;
 };
 
+// Per-frame constants of a (batched) frame, uploaded through a pinned ring.
+struct FrameConst {
+    GsCameraUniforms cams[kMaxViews];
+    ViewInfo views[kMaxViews];
+};
+constexpr int kRing = 8;
+
+// Every launch of one sorted-alpha frame, by value.  Two frames with the same
+// bytes here launch the same kernels with the same arguments, which is what
+// lets a captured hipGraph be replayed.
+struct FramePlan {
+    CullLaunch cull;
+    uint32_t cull_blocks;    // per view
+    SortLaunch depth;
+    BinLaunch bin;
+    SortLaunch tiles;
+    CompositeLaunch comp;
+    uint32_t range_words;
+};
+
 }  // namespace
 
 struct gs_ctx {
@@ -96,7 +116,27 @@ struct gs_ctx {
     int tight_rects = 1;
     int cull_tight = 1;      // tight_rects as the last cull used it
 
-    GsCameraUniforms cam{};
+    GsCameraUniforms cam{};  // view 0 of the current frame
+    uint32_t views = 1;      // views of the current frame
+    uint32_t view_cap = 0;   // views the [view][n] arrays are sized for
+    std::vector<ViewInfo> vinfo;
+    std::vector<GsCameraUniforms> host_cams;
+    uint32_t total_tiles = 0, total_pixels = 0;
+    int vinfo_tile = 0;
+    DevBuf frame_dev;        // FrameConst on the device
+    FrameConst* ring = nullptr;  // pinned, kRing slots
+    hipEvent_t ring_event[kRing] = {};
+    bool ring_used[kRing] = {};
+    int ring_next = 0;
+    int radix_bits = 8;
+    int blocks_per_cu = 0;
+    int use_graph = 0;
+    FramePlan plan{};
+    bool plan_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    uint32_t graph_captures = 0, graph_replays = 0;
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
     bool stats_collected = false;
     int sort_slot = 0;
@@ -158,6 +198,91 @@ int ensure_table(gs_ctx* c, uint32_t elems) {
     return ensure(c, c->table, (size_t)sort_table_words(elems) * 4, "radix tile table");
 }
 
+// Size the [view][n] arrays (records, dense keys, pixel rectangles, the
+// visible list and its sort buffers) for `views` views of the scene.
+int ensure_views(gs_ctx* c, uint32_t views) {
+    if (views <= c->view_cap) return GS_OK;
+    const size_t cap = (size_t)views * c->n;
+    if (cap > 0xFFFFFFFFull) return fail(GS_ERR_LIMIT, "%u views x %u Gaussians exceed 2^32 splats", views, c->n);
+    GS_CHECK(ensure(c, c->records, cap * kRecordVec4 * 16, "projected splats"));
+    for (int k = 0; k < 3; ++k) {
+        GS_CHECK(ensure(c, c->keys[k], cap * 4, "sort keys"));
+        GS_CHECK(ensure(c, c->ids[k], cap * 4, "sort ids"));
+    }
+    const size_t blocks = (size_t)views * ((c->n + 255u) / 256u);
+    GS_CHECK(ensure(c, c->dense_keys, cap * 4, "dense keys"));
+    GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
+    GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
+    GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
+    GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
+    GS_CHECK(ensure_table(c, (uint32_t)std::max<size_t>(cap, c->dup_cap)));
+    c->view_cap = views;
+    return GS_OK;
+}
+
+// Next pinned ring slot, once the copy that last read it has completed.
+int ring_slot(gs_ctx* c, FrameConst** slot, hipEvent_t* ev) {
+    const int k = c->ring_next;
+    c->ring_next = (k + 1) % kRing;
+    if (c->ring_used[k]) GS_HIP(hipEventSynchronize(c->ring_event[k]));
+    c->ring_used[k] = true;
+    *slot = c->ring + k;
+    *ev = c->ring_event[k];
+    return GS_OK;
+}
+
+FrameConst* frame_dev(gs_ctx* c) { return static_cast<FrameConst*>(c->frame_dev.p); }
+
+int upload_cameras(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count) {
+    FrameConst* slot;
+    hipEvent_t ev;
+    GS_CHECK(ring_slot(c, &slot, &ev));
+    memcpy(slot->cams, cams, sizeof(GsCameraUniforms) * count);
+    c->host_cams.assign(cams, cams + count);
+    GS_HIP(hipMemcpyAsync(frame_dev(c)->cams, slot->cams, sizeof(GsCameraUniforms) * count, hipMemcpyHostToDevice,
+                          c->stream));
+    GS_HIP(hipEventRecord(ev, c->stream));
+    c->cam = cams[0];
+    c->views = count;
+    return GS_OK;
+}
+
+// Tile grid and image placement of every view for `tile`, uploaded when it
+// changes.
+int upload_views(gs_ctx* c, const GsCameraUniforms* cams, int tile) {
+    std::vector<ViewInfo> vinfo(c->views);
+    uint64_t tiles = 0, pixels = 0;
+    for (uint32_t v = 0; v < c->views; ++v) {
+        ViewInfo& vi = vinfo[v];
+        vi.width = (int)cams[v].viewport[0];
+        vi.height = (int)cams[v].viewport[1];
+        vi.tiles_x = (vi.width + tile - 1) / tile;
+        vi.tiles_y = (vi.height + tile - 1) / tile;
+        vi.tile_base = (uint32_t)tiles;
+        vi.pixel_base = (uint32_t)pixels;
+        tiles += (uint64_t)vi.tiles_x * vi.tiles_y;
+        pixels += (uint64_t)vi.width * vi.height;
+    }
+    if (tiles > 0x7FFFFFFFull || pixels > 0x7FFFFFFFull)
+        return fail(GS_ERR_LIMIT, "views need %llu tiles / %llu pixels, over 2^31", (unsigned long long)tiles,
+                    (unsigned long long)pixels);
+    if (c->vinfo_tile == tile && c->vinfo.size() == vinfo.size() &&
+        memcmp(c->vinfo.data(), vinfo.data(), sizeof(ViewInfo) * vinfo.size()) == 0)
+        return GS_OK;
+    c->vinfo.swap(vinfo);
+    FrameConst* slot;
+    hipEvent_t ev;
+    GS_CHECK(ring_slot(c, &slot, &ev));
+    memcpy(slot->views, c->vinfo.data(), sizeof(ViewInfo) * c->views);
+    GS_HIP(hipMemcpyAsync(frame_dev(c)->views, slot->views, sizeof(ViewInfo) * c->views, hipMemcpyHostToDevice,
+                          c->stream));
+    GS_HIP(hipEventRecord(ev, c->stream));
+    c->total_tiles = (uint32_t)tiles;
+    c->total_pixels = (uint32_t)pixels;
+    c->vinfo_tile = tile;
+    return GS_OK;
+}
+
 int timed_begin(gs_ctx* c, int stage, TimedLaunch* t) {
     if (!c->profiling) return GS_OK;
     t->stage = stage;
@@ -193,20 +318,22 @@ int check_camera(const GsCameraUniforms* cam) {
     return GS_OK;
 }
 
+// GL framebuffer rows start at the bottom; readback returns the top row first.
+bool gl_rows(const gs_ctx* c) { return c->cam.clip_y_sign > 0.0f; }
+
 int read_u32(gs_ctx* c, const uint32_t* dev, uint32_t* host) {
     GS_HIP(hipMemcpyAsync(host, dev, 4, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
     return GS_OK;
 }
 
-int copy_image(gs_ctx* c, const void* dev, void* host, int w, int h, size_t px_bytes, size_t bytes) {
+int copy_image(gs_ctx* c, const void* dev, void* host, int w, int h, size_t px_bytes, size_t bytes, bool flip) {
     size_t need = (size_t)w * h * px_bytes;
     if (bytes != need) return fail(GS_ERR_INVALID, "image needs %zu bytes, got %zu", need, bytes);
     std::vector<uint8_t> tmp(need);
     GS_HIP(hipMemcpyAsync(tmp.data(), dev, need, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
     size_t row = (size_t)w * px_bytes;
-    bool flip = c->cam.clip_y_sign > 0.0f;  // GL framebuffer rows start at the bottom
     for (int y = 0; y < h; ++y) {
         int src = flip ? h - 1 - y : y;
         memcpy(static_cast<uint8_t*>(host) + (size_t)y * row, tmp.data() + (size_t)src * row, row);
@@ -236,10 +363,14 @@ int gs_create(gs_ctx** out, int device) {
         return fail(GS_ERR_HIP, "device setup failed: %s", hipGetErrorString(e));
     }
     int r = ensure(c, c->counters, sizeof(DevCounters), "counters");
+    if (r == GS_OK) r = ensure(c, c->frame_dev, sizeof(FrameConst), "frame constants");
     if (r == GS_OK) {
         e = hipMemsetAsync(c->counters.p, 0, sizeof(DevCounters), c->stream);
         if (e == hipSuccess) e = hipHostMalloc((void**)&c->dup_host, 4, hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->dup_event, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&c->ring, sizeof(FrameConst) * kRing, hipHostMallocDefault);
+        for (int k = 0; k < kRing && e == hipSuccess; ++k)
+            e = hipEventCreateWithFlags(&c->ring_event[k], hipEventDisableTiming);
         if (e != hipSuccess) r = fail(GS_ERR_HIP, "context init failed: %s", hipGetErrorString(e));
     }
     if (r != GS_OK) {
@@ -265,6 +396,12 @@ void gs_destroy(gs_ctx* c) {
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum};
     for (DevBuf* b : bufs) release(c, *b);
+    release(c, c->frame_dev);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (int k = 0; k < kRing; ++k)
+        if (c->ring_event[k]) (void)hipEventDestroy(c->ring_event[k]);
+    if (c->ring) (void)hipHostFree(c->ring);
     if (c->dup_event) (void)hipEventDestroy(c->dup_event);
     if (c->dup_host) (void)hipHostFree(c->dup_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -301,6 +438,17 @@ int gs_set_option(gs_ctx* c, int option, int value) {
             return ok();
         case GS_OPT_TIGHT_RECTS:
             c->tight_rects = value ? 1 : 0;
+            return ok();
+        case GS_OPT_RADIX_BITS:
+            if (value != 4 && value != 8) return fail(GS_ERR_INVALID, "radix digit width must be 4 or 8");
+            c->radix_bits = value;
+            return ok();
+        case GS_OPT_BLOCKS_PER_CU:
+            if (value < 0 || value > 64) return fail(GS_ERR_INVALID, "blocks per CU must be in [0, 64]");
+            c->blocks_per_cu = value;
+            return ok();
+        case GS_OPT_GRAPH:
+            c->use_graph = value ? 1 : 0;
             return ok();
         default:
             return fail(GS_ERR_INVALID, "unknown option %d", option);
@@ -366,18 +514,8 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     release(c, c->planes);
     c->planes = planes;
     c->n = count;
-    GS_CHECK(ensure(c, c->records, (size_t)count * kRecordVec4 * 16, "projected splats"));
-    for (int k = 0; k < 3; ++k) {
-        GS_CHECK(ensure(c, c->keys[k], (size_t)count * 4, "sort keys"));
-        GS_CHECK(ensure(c, c->ids[k], (size_t)count * 4, "sort ids"));
-    }
-    const size_t blocks = (count + 255u) / 256u;
-    GS_CHECK(ensure(c, c->dense_keys, (size_t)count * 4, "dense keys"));
-    GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
-    GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
-    GS_CHECK(ensure(c, c->rects, (size_t)count * 8, "bin rectangles"));
-    GS_CHECK(ensure(c, c->rect_px, (size_t)count * 8, "pixel rectangles"));
-    GS_CHECK(ensure_table(c, count));
+    c->view_cap = 0;
+    GS_CHECK(ensure_views(c, 1));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
     return ok();
 }
@@ -410,111 +548,82 @@ int gs_set_gps_budget(gs_ctx* c, uint32_t max_work_items) {
     return ok();
 }
 
-int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* effect, const GsCullOptions* opts) {
-    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
-    if (!c->n) return fail(GS_ERR_STATE, "no Gaussians uploaded");
-    GS_CHECK(check_camera(cam));
-    int sh_degree = opts ? opts->sh_degree : 3;
-    if (sh_degree < 0 || sh_degree > 3) return fail(GS_ERR_INVALID, "sh_degree %d outside [0, 3]", sh_degree);
-    GS_CHECK(set_device(c));
-    const uint32_t blocks = (c->n + 255u) / 256u;
-    const int collect = opts ? opts->collect_stats : 0;
-    if (collect) GS_CHECK(ensure(c, c->block_diag, (size_t)blocks * 32, "block diagnostics"));
-    c->cam = *cam;
-    CullLaunch a{};
-    a.planes = ptr<float4>(c->planes);
-    a.n = c->n;
-    a.cam = *cam;
-    if (effect) a.effect = *effect;
-    a.effect_active = effect && effect->params[3] > 0.5f;
-    a.sh_degree = sh_degree;
-    a.sh_lod_distance = opts ? opts->sh_lod_distance : 0.0f;
-    a.collect_stats = collect;
-    a.records = ptr<float4>(c->records);
-    a.dense_keys = ptr<uint32_t>(c->dense_keys);
-    a.block_counts = ptr<uint32_t>(c->block_counts);
-    a.block_diag = collect ? ptr<uint32_t>(c->block_diag) : nullptr;
-    a.rect_px = ptr<uint2>(c->rect_px);
-    a.tight_rects = c->tight_rects;
-    c->cull_tight = c->tight_rects;
-    TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
-    GS_HIP(launch_cull(a, c->stream));
-    GS_HIP(launch_scan_counts(ptr<uint32_t>(c->block_counts), blocks, ptr<uint32_t>(c->block_offsets),
-                              &c->dc()->cull.visible_count, a.block_diag, c->dc()->cull.diag, c->stream));
-    GS_HIP(launch_compact(ptr<uint32_t>(c->dense_keys), c->n, ptr<uint32_t>(c->block_offsets),
-                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), c->stream));
-    GS_CHECK(timed_end(c, &t));
-    c->have_cull = true;
-    c->have_sort = c->have_composite = false;
-    c->stats_collected = a.collect_stats != 0;
-    return ok();
-}
+}  // extern "C"
 
-int gs_sort(gs_ctx* c) {
-    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
-    if (!c->have_cull) return fail(GS_ERR_STATE, "gs_sort needs a gs_cull first");
-    GS_CHECK(set_device(c));
-    SortLaunch s{};
-    for (int k = 0; k < 3; ++k) {
-        s.keys[k] = ptr<uint32_t>(c->keys[k]);
-        s.vals[k] = ptr<uint32_t>(c->ids[k]);
-    }
-    s.count = &c->dc()->cull.visible_count;
-    s.capacity = c->n;
-    s.scratch = c->dc()->depth_sort.scratch;
-    s.table = ptr<uint32_t>(c->table);
-    s.passes = 4;
-    s.bits = 8;
-    TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
-    int slot = 0;
-    GS_HIP(launch_radix_sort(s, &slot, c->stream));
-    GS_CHECK(timed_end(c, &t));
-    c->sort_slot = slot;
-    c->have_sort = true;
-    c->have_composite = false;
-    return ok();
-}
+namespace {
 
 // (tile id, splat id) buffers are sized by capacity so a frame needs no host
-// round trip.  The requested pair count is copied back asynchronously; before
-// any composite result is observed (image readback, stats, synchronize) the
-// count is checked, and a frame that overflowed is re-binned with grown
-// buffers.  A new frame also grows the buffers as soon as the previous
-// frame's count is known.
-static int grow_tile_buffers(gs_ctx* c, uint32_t entries) {
+// round trip; they grow once a frame's requested count is known to exceed it.
+int grow_tile_buffers(gs_ctx* c, uint32_t entries) {
     if (entries > 0x7FFFFFFFu) return fail(GS_ERR_LIMIT, "%u tile entries exceed the 2^31 limit", entries);
     if (entries <= c->dup_cap) return GS_OK;
     for (int k = 0; k < 3; ++k) {
         GS_CHECK(ensure(c, c->dup_keys[k], (size_t)entries * 4, "tile keys"));
         GS_CHECK(ensure(c, c->dup_vals[k], (size_t)entries * 4, "tile values"));
     }
-    GS_CHECK(ensure_table(c, std::max(entries, c->n)));
+    GS_CHECK(ensure_table(c, (uint32_t)std::max<size_t>(entries, (size_t)c->view_cap * c->n)));
     c->dup_cap = entries;
     return GS_OK;
 }
 
-static int enqueue_composite(gs_ctx* c, int tile) {
-    const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
-    const int tiles_x = (W + tile - 1) / tile, tiles_y = (H + tile - 1) / tile;
-    const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
-    GS_CHECK(ensure(c, c->image, (size_t)W * H * 8, "RGBA16F image"));
-    GS_CHECK(ensure(c, c->ranges, (size_t)ntiles * 8, "tile ranges"));
-    if (!c->dup_cap) GS_CHECK(grow_tile_buffers(c, std::max(3u * c->n, 65536u)));
-    int passes = 1;
-    while (passes < 4 && (ntiles - 1u) >> (8 * passes)) ++passes;
+int check_cull_options(const GsCullOptions* opts, uint32_t views) {
+    int sh_degree = opts ? opts->sh_degree : 3;
+    if (sh_degree < 0 || sh_degree > 3) return fail(GS_ERR_INVALID, "sh_degree %d outside [0, 3]", sh_degree);
+    if (opts && opts->collect_stats && views > 1)
+        return fail(GS_ERR_INVALID, "collect_stats needs a single view");
+    return GS_OK;
+}
 
-    BinLaunch b{};
-    b.records = ptr<float4>(c->records);
+CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullOptions* opts) {
+    CullLaunch a{};
+    a.planes = ptr<float4>(c->planes);
+    a.n = c->n;
+    a.cams = frame_dev(c)->cams;
+    a.views = c->views;
+    if (effect) a.effect = *effect;
+    a.effect_active = effect && effect->params[3] > 0.5f;
+    a.sh_degree = opts ? opts->sh_degree : 3;
+    a.sh_lod_distance = opts ? opts->sh_lod_distance : 0.0f;
+    a.collect_stats = opts ? opts->collect_stats : 0;
+    a.records = ptr<float4>(c->records);
+    a.dense_keys = ptr<uint32_t>(c->dense_keys);
+    a.block_counts = ptr<uint32_t>(c->block_counts);
+    a.block_diag = a.collect_stats ? ptr<uint32_t>(c->block_diag) : nullptr;
+    a.rect_px = ptr<uint2>(c->rect_px);
+    a.tight_rects = c->tight_rects;
+    return a;
+}
+
+SortLaunch make_depth_sort(gs_ctx* c) {
+    SortLaunch s{};
+    for (int k = 0; k < 3; ++k) {
+        s.keys[k] = ptr<uint32_t>(c->keys[k]);
+        s.vals[k] = ptr<uint32_t>(c->ids[k]);
+    }
+    s.count = &c->dc()->cull.visible_count;
+    s.capacity = c->views * c->n;
+    s.scratch = c->dc()->depth_sort.scratch;
+    s.table = ptr<uint32_t>(c->table);
+    s.bits = c->radix_bits;
+    s.passes = 32 / c->radix_bits;
+    return s;
+}
+
+// Bin, tile-sort and composite launches for the current views and tile size.
+void make_composite(gs_ctx* c, int tile, FramePlan* p) {
+    const uint32_t ntiles = c->total_tiles;
+    int bits_needed = 1;
+    while (bits_needed < 31 && ((ntiles - 1u) >> bits_needed)) ++bits_needed;
+    const int passes = (bits_needed + c->radix_bits - 1) / c->radix_bits;
+
+    BinLaunch& b = p->bin;
+    b = BinLaunch{};
     b.sorted_ids = ptr<uint32_t>(c->ids[c->sort_slot]);
     b.count = &c->dc()->cull.visible_count;
-    b.capacity = c->n;
-    b.width = W;
-    b.height = H;
+    b.capacity = c->views * c->n;
+    b.n = c->n;
     b.tile = tile;
-    b.tiles_x = tiles_x;
-    b.tiles_y = tiles_y;
+    b.views = frame_dev(c)->views;
     b.counters = c->dc();
     b.rect_px = ptr<uint2>(c->rect_px);
     b.rects = ptr<uint2>(c->rects);
@@ -524,7 +633,8 @@ static int enqueue_composite(gs_ctx* c, int tile) {
     b.dup_vals = ptr<uint32_t>(c->dup_vals[0]);
     b.dup_capacity = c->dup_cap;
 
-    SortLaunch s{};
+    SortLaunch& s = p->tiles;
+    s = SortLaunch{};
     for (int k = 0; k < 3; ++k) {
         s.keys[k] = ptr<uint32_t>(c->dup_keys[k]);
         s.vals[k] = ptr<uint32_t>(c->dup_vals[k]);
@@ -534,46 +644,100 @@ static int enqueue_composite(gs_ctx* c, int tile) {
     s.scratch = c->dc()->tile_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
     s.passes = passes;
-    s.bits = 8;
+    s.bits = c->radix_bits;
 
-    GS_HIP(launch_fill_u32(ptr<uint32_t>(c->ranges), 0u, (size_t)ntiles * 2, c->stream));
-    TimedLaunch t{};
-    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
-    GS_HIP(launch_bin(b, c->stream));
-    int slot = 0;
-    GS_HIP(launch_radix_sort(s, &slot, c->stream));
-    GS_HIP(launch_tile_ranges(s.keys[slot], s.count, c->dup_cap, ptr<uint2>(c->ranges), c->stream));
-    GS_CHECK(timed_end(c, &t));
-
-    CompositeLaunch cl{};
+    // the sorted pairs land in buffer 2 for an even pass count, else 1
+    const int slot = passes % 2 == 0 ? 2 : 1;
+    CompositeLaunch& cl = p->comp;
+    cl = CompositeLaunch{};
     cl.records = ptr<float4>(c->records);
     cl.tile_vals = s.vals[slot];
     cl.ranges = ptr<uint2>(c->ranges);
-    cl.width = W;
-    cl.height = H;
     cl.tile = tile;
-    cl.tiles_x = tiles_x;
-    cl.tiles_y = tiles_y;
+    cl.views = frame_dev(c)->views;
+    cl.nviews = c->views;
+    cl.total_tiles = ntiles;
     cl.image = ptr<uint16_t>(c->image);
     cl.quadrant_cull = c->quadrant_cull;
-    cl.tight_rects = c->cull_tight;
+    cl.tight_rects = c->tight_rects;
+    cl.persistent_blocks = c->blocks_per_cu > 0 ? (uint32_t)(c->blocks_per_cu * c->props.multiProcessorCount) : 0u;
+    cl.queue = c->dc()->composite_queue;
+    p->range_words = ntiles * 2u;
+}
+
+int stage_cull(gs_ctx* c, const FramePlan& p) {
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
+    GS_HIP(launch_cull(p.cull, c->stream));
+    GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->block_offsets),
+                              &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream));
+    GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),
+                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), c->stream));
+    GS_CHECK(timed_end(c, &t));
+    return GS_OK;
+}
+
+int stage_sort(gs_ctx* c, const FramePlan& p, int* slot) {
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
+    GS_HIP(launch_radix_sort(p.depth, slot, c->stream));
+    GS_CHECK(timed_end(c, &t));
+    return GS_OK;
+}
+
+int stage_composite(gs_ctx* c, const FramePlan& p) {
+    GS_HIP(launch_fill_u32(const_cast<uint32_t*>(reinterpret_cast<const uint32_t*>(p.comp.ranges)), 0u,
+                           p.range_words, c->stream));
+    if (p.comp.persistent_blocks) GS_HIP(launch_fill_u32(p.comp.queue, 0u, 8, c->stream));
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));
+    GS_HIP(launch_bin(p.bin, c->stream));
+    int slot = 0;
+    GS_HIP(launch_radix_sort(p.tiles, &slot, c->stream));
+    GS_HIP(launch_tile_ranges(p.tiles.keys[slot], p.tiles.count, p.tiles.capacity,
+                              const_cast<uint2*>(p.comp.ranges), c->stream));
+    GS_CHECK(timed_end(c, &t));
     TimedLaunch t2{};
     GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
-    GS_HIP(launch_composite(cl, c->stream));
+    GS_HIP(launch_composite(p.comp, c->stream));
     GS_CHECK(timed_end(c, &t2));
+    return GS_OK;
+}
+
+// Buffers a composite at `tile` over the current views needs.
+int prepare_composite(gs_ctx* c, int tile) {
+    GS_CHECK(ensure(c, c->image, (size_t)c->total_pixels * 8, "RGBA16F images"));
+    GS_CHECK(ensure(c, c->ranges, (size_t)c->total_tiles * 8, "tile ranges"));
+    if (!c->dup_cap) GS_CHECK(grow_tile_buffers(c, std::max(3u * c->views * c->n, 65536u)));
+    (void)tile;
+    return GS_OK;
+}
+
+// The requested pair count comes back asynchronously: before any composite
+// result is observed (image readback, stats, synchronize) the count is
+// checked, and a frame that overflowed is re-binned with grown buffers.
+int after_composite(gs_ctx* c, int tile) {
     GS_HIP(hipMemcpyAsync(c->dup_host, &c->dc()->bin.dup_total, 4, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipEventRecord(c->dup_event, c->stream));
     c->dup_pending = true;
     c->composite_tile = tile;
-    c->img_w = W;
-    c->img_h = H;
+    c->img_w = (int)c->cam.viewport[0];
+    c->img_h = (int)c->cam.viewport[1];
     return GS_OK;
+}
+
+int enqueue_composite(gs_ctx* c, int tile) {
+    GS_CHECK(upload_views(c, c->host_cams.data(), tile));
+    GS_CHECK(prepare_composite(c, tile));
+    make_composite(c, tile, &c->plan);
+    GS_CHECK(stage_composite(c, c->plan));
+    return after_composite(c, tile);
 }
 
 // Waits for the pending pair count (or only polls when wait is false); grows
 // the buffers on overflow and, when the current composite overflowed and
 // rerun is set, composites again until every pair fits.
-static int settle_composite(gs_ctx* c, bool wait, bool rerun) {
+int settle_composite(gs_ctx* c, bool wait, bool rerun) {
     while (c->dup_pending) {
         if (wait) {
             GS_HIP(hipEventSynchronize(c->dup_event));
@@ -594,6 +758,76 @@ static int settle_composite(gs_ctx* c, bool wait, bool rerun) {
     return GS_OK;
 }
 
+// Replays the captured frame when the plan is byte-identical to the one it
+// was captured from, otherwise captures the plan's launches first.
+int launch_graph(gs_ctx* c, const FramePlan& p) {
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(&p);
+    if (!c->graph_exec || c->graph_key.size() != sizeof(FramePlan) ||
+        memcmp(c->graph_key.data(), b, sizeof(FramePlan)) != 0) {
+        if (c->graph_exec) GS_HIP(hipGraphExecDestroy(c->graph_exec));
+        if (c->graph) GS_HIP(hipGraphDestroy(c->graph));
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        GS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int slot = 0;
+        int r = stage_cull(c, p);
+        if (r == GS_OK) r = stage_sort(c, p, &slot);
+        if (r == GS_OK) r = stage_composite(c, p);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r != GS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        GS_HIP(e);
+        c->graph = g;
+        GS_HIP(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        c->graph_key.assign(b, b + sizeof(FramePlan));
+        c->graph_captures++;
+    } else {
+        c->graph_replays++;
+    }
+    GS_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* effect, const GsCullOptions* opts) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->n) return fail(GS_ERR_STATE, "no Gaussians uploaded");
+    GS_CHECK(check_camera(cam));
+    GS_CHECK(check_cull_options(opts, 1));
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, false, false));
+    const uint32_t blocks = (c->n + 255u) / 256u;
+    const int collect = opts ? opts->collect_stats : 0;
+    if (collect) GS_CHECK(ensure(c, c->block_diag, (size_t)blocks * 32, "block diagnostics"));
+    GS_CHECK(upload_cameras(c, cam, 1));
+    c->plan.cull = make_cull(c, effect, opts);
+    c->plan.cull_blocks = blocks;
+    GS_CHECK(stage_cull(c, c->plan));
+    c->have_cull = true;
+    c->have_sort = c->have_composite = false;
+    c->stats_collected = collect != 0;
+    return ok();
+}
+
+int gs_sort(gs_ctx* c) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_cull) return fail(GS_ERR_STATE, "gs_sort needs a gs_cull first");
+    GS_CHECK(set_device(c));
+    c->plan.depth = make_depth_sort(c);
+    int slot = 0;
+    GS_CHECK(stage_sort(c, c->plan, &slot));
+    c->sort_slot = slot;
+    c->have_sort = true;
+    c->have_composite = false;
+    return ok();
+}
+
 int gs_composite_alpha(gs_ctx* c, int tile) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
     if (!c->have_sort) return fail(GS_ERR_STATE, "gs_composite_alpha needs gs_cull and gs_sort first");
@@ -607,16 +841,149 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     return ok();
 }
 
+int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, const GsSplatEffectParams* effect,
+                    const GsCullOptions* opts, int tile) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->n) return fail(GS_ERR_STATE, "no Gaussians uploaded");
+    if (!cams || count == 0 || count > (uint32_t)kMaxViews)
+        return fail(GS_ERR_INVALID, "view count %u outside [1, %d]", count, kMaxViews);
+    for (uint32_t v = 0; v < count; ++v) GS_CHECK(check_camera(cams + v));
+    GS_CHECK(check_cull_options(opts, count));
+    if (tile == 0) tile = c->default_tile;
+    if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, false, false));
+    const uint32_t blocks = (c->n + 255u) / 256u;
+    const int collect = opts ? opts->collect_stats : 0;
+    if (collect) GS_CHECK(ensure(c, c->block_diag, (size_t)blocks * 32, "block diagnostics"));
+    GS_CHECK(ensure_views(c, count));
+    GS_CHECK(upload_cameras(c, cams, count));
+    GS_CHECK(upload_views(c, cams, tile));
+    GS_CHECK(prepare_composite(c, tile));
+
+    FramePlan& p = c->plan;
+    p.cull = make_cull(c, effect, opts);
+    p.cull_blocks = blocks;
+    p.depth = make_depth_sort(c);
+    // the sorted list lands in buffer 2 for an even pass count, else 1
+    c->sort_slot = p.depth.passes % 2 == 0 ? 2 : 1;
+    make_composite(c, tile, &p);
+    if (c->use_graph && !c->profiling) {
+        GS_CHECK(launch_graph(c, p));
+    } else {
+        int slot = 0;
+        GS_CHECK(stage_cull(c, p));
+        GS_CHECK(stage_sort(c, p, &slot));
+        GS_CHECK(stage_composite(c, p));
+    }
+    GS_CHECK(after_composite(c, tile));
+    c->have_cull = c->have_sort = c->have_composite = true;
+    c->stats_collected = collect != 0;
+    return ok();
+}
+
 int gs_render_alpha(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* effect,
                     const GsCullOptions* opts, int tile) {
-    GS_CHECK(gs_cull(c, cam, effect, opts));
-    GS_CHECK(gs_sort(c));
-    return gs_composite_alpha(c, tile);
+    GS_CHECK(check_camera(cam));
+    return gs_render_views(c, cam, 1, effect, opts, tile);
+}
+
+int gs_read_view_image(gs_ctx* c, uint32_t view, void* host, size_t bytes) {
+    if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
+    if (!c->have_composite) return fail(GS_ERR_STATE, "no composite yet");
+    if (view >= c->views) return fail(GS_ERR_INVALID, "view %u of %u", view, c->views);
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, true, true));
+    const ViewInfo& vi = c->vinfo[view];
+    GS_CHECK(copy_image(c, ptr<uint16_t>(c->image) + (size_t)vi.pixel_base * 4, host, vi.width, vi.height, 8, bytes,
+                        c->host_cams[view].clip_y_sign > 0.0f));
+    return ok();
+}
+
+int gs_autotune(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, const GsCullOptions* opts, int frames,
+                GsTuneResult* out) {
+    if (!c || !cams || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    if (frames < 1 || frames > 1000) return fail(GS_ERR_INVALID, "frames %d outside [1, 1000]", frames);
+    GS_CHECK(set_device(c));
+    const int save_tile = c->default_tile, save_bits = c->radix_bits, save_bpc = c->blocks_per_cu;
+    const bool save_prof = c->profiling;
+    c->profiling = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GS_HIP(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(GS_ERR_HIP, "hipEventCreate failed: %s", hipGetErrorString(he));
+    }
+    const int tiles[3] = {16, 8, 32}, bits[2] = {8, 4}, bpcs[5] = {0, 1, 2, 4, 8};
+    GsTuneResult best{};
+    best.best_ms = 1e30;
+    int r = GS_OK, configs = 0;
+    for (int ti = 0; ti < 3 && r == GS_OK; ++ti)
+        for (int bi = 0; bi < 2 && r == GS_OK; ++bi)
+            for (int pi = 0; pi < 5 && r == GS_OK; ++pi) {
+                c->radix_bits = bits[bi];
+                c->blocks_per_cu = bpcs[pi];
+                // warm-up frame (also settles the tile-entry capacity)
+                r = gs_render_views(c, cams, count, nullptr, opts, tiles[ti]);
+                if (r == GS_OK) r = settle_composite(c, true, true);
+                if (r != GS_OK) break;
+                if (hipEventRecord(e0, c->stream) != hipSuccess) { r = fail(GS_ERR_HIP, "event record failed"); break; }
+                for (int f = 0; f < frames && r == GS_OK; ++f) r = gs_render_views(c, cams, count, nullptr, opts, tiles[ti]);
+                if (r != GS_OK) break;
+                float ms = 0.0f;
+                if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    r = fail(GS_ERR_HIP, "event timing failed");
+                    break;
+                }
+                r = settle_composite(c, true, true);
+                const double per = (double)ms / frames;
+                ++configs;
+                if (ti == 0 && bi == 0 && pi == 0) best.default_ms = per;
+                if (per < best.best_ms) {
+                    best.best_ms = per;
+                    best.tile_size = tiles[ti];
+                    best.radix_bits = bits[bi];
+                    best.blocks_per_cu = bpcs[pi];
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->profiling = save_prof;
+    if (r != GS_OK) {
+        c->default_tile = save_tile;
+        c->radix_bits = save_bits;
+        c->blocks_per_cu = save_bpc;
+        return r;
+    }
+    best.configs = configs;
+    c->default_tile = best.tile_size;
+    c->radix_bits = best.radix_bits;
+    c->blocks_per_cu = best.blocks_per_cu;
+    *out = best;
+    return ok();
+}
+
+int gs_frame_info(gs_ctx* c, GsFrameInfo* out) {
+    if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    memset(out, 0, sizeof(*out));
+    out->views = c->have_cull ? c->views : 0;
+    out->total_tiles = c->have_composite ? c->total_tiles : 0;
+    out->tile_size = c->have_composite ? c->composite_tile : 0;
+    out->radix_bits = c->radix_bits;
+    out->blocks_per_cu = c->blocks_per_cu;
+    out->graph = c->use_graph;
+    out->graph_captures = c->graph_captures;
+    out->graph_replays = c->graph_replays;
+    out->overflow_reruns = c->overflow_reruns;
+    return ok();
 }
 
 int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int linear_color, float background) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
     if (!c->have_cull) return fail(GS_ERR_STATE, "gs_gps_sample needs a gs_cull first");
+    if (c->views != 1) return fail(GS_ERR_STATE, "gs_gps_sample needs a single-view cull");
     GS_CHECK(set_device(c));
     uint32_t ss = supersample < 1u ? 1u : (supersample > 4u ? 4u : supersample);
     const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
@@ -709,19 +1076,19 @@ int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
         case GS_IMAGE_ALPHA_RGBA16F:
             if (!c->have_composite) return fail(GS_ERR_STATE, "no composite yet");
             GS_CHECK(settle_composite(c, true, true));
-            GS_CHECK(copy_image(c, c->image.p, host, c->img_w, c->img_h, 8, bytes));
+            GS_CHECK(copy_image(c, c->image.p, host, c->img_w, c->img_h, 8, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_GPS_SAMPLE_RGBA32F:
             if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
-            GS_CHECK(copy_image(c, c->sample.p, host, c->res_w, c->res_h, 16, bytes));
+            GS_CHECK(copy_image(c, c->sample.p, host, c->res_w, c->res_h, 16, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_GPS_DEPTH_R32F:
             if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
-            GS_CHECK(copy_image(c, c->depth.p, host, c->res_w, c->res_h, 4, bytes));
+            GS_CHECK(copy_image(c, c->depth.p, host, c->res_w, c->res_h, 4, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_ACCUM_RGBA32F:
             if (!c->have_accum) return fail(GS_ERR_STATE, "no accumulation yet");
-            GS_CHECK(copy_image(c, c->accum.p, host, c->acc_w, c->acc_h, 16, bytes));
+            GS_CHECK(copy_image(c, c->accum.p, host, c->acc_w, c->acc_h, 16, bytes, gl_rows(c)));
             return ok();
         default:
             return fail(GS_ERR_INVALID, "unknown image %d", which);
@@ -732,6 +1099,7 @@ int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
     if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
     GS_CHECK(set_device(c));
     if (!c->have_cull) return fail(GS_ERR_STATE, "no cull yet");
+    if (c->views != 1) return fail(GS_ERR_STATE, "buffers are readable after single-view frames only");
     uint32_t V = 0;
     GS_CHECK(read_u32(c, &c->dc()->cull.visible_count, &V));
     const size_t n = c->n;

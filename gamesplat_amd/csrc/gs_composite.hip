@@ -42,7 +42,8 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
     const uint32_t q = blockIdx.x * 256u + tid;
     uint32_t cnt = 0u;
     if (q < V) {
-        const uint2 px = a.rect_px[a.sorted_ids[V - 1u - q]];
+        const uint32_t val = a.sorted_ids[V - 1u - q];
+        const uint2 px = a.rect_px[val];
         uint2 t = make_uint2(0u, 0u);
         if (px.x) {
             const uint32_t T = (uint32_t)a.tile;
@@ -65,6 +66,8 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_start[256];
     __shared__ uint32_t s_id[256];
+    __shared__ uint32_t s_tbase[256];
+    __shared__ uint32_t s_tx[256];
     __shared__ uint2 s_rect[256];
     const uint32_t tid = threadIdx.x;
     const uint32_t V = *a.count;
@@ -75,7 +78,13 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         uint2 r = a.rects[q];
         cnt = ((r.x >> 16) - (r.x & 0xFFFFu)) * ((r.y >> 16) - (r.y & 0xFFFFu));
         s_rect[tid] = r;
-        s_id[tid] = a.sorted_ids[V - 1u - q];
+        const uint32_t val = a.sorted_ids[V - 1u - q];
+        s_id[tid] = val;
+        if (cnt) {
+            const ViewInfo& vi = a.views[val / a.n];
+            s_tbase[tid] = vi.tile_base;
+            s_tx[tid] = (uint32_t)vi.tiles_x;
+        }
     }
     uint32_t total;
     s_start[tid] = block_exclusive_sum<4>(cnt, s_scan, &total);
@@ -88,7 +97,7 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         const uint2 r = s_rect[o];
         const uint32_t x0 = r.x & 0xFFFFu, w = (r.x >> 16) - x0, y0 = r.y & 0xFFFFu;
         const uint32_t k = i - s_start[o];
-        a.dup_keys[base + i] = (y0 + k / w) * (uint32_t)a.tiles_x + x0 + k % w;
+        a.dup_keys[base + i] = s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w;
         a.dup_vals[base + i] = s_id[o];
     }
 }
@@ -115,9 +124,35 @@ struct StagedSplats {
     float4* col;    // r, g, b, -
 };
 
+// The view and tile of a global tile id.
+struct TileRef {
+    int width, height, tx, ty;
+    uint16_t* image;  // the view's RGBA16F image
+    uint2 range;      // [begin, end) of the tile's entries
+};
+
+__device__ __forceinline__ TileRef tile_ref(const CompositeLaunch& a, uint32_t item) {
+    uint32_t lo = 0u, hi = a.nviews - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (a.views[mid].tile_base <= item) lo = mid; else hi = mid - 1u;
+    }
+    const ViewInfo vi = a.views[lo];
+    const uint32_t local = item - vi.tile_base;
+    TileRef t;
+    t.width = vi.width;
+    t.height = vi.height;
+    t.tx = (int)(local % (uint32_t)vi.tiles_x);
+    t.ty = (int)(local / (uint32_t)vi.tiles_x);
+    t.image = a.image + (size_t)vi.pixel_base * 4;
+    t.range = a.ranges[item];
+    return t;
+}
+
 // Stage splat g into slot t: conic from one reciprocal of the clamped
 // determinant, fragment cutoff log((1/255) / max(o, 1e-8)), pixel rectangle.
-__device__ __forceinline__ void stage_splat(const CompositeLaunch& a, uint32_t g, uint32_t t, StagedSplats s) {
+__device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const TileRef& tr, uint32_t g, uint32_t t,
+                                            StagedSplats s) {
     const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
     const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
     const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
@@ -126,7 +161,7 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, uint32_t g
     const float inv = 1.0f / det;
     const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
     const float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
-    s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, a.width, a.height,
+    s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width, tr.height,
                                  a.tight_rects != 0);
     s.geo[t] = make_float4(r0.x, r0.y, -0.5f * k0, -k1);
     s.par[t] = make_float4(-0.5f * k2, r0.z, cutoff, 0.0f);
@@ -153,15 +188,15 @@ __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, i
     tr = fmaf(-alpha, tr, tr);
 }
 
-__device__ __forceinline__ void store_pixel(const CompositeLaunch& a, int px, int py, float cr, float cg, float cb,
+__device__ __forceinline__ void store_pixel(const TileRef& t, int px, int py, float cr, float cg, float cb,
                                             float tr) {
-    size_t o = ((size_t)py * a.width + px) * 4;
+    size_t o = ((size_t)py * t.width + px) * 4;
     ushort4 h;
     h.x = __half_as_ushort(__float2half_rn(cr));
     h.y = __half_as_ushort(__float2half_rn(cg));
     h.z = __half_as_ushort(__float2half_rn(cb));
     h.w = __half_as_ushort(__float2half_rn(1.0f - tr));
-    *reinterpret_cast<ushort4*>(a.image + o) = h;
+    *reinterpret_cast<ushort4*>(t.image + o) = h;
 }
 
 template <int T>
@@ -172,23 +207,14 @@ struct TileShape {
     static constexpr int kBatch = kThreads;
 };
 
-// T x T tiles, one block per tile, PPT pixels per thread; every pixel walks
-// every staged splat of its tile.
+// T x T tile, PPT pixels per thread; every pixel walks every staged splat of
+// its tile.
 template <int T>
-__global__ void __launch_bounds__(TileShape<T>::kThreads) composite_kernel(CompositeLaunch a) {
+__device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
     using S = TileShape<T>;
     constexpr int NT = S::kThreads, PPT = S::kPerThread, BATCH = S::kBatch;
-    __shared__ int4 s_rect[BATCH];
-    __shared__ float4 s_geo[BATCH];
-    __shared__ float4 s_par[BATCH];
-    __shared__ float4 s_col[BATCH];
-    const StagedSplats st{s_rect, s_geo, s_par, s_col};
-
     const uint32_t tid = threadIdx.x;
-    const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
-    const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
-    const uint2 range = a.ranges[tile];
+    const TileRef t = tile_ref(a, item);
 
     int px[PPT], py[PPT];
     float fx[PPT], fy[PPT];
@@ -196,90 +222,135 @@ __global__ void __launch_bounds__(TileShape<T>::kThreads) composite_kernel(Compo
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         int p = (int)tid + k * NT;
-        px[k] = tx * T + p % T;
-        py[k] = ty * T + p / T;
+        px[k] = t.tx * T + p % T;
+        py[k] = t.ty * T + p / T;
         fx[k] = (float)px[k] + 0.5f;
         fy[k] = (float)py[k] + 0.5f;
         cr[k] = cg[k] = cb[k] = 0.0f;
         // pixels outside the image start saturated and are never written
-        tr[k] = px[k] < a.width && py[k] < a.height ? 1.0f : 0.0f;
+        tr[k] = px[k] < t.width && py[k] < t.height ? 1.0f : 0.0f;
     }
 
-    for (uint32_t base = range.x; base < range.y; base += BATCH) {
+    for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
         bool live = false;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) live |= tr[k] >= kTransmittanceMin;
         if (!__syncthreads_or(live)) break;
         uint32_t e = base + tid;
-        if (e < range.y) stage_splat(a, a.tile_vals[e], tid, st);
+        if (e < t.range.y) stage_splat(a, t, a.tile_vals[e], tid, st);
         __syncthreads();
-        const int cnt = (int)min((uint32_t)BATCH, range.y - base);
-        if (!__ballot(live)) continue;
-        for (int j = 0; j < cnt; ++j) {
-            const int4 r = s_rect[j];
-            const float4 geo = s_geo[j], par = s_par[j], col = s_col[j];
+        const int cnt = (int)min((uint32_t)BATCH, t.range.y - base);
+        if (__ballot(live)) {
+            for (int j = 0; j < cnt; ++j) {
+                const int4 r = st.rect[j];
+                const float4 geo = st.geo[j], par = st.par[j], col = st.col[j];
 #pragma unroll
-            for (int k = 0; k < PPT; ++k) blend_step(px[k], py[k], fx[k], fy[k], r, geo, par, col, cr[k], cg[k], cb[k], tr[k]);
+                for (int k = 0; k < PPT; ++k)
+                    blend_step(px[k], py[k], fx[k], fy[k], r, geo, par, col, cr[k], cg[k], cb[k], tr[k]);
+            }
         }
     }
+    __syncthreads();  // the staged batch is free for the next tile
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k)
-        if (px[k] < a.width && py[k] < a.height) store_pixel(a, px[k], py[k], cr[k], cg[k], cb[k], tr[k]);
+        if (px[k] < t.width && py[k] < t.height) store_pixel(t, px[k], py[k], cr[k], cg[k], cb[k], tr[k]);
 }
 
-// 16x16 tiles, one block of four wave64s, each wave owning one 8x8 quadrant.
-// Every splat of an LDS batch is tested against the quadrant's pixel range
-// with one ballot per 64 splats; the wave then walks only the overlapping
-// splats (scalar bit iteration) with one LDS wait per splat, and stops as
-// soon as all 64 of its pixels are saturated.
-__global__ void __launch_bounds__(256) composite_quad_kernel(CompositeLaunch a) {
+// 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
+// of an LDS batch is tested against the quadrant's pixel range with one
+// ballot per 64 splats; the wave then walks only the overlapping splats
+// (scalar bit iteration) with one LDS wait per splat, and stops as soon as
+// all 64 of its pixels are saturated.
+__device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
     constexpr int BATCH = 256;
-    __shared__ int4 s_rect[BATCH];
-    __shared__ float4 s_geo[BATCH];
-    __shared__ float4 s_par[BATCH];
-    __shared__ float4 s_col[BATCH];
-    const StagedSplats st{s_rect, s_geo, s_par, s_col};
-
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
-    const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
-    const uint2 range = a.ranges[tile];
+    const TileRef t = tile_ref(a, item);
 
-    const int qx0 = tx * 16 + (int)(wave & 1u) * 8, qy0 = ty * 16 + (int)(wave >> 1) * 8;
+    const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-    const bool valid = px < a.width && py < a.height;
+    const bool valid = px < t.width && py < t.height;
     // pixels outside the image start saturated and are never written
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? 1.0f : 0.0f;
 
-    for (uint32_t base = range.x; base < range.y; base += BATCH) {
+    for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
         if (!__syncthreads_or(tr >= kTransmittanceMin)) break;
         uint32_t e = base + tid;
-        if (e < range.y) stage_splat(a, a.tile_vals[e], tid, st);
+        if (e < t.range.y) stage_splat(a, t, a.tile_vals[e], tid, st);
         __syncthreads();
-        const uint32_t cnt = min((uint32_t)BATCH, range.y - base);
+        const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
         if (!__ballot(tr >= kTransmittanceMin)) continue;
         for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
             // which of these 64 splats touch this wave's quadrant
             uint32_t j = chunk + lane;
             bool hit = false;
             if (j < cnt) {
-                int4 r = s_rect[j];
+                int4 r = st.rect[j];
                 hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
             }
             uint64_t mask = __ballot(hit);
             while (mask) {
                 const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
-                blend_step(px, py, fx, fy, s_rect[k], s_geo[k], s_par[k], s_col[k], cr, cg, cb, tr);
+                blend_step(px, py, fx, fy, st.rect[k], st.geo[k], st.par[k], st.col[k], cr, cg, cb, tr);
             }
             if (!__ballot(tr >= kTransmittanceMin)) break;
         }
     }
+    __syncthreads();  // the staged batch is free for the next tile
 
-    if (valid) store_pixel(a, px, py, cr, cg, cb, tr);
+    if (valid) store_pixel(t, px, py, cr, cg, cb, tr);
+}
+
+// Next tile of a persistent block: blocks are dealt to the 8 XCDs
+// round-robin, each XCD drains its own contiguous band of tiles (its L2 sees
+// neighbouring tiles), then takes tiles from the other bands.  A plain
+// (agent-coherent) read skips exhausted bands without an atomic.
+__device__ __forceinline__ uint32_t next_tile(const CompositeLaunch& a, uint32_t xcd) {
+    const uint32_t total = a.total_tiles;
+    for (uint32_t k = 0; k < 8u; ++k) {
+        const uint32_t band = (xcd + k) & 7u;
+        const uint32_t lo = (uint32_t)(((uint64_t)total * band) / 8u);
+        const uint32_t hi = (uint32_t)(((uint64_t)total * (band + 1u)) / 8u);
+        if (hi == lo) continue;
+        if (__hip_atomic_load(&a.queue[band], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi - lo) continue;
+        const uint32_t c = atomicAdd(&a.queue[band], 1u);
+        if (c < hi - lo) return lo + c;
+    }
+    return total;
+}
+
+template <int T, bool QUAD>
+struct CompositeKernelShape {
+    static constexpr int kThreads = QUAD ? 256 : TileShape<T>::kThreads;
+};
+
+// One block per tile (persistent_blocks == 0, XCD-aware tile order) or a
+// persistent grid pulling tiles from the per-XCD queues.
+template <int T, bool QUAD>
+__global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads)) composite_kernel(CompositeLaunch a) {
+    constexpr int BATCH = CompositeKernelShape<T, QUAD>::kThreads;
+    __shared__ int4 s_rect[BATCH];
+    __shared__ float4 s_geo[BATCH];
+    __shared__ float4 s_par[BATCH];
+    __shared__ float4 s_col[BATCH];
+    __shared__ uint32_t s_item;
+    const StagedSplats st{s_rect, s_geo, s_par, s_col};
+    if (a.persistent_blocks == 0u) {
+        const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
+        if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
+        return;
+    }
+    const uint32_t xcd = blockIdx.x & 7u;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = next_tile(a, xcd);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= a.total_tiles) break;
+        if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
+    }
 }
 
 }  // namespace
@@ -305,20 +376,20 @@ hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint3
 }
 
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s) {
-    uint32_t ntiles = (uint32_t)(a.tiles_x * a.tiles_y);
-    if (!ntiles) return hipSuccess;
+    if (!a.total_tiles || !a.nviews) return hipSuccess;
+    const uint32_t grid = a.persistent_blocks ? a.persistent_blocks : a.total_tiles;
     switch (a.tile) {
         case 8:
-            hipLaunchKernelGGL(composite_kernel<8>, dim3(ntiles), dim3(TileShape<8>::kThreads), 0, s, a);
+            hipLaunchKernelGGL((composite_kernel<8, false>), dim3(grid), dim3(TileShape<8>::kThreads), 0, s, a);
             break;
         case 16:
             if (a.quadrant_cull)
-                hipLaunchKernelGGL(composite_quad_kernel, dim3(ntiles), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((composite_kernel<16, true>), dim3(grid), dim3(256), 0, s, a);
             else
-                hipLaunchKernelGGL(composite_kernel<16>, dim3(ntiles), dim3(TileShape<16>::kThreads), 0, s, a);
+                hipLaunchKernelGGL((composite_kernel<16, false>), dim3(grid), dim3(TileShape<16>::kThreads), 0, s, a);
             break;
         case 32:
-            hipLaunchKernelGGL(composite_kernel<32>, dim3(ntiles), dim3(TileShape<32>::kThreads), 0, s, a);
+            hipLaunchKernelGGL((composite_kernel<32, false>), dim3(grid), dim3(TileShape<32>::kThreads), 0, s, a);
             break;
         default:
             return hipErrorInvalidValue;

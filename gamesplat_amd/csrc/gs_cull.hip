@@ -1,12 +1,12 @@
 // gs_cull.hip - per-Gaussian cull, EWA projection and SH colour.
 //
 // Replaces cull_reset_cs and cull_cs (shaders/cull.glsl:166-474).  One
-// thread per Gaussian, 256-thread blocks.  The cull writes the projected
-// splat, a dense depth key (0xFFFFFFFF when culled, as the reset leaves it)
-// and a per-block visible count; after a scan of those counts
-// compact_kernel moves the visible (key, id) pairs to one list in index
-// order, the order a stable sort over the reference's sentinel-padded
-// arrays keeps.
+// thread per Gaussian, 256-thread blocks, up to 64 views per launch.  For
+// each view the cull writes the projected splat, a dense depth key
+// (0xFFFFFFFF when culled, as the reset leaves it) and a per-block visible
+// count; after a scan of those counts compact_kernel moves the visible
+// (key, id) pairs to one list in index order per view, the order a stable
+// sort over the reference's sentinel-padded arrays keeps.
 //
 // Float policy: the library is built with -ffp-contract=off and every
 // expression below keeps the operand order of the GLSL source, so each
@@ -57,30 +57,35 @@ constexpr float kShC3_4 = -0.4570457994644658f;
 constexpr float kShC3_5 = 1.445305721320277f;
 constexpr float kShC3_6 = -0.5900435899266435f;
 
+// One thread per Gaussian, projected into every view of the launch.  The
+// Gaussian's planes are read once: P0 (and P1, P2 under the effect) for the
+// visibility tests of all views, then P1-P3 and the SH planes the deepest
+// visible view needs.  The 3D covariance does not depend on the view and is
+// formed once; projection, SH colour, record, pixel rectangle and dense key
+// are per view, with the operations of the single-view cull.
 __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
-    __shared__ uint32_t s_scan[4];
+    __shared__ uint32_t s_cnt[kMaxViews][4];
     __shared__ uint32_t s_diag[8];
 
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 8) s_diag[tid] = 0u;
     __syncthreads();
     const uint32_t block = blockIdx.x;
     const uint32_t idx = block * 256u + tid;
     const uint32_t n = a.n;
-    const GsCameraUniforms& cam = a.cam;
-    const float* V = cam.view;
-    const float* P = cam.proj;
+    const uint32_t views = a.views;
 
-    bool visible = false;
-    uint32_t key = 0u;
+    uint64_t vis = 0ull;
+    float position[3] = {0.0f, 0.0f, 0.0f};
+    float opacity = 0.0f;
+    float effect_wave = 0.0f;
+    float4 p1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p2 = p1;
+    float scale[3] = {0.0f, 0.0f, 0.0f};
+    bool loaded = false;
     if (idx < n) {
         float4 p0 = a.planes[idx];
-        float position[3] = {p0.x, p0.y, p0.z};
-        float opacity = p0.w;
-        float effect_wave = 0.0f;
-        float4 p1, p2;
-        float scale[3];
-        bool loaded = false;
+        position[0] = p0.x; position[1] = p0.y; position[2] = p0.z;
+        opacity = p0.w;
 
         if (a.effect_active) {
             // shockwave (cull.glsl:183-207)
@@ -121,52 +126,98 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             opacity = opacity * (1.0f - effect_wave * 0.20f);
         }
 
-        // p_view (cull.glsl:209-210)
-        float pv[3];
+        // visibility in each view (cull.glsl:209-220)
+        for (uint32_t v = 0; v < views; ++v) {
+            const GsCameraUniforms& cam = a.cams[v];
+            const float* V = cam.view;
+            const float* P = cam.proj;
+            float pv[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-            pv[r] = ((V[0 * 4 + r] * position[0] + V[1 * 4 + r] * position[1]) + V[2 * 4 + r] * position[2]) + V[3 * 4 + r] * 1.0f;
-
-        // visibility (cull.glsl:212-220)
-        visible = pv[2] <= -0.2f;
-        if (visible) {
-            float inv_z = -1.0f / pv[2];
-            float o = cam.orthographic;
-            float ndc_x = (P[0] * pv[0]) * inv_z * (1.0f - o) + P[0] * pv[0] * o;
-            float ndc_y = (P[5] * pv[1]) * inv_z * (1.0f - o) + P[5] * pv[1] * o;
-            visible = __builtin_fabsf(ndc_x) <= 1.3f && __builtin_fabsf(ndc_y) <= 1.3f;
-        }
-
-        if (visible) {
-            key = 0xFFFFFFFFu - fbits(maxf(-pv[2], 0.0f));
-            if (!loaded) {
-                p1 = a.planes[(size_t)1 * n + idx];
-                p2 = a.planes[(size_t)2 * n + idx];
-                scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
+            for (int r = 0; r < 3; ++r)
+                pv[r] = ((V[0 * 4 + r] * position[0] + V[1 * 4 + r] * position[1]) + V[2 * 4 + r] * position[2]) + V[3 * 4 + r] * 1.0f;
+            bool visible = pv[2] <= -0.2f;
+            if (visible) {
+                float inv_z = -1.0f / pv[2];
+                float o = cam.orthographic;
+                float ndc_x = (P[0] * pv[0]) * inv_z * (1.0f - o) + P[0] * pv[0] * o;
+                float ndc_y = (P[5] * pv[1]) * inv_z * (1.0f - o) + P[5] * pv[1] * o;
+                visible = __builtin_fabsf(ndc_x) <= 1.3f && __builtin_fabsf(ndc_y) <= 1.3f;
             }
-            float4 p3 = a.planes[(size_t)3 * n + idx];
-            float qw = p1.x, qx = p1.y, qy = p1.z, qz = p1.w;
-            float dc[3] = {p2.w, p3.x, p3.y};
+            if (visible) vis |= 1ull << v;
+        }
+    }
 
-            // 3D covariance (cull.glsl:234-254)
-            M3 R;
-            R.m[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
-            R.m[0][1] = 2.0f * (qx * qy + qw * qz);
-            R.m[0][2] = 2.0f * (qx * qz - qw * qy);
-            R.m[1][0] = 2.0f * (qx * qy - qw * qz);
-            R.m[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
-            R.m[1][2] = 2.0f * (qy * qz + qw * qx);
-            R.m[2][0] = 2.0f * (qx * qz + qw * qy);
-            R.m[2][1] = 2.0f * (qy * qz - qw * qx);
-            R.m[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
-            M3 S;
+    // view-independent part: the remaining planes and the 3D covariance
+    // (cull.glsl:234-254); SH planes only up to the degree the deepest
+    // visible view evaluates (2 planes for degree 1, 6 for 2, 11 for 3)
+    float dc[3] = {0.0f, 0.0f, 0.0f};
+    M3 cov3d;
+    float sh[46];
+    if (vis) {
+        if (!loaded) {
+            p1 = a.planes[(size_t)1 * n + idx];
+            p2 = a.planes[(size_t)2 * n + idx];
+            scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
+        }
+        float4 p3 = a.planes[(size_t)3 * n + idx];
+        float qw = p1.x, qx = p1.y, qy = p1.z, qz = p1.w;
+        dc[0] = p2.w; dc[1] = p3.x; dc[2] = p3.y;
+        M3 R;
+        R.m[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+        R.m[0][1] = 2.0f * (qx * qy + qw * qz);
+        R.m[0][2] = 2.0f * (qx * qz - qw * qy);
+        R.m[1][0] = 2.0f * (qx * qy - qw * qz);
+        R.m[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+        R.m[1][2] = 2.0f * (qy * qz + qw * qx);
+        R.m[2][0] = 2.0f * (qx * qz + qw * qy);
+        R.m[2][1] = 2.0f * (qy * qz - qw * qx);
+        R.m[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+        M3 S;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) S.m[c][r] = 0.0f;
-            S.m[0][0] = scale[0]; S.m[1][1] = scale[1]; S.m[2][2] = scale[2];
-            M3 M = mul3(R, S);
-            M3 cov3d = mul3(M, transpose3(M));
+            for (int r = 0; r < 3; ++r) S.m[c][r] = 0.0f;
+        S.m[0][0] = scale[0]; S.m[1][1] = scale[1]; S.m[2][2] = scale[2];
+        M3 M = mul3(R, S);
+        cov3d = mul3(M, transpose3(M));
+
+        int need = 0;
+        for (uint32_t v = 0; v < views; ++v) {
+            if (!((vis >> v) & 1ull)) continue;
+            int eff = a.sh_degree;
+            if (a.sh_lod_distance > 0.0f) {
+                const float* V = a.cams[v].view;
+                float z = ((V[2] * position[0] + V[6] * position[1]) + V[10] * position[2]) + V[14] * 1.0f;
+                if ((-z) > a.sh_lod_distance) eff = 0;
+            }
+            need = eff > need ? eff : need;
+        }
+        const int planes_needed = need <= 0 ? 0 : (need == 1 ? 2 : (need == 2 ? 6 : 11));
+        sh[0] = p3.z; sh[1] = p3.w;
+#pragma unroll
+        for (int j = 0; j < 11; ++j) {
+            float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (j < planes_needed) t = a.planes[(size_t)(4 + j) * n + idx];
+            sh[2 + 4 * j] = t.x;
+            sh[3 + 4 * j] = t.y;
+            sh[4 + 4 * j] = t.z;
+            sh[5 + 4 * j] = t.w;
+        }
+    }
+
+    for (uint32_t v = 0; v < views; ++v) {
+        const bool visible = (vis >> v) & 1ull;
+        uint32_t key = 0xFFFFFFFFu;
+        const size_t slot = (size_t)v * n + idx;
+        if (visible) {
+            const GsCameraUniforms& cam = a.cams[v];
+            const float* V = cam.view;
+            const float* P = cam.proj;
+            float pv[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                pv[r] = ((V[0 * 4 + r] * position[0] + V[1 * 4 + r] * position[1]) + V[2 * 4 + r] * position[2]) + V[3 * 4 + r] * 1.0f;
+            key = 0xFFFFFFFFu - fbits(maxf(-pv[2], 0.0f));
 
             // EWA projection (cull.glsl:256-277)
             float z = pv[2], x = pv[0], y = pv[1];
@@ -204,7 +255,7 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             float radius_x = __builtin_ceilf(3.0f * __builtin_sqrtf(ca));
             float radius_y = __builtin_ceilf(3.0f * __builtin_sqrtf(cc));
 
-            // diagnostics (cull.glsl:296-318)
+            // diagnostics (cull.glsl:296-318), single-view launches only
             if (a.collect_stats) {
                 float quad_area = maxf(0.0f, 4.0f * radius_x * radius_y);
                 uint32_t qpx = f2u_sat(minf(quad_area, 4294967295.0f));
@@ -233,21 +284,9 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             float dinv = 1.0f / __builtin_sqrtf((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]);
             float dx = dv[0] * dinv, dy = dv[1] * dinv, dz = dv[2] * dinv;
 
-            // SH (cull.glsl:348-416); only the planes the degree needs are read
+            // SH (cull.glsl:348-416)
             int eff = a.sh_degree;
             if (a.sh_lod_distance > 0.0f && (-z) > a.sh_lod_distance) eff = 0;
-            const int planes_needed = eff <= 0 ? 0 : (eff == 1 ? 2 : (eff == 2 ? 6 : 11));
-            float sh[46];
-            sh[0] = p3.z; sh[1] = p3.w;
-#pragma unroll
-            for (int j = 0; j < 11; ++j) {
-                float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (j < planes_needed) t = a.planes[(size_t)(4 + j) * n + idx];
-                sh[2 + 4 * j] = t.x;
-                sh[3 + 4 * j] = t.y;
-                sh[4 + 4 * j] = t.z;
-                sh[5 + 4 * j] = t.w;
-            }
             float color[3];
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
@@ -283,35 +322,38 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
                 color[ch] = col;
             }
 
-            float4* rec = a.records + (size_t)idx * kRecordVec4;
+            float4* rec = a.records + slot * kRecordVec4;
             rec[0] = make_float4(cx, rcy, opacity, ndc_z);
             rec[1] = make_float4(ca, cb * cam.clip_y_sign, cc, radius_x);
             rec[2] = make_float4(color[0], color[1], color[2], radius_y);
-            a.rect_px[idx] = pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
-                                        (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0);
+            a.rect_px[slot] = pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
+                                         (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0);
         }
+        // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
+        // block's visible count; compaction runs after the counts are scanned
+        if (idx < n) a.dense_keys[slot] = key;
+        const uint64_t bal = __ballot(visible);
+        if (lane == 0) s_cnt[v][wave] = (uint32_t)__popcll(bal);
     }
-
-    // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
-    // block's visible count; compaction runs after the block counts are scanned
-    const uint32_t lane = tid & 63u, wave = tid >> 6;
-    if (idx < n) a.dense_keys[idx] = visible ? key : 0xFFFFFFFFu;
-    uint64_t bal = __ballot(visible);
-    if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
     __syncthreads();
-    if (tid == 0) a.block_counts[block] = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
+    for (uint32_t v = tid; v < views; v += 256u)
+        a.block_counts[(size_t)v * gridDim.x + block] = s_cnt[v][0] + s_cnt[v][1] + s_cnt[v][2] + s_cnt[v][3];
     if (a.collect_stats && tid < 8) a.block_diag[(size_t)block * 8 + tid] = s_diag[tid];
 }
 
-// Order-preserving compaction: block b moves the visible entries of its 256
-// Gaussians to offsets[b] + rank.
+// Order-preserving compaction: block (b, v) moves the visible entries of its
+// 256 Gaussians in view v to offsets[v * blocks + b] + rank.  The value is
+// v * n + index, which addresses the view's records and rectangles, and the
+// list holds view 0's visible Gaussians in index order, then view 1's, ...
 __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict__ dense_keys, uint32_t n,
                                                       const uint32_t* __restrict__ offsets, uint32_t* vis_keys,
                                                       uint32_t* vis_ids) {
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t v = blockIdx.y;
     const uint32_t idx = blockIdx.x * 256u + tid;
-    const uint32_t key = idx < n ? dense_keys[idx] : 0xFFFFFFFFu;
+    const size_t slot = (size_t)v * n + idx;
+    const uint32_t key = idx < n ? dense_keys[slot] : 0xFFFFFFFFu;
     const bool visible = key != 0xFFFFFFFFu;
     const uint64_t bal = __ballot(visible);
     if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
@@ -321,9 +363,10 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     for (int w = 0; w < 4; ++w)
         if ((uint32_t)w < wave) before += s_scan[w];
     if (visible) {
-        const uint32_t pos = offsets[blockIdx.x] + before + (uint32_t)__popcll(bal & lanemask_lt());
+        const uint32_t pos = offsets[(size_t)v * gridDim.x + blockIdx.x] + before +
+                             (uint32_t)__popcll(bal & lanemask_lt());
         vis_keys[pos] = key;
-        vis_ids[pos] = idx;
+        vis_ids[pos] = (uint32_t)slot;
     }
 }
 
@@ -369,17 +412,18 @@ __global__ void __launch_bounds__(kUploadBlock) transpose_upload_kernel(
 }  // namespace
 
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
+    if (a.n == 0 || a.views == 0) return hipSuccess;
+    if (a.views > (uint32_t)kMaxViews) return hipErrorInvalidValue;
     uint32_t blocks = (a.n + 255u) / 256u;
     hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
-                          uint32_t* vis_ids, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(compact_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, dense_keys, n, offsets, vis_keys,
-                       vis_ids);
+hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
+                          uint32_t* vis_keys, uint32_t* vis_ids, hipStream_t s) {
+    if (n == 0 || views == 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_kernel, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n, offsets,
+                       vis_keys, vis_ids);
     return hipGetLastError();
 }
 

@@ -30,6 +30,19 @@ constexpr int kPlanes = 15;
 // reference's operations wherever they are needed.
 constexpr int kRecordVec4 = 3;
 
+// Views per batched launch (one bit each in the cull's visibility mask).
+constexpr int kMaxViews = 64;
+
+// One view of a batched frame.  Per-view arrays (records, pixel rectangles,
+// dense keys) are laid out [view][n], so the compacted value view * n + id
+// addresses them directly; tile ids are global (tile_base + local tile) and
+// the RGBA16F images are packed one after the other (pixel_base).
+struct ViewInfo {
+    int width, height, tiles_x, tiles_y;
+    uint32_t tile_base;
+    uint32_t pixel_base;
+};
+
 // Device counters.  Each stage clears its own group with one fill launch
 // before it runs, so a stage can be re-run without re-running earlier ones.
 // Totals are written by the scan kernels, never accumulated with atomics.
@@ -56,6 +69,7 @@ struct DevCounters {
     BinCounters bin;
     SortCounters tile_sort;
     GpsCounters gps;
+    uint32_t composite_queue[8];  // persistent composite: tiles taken per XCD band
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -127,22 +141,23 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* off
 struct CullLaunch {
     const float4* planes;
     uint32_t n;
-    GsCameraUniforms cam;
+    const GsCameraUniforms* cams;   // device, one per view
+    uint32_t views;                 // 1 .. kMaxViews
     GsSplatEffectParams effect;
     int effect_active;
     int sh_degree;
     float sh_lod_distance;
     int collect_stats;
-    float4* records;
-    uint32_t* dense_keys;    // n: depth key or 0xFFFFFFFF when culled
-    uint32_t* block_counts;  // visible Gaussians per 256-block
-    uint32_t* block_diag;    // 8 diagnostics per block (collect_stats)
-    uint2* rect_px;          // n: covered pixel rectangle of each visible splat
+    float4* records;         // [view][n] x 3 float4
+    uint32_t* dense_keys;    // [view][n]: depth key or 0xFFFFFFFF when culled
+    uint32_t* block_counts;  // [view][block]: visible Gaussians per 256-block
+    uint32_t* block_diag;    // 8 diagnostics per block (collect_stats, one view)
+    uint2* rect_px;          // [view][n]: covered pixel rectangle of each visible splat
     int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
-hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, const uint32_t* offsets, uint32_t* vis_keys,
-                          uint32_t* vis_ids, hipStream_t s);
+hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
+                          uint32_t* vis_keys, uint32_t* vis_ids, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
@@ -166,11 +181,12 @@ uint32_t sort_table_words(uint32_t capacity);
 
 // gs_composite.hip
 struct BinLaunch {
-    const float4* records;
-    const uint32_t* sorted_ids;
-    const uint32_t* count;   // visible count (device)
-    uint32_t capacity;       // Gaussian capacity (grid bound)
-    int width, height, tile, tiles_x, tiles_y;
+    const uint32_t* sorted_ids;  // view * n + id, depth order
+    const uint32_t* count;   // visible count over all views (device)
+    uint32_t capacity;       // views * n (grid bound)
+    uint32_t n;              // per-view stride of the [view][n] arrays
+    int tile;
+    const ViewInfo* views;   // device
     const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
     uint2* rects;            // per front-to-back position: tile rectangle
     uint32_t* block_counts;  // pairs per 256-block
@@ -187,13 +203,20 @@ hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint3
                               hipStream_t s);
 
 struct CompositeLaunch {
-    const float4* records;
+    const float4* records;   // [view][n] x 3 float4
     const uint32_t* tile_vals;
-    const uint2* ranges;
-    int width, height, tile, tiles_x, tiles_y;
-    uint16_t* image;         // RGBA16F, framebuffer row order
+    const uint2* ranges;     // per global tile id
+    int tile;
+    const ViewInfo* views;   // device
+    uint32_t nviews;
+    uint32_t total_tiles;
+    uint16_t* image;         // RGBA16F per view, framebuffer row order, packed by pixel_base
     int quadrant_cull;       // 16x16 tiles: per-wave 8x8 quadrant culling kernel
     int tight_rects;         // per-splat pixel rectangle cut to the opacity-aware ellipse box
+    // 0: one block per tile.  >0: that many persistent blocks pull tiles from
+    // 8 per-XCD queues (queue: 8 counters, cleared before the launch).
+    uint32_t persistent_blocks;
+    uint32_t* queue;
 };
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s);
 

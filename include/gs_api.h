@@ -194,10 +194,57 @@ int gs_gps_sample(gs_ctx* ctx, uint32_t frame_seed, uint32_t supersample,
  * n = sample_count (1 = first sample, history ignored). */
 int gs_accumulate(gs_ctx* ctx, uint32_t sample_count);
 
-/* Whole sorted-alpha frame: gs_cull + gs_sort + gs_composite_alpha. */
+/* Whole sorted-alpha frame: gs_cull + gs_sort + gs_composite_alpha
+ * (renderer_draw_frame's alpha branch, src/renderer.cpp:2139-2289). */
 int gs_render_alpha(gs_ctx* ctx, const GsCameraUniforms* cam,
                     const GsSplatEffectParams* effect, const GsCullOptions* opts,
                     int tile_size);
+
+/* Batched multi-view sorted-alpha frame; no counterpart in the reference,
+ * which draws one camera per frame.  `count` cameras (1..64, any mix of
+ * viewport sizes) over the resident scene: one cull launch reads each
+ * Gaussian once for all views, one depth sort orders every view's visible
+ * list (a stable sort restricted to one view keeps that view's order), one
+ * tile sort and one composite launch cover every (view, tile).  Each view's
+ * image equals what gs_render_alpha gives for that camera.  collect_stats
+ * needs count == 1.  Read the images with gs_read_view_image. */
+int gs_render_views(gs_ctx* ctx, const GsCameraUniforms* cams, uint32_t count,
+                    const GsSplatEffectParams* effect, const GsCullOptions* opts,
+                    int tile_size);
+
+/* RGBA16F image of view `view` of the last gs_render_views (or
+ * gs_render_alpha: view 0), top row first; bytes = width * height * 8. */
+int gs_read_view_image(gs_ctx* ctx, uint32_t view, void* host, size_t bytes);
+
+/* Configuration of the last frame (no counterpart in the reference). */
+typedef struct GsFrameInfo {
+    uint32_t views;
+    uint32_t total_tiles;
+    int32_t tile_size;
+    int32_t radix_bits;
+    int32_t blocks_per_cu;
+    int32_t graph;
+    uint32_t graph_captures;   /* hipGraph captures so far   */
+    uint32_t graph_replays;    /* replays of a captured frame */
+    uint32_t overflow_reruns;  /* frames re-binned after a tile-entry overflow */
+} GsFrameInfo;
+int gs_frame_info(gs_ctx* ctx, GsFrameInfo* out);
+
+/* Autotuner (no counterpart in the reference): renders the given views with
+ * every tile size {8, 16, 32} x radix digit width {4, 8} x composite blocks
+ * per CU {0 (one block per tile), 1, 2, 4, 8}, `frames` timed frames each
+ * after one warm-up frame, keeps the fastest configuration as the context's
+ * options and reports it next to the default (16, 8, 0). */
+typedef struct GsTuneResult {
+    int32_t tile_size;
+    int32_t radix_bits;
+    int32_t blocks_per_cu;
+    int32_t configs;           /* configurations timed */
+    double best_ms;            /* per frame, fastest configuration */
+    double default_ms;         /* per frame, tile 16 / 8-bit digits / one block per tile */
+} GsTuneResult;
+int gs_autotune(gs_ctx* ctx, const GsCameraUniforms* cams, uint32_t count,
+                const GsCullOptions* opts, int frames, GsTuneResult* out);
 
 /* Options (gs_set_option). */
 enum {
@@ -215,7 +262,18 @@ enum {
      * margin that keeps every passing pixel inside; images are identical to
      * 0 = the full quad rectangle, with fewer (tile, splat) pairs.  Takes
      * effect from the next gs_cull. */
-    GS_OPT_TIGHT_RECTS = 3
+    GS_OPT_TIGHT_RECTS = 3,
+    /* Radix digit width of the depth and tile sorts: 8 (default) or 4. */
+    GS_OPT_RADIX_BITS = 4,
+    /* Composite launch: 0 (default) = one block per tile; k > 0 = k
+     * persistent blocks per CU pulling tiles from per-XCD queues. */
+    GS_OPT_BLOCKS_PER_CU = 5,
+    /* 1: gs_render_alpha / gs_render_views capture their frame once into a
+     * hipGraph and replay it while every launch argument is unchanged
+     * (cameras and view sizes live in device memory, so moving cameras
+     * replay; new buffer sizes, options or effect parameters re-capture).
+     * Ignored while profiling is on.  0 (default): direct launches. */
+    GS_OPT_GRAPH = 6
 };
 int gs_set_option(gs_ctx* ctx, int option, int value);
 
