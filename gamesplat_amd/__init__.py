@@ -7,6 +7,7 @@ benchmark.
 """
 from ._lib import (  # noqa: F401
     Context,
+    Denoiser,
     GsCameraUniforms,
     GsError,
     GsSplatEffectParams,

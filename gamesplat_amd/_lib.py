@@ -144,6 +144,19 @@ EXPORTS = {
     "gs_frame_info": (C.c_int, [C.c_void_p, C.POINTER(GsFrameInfo)]),
     "gs_autotune": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.c_uint32, C.POINTER(GsCullOptions),
                               C.c_int, C.POINTER(GsTuneResult)]),
+    "gs_denoiser_param_count": (C.c_uint32, []),
+    "gs_denoiser_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gs_denoiser_destroy": (None, [C.c_void_p]),
+    "gs_denoiser_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "gs_denoiser_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "gs_denoiser_forward_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                               C.c_uint32, C.POINTER(C.c_float), C.c_void_p, C.c_void_p]),
+    "gs_denoiser_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "gs_denoiser_capture": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "gs_denoiser_pool_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "gs_denoiser_train": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_void_p]),
+    "gs_denoise": (C.c_int, [C.c_void_p]),
+    "gs_denoiser_read_image": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "gs_read_image": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "gs_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "gs_stats": (C.c_int, [C.c_void_p, C.POINTER(GsStats)]),
@@ -371,3 +384,80 @@ class Context:
 
     def stream(self) -> int:
         return self._lib.gs_stream(self._h) or 0
+
+
+class Denoiser:
+    """The learned denoiser of a render context (gs_denoiser_*)."""
+
+    def __init__(self, ctx: "Context", seed: int = 0):
+        self._lib = ctx._lib
+        self._ctx = ctx
+        h = C.c_void_p()
+        check(self._lib.gs_denoiser_create(ctx._h, seed, C.byref(h)))
+        self._h = h
+        self.P = int(self._lib.gs_denoiser_param_count())
+
+    def close(self):
+        if self._h:
+            self._lib.gs_denoiser_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.P, np.float32)
+        check(self._lib.gs_denoiser_get_params(self._h, out.ctypes.data_as(C.c_void_p), self.P))
+        return out
+
+    def set_params(self, flat: np.ndarray):
+        f = np.ascontiguousarray(flat, np.float32)
+        assert f.shape == (self.P,)
+        check(self._lib.gs_denoiser_set_params(self._h, f.ctypes.data_as(C.c_void_p), self.P))
+
+    def forward_backward(self, inp: np.ndarray, target: np.ndarray, want_out: bool = True, want_grad: bool = True):
+        x = np.ascontiguousarray(inp, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        n, h, w, c = x.shape
+        assert c == 8 and t.shape == (n, h, w, 4)
+        loss = C.c_float()
+        out = np.empty((n, h, w, 4), np.float32) if want_out else None
+        grad = np.empty(self.P, np.float32) if want_grad else None
+        check(self._lib.gs_denoiser_forward_backward(
+            self._h, x.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p), n, h, w, C.byref(loss),
+            out.ctypes.data_as(C.c_void_p) if out is not None else None,
+            grad.ctypes.data_as(C.c_void_p) if grad is not None else None))
+        return loss.value, out, grad
+
+    def adam_step(self, lr: float):
+        check(self._lib.gs_denoiser_adam_step(self._h, lr))
+
+    def capture(self, crops: int, seed: int):
+        check(self._lib.gs_denoiser_capture(self._h, crops, seed))
+
+    def pool_size(self) -> int:
+        n = C.c_uint32()
+        check(self._lib.gs_denoiser_pool_size(self._h, C.byref(n)))
+        return n.value
+
+    def train(self, steps: int, batch: int, lr: float, seed: int) -> np.ndarray:
+        losses = np.empty(steps, np.float32)
+        check(self._lib.gs_denoiser_train(self._h, steps, batch, lr, seed, losses.ctypes.data_as(C.c_void_p)))
+        return losses
+
+    def denoise(self):
+        check(self._lib.gs_denoise(self._h))
+
+    def read_image(self, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), np.float16)
+        check(self._lib.gs_denoiser_read_image(self._h, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out

@@ -1236,3 +1236,36 @@ int gs_reset_stage_times(gs_ctx* c) {
 void* gs_stream(gs_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 }  // extern "C"
+
+namespace gs {
+
+int api_error(int code, const char* msg) { return fail(code, "%s", msg); }
+int api_ok() { return ok(); }
+
+int ctx_frames(gs_ctx* c, CtxFrames* out) {
+    if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, true, true));
+    memset(out, 0, sizeof(*out));
+    out->stream = c->stream;
+    out->device = c->device;
+    if (c->have_accum && c->acc_w == c->res_w && c->acc_h == c->res_h) {
+        out->accum = ptr<float4>(c->accum);
+        out->depth = ptr<float>(c->depth);
+        out->gps_w = c->acc_w;
+        out->gps_h = c->acc_h;
+    }
+    if (c->have_composite && c->views == 1) {
+        out->alpha = ptr<uint16_t>(c->image);
+        out->alpha_w = c->img_w;
+        out->alpha_h = c->img_h;
+    }
+    out->clip_y_sign = c->cam.clip_y_sign;
+    return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" {
+
+}  // extern "C"

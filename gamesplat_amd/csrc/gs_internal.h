@@ -253,4 +253,85 @@ hipError_t launch_gps_splat(const GpsLaunch& a, hipStream_t s);
 hipError_t launch_gps_resolve(const GpsLaunch& a, hipStream_t s);
 hipError_t launch_accumulate(const float4* sample, float4* accum, uint32_t count, float n, hipStream_t s);
 
+// gs_unet.hip: the denoiser's kernels.  Tensors are NHWC bf16 (raw uint16
+// bits) viewed through a channel stride and a channel offset.
+struct TensorView {
+    uint16_t* p;
+    int cs;   // channel stride of the buffer
+    int co;   // first channel of the view
+};
+struct ConvLaunch {
+    const uint16_t* x;
+    int xs, xo;              // input channel stride / offset
+    uint16_t* y;
+    int ys, yo;              // output channel stride / offset
+    const uint16_t* wt;      // bf16 [cout][kpad]
+    const float* bias;       // [cout] or NULL
+    int n, h, w, cin, cout, kpad, relu;
+};
+hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s);  // nt = cout / 16
+struct WGradLaunch {
+    const uint16_t* x;
+    int xs, xo;
+    const uint16_t* dz;
+    int dzs, dzo;
+    float* partial;          // [chunks][9][cout][cin]
+    float* bias_partial;     // [chunks][cout]
+    int n, h, w, cin, cout;
+    int chunk_pixels;        // multiple of 32
+};
+hipError_t launch_conv3x3_wgrad(const WGradLaunch& a, int chunks, float* gw, float* gb, hipStream_t s);
+hipError_t launch_relu_mask(TensorView dy, TensorView y, long pixels, int c, hipStream_t s);
+hipError_t launch_pool2(TensorView in, TensorView out, int n, int h, int w, int c, hipStream_t s);
+hipError_t launch_pool2_bwd(TensorView dout, TensorView din, int n, int h, int w, int c, hipStream_t s);
+hipError_t launch_up2(TensorView in, TensorView out, int n, int h, int w, int c, hipStream_t s);
+hipError_t launch_up2_bwd(TensorView dout, TensorView din, int n, int h, int w, int c, hipStream_t s);
+struct HeadLaunch {
+    const uint16_t* u;       // [pixels][32]
+    const uint16_t* inp;     // [pixels][8]
+    const float* params;     // head.w (4 x 32) then head.b (4)
+    long pixels;
+    float* out;              // [pixels][4] fp32 or NULL
+    uint16_t* image;         // [pixels][4] RGBA16F or NULL
+    const float* target;     // [pixels][4] or NULL (inference)
+    float inv_count;         // 1 / (pixels * 4)
+    uint16_t* du;            // [pixels][32] bf16 gradient
+    float* partial;          // head_partial_floats(pixels)
+};
+hipError_t launch_head(const HeadLaunch& a, float* loss, float* gw, hipStream_t s);
+uint32_t head_partial_floats(long pixels);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint32_t count, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_pack_weights(const float* w, int cin, int cout, int kpad, int kpad_t, uint16_t* wf, uint16_t* wb,
+                               hipStream_t s);
+struct PackLaunch {
+    const float4* accum;     // width x height running mean
+    const float* depth;      // width x height GPS depth
+    const uint16_t* alpha;   // width x height RGBA16F sorted-alpha target (or NULL)
+    int width, height;
+    int x0, y0, hp, wp;      // region written (pixels beyond the frame are zero)
+    uint16_t* inp;           // [hp][wp][8]
+    float* target;           // [hp][wp][4] or NULL
+};
+hipError_t launch_pack_input(const PackLaunch& a, hipStream_t s);
+hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int width, int height, hipStream_t s);
+hipError_t launch_gather(const uint16_t* pool_in, const float* pool_tgt, const uint32_t* idx, uint32_t pixels,
+                         uint32_t batch, uint16_t* inp, float* tgt, hipStream_t s);
+
+// gs_api.cpp: what the denoiser reads from a render context.
+struct CtxFrames {
+    hipStream_t stream;
+    int device;
+    const float4* accum;     // running mean of GPS samples (NULL before gs_accumulate)
+    const float* depth;      // last GPS depth
+    int gps_w, gps_h;        // accumulation size (0 if none)
+    const uint16_t* alpha;   // last sorted-alpha RGBA16F (NULL before a composite)
+    int alpha_w, alpha_h;
+    float clip_y_sign;       // +1: rows are bottom-first (GL)
+};
+int ctx_frames(gs_ctx* c, CtxFrames* out);
+// Error reporting shared with the C ABI (gs_last_error).
+int api_error(int code, const char* msg);
+int api_ok();
+
 }  // namespace gs

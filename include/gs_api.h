@@ -246,6 +246,45 @@ typedef struct GsTuneResult {
 int gs_autotune(gs_ctx* ctx, const GsCameraUniforms* cams, uint32_t count,
                 const GsCullOptions* opts, int frames, GsTuneResult* out);
 
+/* ---- learned denoiser (no counterpart in the reference) ------------------
+ * A 4-level U-Net (32/48/64/96 channels, 3x3 convolutions + ReLU, 2x2
+ * average pooling, nearest upsampling, skip concatenation, 1x1 head added to
+ * the input mean) that maps the GPS running mean (RGBA) and GPS depth of a
+ * render context to its sorted-alpha image.  bf16 weights and activations on
+ * the matrix cores with fp32 accumulation; fp32 parameters and Adam state.
+ * Every call is ordered on the render context's stream.
+ * Parameter layout (float32): per 3x3 convolution in the order e0a e0b e1a
+ * e1b e2a e2b ba bb d2a d2b d1a d1b d0a d0b, W[cout][tap][cin] (tap =
+ * (dy + 1) * 3 + dx + 1) then b[cout]; then the head W[4][32], b[4]. */
+typedef struct gs_denoiser gs_denoiser;
+uint32_t gs_denoiser_param_count(void);
+/* He-normal initial weights from `seed`. */
+int gs_denoiser_create(gs_ctx* ctx, uint64_t seed, gs_denoiser** out);
+void gs_denoiser_destroy(gs_denoiser* d);
+int gs_denoiser_get_params(gs_denoiser* d, float* host, size_t count);
+/* Also clears the Adam state. */
+int gs_denoiser_set_params(gs_denoiser* d, const float* host, size_t count);
+/* One forward + backward pass over a host batch: input (n, h, w, 8) float
+ * (channels: mean RGBA, depth, 0, 0, 0; rounded to bf16), target (n, h, w,
+ * 4); h, w multiples of 8.  Returns the mean squared error, optionally the
+ * output (n, h, w, 4) and the gradient of every parameter (count floats). */
+int gs_denoiser_forward_backward(gs_denoiser* d, const float* input, const float* target, uint32_t n,
+                                 uint32_t h, uint32_t w, float* loss, float* output, float* grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on the last gradients. */
+int gs_denoiser_adam_step(gs_denoiser* d, float lr);
+/* Adds `crops` random 256x256 crops of the context's current frame (GPS
+ * running mean + depth as input, sorted-alpha RGBA16F as target, same camera
+ * and size) to the training pool (1024 crops, oldest replaced first). */
+int gs_denoiser_capture(gs_denoiser* d, uint32_t crops, uint64_t seed);
+int gs_denoiser_pool_size(gs_denoiser* d, uint32_t* count);
+/* `steps` Adam steps on random batches of pooled crops; losses (optional,
+ * steps floats) receive each step's mean squared error. */
+int gs_denoiser_train(gs_denoiser* d, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses);
+/* Denoises the context's current GPS accumulation into an RGBA16F image. */
+int gs_denoise(gs_denoiser* d);
+/* The last denoised image, top row first; bytes = width * height * 8. */
+int gs_denoiser_read_image(gs_denoiser* d, void* host, size_t bytes);
+
 /* Options (gs_set_option). */
 enum {
     /* 1: the GPS resolve leaves the depth|payload cells in place so they can
