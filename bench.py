@@ -55,6 +55,8 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stochastic", action="store_true")
     ap.add_argument("--no-multiview", action="store_true")
+    ap.add_argument("--no-denoiser", action="store_true")
+    ap.add_argument("--converged-frames", type=int, default=256)
     ap.add_argument("--multiview-steps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -93,6 +95,86 @@ def cpu_baseline(g, cam, width, height, threads):
     orc.composite(proj, si, width, height, mode=2, threads=threads)
     t3 = time.perf_counter()
     return t3 - t0, {"cull_s": t1 - t0, "sort_s": t2 - t1, "composite_s": t3 - t2}
+
+
+WEIGHTS = ROOT / "gamesplat_amd" / "weights" / "unet_denoiser.f32"
+
+
+def _psnr(a, b) -> float:
+    import numpy as np
+
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0.0 else round(10.0 * np.log10(1.0 / mse), 3)
+
+
+def denoised_stochastic(ctx, cam, steps: int, warmup: int, converged: int) -> dict:
+    """Stochastic frames with the learned denoiser: cull + GPS (1 spp) +
+    running mean + U-Net, timed like the sorted-alpha frame; PSNR of the 1
+    and 4 spp means, their denoised images and the converged mean against
+    the sorted-alpha image of the same camera."""
+    import numpy as np
+
+    import gamesplat_amd as gs
+    from gamesplat_amd import abi
+
+    ctx.render_alpha(cam)
+    alpha = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+    W, H = int(cam.viewport[0]), int(cam.viewport[1])
+    out = {}
+    with gs.Denoiser(ctx, seed=1) as den:
+        trained = WEIGHTS.exists()
+        if trained:
+            den.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
+        out["weights"] = str(WEIGHTS.relative_to(ROOT)) if trained else "untrained (He init)"
+
+        def frame(f):
+            ctx.cull(cam)
+            ctx.gps_sample(f, 1)
+            ctx.accumulate(1)
+            den.denoise()
+
+        for f in range(1, warmup + 1):
+            frame(f)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, steps + 1):
+            frame(f)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        # the denoiser alone
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            den.denoise()
+        ctx.synchronize()
+        den_ms = (time.perf_counter() - t0) * 1000.0 / steps
+        out.update({"frames_per_s": round(1000.0 / ms, 2), "ms_per_frame": round(ms, 4),
+                    "denoise_ms": round(den_ms, 4)})
+        # with the 16 Mi budget of the workload, then with a budget no frame
+        # exceeds (no dropped work: the estimator's own error)
+        budget = ctx.stats()["gps_max_work_items"]
+        requested = ctx.stats()["gps_requested_work_items"]
+        for tag, b in (("", budget), ("no_drop_", max(budget, 1 << 27))):
+            ctx.set_gps_budget(b)
+            for spp in (1, 4):
+                ctx.cull(cam)
+                for f in range(1, spp + 1):
+                    ctx.gps_sample(7000 + f, 1)
+                    ctx.accumulate(f)
+                acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
+                den.denoise()
+                d = den.read_image(W, H)
+                out[f"{tag}psnr_{spp}spp_mean"] = _psnr(acc, alpha)
+                out[f"{tag}psnr_{spp}spp_denoised"] = _psnr(d, alpha)
+            ctx.cull(cam)
+            for f in range(1, converged + 1):
+                ctx.gps_sample(20000 + f, 1)
+                ctx.accumulate(f)
+            out[f"{tag}psnr_converged_{converged}_frames"] = _psnr(ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F), alpha)
+        ctx.set_gps_budget(budget)
+        out["requested_work_items"] = requested
+        out["budget"] = budget
+    return out
 
 
 def multiview(steps: int, warmup: int) -> dict:
@@ -220,6 +302,9 @@ def main():
             ctx.accumulate(f)
         sstages = ctx.stage_times()
         ctx.set_profiling(False)
+        if not args.no_denoiser:
+            den = denoised_stochastic(ctx, cam, args.steps, args.warmup, args.converged_frames)
+            print(f"# denoised: {json.dumps(den)}", file=sys.stderr, flush=True)
         stoch = {
             "frames_per_s": round(1000.0 / sms, 2),
             "ms_per_frame": round(sms, 4),
@@ -227,6 +312,7 @@ def main():
             "dropped_work_items": gst["gps_dropped_work_items"],
             "budget": gst["gps_max_work_items"],
             "stage_ms": {k: round(v[0] / v[1], 4) for k, v in sstages.items() if v[1]},
+            "denoised": None if args.no_denoiser else den,
         }
 
     # roofline of the dominant kernel stage (algorithmic bytes per launch)

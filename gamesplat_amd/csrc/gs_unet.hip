@@ -38,64 +38,88 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 
 __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 u) { return __builtin_bit_cast(bf16x8_t, u); }
 
-template <int NT>
-__global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
-    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
-    const int y = (int)blockIdx.y * 4 + wave;
-    const int x0 = (int)blockIdx.x * 32;
-    const int n = (int)blockIdx.z;
-    const int K = 9 * a.cin;
-    const int row = lane & 15, kg = lane >> 4;
+// Block tile: 32 pixels wide, 4 * RG / 2 rows (RG 16-pixel row groups per
+// wave).  The input tile with a one-pixel halo is staged in LDS once (channel
+// pitch cin + 8 spreads the 16-byte reads of consecutive pixels over the
+// banks); all 9 taps then read their A operands from LDS, and the weights
+// (B operands) come through the L1/L2 caches.
+constexpr int kConvTileW = 32;
 
-    f32x4_t acc[2][NT];
+template <int NT, int RG>
+__global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
+    extern __shared__ __align__(16) uint16_t s_in[];
+    constexpr int RPW = RG / 2;            // rows per wave
+    constexpr int BR = 4 * RPW;            // rows per block
+    constexpr int TW = kConvTileW + 2, TR = BR + 2;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int y0 = (int)blockIdx.y * BR;
+    const int x0 = (int)blockIdx.x * kConvTileW;
+    const int n = (int)blockIdx.z;
+    const int cin = a.cin, cp = cin + 8, groups = cin / 8;
+    const size_t img = (size_t)n * a.h;
+
+    // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the image)
+    for (int i = tid; i < TR * TW * groups; i += 256) {
+        const int g = i % groups, pix = i / groups;
+        const int tx = pix % TW, ty = pix / TW;
+        const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+            v = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
+        *reinterpret_cast<uint4*>(s_in + (size_t)pix * cp + g * 8) = v;
+    }
+    __syncthreads();
+
+    const int row = lane & 15, kg = lane >> 4;
+    f32x4_t acc[RG][NT];
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < RG; ++g)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[g][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
-    const size_t img = (size_t)n * a.h;
+    // this lane's k = kc + 8 kg as (tap, channel), advanced by 32 per chunk
+    const int K = 9 * cin;
+    int tap = 0, ci = kg * 8;
+    while (ci >= cin) { ci -= cin; ++tap; }
     for (int kc = 0; kc < a.kpad; kc += 32) {
         const int kk = kc + kg * 8;
-        uint4 av[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
-        if (kk < K && y < a.h) {
-            const int tap = kk / a.cin, ci = kk - tap * a.cin;
-            const int yy = y + tap / 3 - 1, dx = tap % 3 - 1;
-            if (yy >= 0 && yy < a.h) {
-#pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    const int xx = x0 + g * 16 + row + dx;
-                    if (xx >= 0 && xx < a.w)
-                        av[g] = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + ci);
-                }
-            }
-        }
         uint4 bv[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
             bv[t] = *reinterpret_cast<const uint4*>(a.wt + (size_t)(t * 16 + row) * a.kpad + kk);
+        const bool live = kk < K;
+        const int dy = tap / 3, dx = tap - dy * 3;  // 0..2: the halo offsets
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < RG; ++g) {
+            const int r = wave * RPW + g / 2, c = (g & 1) * 16 + row;
+            uint4 av = make_uint4(0u, 0u, 0u, 0u);
+            if (live) av = *reinterpret_cast<const uint4*>(s_in + (size_t)((r + dy) * TW + c + dx) * cp + ci);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-                acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av[g]), as_bf16x8(bv[t]), acc[g][t],
-                                                                     0, 0, 0);
+                acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av), as_bf16x8(bv[t]), acc[g][t], 0,
+                                                                     0, 0);
+        }
+        ci += 32;
+        while (ci >= cin && tap < 9) { ci -= cin; ++tap; }
     }
 
-    if (y >= a.h) return;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + row;
         const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < RG; ++g) {
+            const int y = y0 + wave * RPW + g / 2;
+            if (y >= a.h) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int xx = x0 + g * 16 + kg * 4 + i;
+                const int xx = x0 + (g & 1) * 16 + kg * 4 + i;
                 if (xx >= a.w) continue;
                 float v = acc[g][t][i] + b;
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
                 a.y[((img + y) * a.w + xx) * a.ys + a.yo + co] = f2bf(v);
             }
+        }
     }
 }
 
@@ -482,19 +506,35 @@ inline uint32_t blocks_for(long n, int t = 256) { return (uint32_t)((n + t - 1) 
 
 }  // namespace
 
+template <int NT, int RG>
+hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
+    constexpr int BR = 4 * (RG / 2);
+    const size_t lds = (size_t)(BR + 2) * (kConvTileW + 2) * (a.cin + 8) * 2;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv3x3_kernel<NT, RG>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    dim3 grid((a.w + kConvTileW - 1) / kConvTileW, (a.h + BR - 1) / BR, a.n);
+    hipLaunchKernelGGL((conv3x3_kernel<NT, RG>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s) {
-    dim3 grid((a.w + 31) / 32, (a.h + 3) / 4, a.n);
+    if (a.cin % 8 || a.cin > 160) return hipErrorInvalidValue;
     switch (nt) {
-        case 2: hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(conv3x3_kernel<3>, grid, dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(conv3x3_kernel<4>, grid, dim3(256), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(conv3x3_kernel<5>, grid, dim3(256), 0, s, a); break;
-        case 6: hipLaunchKernelGGL(conv3x3_kernel<6>, grid, dim3(256), 0, s, a); break;
-        case 7: hipLaunchKernelGGL(conv3x3_kernel<7>, grid, dim3(256), 0, s, a); break;
-        case 10: hipLaunchKernelGGL(conv3x3_kernel<10>, grid, dim3(256), 0, s, a); break;
+        case 2: return conv3x3_launch<2, 4>(a, s);
+        case 3: return conv3x3_launch<3, 4>(a, s);
+        case 4: return conv3x3_launch<4, 4>(a, s);
+        case 5: return conv3x3_launch<5, 2>(a, s);
+        case 6: return conv3x3_launch<6, 2>(a, s);
+        case 7: return conv3x3_launch<7, 2>(a, s);
+        case 10: return conv3x3_launch<10, 2>(a, s);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_conv3x3_wgrad(const WGradLaunch& a, int chunks, float* gw, float* gb, hipStream_t s) {
