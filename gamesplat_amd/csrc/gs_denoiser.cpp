@@ -202,10 +202,24 @@ int pack_all(gs_denoiser* d) {
     return GS_OK;
 }
 
+// Conv li with bias and ReLU; optionally its output 2x2-pooled into `pool`
+// and its first `split` input channels upsampled from the half-size x2.
+struct ConvExtra {
+    uint16_t* pool = nullptr;
+    int ps = 0;
+    const uint16_t* x2 = nullptr;
+    int x2s = 0, split = 0;
+};
+
 int conv(gs_denoiser* d, int li, const uint16_t* x, int xs, int xo, uint16_t* y, int ys, int yo, int n, int h,
-         int w) {
+         int w, const ConvExtra& ex = ConvExtra()) {
     const Layer& L = d->layers[li];
     ConvLaunch a{};
+    a.pool = ex.pool;
+    a.ps = ex.ps;
+    a.x2 = ex.x2;
+    a.x2s = ex.x2s;
+    a.split = ex.split;
     a.x = x; a.xs = xs; a.xo = xo;
     a.y = y; a.ys = ys; a.yo = yo;
     a.wt = L.wf;
@@ -254,25 +268,41 @@ int forward(gs_denoiser* d, bool with_target) {
     const int n = d->an, h0 = d->ah, w0 = d->aw;
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8;
     hipStream_t s = d->s;
+    // the encoder's second convolutions also write the pooled tensor; for
+    // inference the decoder's first convolutions read the upsampled half of
+    // their concatenation straight from the half-size tensor (training keeps
+    // the concatenation whole: the weight gradient reads it)
+    auto pooled = [](uint16_t* p, int ps) {
+        ConvExtra e;
+        e.pool = p;
+        e.ps = ps;
+        return e;
+    };
+    auto upsampled = [&](const uint16_t* x2, int c) {
+        ConvExtra e;
+        if (!with_target) {
+            e.x2 = x2;
+            e.x2s = c;
+            e.split = c;
+        }
+        return e;
+    };
     DN_CHECK(conv(d, E0A, d->inp, 8, 0, d->t0, 32, 0, n, h0, w0));
-    DN_CHECK(conv(d, E0B, d->t0, 32, 0, d->cat0, 80, 48, n, h0, w0));
-    DN_HIP(launch_pool2(tv(d->cat0, 80, 48), tv(d->p1, 32, 0), n, h0, w0, 32, s));
+    DN_CHECK(conv(d, E0B, d->t0, 32, 0, d->cat0, 80, 48, n, h0, w0, pooled(d->p1, 32)));
     DN_CHECK(conv(d, E1A, d->p1, 32, 0, d->t1, 48, 0, n, h1, w1));
-    DN_CHECK(conv(d, E1B, d->t1, 48, 0, d->cat1, 112, 64, n, h1, w1));
-    DN_HIP(launch_pool2(tv(d->cat1, 112, 64), tv(d->p2, 48, 0), n, h1, w1, 48, s));
+    DN_CHECK(conv(d, E1B, d->t1, 48, 0, d->cat1, 112, 64, n, h1, w1, pooled(d->p2, 48)));
     DN_CHECK(conv(d, E2A, d->p2, 48, 0, d->t2, 64, 0, n, h2, w2));
-    DN_CHECK(conv(d, E2B, d->t2, 64, 0, d->cat2, 160, 96, n, h2, w2));
-    DN_HIP(launch_pool2(tv(d->cat2, 160, 96), tv(d->p3, 64, 0), n, h2, w2, 64, s));
+    DN_CHECK(conv(d, E2B, d->t2, 64, 0, d->cat2, 160, 96, n, h2, w2, pooled(d->p3, 64)));
     DN_CHECK(conv(d, BA, d->p3, 64, 0, d->t3, 96, 0, n, h3, w3));
     DN_CHECK(conv(d, BB, d->t3, 96, 0, d->u3, 96, 0, n, h3, w3));
-    DN_HIP(launch_up2(tv(d->u3, 96, 0), tv(d->cat2, 160, 0), n, h3, w3, 96, s));
-    DN_CHECK(conv(d, D2A, d->cat2, 160, 0, d->t2d, 64, 0, n, h2, w2));
+    if (with_target) DN_HIP(launch_up2(tv(d->u3, 96, 0), tv(d->cat2, 160, 0), n, h3, w3, 96, s));
+    DN_CHECK(conv(d, D2A, d->cat2, 160, 0, d->t2d, 64, 0, n, h2, w2, upsampled(d->u3, 96)));
     DN_CHECK(conv(d, D2B, d->t2d, 64, 0, d->u2, 64, 0, n, h2, w2));
-    DN_HIP(launch_up2(tv(d->u2, 64, 0), tv(d->cat1, 112, 0), n, h2, w2, 64, s));
-    DN_CHECK(conv(d, D1A, d->cat1, 112, 0, d->t1d, 48, 0, n, h1, w1));
+    if (with_target) DN_HIP(launch_up2(tv(d->u2, 64, 0), tv(d->cat1, 112, 0), n, h2, w2, 64, s));
+    DN_CHECK(conv(d, D1A, d->cat1, 112, 0, d->t1d, 48, 0, n, h1, w1, upsampled(d->u2, 64)));
     DN_CHECK(conv(d, D1B, d->t1d, 48, 0, d->u1, 48, 0, n, h1, w1));
-    DN_HIP(launch_up2(tv(d->u1, 48, 0), tv(d->cat0, 80, 0), n, h1, w1, 48, s));
-    DN_CHECK(conv(d, D0A, d->cat0, 80, 0, d->t0d, 32, 0, n, h0, w0));
+    if (with_target) DN_HIP(launch_up2(tv(d->u1, 48, 0), tv(d->cat0, 80, 0), n, h1, w1, 48, s));
+    DN_CHECK(conv(d, D0A, d->cat0, 80, 0, d->t0d, 32, 0, n, h0, w0, upsampled(d->u1, 48)));
     DN_CHECK(conv(d, D0B, d->t0d, 32, 0, d->u0, 32, 0, n, h0, w0));
     HeadLaunch hl{};
     hl.u = d->u0;

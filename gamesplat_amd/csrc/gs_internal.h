@@ -268,6 +268,13 @@ struct ConvLaunch {
     const uint16_t* wt;      // bf16 [cout][kpad]
     const float* bias;       // [cout] or NULL
     int n, h, w, cin, cout, kpad, relu;
+    // optional: input channels [0, split) are the nearest 2x upsample of x2
+    // (n, h/2, w/2); channels [split, cin) come from x at the same index
+    const uint16_t* x2;
+    int x2s, x2o, split;
+    // optional: 2x2 average pool of the (bf16) output into pool (n, h/2, w/2)
+    uint16_t* pool;
+    int ps, po;
 };
 hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s);  // nt = cout / 16
 struct WGradLaunch {

@@ -58,14 +58,21 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     const int cin = a.cin, cp = cin + 8, groups = cin / 8;
     const size_t img = (size_t)n * a.h;
 
-    // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the image)
+    // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the image);
+    // with x2, channels below split are read from the half-size tensor
+    const size_t img2 = (size_t)n * (a.h / 2);
     for (int i = tid; i < TR * TW * groups; i += 256) {
         const int g = i % groups, pix = i / groups;
         const int tx = pix % TW, ty = pix / TW;
         const int yy = y0 + ty - 1, xx = x0 + tx - 1;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
-            v = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
+        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+            if (a.x2 && g * 8 < a.split)
+                v = *reinterpret_cast<const uint4*>(a.x2 + ((img2 + yy / 2) * (a.w / 2) + xx / 2) * a.x2s + a.x2o +
+                                                    g * 8);
+            else
+                v = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
+        }
         *reinterpret_cast<uint4*>(s_in + (size_t)pix * cp + g * 8) = v;
     }
     __syncthreads();
@@ -81,12 +88,21 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     const int K = 9 * cin;
     int tap = 0, ci = kg * 8;
     while (ci >= cin) { ci -= cin; ++tap; }
+    // weights of the next chunk are loaded while this chunk's MFMAs run
+    const uint16_t* wrow = a.wt + (size_t)row * a.kpad + kg * 8;
+    uint4 bnext[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bnext[t] = *reinterpret_cast<const uint4*>(wrow + (size_t)t * 16 * a.kpad);
     for (int kc = 0; kc < a.kpad; kc += 32) {
         const int kk = kc + kg * 8;
         uint4 bv[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-            bv[t] = *reinterpret_cast<const uint4*>(a.wt + (size_t)(t * 16 + row) * a.kpad + kk);
+        for (int t = 0; t < NT; ++t) bv[t] = bnext[t];
+        if (kc + 32 < a.kpad) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                bnext[t] = *reinterpret_cast<const uint4*>(wrow + (size_t)t * 16 * a.kpad + kc + 32);
+        }
         const bool live = kk < K;
         const int dy = tap / 3, dx = tap - dy * 3;  // 0..2: the halo offsets
 #pragma unroll
@@ -107,18 +123,37 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + row;
         const float b = a.bias ? a.bias[co] : 0.0f;
+        uint16_t q[RG][4];
 #pragma unroll
         for (int g = 0; g < RG; ++g) {
             const int y = y0 + wave * RPW + g / 2;
-            if (y >= a.h) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int xx = x0 + (g & 1) * 16 + kg * 4 + i;
-                if (xx >= a.w) continue;
                 float v = acc[g][t][i] + b;
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
-                a.y[((img + y) * a.w + xx) * a.ys + a.yo + co] = f2bf(v);
+                q[g][i] = f2bf(v);
+                if (y < a.h && xx < a.w) a.y[((img + y) * a.w + xx) * a.ys + a.yo + co] = q[g][i];
             }
+        }
+        // 2x2 average pool: with two rows per wave a lane holds whole 2x2
+        // blocks (rows of groups g and g + 2, pixels i and i + 1); summed in
+        // the order of pool2_kernel, so the result is the same bits
+        if (RG == 4 && a.pool) {
+            const int yp = (y0 + wave * RPW) / 2;
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int xx = x0 + g * 16 + kg * 4 + 2 * j;
+                    if (2 * yp + 1 >= a.h || xx + 1 >= a.w) continue;
+                    float sum = 0.0f;
+                    sum += bf2f(q[g][2 * j]);
+                    sum += bf2f(q[g][2 * j + 1]);
+                    sum += bf2f(q[g + 2][2 * j]);
+                    sum += bf2f(q[g + 2][2 * j + 1]);
+                    a.pool[(((size_t)n * (a.h / 2) + yp) * (a.w / 2) + xx / 2) * a.ps + a.po + co] = f2bf(sum * 0.25f);
+                }
         }
     }
 }
@@ -509,6 +544,8 @@ inline uint32_t blocks_for(long n, int t = 256) { return (uint32_t)((n + t - 1) 
 template <int NT, int RG>
 hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
     constexpr int BR = 4 * (RG / 2);
+    if (a.pool && (RG != 4 || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
+    if (a.x2 && (a.split % 8 || a.split > a.cin || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
     const size_t lds = (size_t)(BR + 2) * (kConvTileW + 2) * (a.cin + 8) * 2;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
