@@ -160,7 +160,10 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
     if (det < 1e-6f) det = 1e-6f;
     const float inv = 1.0f / det;
     const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
-    const float cutoff = logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
+    // hardware log: the composite's fragment test is held to a tolerance (its
+    // quadratic form is FMA-evaluated), and the pixel rectangle's margin
+    // covers the difference
+    const float cutoff = __logf((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
     s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width, tr.height,
                                  a.tight_rects != 0);
     s.geo[t] = make_float4(r0.x, r0.y, -0.5f * k0, -k1);

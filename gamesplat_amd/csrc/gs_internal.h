@@ -52,7 +52,7 @@ struct CullCounters {
     uint32_t pad[7];
 };
 struct SortCounters {
-    uint32_t scratch[512];   // per pass: digit column totals, digit bases
+    uint32_t scratch[512];   // per pass: digit column totals
 };
 struct BinCounters {
     uint32_t dup_total;   // (tile, splat) pairs requested

@@ -8,8 +8,8 @@
 //   colscan    one block per digit: exclusive scan of that digit's column over
 //              the tiles, plus the column total (the reference's single
 //              workgroup loops over every group twice instead);
-//   digitbase  exclusive scan of the column totals;
-//   downsweep  stable ranking inside the tile with wave64 ballots (peers =
+//   downsweep  digit bases from the column totals (every block scans them),
+//              stable ranking inside the tile with wave64 ballots (peers =
 //              lanes whose digit matches on every bit), reorder by digit in
 //              LDS, then contiguous runs of equal digits are written out.
 // No kernel waits on another block or takes a global ticket (device-scope
@@ -35,7 +35,6 @@ struct PassArgs {
     uint32_t* table;        // [RADIX][tiles]: tile digit counts, then column offsets
     uint32_t tiles;         // table stride (grid size)
     uint32_t* col_total;    // [RADIX]
-    uint32_t* digit_base;   // [RADIX]
     int shift;
 };
 
@@ -103,16 +102,6 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
         carry += chunk;
     }
     if (tid == 0) a.col_total[d] = carry;
-}
-
-template <int BITS>
-__global__ void __launch_bounds__(kThreads) digitbase_kernel(PassArgs a) {
-    constexpr int RADIX = 1 << BITS;
-    __shared__ uint32_t s_scan[kWaves];
-    const uint32_t tid = threadIdx.x;
-    uint32_t total;
-    uint32_t base = block_exclusive_sum<kWaves>(tid < (uint32_t)RADIX ? a.col_total[tid] : 0u, s_scan, &total);
-    if (tid < (uint32_t)RADIX) a.digit_base[tid] = base;
 }
 
 template <int BITS>
@@ -188,9 +177,13 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     }
     uint32_t total;
     uint32_t local = block_exclusive_sum<kWaves>(bin_total, s_scan, &total);
+    // digit bases: every block scans the column totals itself (no separate
+    // single-block launch between colscan and downsweep)
+    uint32_t all;
+    const uint32_t dbase = block_exclusive_sum<kWaves>(tid < (uint32_t)RADIX ? a.col_total[tid] : 0u, s_scan, &all);
     if (tid < (uint32_t)RADIX) {
         s_local[tid] = local;
-        s_dest[tid] = a.digit_base[tid] + a.table[(size_t)tid * a.tiles + tile] - local;
+        s_dest[tid] = dbase + a.table[(size_t)tid * a.tiles + tile] - local;
     }
     __syncthreads();
 
@@ -230,11 +223,9 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.table = a.table;
         pa.tiles = tiles;
         pa.col_total = a.scratch;
-        pa.digit_base = a.scratch + 256;
         pa.shift = p * BITS;
         hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
-        hipLaunchKernelGGL(digitbase_kernel<BITS>, dim3(1), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(downsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
