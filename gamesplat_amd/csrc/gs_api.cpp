@@ -434,7 +434,8 @@ int gs_set_option(gs_ctx* c, int option, int value) {
             c->default_tile = value;
             return ok();
         case GS_OPT_QUADRANT_CULL:
-            c->quadrant_cull = value ? 1 : 0;
+            if (value < 0 || value > 2) return fail(GS_ERR_INVALID, "quadrant cull must be 0, 1 or 2");
+            c->quadrant_cull = value;
             return ok();
         case GS_OPT_TIGHT_RECTS:
             c->tight_rects = value ? 1 : 0;

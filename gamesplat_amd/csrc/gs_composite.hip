@@ -116,13 +116,16 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
     if (i == count - 1 || keys[i + 1] != k) ranges[k].y = i + 1;
 }
 
-// One splat as the compositors keep it in LDS (4 x 16 bytes).
+// One splat as the compositors keep it in LDS (4 x 16 bytes).  The
+// quadratic form and the cutoff are in log2 units (scaled by log2 e), so the
+// blend step's alpha is one v_exp_f32.
 struct StagedSplats {
     int4* rect;     // covered pixels [i0, i1) x [j0, j1)
-    float4* geo;    // cx, cy, -k0/2, -k1
-    float4* par;    // -k2/2, opacity, cutoff, -
+    float4* geo;    // cx, cy, -k0/2 log2e, -k1 log2e
+    float4* par;    // -k2/2 log2e, opacity, log2((1/255) / o), -
     float4* col;    // r, g, b, -
 };
+constexpr float kLog2e = 1.4426950408889634f;
 
 // The view and tile of a global tile id.
 struct TileRef {
@@ -160,14 +163,14 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
     if (det < 1e-6f) det = 1e-6f;
     const float inv = 1.0f / det;
     const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
-    // hardware log: the composite's fragment test is held to a tolerance (its
-    // quadratic form is FMA-evaluated), and the pixel rectangle's margin
+    // hardware log2: the composite's fragment test is held to a tolerance
+    // (its quadratic form is FMA-evaluated), and the pixel rectangle's margin
     // covers the difference
-    const float cutoff = __logf((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
+    const float cutoff = __log2f((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
     s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width, tr.height,
                                  a.tight_rects != 0);
-    s.geo[t] = make_float4(r0.x, r0.y, -0.5f * k0, -k1);
-    s.par[t] = make_float4(-0.5f * k2, r0.z, cutoff, 0.0f);
+    s.geo[t] = make_float4(r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1);
+    s.par[t] = make_float4(-0.5f * kLog2e * k2, r0.z, cutoff, 0.0f);
     s.col[t] = make_float4(r2.x, r2.y, r2.z, 0.0f);
 }
 
@@ -175,12 +178,13 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
 // pixel rectangle), the fragment test (power in [cutoff, 0]) and the
 // saturation test select the alpha or 0, and a zero alpha leaves colour and
 // transmittance bit-for-bit unchanged.  The quadratic form is evaluated with
-// fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy.
+// fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy, in
+// log2 units.
 __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, int4 r, float4 geo, float4 par,
                                            float4 col, float& cr, float& cg, float& cb, float& tr) {
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, (par.x * dy) * dy);
-    float alpha = minf(par.y * __expf(power), 0.99f);
+    float alpha = minf(par.y * __builtin_amdgcn_exp2f(power), 0.99f);
     const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power <= 0.0f) &
                       (power >= par.z) & (tr >= kTransmittanceMin);
     alpha = take ? alpha : 0.0f;
@@ -260,6 +264,37 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
         if (px[k] < t.width && py[k] < t.height) store_pixel(t, px[k], py[k], cr[k], cg[k], cb[k], tr[k]);
 }
 
+// Whether any pixel centre of the 8x8 quadrant at (qx0, qy0) that the
+// rectangle r covers can pass the fragment test power >= cutoff.  power is a
+// concave quadratic (log2 units: A dx^2 + B dx dy + C dy^2, A, C < 0), so its
+// maximum over the box of those centres is 0 when the splat centre lies in
+// the box and otherwise sits on an edge, at the clamped vertex of the 1D
+// parabola.  The margin covers the float evaluation of power here and in the
+// blend step (about 10 eps |power| / (1 - |rho|)^2 each, with 1 - rho^2 =
+// r2 = 1 - B^2 / (4 A C)); splats with r2 < 0.01 or a clamped, indefinite
+// conic are always walked.
+__device__ __forceinline__ bool ellipse_meets_box(int4 r, float4 geo, float4 par, int qx0, int qy0) {
+    const float A = geo.z, B = geo.w, C = par.x;
+    if (!(A < 0.0f && C < 0.0f)) return true;
+    const float r2 = 1.0f - (B * B) / (4.0f * A * C);
+    if (!(r2 >= 0.01f)) return true;
+    const float xl = (float)max(qx0, r.x) + 0.5f - geo.x, xh = (float)min(qx0 + 8, r.y) - 0.5f - geo.x;
+    const float yl = (float)max(qy0, r.z) + 0.5f - geo.y, yh = (float)min(qy0 + 8, r.w) - 0.5f - geo.y;
+    if (xl <= 0.0f && xh >= 0.0f && yl <= 0.0f && yh >= 0.0f) return true;
+    const float sy = -B / (2.0f * C), sx = -B / (2.0f * A);  // argmax of the parabola along y (x)
+    auto along_y = [&](float u) {  // max over dy in [yl, yh] at dx = u
+        const float t = clampf(sy * u, yl, yh);
+        return fmaf(fmaf(C, t, B * u), t, (A * u) * u);
+    };
+    auto along_x = [&](float u) {  // max over dx in [xl, xh] at dy = u
+        const float t = clampf(sx * u, xl, xh);
+        return fmaf(fmaf(A, t, B * u), t, (C * u) * u);
+    };
+    const float best = maxf(maxf(along_y(xl), along_y(xh)), maxf(along_x(yl), along_x(yh)));
+    const float margin = __builtin_fabsf(par.z) * (0.003f + 1e-5f / (r2 * r2)) + 1e-5f;
+    return best >= par.z - margin;
+}
+
 // 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
 // of an LDS batch is tested against the quadrant's pixel range with one
 // ballot per 64 splats; the wave then walks only the overlapping splats
@@ -291,6 +326,7 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             if (j < cnt) {
                 int4 r = st.rect[j];
                 hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
+                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, st.geo[j], st.par[j], qx0, qy0);
             }
             uint64_t mask = __ballot(hit);
             while (mask) {

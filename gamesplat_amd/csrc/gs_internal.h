@@ -211,7 +211,7 @@ struct CompositeLaunch {
     uint32_t nviews;
     uint32_t total_tiles;
     uint16_t* image;         // RGBA16F per view, framebuffer row order, packed by pixel_base
-    int quadrant_cull;       // 16x16 tiles: per-wave 8x8 quadrant culling kernel
+    int quadrant_cull;       // 16x16 tiles: 8x8 quadrant kernel; 1 = rectangle + ellipse test, 2 = rectangle only
     int tight_rects;         // per-splat pixel rectangle cut to the opacity-aware ellipse box
     // 0: one block per tile.  >0: that many persistent blocks pull tiles from
     // 8 per-XCD queues (queue: 8 counters, cleared before the launch).

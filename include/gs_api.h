@@ -293,8 +293,11 @@ enum {
     GS_OPT_GPS_KEEP_CELLS = 0,
     /* Default tile size of gs_composite_alpha when 0 is passed (8, 16, 32). */
     GS_OPT_TILE_SIZE = 1,
-    /* 16x16 tiles: 1 (default) = per-wave 8x8 quadrant culling kernel,
-     * 0 = one pixel per thread over every splat of the tile. */
+    /* 16x16 tiles: 1 (default) = per-wave 8x8 quadrant kernel that walks
+     * the splats whose rectangle and alpha >= 1/255 ellipse meet the
+     * quadrant; 2 = the same kernel with the rectangle test only; 0 = one
+     * pixel per thread over every splat of the tile.  Images are identical
+     * for 1 and 2. */
     GS_OPT_QUADRANT_CULL = 2,
     /* 1 (default): each splat's pixel rectangle (binning and compositing) is
      * cut to the box of the ellipse where its fragment test can pass, with a

@@ -98,7 +98,7 @@ def test_all_culled(ctx, orc, gs):
     assert not img.any()
 
 
-@pytest.mark.parametrize("tile,quad", [(8, 1), (16, 1), (16, 0), (32, 1)])
+@pytest.mark.parametrize("tile,quad", [(8, 1), (16, 1), (16, 2), (16, 0), (32, 1)])
 def test_composite_vs_fp64(ctx, orc, c1_scene, tile, quad):
     g, cfg, cam = c1_scene
     print(f"case: composite c1 tile={tile} quadrant_cull={quad}", flush=True)
@@ -131,6 +131,27 @@ def _needle_scene(gs, seed=21, n=20000):
 
 
 @pytest.mark.parametrize("scene", ["c1", "needles"])
+def test_quadrant_ellipse_cull_identical_image(ctx, gs, c1_scene, scene):
+    """Skipping (splat, quadrant) pairs whose alpha >= 1/255 ellipse misses
+    the quadrant must not change a single pixel."""
+    g, cfg, cam = c1_scene
+    if scene == "needles":
+        g = _needle_scene(gs)
+    ctx.upload(g)
+    out = {}
+    try:
+        for quad in (2, 1):
+            ctx.set_option(abi.GS_OPT_QUADRANT_CULL, quad)
+            ctx.render_alpha(cam, tile=16)
+            out[quad] = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy()
+    finally:
+        ctx.set_option(abi.GS_OPT_QUADRANT_CULL, 1)
+    print(f"case: quadrant ellipse cull scene={scene}", flush=True)
+    np.testing.assert_array_equal(out[1], out[2])
+    assert out[1].any()
+
+
+@pytest.mark.parametrize("scene", ["c1", "needles"])
 def test_tight_rects_identical_image(ctx, gs, c1_scene, scene):
     """Cutting each splat's pixel rectangle to the box of its alpha >= 1/255
     ellipse must not change a single pixel, only the (tile, splat) pairs."""
@@ -140,7 +161,7 @@ def test_tight_rects_identical_image(ctx, gs, c1_scene, scene):
     ctx.upload(g)
     fewer = False
     try:
-        for tile, quad in [(16, 1), (16, 0), (8, 1), (32, 1)]:
+        for tile, quad in [(16, 1), (16, 2), (16, 0), (8, 1), (32, 1)]:
             print(f"case: tight rects scene={scene} tile={tile} quadrant_cull={quad}", flush=True)
             ctx.set_option(abi.GS_OPT_QUADRANT_CULL, quad)
             out = {}
