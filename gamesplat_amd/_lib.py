@@ -7,11 +7,14 @@ fail loudly.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libgamesplat_hip.so"
+# GS_LIB_PATH selects another build of the same library (e.g. to compare two
+# builds on one machine); by default the in-tree build is loaded.
+LIB_PATH = Path(os.environ.get("GS_LIB_PATH", str(Path(__file__).resolve().parent / "libgamesplat_hip.so")))
 
 GS_OK = 0
 GS_ERR_INVALID = -1
