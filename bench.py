@@ -52,6 +52,7 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--graph", action="store_true", help="replay each sorted-alpha frame from one hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stochastic", action="store_true")
     ap.add_argument("--no-multiview", action="store_true")
@@ -251,6 +252,8 @@ def main():
     cam = scenes.random_view_camera(cfg["seed"], W, H)
     ctx = gs.Context(0)
     ctx.upload(g)
+    if args.graph:
+        ctx.set_option(gs.abi.GS_OPT_GRAPH, 1)
     print(f"# {ctx.device_info()}", file=sys.stderr, flush=True)
 
     # sorted-alpha frames

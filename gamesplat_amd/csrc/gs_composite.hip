@@ -175,9 +175,11 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
 }
 
 // One front-to-back blend step of pixel (px, py), branch-free: coverage (the
-// pixel rectangle), the fragment test (power in [cutoff, 0]) and the
-// saturation test select the alpha or 0, and a zero alpha leaves colour and
-// transmittance bit-for-bit unchanged.  The quadratic form is evaluated with
+// pixel rectangle) and the fragment test (power in [cutoff, 0]) select the
+// alpha or 0, and a zero alpha leaves colour and transmittance bit-for-bit
+// unchanged.  A saturated pixel (T < 1e-4) keeps blending until its whole
+// wave is saturated; what it adds is below 1e-4 of a colour, and it only
+// brings the result closer to blending every splat.  The quadratic form is evaluated with
 // fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy, in
 // log2 units.
 __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, int4 r, float4 geo, float4 par,
@@ -185,8 +187,7 @@ __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, i
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, (par.x * dy) * dy);
     float alpha = minf(par.y * __builtin_amdgcn_exp2f(power), 0.99f);
-    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power <= 0.0f) &
-                      (power >= par.z) & (tr >= kTransmittanceMin);
+    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power <= 0.0f) & (power >= par.z);
     alpha = take ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
