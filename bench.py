@@ -83,6 +83,44 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
+def frame_times(ctx, frame, count: int):
+    """Per-frame GPU times (ms) from HIP events recorded on the context's
+    stream around each frame, frames issued back to back."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p(ctx._lib.gs_stream(ctx.handle))
+    evs = [C.c_void_p() for _ in range(count + 1)]
+    for e in evs:
+        assert hip.hipEventCreate(C.byref(e)) == 0
+    for k in range(count):
+        assert hip.hipEventRecord(evs[k], stream) == 0
+        frame(k)
+    assert hip.hipEventRecord(evs[count], stream) == 0
+    assert hip.hipEventSynchronize(evs[count]) == 0
+    out = []
+    for k in range(count):
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), evs[k], evs[k + 1]) == 0
+        out.append(ms.value)
+    for e in evs:
+        hip.hipEventDestroy(e)
+    return out
+
+
+def unet_flops(width: int, height: int) -> float:
+    """Multiply-add FLOPs of one denoiser pass at width x height (3x3
+    convolutions at their levels plus the 1x1 head)."""
+    # (input channels, output channels, level) of the 14 3x3 convolutions
+    CONVS_LEVELS = [(8, 32, 0), (32, 32, 0), (32, 48, 1), (48, 48, 1), (48, 64, 2), (64, 64, 2), (64, 96, 3),
+                    (96, 96, 3), (160, 64, 2), (64, 64, 2), (112, 48, 1), (48, 48, 1), (80, 32, 0), (32, 32, 0)]
+    total = 0.0
+    for cin, cout, level in CONVS_LEVELS:
+        px = (width >> level) * (height >> level)
+        total += 2.0 * 9 * cin * cout * px
+    return total + 2.0 * 32 * 4 * width * height
+
+
 def cpu_baseline(g, cam, width, height, threads):
     """The oracle's restatement of the reference pipeline on host cores: one
     full frame (cull + SH-3, LSD radix sort, fp16-per-blend composite)."""
@@ -149,8 +187,11 @@ def denoised_stochastic(ctx, cam, steps: int, warmup: int, converged: int) -> di
             den.denoise()
         ctx.synchronize()
         den_ms = (time.perf_counter() - t0) * 1000.0 / steps
+        fl = unet_flops(W + (-W) % 8, H + (-H) % 8)
         out.update({"frames_per_s": round(1000.0 / ms, 2), "ms_per_frame": round(ms, 4),
-                    "denoise_ms": round(den_ms, 4)})
+                    "denoise_ms": round(den_ms, 4), "denoise_gflop": round(fl / 1e9, 2),
+                    "denoise_mfma_tflops": round(fl / (den_ms * 1e-3) / 1e12, 2),
+                    "denoise_mfma_frac_of_bf16_peak": round(fl / (den_ms * 1e-3) / 2.5e15, 4)})
         # with the 16 Mi budget of the workload, then with a budget no frame
         # exceeds (no dropped work: the estimator's own error)
         budget = ctx.stats()["gps_max_work_items"]
@@ -269,6 +310,10 @@ def main():
     st = ctx.stats()
     V, D = st["visible_count"], st["tile_entries"]
 
+    # per-frame GPU times from events on the context stream
+    ft = sorted(frame_times(ctx, lambda k: ctx.render_alpha(cam, tile=args.tile), max(100, args.steps)))
+    frame_p50, frame_p95 = ft[len(ft) // 2], ft[int(len(ft) * 0.95)]
+
     # per-stage HIP-event times (separate pass: events add launch gaps)
     ctx.set_profiling(True)
     ctx.reset_stage_times()
@@ -315,6 +360,10 @@ def main():
             "dropped_work_items": gst["gps_dropped_work_items"],
             "budget": gst["gps_max_work_items"],
             "stage_ms": {k: round(v[0] / v[1], 4) for k, v in sstages.items() if v[1]},
+            # one 64-bit atomicMin per work item kept (ss = 1)
+            "gps_splat_atomics_per_s": round(min(gst["gps_requested_work_items"], gst["gps_max_work_items"]) /
+                                             (sstages["gps_splat"][0] / sstages["gps_splat"][1] * 1e-3), 1)
+            if sstages["gps_splat"][1] else None,
             "denoised": None if args.no_denoiser else den,
         }
 
@@ -369,6 +418,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "frame_ms_p50": round(frame_p50, 4),
+        "frame_ms_p95": round(frame_p95, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -378,6 +429,7 @@ def main():
                    "gaussians": N, "width": W, "height": H, "visible": V, "tile_entries": D,
                    "seed": cfg["seed"], "parallelism": "single GPU"},
         "gaussians_per_s": round(N * 1000.0 / ms, 1),
+        "tile_entries_per_s": round(D / (stage_ms["composite"] * 1e-3), 1) if stage_ms["composite"] else None,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
         "stochastic": stoch,
         "roofline": roofline,
