@@ -107,13 +107,21 @@ __global__ void clamp_count_kernel(const uint32_t* total, uint32_t capacity, uin
     if (threadIdx.x == 0) *clamped = min(*total, capacity);
 }
 
+// [begin, end) of every tile's entries in the tile-sorted keys: each run
+// start writes its own begin and the previous run's end; one key load per
+// entry, the previous key comes from the neighbouring lane.
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint32_t* count_ptr, uint2* ranges) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *count_ptr;
+    const uint32_t k = i < count ? keys[i] : 0xFFFFFFFFu;
+    uint32_t prev = __shfl_up(k, 1, 64);
+    if (__lane_id() == 0) prev = (i > 0 && i - 1 < count) ? keys[i - 1] : 0xFFFFFFFFu;
     if (i >= count) return;
-    uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) ranges[k].x = i;
-    if (i == count - 1 || keys[i + 1] != k) ranges[k].y = i + 1;
+    if (i == 0 || prev != k) {
+        ranges[k].x = i;
+        if (i > 0) ranges[prev].y = i;
+    }
+    if (i == count - 1) ranges[k].y = count;
 }
 
 // One splat as the compositors keep it in LDS (4 x 16 bytes).  The

@@ -14,7 +14,55 @@ namespace gs {
 namespace {
 
 constexpr int kScanThreads = 1024;
-constexpr int kRun = 32;  // consecutive counts per thread per round
+constexpr int kRun = 32;             // consecutive counts per thread per round
+constexpr uint32_t kLdsCounts = 32768;  // counts staged in LDS (128 KiB)
+
+__device__ __forceinline__ void reduce_diag(const uint32_t* __restrict__ block_diag, uint32_t nb, uint32_t* diag_out,
+                                            uint32_t* s_diag) {
+    const uint32_t tid = threadIdx.x;
+    // SplatDiagnostics: sums except the two maxima (fields 1 and 5)
+    if (tid < 8) s_diag[tid] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < nb * 8u; i += kScanThreads) {
+        uint32_t f = i & 7u, v = block_diag[i];
+        if (!v) continue;
+        if (f == 1u || f == 5u) atomicMax(&s_diag[f], v);
+        else atomicAdd(&s_diag[f], v);
+    }
+    __syncthreads();
+    if (tid < 8) diag_out[tid] = s_diag[tid];
+}
+
+// Up to kLdsCounts counts: coalesced loads into LDS, each thread scans one
+// contiguous run out of LDS (runs of odd length stay clear of bank
+// conflicts), one block scan of the run sums, coalesced stores.
+__global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uint32_t* __restrict__ counts,
+                                                                       uint32_t nb, uint32_t* offsets,
+                                                                       uint32_t* total,
+                                                                       const uint32_t* __restrict__ block_diag,
+                                                                       uint32_t* diag_out) {
+    extern __shared__ uint32_t s_c[];
+    __shared__ uint32_t s_scan[kScanThreads / 64];
+    __shared__ uint32_t s_diag[8];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < nb; i += kScanThreads) s_c[i] = counts[i];
+    __syncthreads();
+    const uint32_t run = (nb + kScanThreads - 1) / kScanThreads;
+    const uint32_t r0 = tid * run, r1 = min(r0 + run, nb);
+    uint32_t sum = 0u;
+    for (uint32_t i = r0; i < r1; ++i) sum += s_c[i];
+    uint32_t all;
+    uint32_t acc = block_exclusive_sum<kScanThreads / 64>(sum, s_scan, &all);
+    for (uint32_t i = r0; i < r1; ++i) {
+        const uint32_t v = s_c[i];
+        s_c[i] = acc;
+        acc += v;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nb; i += kScanThreads) offsets[i] = s_c[i];
+    if (tid == 0) *total = all;
+    if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
+}
 
 __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_t* __restrict__ counts,
                                                                    uint32_t nb, uint32_t* offsets,
@@ -47,27 +95,27 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
         carry += chunk;
     }
     if (tid == 0) *total = carry;
-    if (block_diag) {
-        // SplatDiagnostics: sums except the two maxima (fields 1 and 5)
-        if (tid < 8) s_diag[tid] = 0u;
-        __syncthreads();
-        for (uint32_t i = tid; i < nb * 8u; i += kScanThreads) {
-            uint32_t f = i & 7u, v = block_diag[i];
-            if (!v) continue;
-            if (f == 1u || f == 5u) atomicMax(&s_diag[f], v);
-            else atomicAdd(&s_diag[f], v);
-        }
-        __syncthreads();
-        if (tid < 8) diag_out[tid] = s_diag[tid];
-    }
+    if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
 }
 
 }  // namespace
 
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
                               const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s) {
-    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, offsets, total,
-                       block_diag, diag_out);
+    if (nb <= kLdsCounts) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_counts_lds_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCounts * 4);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(scan_counts_lds_kernel, dim3(1), dim3(kScanThreads), (size_t)(nb ? nb : 1) * 4, s, counts,
+                           nb, offsets, total, block_diag, diag_out);
+    } else {
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, offsets, total,
+                           block_diag, diag_out);
+    }
     return hipGetLastError();
 }
 
