@@ -139,6 +139,7 @@ EXPORTS = {
     "gs_composite_alpha": (C.c_int, [C.c_void_p, C.c_int]),
     "gs_gps_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_float]),
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "gs_stochastic_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float]),
     "gs_render_alpha": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
                                   C.POINTER(GsCullOptions), C.c_int]),
     "gs_render_views": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.c_uint32,
@@ -335,6 +336,9 @@ class Context:
     def gps_sample(self, frame_seed: int, supersample: int = 1, linear_color: bool = False,
                    background: float = 0.0):
         check(self._lib.gs_gps_sample(self._h, frame_seed, supersample, int(linear_color), background))
+
+    def stochastic_sample(self, frame_seed: int, background: float = 0.0):
+        check(self._lib.gs_stochastic_sample(self._h, frame_seed, background))
 
     def accumulate(self, sample_count: int):
         check(self._lib.gs_accumulate(self._h, sample_count))

@@ -104,6 +104,9 @@ struct gs_ctx {
     hipEvent_t dup_event = nullptr;
     bool dup_pending = false;
     int composite_tile = 16;
+    int composite_kind = 0;       // the pending frame: 0 sorted alpha, 1 stochastic splats
+    uint32_t stoch_seed = 0;
+    float stoch_bg = 0.0f;
     int img_w = 0, img_h = 0;
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
@@ -686,7 +689,8 @@ int stage_sort(gs_ctx* c, const FramePlan& p, int* slot) {
     return GS_OK;
 }
 
-int stage_composite(gs_ctx* c, const FramePlan& p) {
+// (tile, splat) pairs, their stable sort by tile and the tile ranges.
+int stage_bin(gs_ctx* c, const FramePlan& p) {
     GS_HIP(launch_fill_u32(const_cast<uint32_t*>(reinterpret_cast<const uint32_t*>(p.comp.ranges)), 0u,
                            p.range_words, c->stream));
     if (p.comp.persistent_blocks) GS_HIP(launch_fill_u32(p.comp.queue, 0u, 8, c->stream));
@@ -698,6 +702,11 @@ int stage_composite(gs_ctx* c, const FramePlan& p) {
     GS_HIP(launch_tile_ranges(p.tiles.keys[slot], p.tiles.count, p.tiles.capacity,
                               const_cast<uint2*>(p.comp.ranges), c->stream));
     GS_CHECK(timed_end(c, &t));
+    return GS_OK;
+}
+
+int stage_composite(gs_ctx* c, const FramePlan& p) {
+    GS_CHECK(stage_bin(c, p));
     TimedLaunch t2{};
     GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t2));
     GS_HIP(launch_composite(p.comp, c->stream));
@@ -727,7 +736,36 @@ int after_composite(gs_ctx* c, int tile) {
     return GS_OK;
 }
 
+// Bin + StochasticSplats sample into the sample / depth images.
+int enqueue_stochastic(gs_ctx* c) {
+    const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
+    GS_CHECK(upload_views(c, c->host_cams.data(), 16));
+    GS_CHECK(prepare_composite(c, 16));
+    GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "sample image"));
+    GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "sample depth"));
+    make_composite(c, 16, &c->plan);
+    GS_CHECK(stage_bin(c, c->plan));
+    StochasticLaunch sl{};
+    sl.tiles = c->plan.comp;
+    sl.frame_seed = (float)c->stoch_seed;
+    sl.background = c->stoch_bg;
+    sl.clip_z_01 = c->cam.clip_z_01;
+    sl.sample = ptr<float4>(c->sample);
+    sl.depth = ptr<float>(c->depth);
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_COMPOSITE, &t));
+    GS_HIP(launch_stochastic(sl, c->stream));
+    GS_CHECK(timed_end(c, &t));
+    c->res_w = W;
+    c->res_h = H;
+    return GS_OK;
+}
+
 int enqueue_composite(gs_ctx* c, int tile) {
+    if (c->composite_kind == 1) {
+        GS_CHECK(enqueue_stochastic(c));
+        return after_composite(c, 16);
+    }
     GS_CHECK(upload_views(c, c->host_cams.data(), tile));
     GS_CHECK(prepare_composite(c, tile));
     make_composite(c, tile, &c->plan);
@@ -752,7 +790,7 @@ int settle_composite(gs_ctx* c, bool wait, bool rerun) {
         c->last_dup = total;
         if (total <= c->dup_cap) return GS_OK;
         GS_CHECK(grow_tile_buffers(c, total + total / 4u));
-        if (!(rerun && c->have_composite)) return GS_OK;
+        if (!(rerun && (c->have_composite || c->composite_kind == 1))) return GS_OK;
         c->overflow_reruns++;
         GS_CHECK(enqueue_composite(c, c->composite_tile));
     }
@@ -837,8 +875,31 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     GS_CHECK(set_device(c));
     GS_CHECK(settle_composite(c, false, false));
     c->have_composite = false;
+    c->composite_kind = 0;
     GS_CHECK(enqueue_composite(c, tile));
     c->have_composite = true;
+    return ok();
+}
+
+int gs_stochastic_sample(gs_ctx* c, uint32_t frame_seed, float background) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_cull) return fail(GS_ERR_STATE, "gs_stochastic_sample needs a gs_cull first");
+    if (c->views != 1) return fail(GS_ERR_STATE, "gs_stochastic_sample needs a single-view cull");
+    GS_CHECK(set_device(c));
+    GS_CHECK(settle_composite(c, false, false));
+    if (!c->have_sort) {
+        c->plan.depth = make_depth_sort(c);
+        int slot = 0;
+        GS_CHECK(stage_sort(c, c->plan, &slot));
+        c->sort_slot = slot;
+        c->have_sort = true;
+    }
+    c->have_composite = false;
+    c->composite_kind = 1;
+    c->stoch_seed = frame_seed;
+    c->stoch_bg = background;
+    GS_CHECK(enqueue_composite(c, 16));
+    c->have_gps = true;  // the sample and depth images hold this frame
     return ok();
 }
 
@@ -877,6 +938,7 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
         GS_CHECK(stage_sort(c, p, &slot));
         GS_CHECK(stage_composite(c, p));
     }
+    c->composite_kind = 0;
     GS_CHECK(after_composite(c, tile));
     c->have_cull = c->have_sort = c->have_composite = true;
     c->stats_collected = collect != 0;
@@ -986,6 +1048,8 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     if (!c->have_cull) return fail(GS_ERR_STATE, "gs_gps_sample needs a gs_cull first");
     if (c->views != 1) return fail(GS_ERR_STATE, "gs_gps_sample needs a single-view cull");
     GS_CHECK(set_device(c));
+    // a pending stochastic-splat frame writes the same images: settle it first
+    if (c->composite_kind == 1) GS_CHECK(settle_composite(c, true, true));
     uint32_t ss = supersample < 1u ? 1u : (supersample > 4u ? 4u : supersample);
     const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
     const int gw = W * (int)ss, gh = H * (int)ss;
@@ -1054,8 +1118,10 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
 
 int gs_accumulate(gs_ctx* c, uint32_t sample_count) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
-    if (!c->have_gps) return fail(GS_ERR_STATE, "gs_accumulate needs a gs_gps_sample first");
+    if (!c->have_gps) return fail(GS_ERR_STATE, "gs_accumulate needs a gs_gps_sample or gs_stochastic_sample first");
     GS_CHECK(set_device(c));
+    // a stochastic sample whose tile lists overflowed is redone before it is averaged
+    if (c->composite_kind == 1) GS_CHECK(settle_composite(c, true, true));
     if (c->acc_w != c->res_w || c->acc_h != c->res_h) {
         if (sample_count > 1)
             return fail(GS_ERR_STATE, "accumulation history is %dx%d, sample is %dx%d; reset with sample_count 1",
@@ -1081,10 +1147,12 @@ int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
             return ok();
         case GS_IMAGE_GPS_SAMPLE_RGBA32F:
             if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
+            GS_CHECK(settle_composite(c, true, true));
             GS_CHECK(copy_image(c, c->sample.p, host, c->res_w, c->res_h, 16, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_GPS_DEPTH_R32F:
             if (!c->have_gps) return fail(GS_ERR_STATE, "no GPS sample yet");
+            GS_CHECK(settle_composite(c, true, true));
             GS_CHECK(copy_image(c, c->depth.p, host, c->res_w, c->res_h, 4, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_ACCUM_RGBA32F:

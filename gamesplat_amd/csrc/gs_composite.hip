@@ -351,6 +351,115 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
     if (valid) store_pixel(t, px, py, cr, cg, cb, tr);
 }
 
+// StochasticSplats sample (shaders/splat.glsl:107-146): every covered
+// fragment that passes the alpha test is kept when hash13(pixel centre,
+// id + frame_seed) < alpha, and the kept fragment with the smallest 24-bit
+// window depth wins, ties to the lowest index (the reference draws the
+// splats in index order with a LESS depth test).  Along a tile list the
+// depth does not decrease, so a pixel is settled once the current splat lies
+// more than a few D24 steps behind its winner.  The fragment arithmetic is
+// the reference's, in source order, with the deterministic exp and log, so
+// keep decisions are those of the CPU restatement.
+__device__ __forceinline__ float fract_(float x) { return x - __builtin_floorf(x); }
+
+__device__ __forceinline__ float hash13(float x, float y, float z) {
+    float px = fract_(x * 0.1031f), py = fract_(y * 0.1031f), pz = fract_(z * 0.1031f);
+    const float d = (px * (py + 33.33f) + py * (pz + 33.33f)) + pz * (px + 33.33f);
+    px = px + d;
+    py = py + d;
+    pz = pz + d;
+    return fract_((px + py) * pz);
+}
+
+__device__ __forceinline__ uint32_t depth24(float ndc_z, float clip_z_01) {
+    float d = clip_z_01 > 0.5f ? ndc_z : ndc_z * 0.5f + 0.5f;
+    d = minf(maxf(d, 0.0f), 1.0f);
+    return (uint32_t)__builtin_rint((double)d * 16777215.0);
+}
+
+__global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
+    constexpr int BATCH = 256;
+    constexpr uint32_t kDepthSlack = 4u;  // D24 steps: float ndc_z is monotone in depth up to rounding
+    __shared__ int4 s_rect[BATCH];
+    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
+    __shared__ float4 s_par[BATCH];   // k2, opacity, cutoff, D24 depth (bits)
+    __shared__ float4 s_col[BATCH];   // r, g, b (half precision), id (bits)
+    const CompositeLaunch& a = sl.tiles;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
+    const TileRef t = tile_ref(a, item);
+
+    const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
+    const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const bool valid = px < t.width && py < t.height;
+    uint32_t best_q = 16777215u, best_id = 0xFFFFFFFFu;
+    float cr = sl.background, cg = sl.background, cb = sl.background;
+    bool kept = false, open = valid;
+
+    for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
+        if (!__syncthreads_or(open)) break;
+        const uint32_t e = base + tid;
+        if (e < t.range.y) {
+            const uint32_t g = a.tile_vals[e];
+            const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
+            const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
+            const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
+            float det = r1.x * r1.z - r1.y * r1.y;
+            if (det < 1e-6f) det = 1e-6f;
+            const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+            s_rect[tid] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width, t.height, false);
+            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
+            s_par[tid] = make_float4(k2, r0.z, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
+                                     __uint_as_float(depth24(r0.w, sl.clip_z_01)));
+            s_col[tid] = make_float4(__half2float(__float2half_rn(r2.x)), __half2float(__float2half_rn(r2.y)),
+                                     __half2float(__float2half_rn(r2.z)), __uint_as_float(g));
+        }
+        __syncthreads();
+        const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
+        if (!__ballot(open)) continue;
+        for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
+            const uint32_t j = chunk + lane;
+            bool hit = false;
+            if (j < cnt) {
+                const int4 r = s_rect[j];
+                hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
+            }
+            uint64_t mask = __ballot(hit);
+            while (mask) {
+                const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                const int4 r = s_rect[k];
+                const float4 geo = s_geo[k], par = s_par[k], col = s_col[k];
+                const uint32_t q = __float_as_uint(par.w), id = __float_as_uint(col.w);
+                const float dx = fx - geo.x, dy = fy - geo.y;
+                const float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + par.x * dy * dy);
+                const bool covered = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & !(power > 0.0f) &
+                                     !(power < par.z);
+                const float alpha = minf(par.y * expf_(power), 0.99f);
+                const bool win = covered && hash13(fx, fy, (float)id + sl.frame_seed) < alpha &&
+                                 (q < best_q || (q == best_q && id < best_id));
+                if (win) {
+                    best_q = q;
+                    best_id = id;
+                    cr = col.x;
+                    cg = col.y;
+                    cb = col.z;
+                    kept = true;
+                }
+                if (q > best_q + kDepthSlack) open = false;
+            }
+            if (!__ballot(open)) break;
+        }
+    }
+
+    if (valid) {
+        const size_t o = (size_t)py * t.width + px;
+        sl.sample[o] = kept ? make_float4(cr, cg, cb, 1.0f) : make_float4(cr, cg, cb, 0.0f);
+        sl.depth[o] = kept ? (float)((double)best_q / 16777215.0) : 1.0f;
+    }
+}
+
 // Next tile of a persistent block: blocks are dealt to the 8 XCDs
 // round-robin, each XCD drains its own contiguous band of tiles (its L2 sees
 // neighbouring tiles), then takes tiles from the other bands.  A plain
@@ -420,6 +529,13 @@ hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint3
                               hipStream_t s) {
     if (!capacity) return hipSuccess;
     hipLaunchKernelGGL(tile_ranges_kernel, dim3((capacity + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_stochastic(const StochasticLaunch& a, hipStream_t s) {
+    if (!a.tiles.total_tiles) return hipSuccess;
+    if (a.tiles.tile != 16 || a.tiles.nviews != 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stochastic_kernel, dim3(a.tiles.total_tiles), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

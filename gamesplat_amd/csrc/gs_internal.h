@@ -220,6 +220,17 @@ struct CompositeLaunch {
 };
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s);
 
+// StochasticSplats sample over the 16x16 tile lists of one view.
+struct StochasticLaunch {
+    CompositeLaunch tiles;   // records, tile lists and ranges, tile = 16
+    float frame_seed;
+    float background;
+    float clip_z_01;
+    float4* sample;          // W x H RGBA32F, framebuffer rows
+    float* depth;            // W x H window depth of the kept splat (1 where none)
+};
+hipError_t launch_stochastic(const StochasticLaunch& a, hipStream_t s);
+
 // gs_gps.hip
 struct GpsLaunch {
     const float4* records;

@@ -9,7 +9,7 @@
 // Flags (defaults in brackets):
 //   ply=<path>                 3DGS binary PLY; otherwise a synthetic scene:
 //   gaussians=[100000] clusters=[16] seed=[0x5EED0001] opacity_mean=[0]
-//   width=[1280] height=[720] render_mode=[alpha|gps]
+//   width=[1280] height=[720] render_mode=[alpha|gps|stochastic] stochastic_spp=[1]
 //   view_seed=[seed] radius=[15] orbit_deg=[0]   camera on a seeded direction
 //   gps_ss=[1] gps_budget_m=[16] gps_accumulation=[1] sh_degree=[3]
 //   gpu_budget_mb=[0] diagnostics=[0]
@@ -165,7 +165,11 @@ int main(int argc, char** argv) {
     Renderer r;
     if (!renderer_init(&r)) return 1;
     const std::string mode = arg("render_mode", "alpha");
-    r.splat_render_mode = mode == "gps" ? SplatRenderMode::GaussianPointSplatting : SplatRenderMode::AlphaBlendSorted;
+    r.splat_render_mode = mode == "gps"          ? SplatRenderMode::GaussianPointSplatting
+                          : mode == "stochastic" ? SplatRenderMode::StochasticSplats
+                                                 : SplatRenderMode::AlphaBlendSorted;
+    r.stochastic_taa_enabled = false;  // plain accumulation for StochasticSplats
+    r.stochastic_samples_per_frame = (uint32_t)argi("stochastic_spp", 1);
     r.black_background = true;
     r.sh_degree = (int)argi("sh_degree", 3);
     r.gpu_budget_bytes = (size_t)argi("gpu_budget_mb", 0) * 1048576u;

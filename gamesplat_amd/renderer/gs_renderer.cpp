@@ -127,8 +127,10 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         return;
     }
     const bool gps = r->splat_render_mode == SplatRenderMode::GaussianPointSplatting;
-    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) {
-        fprintf(stderr, "[renderer] StochasticSplats mode is not supported by this build\n");
+    const bool stochastic = r->splat_render_mode == SplatRenderMode::StochasticSplats;
+    if (stochastic && r->stochastic_taa_enabled) {
+        fprintf(stderr, "[renderer] StochasticSplats with TAA is not supported by this build; "
+                        "set stochastic_taa_enabled = false for plain accumulation\n");
         r->last_frame_status = GS_ERR_INVALID;
         return;
     }
@@ -155,7 +157,7 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     // no indirect draw count in the reference either (src/renderer.cpp:2177,2187)
     scene->visible_count = scene->gaussian_count;
 
-    if (!gps) {
+    if (!gps && !stochastic) {
         st = gs_sort(r->ctx);
         if (st == GS_OK) st = gs_composite_alpha(r->ctx, 0);
         if (st != GS_OK) {
@@ -172,6 +174,35 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         renderer_reset_stochastic_accumulation(r);
     memcpy(&r->stochastic_prev_cam, cam, sizeof(*cam));
     r->stochastic_prev_cam_valid = true;
+
+    if (stochastic) {
+        // StochasticSplats without TAA (src/renderer.cpp:2331-2420): samples_per_frame
+        // samples, each with the next frame seed, folded into the running mean
+        const float bg = r->black_background ? 0.0f : 0.1f;
+        uint32_t samples = r->stochastic_samples_per_frame < 1u ? 1u : r->stochastic_samples_per_frame;
+        if (samples > 16u) samples = 16u;
+        for (uint32_t i = 0; i < samples; ++i) {
+            r->stochastic_frame_seed++;
+            st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg);
+            if (st == GS_OK && (r->stochastic_accumulation_enabled || samples > 1u)) {
+                const uint32_t next = r->stochastic_accumulation_enabled ? r->stochastic_sample_count + 1u : i + 1u;
+                st = gs_accumulate(r->ctx, next);
+                if (st == GS_OK && r->stochastic_accumulation_enabled) {
+                    r->stochastic_sample_count = next;
+                    r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
+                }
+            }
+            if (st != GS_OK) {
+                log_error("stochastic splats", st);
+                r->last_frame_status = st;
+                return;
+            }
+        }
+        r->stochastic_width = (uint32_t)cam->viewport[0];
+        r->stochastic_height = (uint32_t)cam->viewport[1];
+        r->stochastic_display_accum = r->stochastic_accumulation_enabled || samples > 1u;
+        return;
+    }
     r->stochastic_frame_seed++;
 
     uint32_t ss = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
@@ -227,9 +258,11 @@ bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
         for (size_t i = 0; i < h.size(); ++i) out[i] = half_to_float(h[i]);
         return true;
     }
-    size_t px = (size_t)r->gps_gpu.width * r->gps_gpu.height;
+    size_t px = (size_t)r->stochastic_width * r->stochastic_height;
     if (floats != px * 4) return false;
-    int which = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
+    const bool accum = r->splat_render_mode == SplatRenderMode::StochasticSplats ? r->stochastic_display_accum
+                                                                               : r->stochastic_accumulation_enabled;
+    int which = accum ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
     int st = gs_read_image(r->ctx, which, out, floats * 4);
     if (st != GS_OK) {
         log_error("read stochastic image", st);

@@ -122,6 +122,7 @@ struct Renderer {
     GaussianPointSplatGpu gps_gpu;
     uint32_t stochastic_width;
     uint32_t stochastic_height;
+    bool stochastic_display_accum;   // StochasticSplats: the running mean (not the last sample) is shown
     uint32_t stochastic_sample_count;
     uint32_t stochastic_samples_per_frame;
     uint32_t stochastic_taa_current_samples;

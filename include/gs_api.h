@@ -190,6 +190,19 @@ int gs_composite_alpha(gs_ctx* ctx, int tile_size);
 int gs_gps_sample(gs_ctx* ctx, uint32_t frame_seed, uint32_t supersample,
                   int linear_color, float background);
 
+/* Replaces the StochasticSplats sample pass (splat_stochastic_fs,
+ * shaders/splat.glsl:107-146; pass src/renderer.cpp:2331-2352) without TAA
+ * jitter: every covered fragment that passes the alpha test is kept when
+ * hash13(pixel centre, id + frame_seed) < alpha, the kept fragment with the
+ * smallest 24-bit depth wins (ties: lowest index, the reference's draw
+ * order), and the sample image (GS_IMAGE_GPS_SAMPLE_RGBA32F) receives its
+ * colour (half precision, alpha 1) or (background, background, background,
+ * 0); the depth image receives its D24 depth / (2^24 - 1) or 1.  Needs
+ * gs_cull (single view); sorts if gs_sort has not run.  gs_accumulate
+ * averages these samples as it does GPS samples.  TAA reprojection
+ * (shaders/accum.glsl:47-179) is not part of this entry point. */
+int gs_stochastic_sample(gs_ctx* ctx, uint32_t frame_seed, float background);
+
 /* Replaces accum_fs (shaders/accum.glsl:19-45): accum = h + (s - h) / n with
  * n = sample_count (1 = first sample, history ignored). */
 int gs_accumulate(gs_ctx* ctx, uint32_t sample_count);

@@ -838,6 +838,90 @@ void or_gps_resolve(const uint64_t* cells, int width, int height, int ss_in, flo
     }
 }
 
+/* hash13 of shaders/splat.glsl:121-125, float operations in source order
+ * (dot product summed left to right). */
+float or_hash13(float x, float y, float z) {
+    float px = or_fract(x * 0.1031f), py = or_fract(y * 0.1031f), pz = or_fract(z * 0.1031f);
+    float d = (px * (py + 33.33f) + py * (pz + 33.33f)) + pz * (px + 33.33f);
+    px = px + d;
+    py = py + d;
+    pz = pz + d;
+    return or_fract((px + py) * pz);
+}
+
+/* 24-bit window depth of an ndc depth: GL depth range [0, 1]
+ * (z_w = 0.5 z + 0.5, or z itself for clip_z_01), clamped, to a D24 value
+ * with round to nearest. */
+uint32_t or_depth24(float ndc_z, float clip_z_01) {
+    float d = clip_z_01 > 0.5f ? ndc_z : ndc_z * 0.5f + 0.5f;
+    d = fminf(fmaxf(d, 0.0f), 1.0f);
+    return (uint32_t)rint((double)d * 16777215.0);
+}
+
+/* StochasticSplats sample pass (shaders/splat.glsl:107-146 with the splat_vs
+ * quad, pipeline src/renderer.cpp:302-320, pass src/renderer.cpp:2331-2352,
+ * draw order src/renderer.cpp:1688-1716): the visible splats are drawn in
+ * index order (ids[i] = i or 0xFFFFFFFF); a covered fragment that passes the
+ * alpha test is kept when hash13(gl_FragCoord.xy, id + frame_seed) < alpha,
+ * and written as (rgb, 1) when its 24-bit depth is LESS than the stored one
+ * (cleared to 1.0); the colour target holds half floats (RGBA16F).  Clear
+ * colour (bg, bg, bg, 0).  color: W*H*4, depth: W*H D24 depths of the kept
+ * splat / (2^24 - 1) (1 where nothing was kept), framebuffer row order. */
+void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,
+                          float clip_z_01, float frame_seed, float background, float* color, float* depth,
+                          int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int j = 0; j < height; ++j) {
+        uint32_t* best = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)width);
+        for (int i = 0; i < width; ++i) {
+            best[i] = 16777215u;
+            float* c = color + ((size_t)j * width + i) * 4;
+            c[0] = c[1] = c[2] = background;
+            c[3] = 0.0f;
+            depth[(size_t)j * width + i] = 1.0f;
+        }
+        for (uint32_t s = 0; s < n; ++s) {
+            const uint32_t id = ids[s];
+            if (id == 0xFFFFFFFFu) continue;
+            const float* rec = projected + (size_t)id * 16;
+            int32_t q[4];
+            or_quad_fixed(rec, (float)width, (float)height, q);
+            int32_t i_lo = or_ceil_div256(q[0] - 128), i_hi = or_ceil_div256(q[1] - 128);
+            int32_t y_lo = or_ceil_div256(q[2] - 128), y_hi = or_ceil_div256(q[3] - 128);
+            if (j < y_lo || j >= y_hi) continue;
+            if (i_lo < 0) i_lo = 0;
+            if (i_hi > width) i_hi = width;
+            const float o = rec[3];
+            const float cutoff = or_logf((1.0f / 255.0f) / fmaxf(o, 1e-8f));
+            const float cx = rec[4], cy = rec[5];
+            const float k0 = rec[8], k1 = rec[9], k2 = rec[10];
+            const uint32_t dq = or_depth24(rec[11], clip_z_01);
+            const float fy = (float)j + 0.5f;
+            const float dy = fy - cy;
+            const float seed = (float)id + frame_seed;
+            for (int32_t i = i_lo; i < i_hi; ++i) {
+                const float fx = (float)i + 0.5f;
+                const float dx = fx - cx;
+                const float power = -0.5f * (k0 * dx * dx + 2.0f * k1 * dx * dy + k2 * dy * dy);
+                if (power > 0.0f || power < cutoff) continue;
+                const float alpha = fminf(o * or_expf(power), 0.99f);
+                if (or_hash13(fx, fy, seed) >= alpha) continue;
+                if (!(dq < best[i])) continue;
+                best[i] = dq;
+                float* c = color + ((size_t)j * width + i) * 4;
+                for (int ch = 0; ch < 3; ++ch) c[ch] = or_f16_to_f32(or_f32_to_f16(rec[ch]));
+                c[3] = 1.0f;
+                depth[(size_t)j * width + i] = (float)((double)dq / 16777215.0);
+            }
+        }
+        free(best);
+    }
+    (void)threads;
+}
+
 void or_accumulate(const float* history, const float* sample, float sample_count,
                    float* out, uint32_t count) {
     float n = fmaxf(sample_count, 1.0f);

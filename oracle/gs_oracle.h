@@ -82,6 +82,16 @@ void or_gps_splat(const uint32_t* prepared, const uint32_t* work, uint32_t work_
  * framebuffer row order (row 0 = first row of the cell buffer). */
 void or_gps_resolve(const uint64_t* cells, int width, int height, int ss, float background,
                     int linear_color, const float* projected, float* color, float* depth);
+/* hash13 (shaders/splat.glsl:121-125) and the D24 window depth. */
+float or_hash13(float x, float y, float z);
+uint32_t or_depth24(float ndc_z, float clip_z_01);
+/* StochasticSplats sample (shaders/splat.glsl:107-146): splats drawn in index
+ * order, kept when hash13 < alpha, nearest 24-bit depth wins (LESS), colour
+ * stored as half floats; clear (bg, bg, bg, 0).  color W*H*4, depth W*H,
+ * framebuffer row order. */
+void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,
+                          float clip_z_01, float frame_seed, float background, float* color, float* depth,
+                          int threads);
 /* accum_fs (shaders/accum.glsl:39-44), elementwise over count floats. */
 void or_accumulate(const float* history, const float* sample, float sample_count,
                    float* out, uint32_t count);

@@ -46,6 +46,10 @@ _SIGS = {
     "or_gps_resolve": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p,
                               C.c_void_p, C.c_void_p]),
     "or_accumulate": (None, [C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_uint32]),
+    "or_hash13": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+    "or_depth24": (C.c_uint32, [C.c_float, C.c_float]),
+    "or_stochastic_splats": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_float,
+                                    C.c_float, C.c_void_p, C.c_void_p, C.c_int]),
 }
 
 
@@ -149,6 +153,18 @@ def gps_resolve(cells, width, height, ss=1, background=0.0, linear_color=False, 
     depth = np.empty((height, width), np.float32)
     proj = np.ascontiguousarray(projected, dtype=np.float32) if projected is not None else np.zeros(16, np.float32)
     lib.or_gps_resolve(_p(cells), width, height, ss, background, int(linear_color), _p(proj), _p(color), _p(depth))
+    return color, depth
+
+
+def stochastic_splats(projected, ids, width, height, frame_seed, clip_z_01=0.0, background=0.0, threads=None):
+    """StochasticSplats sample: (color (H, W, 4), depth (H, W)) in framebuffer row order."""
+    lib = load()
+    p = np.ascontiguousarray(projected, dtype=np.float32)
+    i = np.ascontiguousarray(ids, dtype=np.uint32)
+    color = np.empty((height, width, 4), np.float32)
+    depth = np.empty((height, width), np.float32)
+    lib.or_stochastic_splats(_p(p), _p(i), i.size, width, height, clip_z_01, float(frame_seed), background,
+                             _p(color), _p(depth), threads_default() if threads is None else threads)
     return color, depth
 
 

@@ -90,7 +90,7 @@ def test_frames_are_deterministic(gs, c1_scene):
 
 def test_headless_driver_runs(tmp_path):
     exe = ROOT / "gamesplat_amd" / "gs_headless"
-    for mode in ("alpha", "gps"):
+    for mode in ("alpha", "gps", "stochastic"):
         cap = tmp_path / f"{mode}.f32"
         r = subprocess.run([str(exe), "gaussians=20000", "width=320", "height=240", f"render_mode={mode}",
                             "warmup=1", "frames=4", f"linear_capture={cap}"], capture_output=True, text=True,

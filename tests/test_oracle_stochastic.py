@@ -1,0 +1,98 @@
+"""Pins of the StochasticSplats oracle (or_stochastic_splats), CPU only.
+
+- hash13 against an independent float32 numpy evaluation of the GLSL source;
+- the D24 depth against its closed forms;
+- the depth test's tie rule (first drawn = lowest index wins) on two
+  coincident splats, decided pixel by pixel from the hash values;
+- unbiasedness: the mean of many samples converges to the FP64 front-to-back
+  composite (each splat is kept with probability alpha and the nearest kept
+  one wins, so P(splat wins) is its front-to-back weight), with the error
+  shrinking as the sample count grows.
+"""
+import numpy as np
+
+from gamesplat_amd import scenes
+from oracle import oracle as orc
+
+
+def _hash13_np(x, y, z):
+    f = np.float32
+    p = [np.float32(v) * f(0.1031) for v in (x, y, z)]
+    p = [v - np.floor(v) for v in p]
+    d = (p[0] * (p[1] + f(33.33)) + p[1] * (p[2] + f(33.33))) + p[2] * (p[0] + f(33.33))
+    p = [v + d for v in p]
+    r = (p[0] + p[1]) * p[2]
+    return r - np.floor(r)
+
+
+def test_hash13_matches_float32_source():
+    rng = np.random.default_rng(1)
+    for _ in range(500):
+        x = np.float32(rng.integers(0, 4096) + 0.5)
+        y = np.float32(rng.integers(0, 4096) + 0.5)
+        z = np.float32(rng.integers(0, 6_000_000)) + np.float32(rng.integers(1, 300))
+        h = orc.scalar("or_hash13", float(x), float(y), float(z))
+        assert np.float32(h) == _hash13_np(x, y, z)
+        assert 0.0 <= h < 1.0
+
+
+def test_depth24_closed_forms():
+    assert orc.scalar("or_depth24", -1.0, 0.0) == 0
+    assert orc.scalar("or_depth24", 1.0, 0.0) == 16777215
+    assert orc.scalar("or_depth24", 0.0, 0.0) == 8388608  # 8388607.5 rounds to even
+    assert orc.scalar("or_depth24", 0.25, 1.0) == round(0.25 * 16777215)
+    assert orc.scalar("or_depth24", 2.0, 0.0) == 16777215  # clamped
+
+
+def _record(cx, cy, rgb, opacity, ndc_z, sigma):
+    """A 64-float-row projected splat (color_opacity, center_radius, conic_depth, covariance_det)."""
+    a = c = sigma * sigma + 0.3
+    det = a * c
+    rad = float(np.ceil(3.0 * np.sqrt(a)))
+    return np.array([*rgb, opacity, cx, cy, rad, rad, c / det, 0.0, a / det, ndc_z, a, 0.0, c, det], np.float32)
+
+
+def test_ties_go_to_the_lowest_index():
+    W = H = 24
+    rec = np.stack([_record(12.0, 12.0, (1.0, 0.0, 0.0), 0.6, 0.3, 3.0),
+                    _record(12.0, 12.0, (0.0, 1.0, 0.0), 0.6, 0.3, 3.0)])
+    ids = np.array([0, 1], np.uint32)
+    seed = 5.0
+    color, _ = orc.stochastic_splats(rec, ids, W, H, seed)
+    red = green = 0
+    for j in range(H):
+        for i in range(W):
+            fx, fy = i + 0.5, j + 0.5
+            dx, dy = np.float32(fx - 12.0), np.float32(fy - 12.0)
+            k0, k2 = rec[0, 8], rec[0, 10]
+            power = np.float32(-0.5) * (k0 * dx * dx + np.float32(0.0) + k2 * dy * dy)
+            alpha = min(np.float32(0.6) * np.float32(orc.scalar("or_expf", float(power))), np.float32(0.99))
+            cutoff = orc.scalar("or_logf", (1.0 / 255.0) / 0.6)
+            covered = power >= cutoff
+            keep0 = covered and orc.scalar("or_hash13", fx, fy, 0.0 + seed) < alpha
+            keep1 = covered and orc.scalar("or_hash13", fx, fy, 1.0 + seed) < alpha
+            want = (1.0, 0.0, 0.0) if keep0 else ((0.0, 1.0, 0.0) if keep1 else (0.0, 0.0, 0.0))
+            np.testing.assert_array_equal(color[j, i, :3], want)
+            red += keep0
+            green += keep1 and not keep0
+    assert red > 0 and green > 0
+
+
+def test_mean_converges_to_front_to_back_composite():
+    from gamesplat_amd import generate_scene
+
+    g = generate_scene(0x5EED5701, 4000, 4, 0.0)
+    cam = scenes.random_view_camera(0x5EED5701, 48, 40)
+    keys, ids, proj, vis, _ = orc.cull(g, cam)
+    sk, si = orc.radix_sort(keys, ids)
+    ref = orc.composite(proj, si, 48, 40, mode=0)[::-1]  # framebuffer row order
+    errs = {}
+    acc = np.zeros((40, 48, 4))
+    for f in range(1, 257):
+        c, _ = orc.stochastic_splats(proj, ids, 48, 40, f)
+        acc += c
+        if f in (16, 256):
+            errs[f] = np.sqrt(np.mean((acc / f - ref) ** 2))
+    # Monte Carlo: error ~ 1/sqrt(K); 16x the samples should give ~4x less
+    assert errs[256] < 0.4 * errs[16], errs
+    assert errs[256] < 0.03, errs
