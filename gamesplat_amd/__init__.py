@@ -11,10 +11,13 @@ from ._lib import (  # noqa: F401
     GsCameraUniforms,
     GsError,
     GsSplatEffectParams,
+    GsTaaUniforms,
     camera_from_pose,
     generate_scene,
     load,
     look_at_origin_pose,
+    taa_jitter,
+    taa_uniforms,
 )
 from . import _lib as abi  # noqa: F401
 from .scenes import orbit_camera, random_view_camera, scene_config  # noqa: F401

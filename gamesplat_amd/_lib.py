@@ -27,6 +27,9 @@ GS_IMAGE_ALPHA_RGBA16F = 0
 GS_IMAGE_GPS_SAMPLE_RGBA32F = 1
 GS_IMAGE_GPS_DEPTH_R32F = 2
 GS_IMAGE_ACCUM_RGBA32F = 3
+GS_IMAGE_FRAME_AVG_RGBA32F = 4
+GS_IMAGE_TAA_RGBA32F = 5
+GS_IMAGE_TAA_XYZ_RGBA32F = 6
 
 GS_BUFFER_SORTED_KEYS = 0
 GS_BUFFER_SORTED_IDS = 1
@@ -69,6 +72,30 @@ class GsCameraUniforms(C.Structure):
     def from_array(cls, a) -> "GsCameraUniforms":
         a = np.ascontiguousarray(a, dtype=np.float32)
         assert a.size == 40
+        return cls.from_buffer_copy(a.tobytes())
+
+
+class GsTaaUniforms(C.Structure):
+    """TaaAccumUBO (shaders/accum.glsl:64-75); matrices column-major."""
+    _fields_ = [
+        ("inv_view", C.c_float * 16),
+        ("prev_view", C.c_float * 16),
+        ("viewport_orthographic", C.c_float * 4),
+        ("focal_clip", C.c_float * 4),
+        ("prev_focal_clip", C.c_float * 4),
+        ("proj_z", C.c_float * 4),
+        ("history_valid", C.c_float),
+        ("view_changed", C.c_float),
+        ("pad", C.c_float * 2),
+    ]
+
+    def as_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+    @classmethod
+    def from_array(cls, a) -> "GsTaaUniforms":
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.size == 52
         return cls.from_buffer_copy(a.tobytes())
 
 
@@ -139,7 +166,12 @@ EXPORTS = {
     "gs_composite_alpha": (C.c_int, [C.c_void_p, C.c_int]),
     "gs_gps_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_float]),
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
-    "gs_stochastic_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float]),
+    "gs_stochastic_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
+    "gs_taa_jitter": (None, [C.c_uint32, C.POINTER(C.c_float)]),
+    "gs_frame_average": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "gs_taa_uniforms": (C.c_int, [C.POINTER(GsCameraUniforms), C.POINTER(GsCameraUniforms), C.c_int,
+                                  C.POINTER(GsTaaUniforms)]),
+    "gs_taa_accumulate": (C.c_int, [C.c_void_p, C.POINTER(GsTaaUniforms), C.c_int]),
     "gs_render_alpha": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
                                   C.POINTER(GsCullOptions), C.c_int]),
     "gs_render_views": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.c_uint32,
@@ -207,6 +239,23 @@ def check(code: int) -> None:
 
 def _fptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def taa_jitter(frame_seed: int) -> tuple[float, float]:
+    """Halton (2, 3) jitter of a StochasticSplats sample, in pixels."""
+    out = (C.c_float * 2)()
+    load().gs_taa_jitter(frame_seed, out)
+    return float(out[0]), float(out[1])
+
+
+def taa_uniforms(cam: GsCameraUniforms, prev: GsCameraUniforms | None = None,
+                 view_changed: bool = False) -> GsTaaUniforms:
+    """TAA uniforms for cam with prev's history (None = no history).  A view
+    that cannot be inverted raises GsError."""
+    u = GsTaaUniforms()
+    check(load().gs_taa_uniforms(C.byref(cam), None if prev is None else C.byref(prev), int(view_changed),
+                                 C.byref(u)))
+    return u
 
 
 def generate_scene(seed: int, count: int, clusters: int, opacity_logit_mean: float = 0.0) -> np.ndarray:
@@ -337,8 +386,15 @@ class Context:
                    background: float = 0.0):
         check(self._lib.gs_gps_sample(self._h, frame_seed, supersample, int(linear_color), background))
 
-    def stochastic_sample(self, frame_seed: int, background: float = 0.0):
-        check(self._lib.gs_stochastic_sample(self._h, frame_seed, background))
+    def stochastic_sample(self, frame_seed: int, background: float = 0.0, jitter=None):
+        j = None if jitter is None else (C.c_float * 2)(float(jitter[0]), float(jitter[1]))
+        check(self._lib.gs_stochastic_sample(self._h, frame_seed, background, j))
+
+    def frame_average(self, sample_index: int):
+        check(self._lib.gs_frame_average(self._h, sample_index))
+
+    def taa_accumulate(self, uniforms: GsTaaUniforms, use_frame_average: bool = False):
+        check(self._lib.gs_taa_accumulate(self._h, C.byref(uniforms), int(use_frame_average)))
 
     def accumulate(self, sample_count: int):
         check(self._lib.gs_accumulate(self._h, sample_count))

@@ -10,6 +10,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -107,11 +109,17 @@ struct gs_ctx {
     int composite_kind = 0;       // the pending frame: 0 sorted alpha, 1 stochastic splats
     uint32_t stoch_seed = 0;
     float stoch_bg = 0.0f;
+    float stoch_jitter[2] = {0.0f, 0.0f};
     int img_w = 0, img_h = 0;
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
     DevBuf work, prepared, item_count, item_base, cells, sample, depth, accum;
     int gps_w = 0, gps_h = 0, gps_ss = 1, res_w = 0, res_h = 0, acc_w = 0, acc_h = 0;
+    // StochasticSplats + TAA: frame average and the double-buffered history
+    DevBuf frame_avg, taa_color[2], taa_xyz[2];
+    int avg_w = 0, avg_h = 0, taa_w = 0, taa_h = 0;
+    int taa_last = 0;        // history buffer the last TAA pass wrote
+    bool have_avg = false, have_taa = false;
     bool cells_dirty = true;
     int keep_cells = 0;
     int default_tile = 16;
@@ -397,7 +405,8 @@ void gs_destroy(gs_ctx* c) {
                       &c->ids[2], &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
-                      &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum};
+                      &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
+                      &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1]};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -744,12 +753,23 @@ int enqueue_stochastic(gs_ctx* c) {
     GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "sample image"));
     GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "sample depth"));
     make_composite(c, 16, &c->plan);
+    if (c->stoch_jitter[0] != 0.0f || c->stoch_jitter[1] != 0.0f) {
+        // jittered quads: the tile rectangles come from the moved quads
+        BinLaunch& b = c->plan.bin;
+        b.jittered = 1;
+        b.records = ptr<float4>(c->records);
+        b.tight = c->cull_tight;
+        b.jitter_x = c->stoch_jitter[0];
+        b.jitter_y = c->stoch_jitter[1];
+    }
     GS_CHECK(stage_bin(c, c->plan));
     StochasticLaunch sl{};
     sl.tiles = c->plan.comp;
     sl.frame_seed = (float)c->stoch_seed;
     sl.background = c->stoch_bg;
     sl.clip_z_01 = c->cam.clip_z_01;
+    sl.jitter_x = c->stoch_jitter[0];
+    sl.jitter_y = c->stoch_jitter[1];
     sl.sample = ptr<float4>(c->sample);
     sl.depth = ptr<float>(c->depth);
     TimedLaunch t{};
@@ -881,8 +901,10 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     return ok();
 }
 
-int gs_stochastic_sample(gs_ctx* c, uint32_t frame_seed, float background) {
+int gs_stochastic_sample(gs_ctx* c, uint32_t frame_seed, float background, const float* jitter_px) {
     if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (jitter_px && !(std::isfinite(jitter_px[0]) && std::isfinite(jitter_px[1])))
+        return fail(GS_ERR_INVALID, "jitter must be finite");
     if (!c->have_cull) return fail(GS_ERR_STATE, "gs_stochastic_sample needs a gs_cull first");
     if (c->views != 1) return fail(GS_ERR_STATE, "gs_stochastic_sample needs a single-view cull");
     GS_CHECK(set_device(c));
@@ -898,6 +920,8 @@ int gs_stochastic_sample(gs_ctx* c, uint32_t frame_seed, float background) {
     c->composite_kind = 1;
     c->stoch_seed = frame_seed;
     c->stoch_bg = background;
+    c->stoch_jitter[0] = jitter_px ? jitter_px[0] : 0.0f;
+    c->stoch_jitter[1] = jitter_px ? jitter_px[1] : 0.0f;
     GS_CHECK(enqueue_composite(c, 16));
     c->have_gps = true;  // the sample and depth images hold this frame
     return ok();
@@ -1136,6 +1160,145 @@ int gs_accumulate(gs_ctx* c, uint32_t sample_count) {
     return ok();
 }
 
+void gs_taa_jitter(uint32_t frame_seed, float jitter_px[2]) {
+    if (!jitter_px) return;
+    const uint32_t index = frame_seed % 1024u + 1u;
+    const uint32_t bases[2] = {2u, 3u};
+    for (int k = 0; k < 2; ++k) {
+        // radical inverse of index in the base, in float as the reference
+        float f = 1.0f, r = 0.0f;
+        for (uint32_t i = index; i > 0; i /= bases[k]) {
+            f /= (float)bases[k];
+            r += f * (float)(i % bases[k]);
+        }
+        jitter_px[k] = r - 0.5f;
+    }
+}
+
+int gs_frame_average(gs_ctx* c, uint32_t sample_index) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->have_gps) return fail(GS_ERR_STATE, "gs_frame_average needs a gs_stochastic_sample first");
+    GS_CHECK(set_device(c));
+    if (c->composite_kind == 1) GS_CHECK(settle_composite(c, true, true));
+    if (sample_index > 0 && (c->avg_w != c->res_w || c->avg_h != c->res_h || !c->have_avg))
+        return fail(GS_ERR_STATE, "frame average is %dx%d, sample is %dx%d; start with sample_index 0", c->avg_w,
+                    c->avg_h, c->res_w, c->res_h);
+    GS_CHECK(ensure(c, c->frame_avg, (size_t)c->res_w * c->res_h * 16, "frame average image"));
+    GS_HIP(launch_accumulate(ptr<float4>(c->sample), ptr<float4>(c->frame_avg), (uint32_t)(c->res_w * c->res_h),
+                             (float)sample_index + 1.0f, c->stream));
+    c->avg_w = c->res_w;
+    c->avg_h = c->res_h;
+    c->have_avg = true;
+    return ok();
+}
+
+namespace {
+
+// Inverse of a column-major 4x4 matrix: Gauss-Jordan elimination with
+// partial pivoting in double precision, rounded to float.  False when
+// |det| <= 1e-8, the reference's singularity threshold.
+bool invert4(const float* m, float* out) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            a[r][k] = m[k * 4 + r];
+            a[r][4 + k] = r == k ? 1.0 : 0.0;
+        }
+    double det = 1.0;
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col) {
+            for (int k = 0; k < 8; ++k) std::swap(a[piv][k], a[col][k]);
+            det = -det;
+        }
+        det *= a[col][col];
+        const double inv = 1.0 / a[col][col];
+        for (int k = 0; k < 8; ++k) a[col][k] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col || a[r][col] == 0.0) continue;
+            const double f = a[r][col];
+            for (int k = 0; k < 8; ++k) a[r][k] -= f * a[col][k];
+        }
+    }
+    if (std::fabs(det) <= 1e-8) return false;
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) out[k * 4 + r] = (float)a[r][4 + k];
+    return true;
+}
+
+}  // namespace
+
+int gs_taa_uniforms(const GsCameraUniforms* cam, const GsCameraUniforms* prev, int view_changed,
+                    GsTaaUniforms* out) {
+    if (!cam || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    *out = GsTaaUniforms{};
+    const bool invertible = invert4(cam->view, out->inv_view);
+    if (!invertible) {
+        std::memcpy(out->inv_view, cam->view, sizeof(out->inv_view));
+        prev = nullptr;
+    }
+    const GsCameraUniforms* p = prev ? prev : cam;
+    std::memcpy(out->prev_view, p->view, sizeof(out->prev_view));
+    out->viewport_orthographic[0] = cam->viewport[0];
+    out->viewport_orthographic[1] = cam->viewport[1];
+    out->viewport_orthographic[2] = cam->orthographic;
+    out->viewport_orthographic[3] = p->orthographic;
+    out->focal_clip[0] = cam->persp_focal;
+    out->focal_clip[1] = cam->ortho_focal;
+    out->focal_clip[2] = cam->clip_y_sign;
+    out->focal_clip[3] = cam->clip_z_01;
+    out->prev_focal_clip[0] = p->persp_focal;
+    out->prev_focal_clip[1] = p->ortho_focal;
+    out->prev_focal_clip[2] = p->clip_y_sign;
+    out->proj_z[0] = cam->proj[10];
+    out->proj_z[1] = cam->proj[14];
+    out->history_valid = prev ? 1.0f : 0.0f;
+    out->view_changed = view_changed ? 1.0f : 0.0f;
+    if (!invertible) return fail(GS_ERR_INVALID, "view matrix is not invertible; history dropped");
+    return ok();
+}
+
+int gs_taa_accumulate(gs_ctx* c, const GsTaaUniforms* u, int use_frame_average) {
+    if (!c || !u) return fail(GS_ERR_INVALID, "bad arguments");
+    if (!c->have_gps) return fail(GS_ERR_STATE, "gs_taa_accumulate needs a gs_stochastic_sample first");
+    GS_CHECK(set_device(c));
+    if (c->composite_kind == 1) GS_CHECK(settle_composite(c, true, true));
+    const int W = c->res_w, H = c->res_h;
+    if (use_frame_average && (!c->have_avg || c->avg_w != W || c->avg_h != H))
+        return fail(GS_ERR_STATE, "no frame average of this size; call gs_frame_average");
+    if (u->history_valid != 0.0f && (!c->have_taa || c->taa_w != W || c->taa_h != H))
+        return fail(GS_ERR_STATE, "TAA history is %dx%d, frame is %dx%d; pass history_valid 0 to restart",
+                    c->taa_w, c->taa_h, W, H);
+    if (u->viewport_orthographic[0] != (float)W || u->viewport_orthographic[1] != (float)H)
+        return fail(GS_ERR_INVALID, "uniform viewport %gx%g does not match the %dx%d sample",
+                    (double)u->viewport_orthographic[0], (double)u->viewport_orthographic[1], W, H);
+    const size_t bytes = (size_t)W * H * 16;
+    for (int k = 0; k < 2; ++k) {
+        GS_CHECK(ensure(c, c->taa_color[k], bytes, "TAA colour history"));
+        GS_CHECK(ensure(c, c->taa_xyz[k], bytes, "TAA position history"));
+    }
+    const int rd = c->taa_last, wr = c->have_taa ? 1 - c->taa_last : 0;
+    TaaLaunch t{};
+    t.u = *u;
+    t.width = W;
+    t.height = H;
+    t.color = use_frame_average ? ptr<float4>(c->frame_avg) : ptr<float4>(c->sample);
+    t.depth = ptr<float>(c->depth);
+    t.hist_color = ptr<float4>(c->taa_color[rd]);
+    t.hist_xyz = ptr<float4>(c->taa_xyz[rd]);
+    t.out_color = ptr<float4>(c->taa_color[wr]);
+    t.out_xyz = ptr<float4>(c->taa_xyz[wr]);
+    GS_HIP(launch_taa(t, c->stream));
+    c->taa_last = wr;
+    c->taa_w = W;
+    c->taa_h = H;
+    c->have_taa = true;
+    return ok();
+}
+
 int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
     if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
     GS_CHECK(set_device(c));
@@ -1159,6 +1322,17 @@ int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
             if (!c->have_accum) return fail(GS_ERR_STATE, "no accumulation yet");
             GS_CHECK(copy_image(c, c->accum.p, host, c->acc_w, c->acc_h, 16, bytes, gl_rows(c)));
             return ok();
+        case GS_IMAGE_FRAME_AVG_RGBA32F:
+            if (!c->have_avg) return fail(GS_ERR_STATE, "no frame average yet");
+            GS_CHECK(copy_image(c, c->frame_avg.p, host, c->avg_w, c->avg_h, 16, bytes, gl_rows(c)));
+            return ok();
+        case GS_IMAGE_TAA_RGBA32F:
+        case GS_IMAGE_TAA_XYZ_RGBA32F: {
+            if (!c->have_taa) return fail(GS_ERR_STATE, "no TAA pass yet");
+            const DevBuf& b = which == GS_IMAGE_TAA_RGBA32F ? c->taa_color[c->taa_last] : c->taa_xyz[c->taa_last];
+            GS_CHECK(copy_image(c, b.p, host, c->taa_w, c->taa_h, 16, bytes, gl_rows(c)));
+            return ok();
+        }
         default:
             return fail(GS_ERR_INVALID, "unknown image %d", which);
     }

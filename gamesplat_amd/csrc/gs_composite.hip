@@ -43,7 +43,17 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
     uint32_t cnt = 0u;
     if (q < V) {
         const uint32_t val = a.sorted_ids[V - 1u - q];
-        const uint2 px = a.rect_px[val];
+        uint2 px;
+        if (a.jittered) {
+            const float4 r0 = a.records[(size_t)val * kRecordVec4 + 0];
+            const float4 r1 = a.records[(size_t)val * kRecordVec4 + 1];
+            const float4 r2 = a.records[(size_t)val * kRecordVec4 + 2];
+            const ViewInfo& vi = a.views[val / a.n];
+            px = pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, vi.width, vi.height, a.tight != 0,
+                            a.jitter_x, a.jitter_y);
+        } else {
+            px = a.rect_px[val];
+        }
         uint2 t = make_uint2(0u, 0u);
         if (px.x) {
             const uint32_t T = (uint32_t)a.tile;
@@ -408,8 +418,9 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
             float det = r1.x * r1.z - r1.y * r1.y;
             if (det < 1e-6f) det = 1e-6f;
             const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
-            s_rect[tid] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width, t.height, false);
-            s_geo[tid] = make_float4(r0.x, r0.y, k0, 2.0f * k1);
+            s_rect[tid] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width, t.height, false,
+                                           sl.jitter_x, sl.jitter_y);
+            s_geo[tid] = make_float4(r0.x + sl.jitter_x, r0.y + sl.jitter_y, k0, 2.0f * k1);
             s_par[tid] = make_float4(k2, r0.z, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
                                      __uint_as_float(depth24(r0.w, sl.clip_z_01)));
             s_col[tid] = make_float4(__half2float(__float2half_rn(r2.x)), __half2float(__float2half_rn(r2.y)),

@@ -188,6 +188,12 @@ struct BinLaunch {
     int tile;
     const ViewInfo* views;   // device
     const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
+    // jittered StochasticSplats samples: the rectangle is recomputed from
+    // the record with the quad moved by (jitter_x, jitter_y)
+    const float4* records;
+    int jittered;
+    int tight;               // the cull's tight-rectangle setting
+    float jitter_x, jitter_y;
     uint2* rects;            // per front-to-back position: tile rectangle
     uint32_t* block_counts;  // pairs per 256-block
     uint32_t* block_offsets;
@@ -226,6 +232,7 @@ struct StochasticLaunch {
     float frame_seed;
     float background;
     float clip_z_01;
+    float jitter_x, jitter_y;  // TAA jitter of the quads and fragment centres (pixels)
     float4* sample;          // W x H RGBA32F, framebuffer rows
     float* depth;            // W x H window depth of the kept splat (1 where none)
 };
@@ -263,6 +270,19 @@ hipError_t launch_gps_expand(const GpsLaunch& a, hipStream_t s);
 hipError_t launch_gps_splat(const GpsLaunch& a, hipStream_t s);
 hipError_t launch_gps_resolve(const GpsLaunch& a, hipStream_t s);
 hipError_t launch_accumulate(const float4* sample, float4* accum, uint32_t count, float n, hipStream_t s);
+
+// gs_taa.hip: taa_accum_fs over W x H framebuffer-row images.
+struct TaaLaunch {
+    GsTaaUniforms u;
+    int width, height;
+    const float4* color;       // current colour (frame average or sample)
+    const float* depth;        // current sample depth
+    const float4* hist_color;  // history read buffer
+    const float4* hist_xyz;
+    float4* out_color;         // history write buffer
+    float4* out_xyz;
+};
+hipError_t launch_taa(const TaaLaunch& a, hipStream_t s);
 
 // gs_unet.hip: the denoiser's kernels.  Tensors are NHWC bf16 (raw uint16
 // bits) viewed through a channel stride and a channel offset.

@@ -25,10 +25,13 @@ __device__ __forceinline__ int32_t snap256(float w) {
 
 __device__ __forceinline__ int32_t ceil_div256(int32_t a) { return -((-a) >> 8); }
 
-// Quad edges x0, x1, y0, y1 in 24.8 fixed point window coordinates.
-__device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float ry, float vw, float vh) {
-    float x0 = cx + -1.0f * rx, x1 = cx + 1.0f * rx;
-    float y0 = cy + -1.0f * ry, y1 = cy + 1.0f * ry;
+// Quad edges x0, x1, y0, y1 in 24.8 fixed point window coordinates; the
+// corners are centre + corner * radius + jitter, in that order (TAA jitter
+// of StochasticSplats samples, zero otherwise: adding 0 changes no value).
+__device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float ry, float vw, float vh,
+                                           float jx = 0.0f, float jy = 0.0f) {
+    float x0 = (cx + -1.0f * rx) + jx, x1 = (cx + 1.0f * rx) + jx;
+    float y0 = (cy + -1.0f * ry) + jy, y1 = (cy + 1.0f * ry) + jy;
     float hw = 0.5f * vw, hh = 0.5f * vh;
     int4 q;
     q.x = snap256((2.0f * x0 / vw - 1.0f + 1.0f) * hw);
@@ -51,10 +54,14 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
 // so the box is grown by that relative margin (plus 0.3% and an absolute
 // 1e-5 on c, and 1e-3 px) and is only used for 1 - rho^2 >= 0.01.  Every
 // pixel that can pass the fragment test stays inside, so composites with and
-// without the cut are identical; it only shortens the tile lists.
+// without the cut are identical; it only shortens the tile lists.  A jitter
+// (jx, jy) moves the quad and the fragment centre (cx + jx, cy + jy).
 __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx,
-                                                 float ry, float opacity, int width, int height, bool tight) {
-    int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height);
+                                                 float ry, float opacity, int width, int height, bool tight,
+                                                 float jx = 0.0f, float jy = 0.0f) {
+    int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height, jx, jy);
+    cx = cx + jx;
+    cy = cy + jy;
     int i0 = max(ceil_div256(q.x - 128), 0), i1 = min(ceil_div256(q.y - 128), width);
     int j0 = max(ceil_div256(q.z - 128), 0), j1 = min(ceil_div256(q.w - 128), height);
     if (tight) {
@@ -79,9 +86,10 @@ __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, f
 // no fragment can survive: the fragment test power >= log((1/255) / max(o,
 // 1e-8)) fails everywhere when that log is positive, since power <= 0.
 __device__ __forceinline__ uint2 pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx, float ry,
-                                            float opacity, int width, int height, bool tight) {
+                                            float opacity, int width, int height, bool tight, float jx = 0.0f,
+                                            float jy = 0.0f) {
     if ((1.0f / 255.0f) / maxf(opacity, 1e-8f) > 1.0f) return make_uint2(0u, 0u);
-    int4 r = splat_pixel_rect(cx, cy, sa, sb, sc, rx, ry, opacity, width, height, tight);
+    int4 r = splat_pixel_rect(cx, cy, sa, sb, sc, rx, ry, opacity, width, height, tight, jx, jy);
     if (r.x >= r.y || r.z >= r.w) return make_uint2(0u, 0u);
     return make_uint2((uint32_t)r.x | ((uint32_t)r.y << 16), (uint32_t)r.z | ((uint32_t)r.w << 16));
 }

@@ -183,7 +183,7 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         if (samples > 16u) samples = 16u;
         for (uint32_t i = 0; i < samples; ++i) {
             r->stochastic_frame_seed++;
-            st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg);
+            st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg, nullptr);
             if (st == GS_OK && (r->stochastic_accumulation_enabled || samples > 1u)) {
                 const uint32_t next = r->stochastic_accumulation_enabled ? r->stochastic_sample_count + 1u : i + 1u;
                 st = gs_accumulate(r->ctx, next);

@@ -103,7 +103,10 @@ enum {
     GS_IMAGE_ALPHA_RGBA16F = 0,   /* sorted-alpha composite, 8 B/pixel (half RGBA) */
     GS_IMAGE_GPS_SAMPLE_RGBA32F = 1, /* GPS resolved sample, premultiplied rgb + coverage */
     GS_IMAGE_GPS_DEPTH_R32F = 2,  /* GPS resolved depth                         */
-    GS_IMAGE_ACCUM_RGBA32F = 3    /* running mean of GPS samples               */
+    GS_IMAGE_ACCUM_RGBA32F = 3,   /* running mean of GPS samples               */
+    GS_IMAGE_FRAME_AVG_RGBA32F = 4, /* mean of this frame's StochasticSplats samples */
+    GS_IMAGE_TAA_RGBA32F = 5,     /* TAA history: rgb, alpha = history length  */
+    GS_IMAGE_TAA_XYZ_RGBA32F = 6  /* TAA history: world position, 1            */
 };
 
 /* Buffers that gs_read_buffer can return (element counts in elements). */
@@ -191,17 +194,63 @@ int gs_gps_sample(gs_ctx* ctx, uint32_t frame_seed, uint32_t supersample,
                   int linear_color, float background);
 
 /* Replaces the StochasticSplats sample pass (splat_stochastic_fs,
- * shaders/splat.glsl:107-146; pass src/renderer.cpp:2331-2352) without TAA
- * jitter: every covered fragment that passes the alpha test is kept when
- * hash13(pixel centre, id + frame_seed) < alpha, the kept fragment with the
- * smallest 24-bit depth wins (ties: lowest index, the reference's draw
- * order), and the sample image (GS_IMAGE_GPS_SAMPLE_RGBA32F) receives its
- * colour (half precision, alpha 1) or (background, background, background,
- * 0); the depth image receives its D24 depth / (2^24 - 1) or 1.  Needs
- * gs_cull (single view); sorts if gs_sort has not run.  gs_accumulate
- * averages these samples as it does GPS samples.  TAA reprojection
- * (shaders/accum.glsl:47-179) is not part of this entry point. */
-int gs_stochastic_sample(gs_ctx* ctx, uint32_t frame_seed, float background);
+ * shaders/splat.glsl:107-146; pass src/renderer.cpp:2331-2352): every
+ * covered fragment that passes the alpha test is kept when hash13(pixel
+ * centre, id + frame_seed) < alpha, the kept fragment with the smallest
+ * 24-bit depth wins (ties: lowest index, the reference's draw order), and
+ * the sample image (GS_IMAGE_GPS_SAMPLE_RGBA32F) receives its colour (half
+ * precision, alpha 1) or (background, background, background, 0); the depth
+ * image receives its D24 depth / (2^24 - 1) or 1.  jitter_px (NULL = none)
+ * is draw_world's splat_jitter_px: it moves every quad and fragment centre
+ * by that many pixels (shaders/splat.glsl:61,72; finite, gs_taa_jitter
+ * gives |jitter| <= 0.5).  Needs gs_cull (single
+ * view); sorts if gs_sort has not run.  gs_accumulate averages these
+ * samples as it does GPS samples. */
+int gs_stochastic_sample(gs_ctx* ctx, uint32_t frame_seed, float background, const float* jitter_px);
+
+/* TAA jitter of a StochasticSplats sample (src/renderer.cpp:2335-2340):
+ * index = frame_seed % 1024 + 1, jitter = (halton(index, 2) - 0.5,
+ * halton(index, 3) - 0.5) with halton of src/renderer.cpp:246-255.  Host
+ * only. */
+void gs_taa_jitter(uint32_t frame_seed, float jitter_px[2]);
+
+/* Mean of the StochasticSplats samples of one TAA frame (the avg pass,
+ * src/renderer.cpp:2362-2385): frame_avg = h + (s - h) / (sample_index + 1)
+ * with s the current sample image; sample_index 0 copies the sample. */
+int gs_frame_average(gs_ctx* ctx, uint32_t sample_index);
+
+/* TaaAccumUBO (shaders/accum.glsl:64-75), 208 bytes; matrices column-major. */
+typedef struct GsTaaUniforms {
+    float inv_view[16];
+    float prev_view[16];
+    float viewport_orthographic[4]; /* viewport xy, orthographic now, orthographic before */
+    float focal_clip[4];            /* persp focal, ortho focal, clip_y_sign, clip_z_01 */
+    float prev_focal_clip[4];       /* previous persp focal, ortho focal, clip_y_sign, unused */
+    float proj_z[4];                /* proj[2][2], proj[3][2], unused, unused */
+    float history_valid;
+    float view_changed;
+    float pad[2];
+} GsTaaUniforms;
+
+/* The TAA uniforms renderer_draw_frame builds (src/renderer.cpp:2320-2327,
+ * 2449-2466): inv_view = inverse(cam->view); the previous camera's fields
+ * when prev_cam is non-NULL (history valid), the current camera's
+ * otherwise; view_changed as given.  A view matrix that cannot be inverted
+ * gives inv_view = view and no history, as the reference's fallback, and
+ * GS_ERR_INVALID so the caller can reset its accumulation.  Host only. */
+int gs_taa_uniforms(const GsCameraUniforms* cam, const GsCameraUniforms* prev_cam, int view_changed,
+                    GsTaaUniforms* out);
+
+/* Replaces taa_accum_fs (shaders/accum.glsl:47-179; pass
+ * src/renderer.cpp:2425-2478): per pixel, the current colour (the frame
+ * average when use_frame_average, else the last sample) and the last
+ * sample's depth give a world position; its reprojection into the previous
+ * view samples the colour and position history bilinearly; a valid history
+ * (history_valid, inside the previous view, alpha > 0 and, when the view
+ * changed, within 100 units) is blended with weight n = min(alpha, 128).
+ * The history is double buffered: this call reads one buffer and writes the
+ * other, which GS_IMAGE_TAA_RGBA32F / GS_IMAGE_TAA_XYZ_RGBA32F then read. */
+int gs_taa_accumulate(gs_ctx* ctx, const GsTaaUniforms* uniforms, int use_frame_average);
 
 /* Replaces accum_fs (shaders/accum.glsl:19-45): accum = h + (s - h) / n with
  * n = sample_count (1 = first sample, history ignored). */

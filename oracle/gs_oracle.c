@@ -595,12 +595,12 @@ static int32_t or_snap(float w) {
     return (int32_t)rintf(s);
 }
 
-void or_quad_fixed(const float* rec, float width, float height, int32_t out[4]) {
+void or_quad_fixed(const float* rec, float width, float height, float jitter_x, float jitter_y, int32_t out[4]) {
     float cx = rec[4], cy = rec[5], rx = rec[6], ry = rec[7];
     for (int k = 0; k < 2; ++k) {
         float corner = k ? 1.0f : -1.0f;
-        float px = cx + corner * rx;
-        float py = cy + corner * ry;
+        float px = (cx + corner * rx) + jitter_x;  /* splat_vs: centre + corner * radius + jitter */
+        float py = (cy + corner * ry) + jitter_y;
         float nx = 2.0f * px / width - 1.0f;
         float ny = 2.0f * py / height - 1.0f;
         float wx = (nx + 1.0f) * (0.5f * width);
@@ -633,7 +633,7 @@ void or_composite(const float* projected, const uint32_t* sorted_ids, uint32_t n
             if (id == 0xFFFFFFFFu) continue;
             const float* rec = projected + (size_t)id * 16;
             int32_t q[4];
-            or_quad_fixed(rec, (float)width, (float)height, q);
+            or_quad_fixed(rec, (float)width, (float)height, 0.0f, 0.0f, q);
             int32_t i_lo = or_ceil_div256(q[0] - 128), i_hi = or_ceil_div256(q[1] - 128);
             int32_t y_lo = or_ceil_div256(q[2] - 128), y_hi = or_ceil_div256(q[3] - 128);
             if (i_lo < 0) i_lo = 0;
@@ -868,8 +868,8 @@ uint32_t or_depth24(float ndc_z, float clip_z_01) {
  * colour (bg, bg, bg, 0).  color: W*H*4, depth: W*H D24 depths of the kept
  * splat / (2^24 - 1) (1 where nothing was kept), framebuffer row order. */
 void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,
-                          float clip_z_01, float frame_seed, float background, float* color, float* depth,
-                          int threads) {
+                          float clip_z_01, float frame_seed, float background, float jitter_x, float jitter_y,
+                          float* color, float* depth, int threads) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -887,8 +887,10 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
             const uint32_t id = ids[s];
             if (id == 0xFFFFFFFFu) continue;
             const float* rec = projected + (size_t)id * 16;
+            /* the quad corners and the fragment centre move by the TAA jitter
+             * (shaders/splat.glsl:61,72) */
             int32_t q[4];
-            or_quad_fixed(rec, (float)width, (float)height, q);
+            or_quad_fixed(rec, (float)width, (float)height, jitter_x, jitter_y, q);
             int32_t i_lo = or_ceil_div256(q[0] - 128), i_hi = or_ceil_div256(q[1] - 128);
             int32_t y_lo = or_ceil_div256(q[2] - 128), y_hi = or_ceil_div256(q[3] - 128);
             if (j < y_lo || j >= y_hi) continue;
@@ -896,7 +898,7 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
             if (i_hi > width) i_hi = width;
             const float o = rec[3];
             const float cutoff = or_logf((1.0f / 255.0f) / fmaxf(o, 1e-8f));
-            const float cx = rec[4], cy = rec[5];
+            const float cx = rec[4] + jitter_x, cy = rec[5] + jitter_y;
             const float k0 = rec[8], k1 = rec[9], k2 = rec[10];
             const uint32_t dq = or_depth24(rec[11], clip_z_01);
             const float fy = (float)j + 0.5f;
@@ -920,6 +922,113 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
         free(best);
     }
     (void)threads;
+}
+
+/* halton (src/renderer.cpp:246-255). */
+float or_halton(uint32_t index, uint32_t base) {
+    float f = 1.0f, r = 0.0f;
+    while (index > 0) {
+        f /= (float)base;
+        r += f * (float)(index % base);
+        index /= base;
+    }
+    return r;
+}
+
+/* GLSL mat4 * vec4, column-major, summed left to right. */
+static void or_mat4_vec4(const float* m, const float v[4], float out[4]) {
+    for (int r = 0; r < 4; ++r)
+        out[r] = ((m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1]) + m[2 * 4 + r] * v[2]) + m[3 * 4 + r] * v[3];
+}
+
+static void or_texel(const float* img, int w, int h, int x, int y, float out[4]) {
+    x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+    y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    memcpy(out, img + ((size_t)y * w + x) * 4, 16);
+}
+
+/* sample_history_*_bilinear (shaders/accum.glsl:120-152). */
+static void or_bilinear(const float* img, int w, int h, float u, float v, float out[4]) {
+    float px = u * (float)w - 0.5f, py = v * (float)h - 0.5f;
+    int x0 = (int)floorf(px), y0 = (int)floorf(py);
+    float fx = or_fract(px), fy = or_fract(py);
+    float c00[4], c10[4], c01[4], c11[4];
+    or_texel(img, w, h, x0, y0, c00);
+    or_texel(img, w, h, x0 + 1, y0, c10);
+    or_texel(img, w, h, x0, y0 + 1, c01);
+    or_texel(img, w, h, x0 + 1, y0 + 1, c11);
+    for (int k = 0; k < 4; ++k)
+        out[k] = or_mix(or_mix(c00[k], c10[k], fx), or_mix(c01[k], c11[k], fx), fy);
+}
+
+/* taa_accum_fs (shaders/accum.glsl:47-179) over a W x H frame; images are
+ * RGBA (W*H*4) or R (W*H) floats in framebuffer row order, u = TaaAccumUBO
+ * as 52 floats (inv_view, prev_view, viewport_orthographic, focal_clip,
+ * prev_focal_clip, proj_z, history_valid, view_changed, pad, pad). */
+void or_taa_accumulate(const float* cur_color, const float* cur_depth, const float* hist_color,
+                       const float* hist_xyz, int w, int h, const float* u, float* out_color, float* out_xyz) {
+    const float* inv_view = u;
+    const float* prev_view = u + 16;
+    const float* vo = u + 32;   /* viewport xy, orthographic now, orthographic before */
+    const float* fc = u + 36;   /* persp focal, ortho focal, clip_y_sign, clip_z_01 */
+    const float* pfc = u + 40;  /* previous persp focal, ortho focal, clip_y_sign */
+    const float* pz = u + 44;   /* proj[2][2], proj[3][2] */
+    const float history_valid = u[48], view_changed = u[49];
+    for (int j = 0; j < h; ++j)
+        for (int i = 0; i < w; ++i) {
+            const size_t p = (size_t)j * w + i;
+            const float uvx = ((float)i + 0.5f) / (float)w, uvy = ((float)j + 0.5f) / (float)h;
+            const float* cc = cur_color + p * 4;
+            const float depth = cur_depth[p];
+            /* reconstruct_view_pos */
+            const float z_ndc = depth * 2.0f - 1.0f;
+            const float view_z = -pz[1] / (z_ndc + pz[0]);
+            float pxx = uvx * vo[0], pxy = uvy * vo[1];
+            const float unflip = (1.0f - fc[2]) * 0.5f;
+            pxy = or_mix(pxy, vo[1] - pxy, unflip);
+            const float cx = pxx - vo[0] * 0.5f, cy = pxy - vo[1] * 0.5f;
+            const float denom = or_mix(fc[0] / view_z, -fc[1], vo[2]);
+            const float view[4] = {cx / denom, cy / denom, view_z, 1.0f};
+            float world[4];
+            or_mat4_vec4(inv_view, view, world);
+            /* project_prev_uv */
+            const float wv[4] = {world[0], world[1], world[2], 1.0f};
+            float pv[4];
+            or_mat4_vec4(prev_view, wv, pv);
+            float pu = -1.0f, pvv = -1.0f;
+            if (!(pv[2] >= -0.001f)) {
+                const float t = vo[3];
+                const float ppx = pfc[0] * pv[0] / pv[2] + vo[0] * 0.5f;
+                const float ppy = vo[1] * 0.5f + pfc[0] * pv[1] / pv[2];
+                const float opx = -pfc[1] * pv[0] + vo[0] * 0.5f;
+                const float opy = vo[1] * 0.5f - pfc[1] * pv[1];
+                float qx = or_mix(ppx, opx, t), qy = or_mix(ppy, opy, t);
+                const float y_flip = (1.0f - pfc[2]) * 0.5f;
+                qy = or_mix(qy, vo[1] - qy, y_flip);
+                pu = qx / vo[0];
+                pvv = qy / vo[1];
+            }
+            int valid = history_valid != 0.0f && pu >= 0.0f && pu <= 1.0f && pvv >= 0.0f && pvv <= 1.0f;
+            float hc[4] = {0, 0, 0, 0}, hx[4] = {0, 0, 0, 0};
+            if (valid) {
+                or_bilinear(hist_color, w, h, pu, pvv, hc);
+                or_bilinear(hist_xyz, w, h, pu, pvv, hx);
+                const float dx = world[0] - hx[0], dy = world[1] - hx[1], dz = world[2] - hx[2];
+                valid = hc[3] > 0.0f && (view_changed == 0.0f || sqrtf((dx * dx + dy * dy) + dz * dz) < 100.0f);
+            }
+            float* oc = out_color + p * 4;
+            float* ox = out_xyz + p * 4;
+            if (valid) {
+                const float n = fminf(hc[3], 128.0f);
+                for (int k = 0; k < 3; ++k) oc[k] = (hc[k] * n + cc[k]) / (n + 1.0f);
+                oc[3] = fminf(n + 1.0f, 128.0f);
+                for (int k = 0; k < 3; ++k) ox[k] = (hx[k] * n + world[k]) / (n + 1.0f);
+                ox[3] = 1.0f;
+            } else {
+                oc[0] = cc[0]; oc[1] = cc[1]; oc[2] = cc[2]; oc[3] = 1.0f;
+                ox[0] = world[0]; ox[1] = world[1]; ox[2] = world[2]; ox[3] = 1.0f;
+            }
+        }
 }
 
 void or_accumulate(const float* history, const float* sample, float sample_count,

@@ -65,7 +65,7 @@ void or_composite(const float* projected, const uint32_t* sorted_ids, uint32_t n
 
 /* Quad coverage of one projected splat in 24.8 fixed point window
  * coordinates: x0,x1,y0,y1; pixel i is covered iff x0 <= 256 i + 128 < x1. */
-void or_quad_fixed(const float* rec, float width, float height, int32_t out[4]);
+void or_quad_fixed(const float* rec, float width, float height, float jitter_x, float jitter_y, int32_t out[4]);
 
 /* GPS expand (shaders/gps.glsl:120-174); items are reserved in index order.
  * prepared: 12 words per Gaussian (mean_basis, sampling, packed) indexed by
@@ -90,8 +90,13 @@ uint32_t or_depth24(float ndc_z, float clip_z_01);
  * stored as half floats; clear (bg, bg, bg, 0).  color W*H*4, depth W*H,
  * framebuffer row order. */
 void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,
-                          float clip_z_01, float frame_seed, float background, float* color, float* depth,
-                          int threads);
+                          float clip_z_01, float frame_seed, float background, float jitter_x, float jitter_y,
+                          float* color, float* depth, int threads);
+/* halton (src/renderer.cpp:246-255). */
+float or_halton(uint32_t index, uint32_t base);
+/* taa_accum_fs (shaders/accum.glsl:47-179); u = TaaAccumUBO as 52 floats. */
+void or_taa_accumulate(const float* cur_color, const float* cur_depth, const float* hist_color,
+                       const float* hist_xyz, int w, int h, const float* u, float* out_color, float* out_xyz);
 /* accum_fs (shaders/accum.glsl:39-44), elementwise over count floats. */
 void or_accumulate(const float* history, const float* sample, float sample_count,
                    float* out, uint32_t count);

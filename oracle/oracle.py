@@ -38,7 +38,7 @@ _SIGS = {
     "or_radix_sort": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "or_composite": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_int,
                             C.c_void_p, C.c_int]),
-    "or_quad_fixed": (None, [C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
+    "or_quad_fixed": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "or_gps_expand": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_int,
                              C.c_void_p, C.c_void_p, C.c_void_p]),
     "or_gps_splat": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_float,
@@ -49,7 +49,10 @@ _SIGS = {
     "or_hash13": (C.c_float, [C.c_float, C.c_float, C.c_float]),
     "or_depth24": (C.c_uint32, [C.c_float, C.c_float]),
     "or_stochastic_splats": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_float,
-                                    C.c_float, C.c_void_p, C.c_void_p, C.c_int]),
+                                    C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_int]),
+    "or_halton": (C.c_float, [C.c_uint32, C.c_uint32]),
+    "or_taa_accumulate": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]),
 }
 
 
@@ -118,11 +121,11 @@ def composite(projected: np.ndarray, sorted_ids: np.ndarray, width: int, height:
     return out
 
 
-def quad_fixed(rec: np.ndarray, width: float, height: float) -> np.ndarray:
+def quad_fixed(rec: np.ndarray, width: float, height: float, jitter=(0.0, 0.0)) -> np.ndarray:
     lib = load()
     r = np.ascontiguousarray(rec, dtype=np.float32)
     out = np.empty(4, np.int32)
-    lib.or_quad_fixed(_p(r), width, height, _p(out))
+    lib.or_quad_fixed(_p(r), width, height, float(jitter[0]), float(jitter[1]), _p(out))
     return out
 
 
@@ -156,7 +159,8 @@ def gps_resolve(cells, width, height, ss=1, background=0.0, linear_color=False, 
     return color, depth
 
 
-def stochastic_splats(projected, ids, width, height, frame_seed, clip_z_01=0.0, background=0.0, threads=None):
+def stochastic_splats(projected, ids, width, height, frame_seed, clip_z_01=0.0, background=0.0, jitter=(0.0, 0.0),
+                      threads=None):
     """StochasticSplats sample: (color (H, W, 4), depth (H, W)) in framebuffer row order."""
     lib = load()
     p = np.ascontiguousarray(projected, dtype=np.float32)
@@ -164,8 +168,25 @@ def stochastic_splats(projected, ids, width, height, frame_seed, clip_z_01=0.0, 
     color = np.empty((height, width, 4), np.float32)
     depth = np.empty((height, width), np.float32)
     lib.or_stochastic_splats(_p(p), _p(i), i.size, width, height, clip_z_01, float(frame_seed), background,
-                             _p(color), _p(depth), threads_default() if threads is None else threads)
+                             float(jitter[0]), float(jitter[1]), _p(color), _p(depth),
+                             threads_default() if threads is None else threads)
     return color, depth
+
+
+def taa_accumulate(cur_color, cur_depth, hist_color, hist_xyz, uniforms):
+    """taa_accum_fs over framebuffer-row images; uniforms: 52 floats (TaaAccumUBO).  Returns (color, xyz)."""
+    lib = load()
+    cc = np.ascontiguousarray(cur_color, dtype=np.float32)
+    cd = np.ascontiguousarray(cur_depth, dtype=np.float32)
+    hc = np.ascontiguousarray(hist_color, dtype=np.float32)
+    hx = np.ascontiguousarray(hist_xyz, dtype=np.float32)
+    u = np.ascontiguousarray(uniforms, dtype=np.float32)
+    assert u.size == 52
+    h, w = cd.shape
+    oc = np.empty((h, w, 4), np.float32)
+    ox = np.empty((h, w, 4), np.float32)
+    lib.or_taa_accumulate(_p(cc), _p(cd), _p(hc), _p(hx), w, h, _p(u), _p(oc), _p(ox))
+    return oc, ox
 
 
 def accumulate(history, sample, n):
