@@ -61,7 +61,7 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     # headless renderer driver (C++ host code over the C ABI)
     rsrc = sorted((PKG / "renderer").glob("*.cpp"))
-    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-o", str(HEADLESS), *map(str, rsrc), "-L", str(PKG),
+    cmd = [HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-o", str(HEADLESS), *map(str, rsrc), "-L", str(PKG),
            "-lgamesplat_hip", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -10,6 +10,7 @@
 //   ply=<path>                 3DGS binary PLY; otherwise a synthetic scene:
 //   gaussians=[100000] clusters=[16] seed=[0x5EED0001] opacity_mean=[0]
 //   width=[1280] height=[720] render_mode=[alpha|gps|stochastic] stochastic_spp=[1]
+//   taa=[0] taa_samples=[4]    StochasticSplats with TAA (samples per frame)
 //   view_seed=[seed] radius=[15] orbit_deg=[0]   camera on a seeded direction
 //   gps_ss=[1] gps_budget_m=[16] gps_accumulation=[1] sh_degree=[3]
 //   gpu_budget_mb=[0] diagnostics=[0]
@@ -168,7 +169,8 @@ int main(int argc, char** argv) {
     r.splat_render_mode = mode == "gps"          ? SplatRenderMode::GaussianPointSplatting
                           : mode == "stochastic" ? SplatRenderMode::StochasticSplats
                                                  : SplatRenderMode::AlphaBlendSorted;
-    r.stochastic_taa_enabled = false;  // plain accumulation for StochasticSplats
+    r.stochastic_taa_enabled = argi("taa", 0) != 0;
+    r.stochastic_taa_current_samples = (uint32_t)argi("taa_samples", 4);
     r.stochastic_samples_per_frame = (uint32_t)argi("stochastic_spp", 1);
     r.black_background = true;
     r.sh_degree = (int)argi("sh_degree", 3);

@@ -112,6 +112,85 @@ void renderer_reset_stochastic_accumulation(Renderer* r) {
     r->stochastic_sample_count = 0;
     r->stochastic_accum_write_index = 0;
     r->stochastic_prev_cam_valid = false;
+    r->stochastic_prev_effect_valid = false;
+    r->stochastic_taa_prev_view_proj_valid = false;
+}
+
+static uint32_t clamp_samples(uint32_t n) { return n < 1u ? 1u : (n > 16u ? 16u : n); }
+
+// StochasticSplats (src/renderer.cpp:2290-2480): per frame, render_samples
+// samples with consecutive frame seeds; with TAA each is Halton-jittered,
+// they are averaged, and the average is reprojected into the history; without
+// TAA they fold into the running mean, which a camera change resets.
+static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEffectParams* splat_effect) {
+    const GsCameraUniforms* gcam = reinterpret_cast<const GsCameraUniforms*>(cam);
+    const float bg = r->black_background ? 0.0f : 0.1f;
+    // new target sizes start a new accumulation (renderer_ensure_stochastic_targets)
+    if (r->stochastic_width != (uint32_t)cam->viewport[0] || r->stochastic_height != (uint32_t)cam->viewport[1])
+        renderer_reset_stochastic_accumulation(r);
+    SplatEffectParams no_effect;
+    memset(&no_effect, 0, sizeof(no_effect));
+    const SplatEffectParams* effect = splat_effect ? splat_effect : &no_effect;
+    const CameraUniforms prev_cam = r->stochastic_prev_cam;
+    bool prev_cam_valid = r->stochastic_prev_cam_valid;
+    const bool cam_changed = r->stochastic_prev_cam_valid && memcmp(&r->stochastic_prev_cam, cam, sizeof(*cam)) != 0;
+    const bool effect_changed =
+        r->stochastic_prev_effect_valid && memcmp(&r->stochastic_prev_effect, effect, sizeof(*effect)) != 0;
+    if ((!r->stochastic_taa_enabled && cam_changed) || effect_changed) {
+        renderer_reset_stochastic_accumulation(r);
+        prev_cam_valid = false;
+    }
+    r->stochastic_prev_cam = *cam;
+    r->stochastic_prev_effect = *effect;
+    r->stochastic_prev_cam_valid = true;
+    r->stochastic_prev_effect_valid = true;
+
+    const bool use_taa = r->stochastic_taa_enabled;
+    const uint32_t samples = clamp_samples(use_taa ? r->stochastic_taa_current_samples
+                                                   : r->stochastic_samples_per_frame);
+    GsTaaUniforms taa;
+    if (use_taa) {
+        const bool has_history = r->stochastic_taa_prev_view_proj_valid && prev_cam_valid;
+        const GsCameraUniforms* prev = has_history ? reinterpret_cast<const GsCameraUniforms*>(&prev_cam) : nullptr;
+        if (gs_taa_uniforms(gcam, prev, cam_changed ? 1 : 0, &taa) != GS_OK) {
+            // a view without an inverse: the uniforms fall back to no history
+            renderer_reset_stochastic_accumulation(r);
+        }
+    }
+    int st = GS_OK;
+    for (uint32_t i = 0; i < samples && st == GS_OK; ++i) {
+        r->stochastic_frame_seed++;
+        float jitter[2] = {0.0f, 0.0f};
+        if (use_taa) gs_taa_jitter(r->stochastic_frame_seed, jitter);
+        st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg, jitter);
+        if (st != GS_OK) break;
+        if (use_taa && samples > 1u) {
+            st = gs_frame_average(r->ctx, i);
+        } else if (!use_taa && (r->stochastic_accumulation_enabled || samples > 1u)) {
+            const uint32_t next = r->stochastic_accumulation_enabled ? r->stochastic_sample_count + 1u : i + 1u;
+            st = gs_accumulate(r->ctx, next);
+            if (st == GS_OK && r->stochastic_accumulation_enabled) {
+                r->stochastic_sample_count = next;
+                r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
+            }
+        }
+    }
+    if (st == GS_OK && use_taa) {
+        st = gs_taa_accumulate(r->ctx, &taa, samples > 1u ? 1 : 0);
+        if (st == GS_OK) {
+            r->stochastic_taa_prev_view_proj_valid = true;
+            r->stochastic_sample_count++;
+            r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
+        }
+    }
+    if (st != GS_OK) return st;
+    r->stochastic_width = (uint32_t)cam->viewport[0];
+    r->stochastic_height = (uint32_t)cam->viewport[1];
+    r->stochastic_display_image =
+        use_taa ? GS_IMAGE_TAA_RGBA32F
+                : (r->stochastic_accumulation_enabled || samples > 1u ? GS_IMAGE_ACCUM_RGBA32F
+                                                                      : GS_IMAGE_GPS_SAMPLE_RGBA32F);
+    return GS_OK;
 }
 
 void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms* cam, const OverlayParams*,
@@ -128,12 +207,6 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     }
     const bool gps = r->splat_render_mode == SplatRenderMode::GaussianPointSplatting;
     const bool stochastic = r->splat_render_mode == SplatRenderMode::StochasticSplats;
-    if (stochastic && r->stochastic_taa_enabled) {
-        fprintf(stderr, "[renderer] StochasticSplats with TAA is not supported by this build; "
-                        "set stochastic_taa_enabled = false for plain accumulation\n");
-        r->last_frame_status = GS_ERR_INVALID;
-        return;
-    }
     if (scene->gaussian_count == 0 || r->gaussian_count == 0) {
         scene->visible_count = 0;
         return;
@@ -170,39 +243,19 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         return;
     }
 
+    if (stochastic) {
+        st = draw_stochastic(r, cam, splat_effect);
+        if (st != GS_OK) {
+            log_error("stochastic splats", st);
+            r->last_frame_status = st;
+        }
+        return;
+    }
+
     if (r->stochastic_prev_cam_valid && memcmp(&r->stochastic_prev_cam, cam, sizeof(*cam)) != 0)
         renderer_reset_stochastic_accumulation(r);
     memcpy(&r->stochastic_prev_cam, cam, sizeof(*cam));
     r->stochastic_prev_cam_valid = true;
-
-    if (stochastic) {
-        // StochasticSplats without TAA (src/renderer.cpp:2331-2420): samples_per_frame
-        // samples, each with the next frame seed, folded into the running mean
-        const float bg = r->black_background ? 0.0f : 0.1f;
-        uint32_t samples = r->stochastic_samples_per_frame < 1u ? 1u : r->stochastic_samples_per_frame;
-        if (samples > 16u) samples = 16u;
-        for (uint32_t i = 0; i < samples; ++i) {
-            r->stochastic_frame_seed++;
-            st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg, nullptr);
-            if (st == GS_OK && (r->stochastic_accumulation_enabled || samples > 1u)) {
-                const uint32_t next = r->stochastic_accumulation_enabled ? r->stochastic_sample_count + 1u : i + 1u;
-                st = gs_accumulate(r->ctx, next);
-                if (st == GS_OK && r->stochastic_accumulation_enabled) {
-                    r->stochastic_sample_count = next;
-                    r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
-                }
-            }
-            if (st != GS_OK) {
-                log_error("stochastic splats", st);
-                r->last_frame_status = st;
-                return;
-            }
-        }
-        r->stochastic_width = (uint32_t)cam->viewport[0];
-        r->stochastic_height = (uint32_t)cam->viewport[1];
-        r->stochastic_display_accum = r->stochastic_accumulation_enabled || samples > 1u;
-        return;
-    }
     r->stochastic_frame_seed++;
 
     uint32_t ss = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
@@ -260,9 +313,8 @@ bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
     }
     size_t px = (size_t)r->stochastic_width * r->stochastic_height;
     if (floats != px * 4) return false;
-    const bool accum = r->splat_render_mode == SplatRenderMode::StochasticSplats ? r->stochastic_display_accum
-                                                                               : r->stochastic_accumulation_enabled;
-    int which = accum ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
+    int which = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
+    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) which = r->stochastic_display_image;
     int st = gs_read_image(r->ctx, which, out, floats * 4);
     if (st != GS_OK) {
         log_error("read stochastic image", st);

@@ -122,7 +122,7 @@ struct Renderer {
     GaussianPointSplatGpu gps_gpu;
     uint32_t stochastic_width;
     uint32_t stochastic_height;
-    bool stochastic_display_accum;   // StochasticSplats: the running mean (not the last sample) is shown
+    int stochastic_display_image;    // StochasticSplats: the GS_IMAGE_* shown (TAA history, running mean or sample)
     uint32_t stochastic_sample_count;
     uint32_t stochastic_samples_per_frame;
     uint32_t stochastic_taa_current_samples;
@@ -131,7 +131,10 @@ struct Renderer {
     uint32_t stochastic_frame_seed;
     uint32_t stochastic_accum_write_index;
     CameraUniforms stochastic_prev_cam;
+    SplatEffectParams stochastic_prev_effect;
     bool stochastic_prev_cam_valid;
+    bool stochastic_prev_effect_valid;
+    bool stochastic_taa_prev_view_proj_valid;
     bool stochastic_accumulation_enabled;
     bool stochastic_taa_enabled;
     int last_frame_status;              // gs status of the last renderer_draw_frame (0 = ok)

@@ -90,11 +90,11 @@ def test_frames_are_deterministic(gs, c1_scene):
 
 def test_headless_driver_runs(tmp_path):
     exe = ROOT / "gamesplat_amd" / "gs_headless"
-    for mode in ("alpha", "gps", "stochastic"):
-        cap = tmp_path / f"{mode}.f32"
+    for mode, extra in (("alpha", []), ("gps", []), ("stochastic", []), ("stochastic", ["taa=1", "orbit_deg=1"])):
+        cap = tmp_path / f"{mode}{len(extra)}.f32"
         r = subprocess.run([str(exe), "gaussians=20000", "width=320", "height=240", f"render_mode={mode}",
-                            "warmup=1", "frames=4", f"linear_capture={cap}"], capture_output=True, text=True,
-                           timeout=120)
+                            "warmup=1", "frames=4", f"linear_capture={cap}", *extra], capture_output=True,
+                           text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         line = [l for l in r.stdout.splitlines() if l.startswith("RESULT ")]
         assert line and '"frames":4' in line[0]
