@@ -6,7 +6,8 @@ One step = one sorted-alpha frame (cull + SH-3 + depth sort + tile composite
 into RGBA16F) of a resident synthetic scene at the workload's size, inputs
 already in HBM.  Default workload: 6M Gaussians, SH degree 3, 1920x1080.
 Prints one JSON line.  Extra fields report the stochastic (GPS, 1 spp, 16 Mi
-work budget) frame rate on the same scene, per-stage times from HIP events on
+work budget) frame rate on the same scene, StochasticSplats with TAA (4
+jittered samples per frame), per-stage times from HIP events on
 the context's stream, and the batched multi-view workload (3M Gaussians, 64
 cameras of mixed sizes) as separate, batched, graph-replayed and autotuned
 frames.
@@ -219,6 +220,45 @@ def denoised_stochastic(ctx, cam, steps: int, warmup: int, converged: int) -> di
     return out
 
 
+def stochastic_taa(ctx, cam, steps: int, warmup: int, samples: int = 4) -> dict:
+    """StochasticSplats with TAA, the reference renderer's default stochastic
+    mode: per frame a cull, `samples` Halton-jittered samples averaged, and
+    the TAA pass over the history (same camera, so the history stays valid)."""
+    import gamesplat_amd as gs
+
+    seed = [0]
+
+    def frame(first: bool):
+        u = gs.taa_uniforms(cam, None if first else cam, False)
+        ctx.cull(cam)
+        for i in range(samples):
+            seed[0] += 1
+            ctx.stochastic_sample(seed[0], 0.0, gs.taa_jitter(seed[0]))
+            ctx.frame_average(i)
+        ctx.taa_accumulate(u, use_frame_average=samples > 1)
+
+    for k in range(warmup):
+        frame(k == 0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame(False)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    ft = sorted(frame_times(ctx, lambda k: frame(False), max(50, steps)))
+    ctx.set_profiling(True)
+    ctx.reset_stage_times()
+    for _ in range(steps):
+        frame(False)
+    st = ctx.stage_times()
+    ctx.set_profiling(False)
+    return {"samples_per_frame": samples, "frames_per_s": round(1000.0 / ms, 2), "ms_per_frame": round(ms, 4),
+            "frame_ms_p50": round(ft[len(ft) // 2], 4), "frame_ms_p95": round(ft[int(len(ft) * 0.95)], 4),
+            # the composite stage holds the stochastic sample kernel (one per sample)
+            "stage_ms": {k: round(v[0] / v[1], 4) for k, v in st.items() if v[1]},
+            "stage_launches_per_frame": {k: round(v[1] / steps, 2) for k, v in st.items() if v[1]}}
+
+
 def multiview(steps: int, warmup: int) -> dict:
     """Batched multi-view: one resident 3M-Gaussian SH-3 scene, 64 cameras of
     mixed sizes (16 each of 256x256, 640x480, 1280x720, 1920x1080) at radius
@@ -353,6 +393,8 @@ def main():
         if not args.no_denoiser:
             den = denoised_stochastic(ctx, cam, args.steps, args.warmup, args.converged_frames)
             print(f"# denoised: {json.dumps(den)}", file=sys.stderr, flush=True)
+        taa = stochastic_taa(ctx, cam, args.steps, args.warmup)
+        print(f"# stochastic splats + TAA: {json.dumps(taa)}", file=sys.stderr, flush=True)
         stoch = {
             "frames_per_s": round(1000.0 / sms, 2),
             "ms_per_frame": round(sms, 4),
@@ -365,6 +407,7 @@ def main():
                                              (sstages["gps_splat"][0] / sstages["gps_splat"][1] * 1e-3), 1)
             if sstages["gps_splat"][1] else None,
             "denoised": None if args.no_denoiser else den,
+            "stochastic_splats_taa": taa,
         }
 
     # roofline of the dominant kernel stage (algorithmic bytes per launch)

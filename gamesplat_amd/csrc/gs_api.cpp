@@ -110,6 +110,15 @@ struct gs_ctx {
     uint32_t stoch_seed = 0;
     float stoch_bg = 0.0f;
     float stoch_jitter[2] = {0.0f, 0.0f};
+    // Tile lists of StochasticSplats samples depend on the cull and sort
+    // only, so the samples of one cull share them.  bin_epoch advances with
+    // every launch that may rewrite the tile buffers; the lists are reused
+    // while it equals stoch_bins_epoch.  stoch_bins_mode: 0 the cull's
+    // rectangles (unjittered samples), 1 grown by one pixel (any jitter up
+    // to half a pixel), 2 moved by stoch_bins_jitter exactly.
+    uint64_t bin_epoch = 1, stoch_bins_epoch = 0;
+    int stoch_bins_mode = 0;
+    float stoch_bins_jitter[2] = {0.0f, 0.0f};
     int img_w = 0, img_h = 0;
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
@@ -570,6 +579,7 @@ namespace {
 int grow_tile_buffers(gs_ctx* c, uint32_t entries) {
     if (entries > 0x7FFFFFFFu) return fail(GS_ERR_LIMIT, "%u tile entries exceed the 2^31 limit", entries);
     if (entries <= c->dup_cap) return GS_OK;
+    c->bin_epoch++;
     for (int k = 0; k < 3; ++k) {
         GS_CHECK(ensure(c, c->dup_keys[k], (size_t)entries * 4, "tile keys"));
         GS_CHECK(ensure(c, c->dup_vals[k], (size_t)entries * 4, "tile values"));
@@ -679,6 +689,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
 }
 
 int stage_cull(gs_ctx* c, const FramePlan& p) {
+    c->bin_epoch++;
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
     GS_HIP(launch_cull(p.cull, c->stream));
@@ -691,6 +702,7 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
 }
 
 int stage_sort(gs_ctx* c, const FramePlan& p, int* slot) {
+    c->bin_epoch++;
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_SORT, &t));
     GS_HIP(launch_radix_sort(p.depth, slot, c->stream));
@@ -700,6 +712,7 @@ int stage_sort(gs_ctx* c, const FramePlan& p, int* slot) {
 
 // (tile, splat) pairs, their stable sort by tile and the tile ranges.
 int stage_bin(gs_ctx* c, const FramePlan& p) {
+    c->bin_epoch++;
     GS_HIP(launch_fill_u32(const_cast<uint32_t*>(reinterpret_cast<const uint32_t*>(p.comp.ranges)), 0u,
                            p.range_words, c->stream));
     if (p.comp.persistent_blocks) GS_HIP(launch_fill_u32(p.comp.queue, 0u, 8, c->stream));
@@ -745,31 +758,46 @@ int after_composite(gs_ctx* c, int tile) {
     return GS_OK;
 }
 
-// Bin + StochasticSplats sample into the sample / depth images.
-int enqueue_stochastic(gs_ctx* c) {
+// StochasticSplats sample into the sample / depth images, binning first
+// unless the tile lists of this cull can be reused (*binned tells which).
+int enqueue_stochastic(gs_ctx* c, bool* binned) {
     const int W = (int)c->cam.viewport[0], H = (int)c->cam.viewport[1];
-    GS_CHECK(upload_views(c, c->host_cams.data(), 16));
-    GS_CHECK(prepare_composite(c, 16));
+    const float jx = c->stoch_jitter[0], jy = c->stoch_jitter[1];
+    const int mode = (jx == 0.0f && jy == 0.0f) ? 0 : (std::fabs(jx) <= 0.5f && std::fabs(jy) <= 0.5f ? 1 : 2);
+    const bool lists_ok = c->stoch_bins_epoch == c->bin_epoch && !c->dup_pending && c->last_dup <= c->dup_cap;
+    const bool reuse = lists_ok && (c->stoch_bins_mode == mode || (c->stoch_bins_mode == 1 && mode == 0) ||
+                                    (mode == 2 && c->stoch_bins_jitter[0] == jx && c->stoch_bins_jitter[1] == jy));
+    *binned = !reuse;
+    if (!reuse) {
+        GS_CHECK(upload_views(c, c->host_cams.data(), 16));
+        GS_CHECK(prepare_composite(c, 16));
+        make_composite(c, 16, &c->plan);
+        if (mode) {
+            // the tile rectangles come from the records: grown by a pixel,
+            // which holds every jitter up to half a pixel, or moved exactly
+            BinLaunch& b = c->plan.bin;
+            b.jittered = 1;
+            b.records = ptr<float4>(c->records);
+            b.tight = c->cull_tight;
+            b.pad = mode == 1 ? 1 : 0;
+            b.jitter_x = mode == 2 ? jx : 0.0f;
+            b.jitter_y = mode == 2 ? jy : 0.0f;
+        }
+        GS_CHECK(stage_bin(c, c->plan));
+        c->stoch_bins_epoch = c->bin_epoch;
+        c->stoch_bins_mode = mode;
+        c->stoch_bins_jitter[0] = jx;
+        c->stoch_bins_jitter[1] = jy;
+    }
     GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "sample image"));
     GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "sample depth"));
-    make_composite(c, 16, &c->plan);
-    if (c->stoch_jitter[0] != 0.0f || c->stoch_jitter[1] != 0.0f) {
-        // jittered quads: the tile rectangles come from the moved quads
-        BinLaunch& b = c->plan.bin;
-        b.jittered = 1;
-        b.records = ptr<float4>(c->records);
-        b.tight = c->cull_tight;
-        b.jitter_x = c->stoch_jitter[0];
-        b.jitter_y = c->stoch_jitter[1];
-    }
-    GS_CHECK(stage_bin(c, c->plan));
     StochasticLaunch sl{};
     sl.tiles = c->plan.comp;
     sl.frame_seed = (float)c->stoch_seed;
     sl.background = c->stoch_bg;
     sl.clip_z_01 = c->cam.clip_z_01;
-    sl.jitter_x = c->stoch_jitter[0];
-    sl.jitter_y = c->stoch_jitter[1];
+    sl.jitter_x = jx;
+    sl.jitter_y = jy;
     sl.sample = ptr<float4>(c->sample);
     sl.depth = ptr<float>(c->depth);
     TimedLaunch t{};
@@ -783,8 +811,9 @@ int enqueue_stochastic(gs_ctx* c) {
 
 int enqueue_composite(gs_ctx* c, int tile) {
     if (c->composite_kind == 1) {
-        GS_CHECK(enqueue_stochastic(c));
-        return after_composite(c, 16);
+        bool binned = false;
+        GS_CHECK(enqueue_stochastic(c, &binned));
+        return binned ? after_composite(c, 16) : GS_OK;
     }
     GS_CHECK(upload_views(c, c->host_cams.data(), tile));
     GS_CHECK(prepare_composite(c, tile));
@@ -820,6 +849,7 @@ int settle_composite(gs_ctx* c, bool wait, bool rerun) {
 // Replays the captured frame when the plan is byte-identical to the one it
 // was captured from, otherwise captures the plan's launches first.
 int launch_graph(gs_ctx* c, const FramePlan& p) {
+    c->bin_epoch++;
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&p);
     if (!c->graph_exec || c->graph_key.size() != sizeof(FramePlan) ||
         memcmp(c->graph_key.data(), b, sizeof(FramePlan)) != 0) {

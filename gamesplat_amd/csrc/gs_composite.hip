@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
             const float4 r2 = a.records[(size_t)val * kRecordVec4 + 2];
             const ViewInfo& vi = a.views[val / a.n];
             px = pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, vi.width, vi.height, a.tight != 0,
-                            a.jitter_x, a.jitter_y);
+                            a.jitter_x, a.jitter_y, a.pad);
         } else {
             px = a.rect_px[val];
         }
@@ -447,9 +447,26 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
                 const float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + par.x * dy * dy);
                 const bool covered = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & !(power > 0.0f) &
                                      !(power < par.z);
-                const float alpha = minf(par.y * expf_(power), 0.99f);
-                const bool win = covered && hash13(fx, fy, (float)id + sl.frame_seed) < alpha &&
-                                 (q < best_q || (q == best_q && id < best_id));
+                const bool nearer = q < best_q || (q == best_q && id < best_id);
+                bool keep = false;
+                if (covered && nearer) {
+                    // keep iff h < alpha = min(o exp(power), 0.99).  The
+                    // hardware exp2 brackets alpha within 2e-5 relative: its
+                    // 1-ulp error, the rounding of power * log2(e) (1.2e-7
+                    // |power| relative) and the float rounding of exp stay
+                    // below 5e-6 while power >= -40, and the fragment test
+                    // keeps power >= log(1 / (255 o)) >= -5.6 for o <= 1.
+                    // Only a hash inside the bracket needs the deterministic
+                    // exp, so keep decisions match the CPU restatement.
+                    const float h = hash13(fx, fy, (float)id + sl.frame_seed);
+                    const float oe = par.y * __builtin_amdgcn_exp2f(power * 1.44269504f);
+                    if (h < minf(oe * (1.0f - 2e-5f), 0.99f)) {
+                        keep = true;
+                    } else if (h < minf(oe * (1.0f + 2e-5f), 0.99f)) {
+                        keep = h < minf(par.y * expf_(power), 0.99f);
+                    }
+                }
+                const bool win = keep;
                 if (win) {
                     best_q = q;
                     best_id = id;

@@ -189,10 +189,12 @@ struct BinLaunch {
     const ViewInfo* views;   // device
     const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
     // jittered StochasticSplats samples: the rectangle is recomputed from
-    // the record with the quad moved by (jitter_x, jitter_y)
+    // the record with the quad moved by (jitter_x, jitter_y) and grown by
+    // pad pixels on each side
     const float4* records;
     int jittered;
     int tight;               // the cull's tight-rectangle setting
+    int pad;
     float jitter_x, jitter_y;
     uint2* rects;            // per front-to-back position: tile rectangle
     uint32_t* block_counts;  // pairs per 256-block

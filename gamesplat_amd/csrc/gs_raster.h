@@ -55,15 +55,18 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
 // 1e-5 on c, and 1e-3 px) and is only used for 1 - rho^2 >= 0.01.  Every
 // pixel that can pass the fragment test stays inside, so composites with and
 // without the cut are identical; it only shortens the tile lists.  A jitter
-// (jx, jy) moves the quad and the fragment centre (cx + jx, cy + jy).
+// (jx, jy) moves the quad and the fragment centre (cx + jx, cy + jy).  pad
+// grows the rectangle by that many pixels on each side before it is cut to
+// the image: with pad 1 it holds the rectangle of every jitter up to half a
+// pixel (each bound moves by less than one pixel, rounding included).
 __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx,
                                                  float ry, float opacity, int width, int height, bool tight,
-                                                 float jx = 0.0f, float jy = 0.0f) {
+                                                 float jx = 0.0f, float jy = 0.0f, int pad = 0) {
     int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height, jx, jy);
     cx = cx + jx;
     cy = cy + jy;
-    int i0 = max(ceil_div256(q.x - 128), 0), i1 = min(ceil_div256(q.y - 128), width);
-    int j0 = max(ceil_div256(q.z - 128), 0), j1 = min(ceil_div256(q.w - 128), height);
+    int i0 = ceil_div256(q.x - 128), i1 = ceil_div256(q.y - 128);
+    int j0 = ceil_div256(q.z - 128), j1 = ceil_div256(q.w - 128);
     if (tight) {
         const float det = sa * sc - sb * sb;
         const float r2 = det / (sa * sc);
@@ -79,7 +82,7 @@ __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, f
             j1 = min(j1, (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1);
         }
     }
-    return make_int4(i0, i1, j0, j1);
+    return make_int4(max(i0 - pad, 0), min(i1 + pad, width), max(j0 - pad, 0), min(j1 + pad, height));
 }
 
 // The rectangle above packed (i0 | i1 << 16, j0 | j1 << 16); all zero when
@@ -87,9 +90,9 @@ __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, f
 // 1e-8)) fails everywhere when that log is positive, since power <= 0.
 __device__ __forceinline__ uint2 pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx, float ry,
                                             float opacity, int width, int height, bool tight, float jx = 0.0f,
-                                            float jy = 0.0f) {
+                                            float jy = 0.0f, int pad = 0) {
     if ((1.0f / 255.0f) / maxf(opacity, 1e-8f) > 1.0f) return make_uint2(0u, 0u);
-    int4 r = splat_pixel_rect(cx, cy, sa, sb, sc, rx, ry, opacity, width, height, tight, jx, jy);
+    int4 r = splat_pixel_rect(cx, cy, sa, sb, sc, rx, ry, opacity, width, height, tight, jx, jy, pad);
     if (r.x >= r.y || r.z >= r.w) return make_uint2(0u, 0u);
     return make_uint2((uint32_t)r.x | ((uint32_t)r.y << 16), (uint32_t)r.z | ((uint32_t)r.w << 16));
 }

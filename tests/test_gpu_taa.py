@@ -228,3 +228,25 @@ def test_renderer_mirror_taa_matches_library_sequence(gs, tmp_path):
         want = c.read_image(abi.GS_IMAGE_TAA_RGBA32F)
     assert want[..., 3].max() > 3.5  # the history spans the frames
     np.testing.assert_array_equal(got, want)
+
+
+def test_jitter_extremes_and_list_reuse(gs, orc):
+    """Samples of one cull share tile lists: unjittered, then jitters at the
+    half-pixel limits (lists grown by a pixel), a jitter beyond it (lists
+    moved exactly), and back; every sample bit-exact, near the image edges
+    too (a wide view of a large scene)."""
+    g = gs.generate_scene(0x7AA0005, 40000, 32, 0.0)
+    cam = scenes.random_view_camera(0x7AA0005, 200, 136, radius=9.0)
+    _, ids, proj, _, _ = orc.cull(g, cam)
+    jitters = [(0.0, 0.0), (-0.5, -0.5), (0.5, 0.5), (0.5, -0.5), (1.5, -2.25), (0.0, 0.0), (-0.25, 0.5)]
+    with gs.Context(0) as c:
+        c.upload(g)
+        c.cull(cam)
+        for k, jit in enumerate(jitters):
+            print(f"case: sample {k} jitter {jit}", flush=True)
+            c.stochastic_sample(100 + k, 0.0, jit)
+            col = c.read_image(abi.GS_IMAGE_GPS_SAMPLE_RGBA32F)
+            rc, _ = orc.stochastic_splats(proj, ids, 200, 136, 100 + k, jitter=jit)
+            np.testing.assert_array_equal(col, rc[::-1])
+            # covered pixels on the image border
+            assert (rc[0, :, 3] > 0).any() and (rc[:, -1, 3] > 0).any()
