@@ -30,6 +30,7 @@ GS_IMAGE_ACCUM_RGBA32F = 3
 GS_IMAGE_FRAME_AVG_RGBA32F = 4
 GS_IMAGE_TAA_RGBA32F = 5
 GS_IMAGE_TAA_XYZ_RGBA32F = 6
+GS_IMAGE_DISPLAY_RGBA32F = 7
 
 GS_BUFFER_SORTED_KEYS = 0
 GS_BUFFER_SORTED_IDS = 1
@@ -97,6 +98,22 @@ class GsTaaUniforms(C.Structure):
         a = np.ascontiguousarray(a, dtype=np.float32)
         assert a.size == 52
         return cls.from_buffer_copy(a.tobytes())
+
+
+class GsAnimation(C.Structure):
+    """GSAnim appearance data (include/gs_api.h)."""
+    _fields_ = [
+        ("rows", C.c_uint32),
+        ("state_count", C.c_uint32),
+        ("deltas", C.c_void_p),
+        ("scales", C.c_float * 48),
+        ("layer_count", C.c_uint32),
+        ("layer_data", C.c_void_p),
+        ("lut_size", C.c_uint32),
+        ("display_lut", C.c_void_p),
+        ("log2_min", C.c_float),
+        ("log2_max", C.c_float),
+    ]
 
 
 class GsSplatEffectParams(C.Structure):
@@ -168,6 +185,10 @@ EXPORTS = {
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gs_stochastic_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "gs_taa_jitter": (None, [C.c_uint32, C.POINTER(C.c_float)]),
+    "gs_upload_animation": (C.c_int, [C.c_void_p, C.POINTER(GsAnimation)]),
+    "gs_set_animation_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
+                                         C.POINTER(C.c_float)]),
+    "gs_display_layers": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
     "gs_frame_average": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gs_taa_uniforms": (C.c_int, [C.POINTER(GsCameraUniforms), C.POINTER(GsCameraUniforms), C.c_int,
                                   C.POINTER(GsTaaUniforms)]),
@@ -389,6 +410,45 @@ class Context:
     def stochastic_sample(self, frame_seed: int, background: float = 0.0, jitter=None):
         j = None if jitter is None else (C.c_float * 2)(float(jitter[0]), float(jitter[1]))
         check(self._lib.gs_stochastic_sample(self._h, frame_seed, background, j))
+
+    def upload_animation(self, deltas=None, scales=None, layers=None, lut=None, log2_min=0.0, log2_max=1.0):
+        """GSAnim appearance data: deltas (states, rows, 48) int8 with scales (48,),
+        or layers (layers, rows, 48) half bits; optional LUT (size, size, size, 3)
+        uint16 indexed [b][g][r].  None for everything drops the animation."""
+        if deltas is None and layers is None:
+            check(self._lib.gs_upload_animation(self._h, None))
+            return
+        a = GsAnimation()
+        keep = []
+        if deltas is not None:
+            d = np.ascontiguousarray(deltas, np.int8)
+            a.rows, a.state_count = d.shape[1], d.shape[0]
+            a.deltas = d.ctypes.data
+            a.scales[:] = [float(x) for x in np.asarray(scales, np.float32)]
+            keep.append(d)
+        else:
+            lay = np.ascontiguousarray(layers, np.uint16)
+            a.rows, a.layer_count = lay.shape[1], lay.shape[0]
+            a.layer_data = lay.ctypes.data
+            keep.append(lay)
+        if lut is not None:
+            t = np.ascontiguousarray(lut, np.uint16)
+            a.lut_size = t.shape[0]
+            a.display_lut = t.ctypes.data
+            a.log2_min, a.log2_max = log2_min, log2_max
+            keep.append(t)
+        check(self._lib.gs_upload_animation(self._h, C.byref(a)))
+
+    def set_animation_frame(self, state_a: int = 0, state_b: int = 0, weight: float = 0.0, layer_weights=None):
+        if layer_weights is None:
+            check(self._lib.gs_set_animation_frame(self._h, state_a, state_b, weight, 0, None))
+            return
+        w = np.ascontiguousarray(layer_weights, np.float32).reshape(-1, 3)
+        check(self._lib.gs_set_animation_frame(self._h, state_a, state_b, weight, w.shape[0],
+                                               w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def display_layers(self, source: int, exposure_ev: float = 0.0):
+        check(self._lib.gs_display_layers(self._h, source, exposure_ev))
 
     def frame_average(self, sample_index: int):
         check(self._lib.gs_frame_average(self._h, sample_index))

@@ -67,9 +67,11 @@ struct TimedLaunch {
 };
 
 // Per-frame constants of a (batched) frame, uploaded through a pinned ring.
+constexpr uint32_t kMaxAnimLayers = 256;  // the GSAnim loader's state / layer limit
 struct FrameConst {
     GsCameraUniforms cams[kMaxViews];
     ViewInfo views[kMaxViews];
+    float layer_weights[kMaxAnimLayers * 3];
 };
 constexpr int kRing = 8;
 
@@ -110,6 +112,19 @@ struct gs_ctx {
     uint32_t stoch_seed = 0;
     float stoch_bg = 0.0f;
     float stoch_jitter[2] = {0.0f, 0.0f};
+    // GSAnim appearance data (anim_kind 1 deltas, 2 layers) and frame state
+    DevBuf anim_data, lut, display;
+    int anim_kind = 0;
+    uint32_t anim_sets = 0;  // states (excluding state 0) or layers uploaded
+    float anim_scales[48] = {};
+    uint32_t lut_size = 0;
+    float lut_log2_min = 0.0f, lut_log2_max = 0.0f;
+    int32_t anim_state_a = 0, anim_state_b = 0;
+    float anim_weight = 0.0f;
+    uint32_t anim_layer_count = 0;
+    std::vector<float> anim_layer_weights;
+    int disp_w = 0, disp_h = 0;
+    bool have_display = false;
     // Tile lists of StochasticSplats samples depend on the cull and sort
     // only, so the samples of one cull share them.  bin_epoch advances with
     // every launch that may rewrite the tile buffers; the lists are reused
@@ -261,6 +276,11 @@ int upload_cameras(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count) {
     c->host_cams.assign(cams, cams + count);
     GS_HIP(hipMemcpyAsync(frame_dev(c)->cams, slot->cams, sizeof(GsCameraUniforms) * count, hipMemcpyHostToDevice,
                           c->stream));
+    if (c->anim_kind == 2 && c->anim_layer_count) {
+        const size_t wb = sizeof(float) * 3 * c->anim_layer_count;
+        memcpy(slot->layer_weights, c->anim_layer_weights.data(), wb);
+        GS_HIP(hipMemcpyAsync(frame_dev(c)->layer_weights, slot->layer_weights, wb, hipMemcpyHostToDevice, c->stream));
+    }
     GS_HIP(hipEventRecord(ev, c->stream));
     c->cam = cams[0];
     c->views = count;
@@ -415,7 +435,8 @@ void gs_destroy(gs_ctx* c) {
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
-                      &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1]};
+                      &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1],
+                      &c->anim_data, &c->lut, &c->display};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -539,6 +560,87 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     c->view_cap = 0;
     GS_CHECK(ensure_views(c, 1));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
+    // a new scene starts without animation
+    release(c, c->anim_data);
+    release(c, c->lut);
+    c->anim_kind = 0;
+    c->anim_sets = 0;
+    c->lut_size = 0;
+    c->anim_state_a = c->anim_state_b = 0;
+    c->anim_weight = 0.0f;
+    c->anim_layer_count = 0;
+    return ok();
+}
+
+int gs_upload_animation(gs_ctx* c, const GsAnimation* an) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    GS_CHECK(set_device(c));
+    if (!an) {
+        release(c, c->anim_data);
+        release(c, c->lut);
+        c->anim_kind = 0;
+        c->anim_sets = 0;
+        c->lut_size = 0;
+        c->anim_state_a = c->anim_state_b = 0;
+        c->anim_weight = 0.0f;
+        c->anim_layer_count = 0;
+        return ok();
+    }
+    if (!c->n) return fail(GS_ERR_STATE, "upload the Gaussians before their animation");
+    if (an->rows != c->n) return fail(GS_ERR_INVALID, "animation has %u rows, the scene %u Gaussians", an->rows, c->n);
+    const bool deltas = an->state_count > 0, layers = an->layer_count > 0;
+    if (deltas == layers) return fail(GS_ERR_INVALID, "an animation holds either delta states or layers");
+    const uint32_t sets = deltas ? an->state_count : an->layer_count;
+    if (sets > kMaxAnimLayers) return fail(GS_ERR_LIMIT, "%u animation states or layers exceed %u", sets, kMaxAnimLayers);
+    if (deltas ? !an->deltas : !an->layer_data) return fail(GS_ERR_INVALID, "animation data is NULL");
+    for (int j = 0; j < 48; ++j)
+        if (!std::isfinite(an->scales[j])) return fail(GS_ERR_INVALID, "delta scale %d is not finite", j);
+    if (an->lut_size == 1 || an->lut_size > 256 || (an->lut_size && !an->display_lut))
+        return fail(GS_ERR_INVALID, "display LUT size %u outside [2, 256]", an->lut_size);
+    if (an->lut_size && !(std::isfinite(an->log2_min) && std::isfinite(an->log2_max) && an->log2_max > an->log2_min))
+        return fail(GS_ERR_INVALID, "display LUT range [%g, %g] is empty", (double)an->log2_min, (double)an->log2_max);
+    const size_t bytes = (size_t)sets * c->n * 48 * (deltas ? 1 : 2);
+    const size_t lut_bytes = (size_t)an->lut_size * an->lut_size * an->lut_size * 3 * 2;
+    release(c, c->anim_data);
+    release(c, c->lut);
+    c->anim_kind = 0;
+    GS_CHECK(ensure(c, c->anim_data, bytes, "animation coefficients"));
+    GS_HIP(hipMemcpy(c->anim_data.p, deltas ? (const void*)an->deltas : (const void*)an->layer_data, bytes,
+                     hipMemcpyHostToDevice));
+    if (an->lut_size) {
+        GS_CHECK(ensure(c, c->lut, lut_bytes, "display LUT"));
+        GS_HIP(hipMemcpy(c->lut.p, an->display_lut, lut_bytes, hipMemcpyHostToDevice));
+    }
+    c->anim_kind = deltas ? 1 : 2;
+    c->anim_sets = sets;
+    memcpy(c->anim_scales, an->scales, sizeof(c->anim_scales));
+    c->lut_size = an->lut_size;
+    c->lut_log2_min = an->log2_min;
+    c->lut_log2_max = an->log2_max;
+    c->anim_state_a = c->anim_state_b = 0;
+    c->anim_weight = 0.0f;
+    c->anim_layer_count = 0;
+    c->have_cull = c->have_sort = c->have_composite = false;
+    return ok();
+}
+
+int gs_set_animation_frame(gs_ctx* c, int32_t state_a, int32_t state_b, float weight, uint32_t layer_count,
+                           const float* layer_weights) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!std::isfinite(weight)) return fail(GS_ERR_INVALID, "animation weight is not finite");
+    const int32_t states = c->anim_kind == 1 ? (int32_t)c->anim_sets : 0;
+    if (state_a < 0 || state_a > states || state_b < 0 || state_b > states)
+        return fail(GS_ERR_INVALID, "animation states %d, %d outside [0, %d]", state_a, state_b, states);
+    const uint32_t layers = c->anim_kind == 2 ? c->anim_sets : 0;
+    if (layer_count > layers) return fail(GS_ERR_INVALID, "%u layers requested, %u uploaded", layer_count, layers);
+    if (layer_count && !layer_weights) return fail(GS_ERR_INVALID, "layer weights are NULL");
+    for (uint32_t k = 0; k < layer_count * 3; ++k)
+        if (!std::isfinite(layer_weights[k])) return fail(GS_ERR_INVALID, "layer weight %u is not finite", k);
+    c->anim_state_a = state_a;
+    c->anim_state_b = state_b;
+    c->anim_weight = weight;
+    c->anim_layer_count = layer_count;
+    c->anim_layer_weights.assign(layer_weights, layer_weights + layer_count * 3);
     return ok();
 }
 
@@ -614,6 +716,20 @@ CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullO
     a.block_diag = a.collect_stats ? ptr<uint32_t>(c->block_diag) : nullptr;
     a.rect_px = ptr<uint2>(c->rect_px);
     a.tight_rects = c->tight_rects;
+    // GSAnim: the delta branch only when a state differs from the base, the
+    // layer branch when layers are summed
+    if (c->anim_kind == 1 && (c->anim_state_a != 0 || (c->anim_weight > 0.0f && c->anim_state_b != 0))) {
+        a.anim = 1;
+        a.anim_state_a = c->anim_state_a;
+        a.anim_state_b = c->anim_state_b;
+        a.anim_weight = c->anim_weight;
+        memcpy(a.anim_scales, c->anim_scales, sizeof(a.anim_scales));
+    } else if (c->anim_kind == 2 && c->anim_layer_count) {
+        a.anim = 2;
+        a.anim_layers = (int)c->anim_layer_count;
+        a.layer_weights = frame_dev(c)->layer_weights;
+    }
+    if (a.anim) a.anim_data = ptr<uint4>(c->anim_data);
     return a;
 }
 
@@ -1329,6 +1445,60 @@ int gs_taa_accumulate(gs_ctx* c, const GsTaaUniforms* u, int use_frame_average) 
     return ok();
 }
 
+int gs_display_layers(gs_ctx* c, int source, float exposure_ev) {
+    if (!c) return fail(GS_ERR_INVALID, "ctx is NULL");
+    if (!c->lut_size) return fail(GS_ERR_STATE, "no display LUT uploaded");
+    if (!std::isfinite(exposure_ev)) return fail(GS_ERR_INVALID, "exposure is not finite");
+    GS_CHECK(set_device(c));
+    LayersDisplayLaunch d{};
+    int w = 0, h = 0;
+    switch (source) {
+        case GS_IMAGE_ALPHA_RGBA16F:
+            if (!c->have_composite) return fail(GS_ERR_STATE, "no composite yet");
+            if (c->views != 1) return fail(GS_ERR_STATE, "display of a single-view composite only");
+            GS_CHECK(settle_composite(c, true, true));
+            d.source = c->image.p;
+            d.half_source = 1;
+            w = c->img_w;
+            h = c->img_h;
+            break;
+        case GS_IMAGE_GPS_SAMPLE_RGBA32F:
+            if (!c->have_gps) return fail(GS_ERR_STATE, "no sample yet");
+            if (c->composite_kind == 1) GS_CHECK(settle_composite(c, true, true));
+            d.source = c->sample.p;
+            w = c->res_w;
+            h = c->res_h;
+            break;
+        case GS_IMAGE_ACCUM_RGBA32F:
+            if (!c->have_accum) return fail(GS_ERR_STATE, "no accumulation yet");
+            d.source = c->accum.p;
+            w = c->acc_w;
+            h = c->acc_h;
+            break;
+        case GS_IMAGE_TAA_RGBA32F:
+            if (!c->have_taa) return fail(GS_ERR_STATE, "no TAA pass yet");
+            d.source = c->taa_color[c->taa_last].p;
+            w = c->taa_w;
+            h = c->taa_h;
+            break;
+        default:
+            return fail(GS_ERR_INVALID, "image %d is not a linear colour image", source);
+    }
+    GS_CHECK(ensure(c, c->display, (size_t)w * h * 16, "display image"));
+    d.count = (uint32_t)(w * h);
+    d.lut = ptr<uint16_t>(c->lut);
+    d.lut_size = (int)c->lut_size;
+    d.log2_min = c->lut_log2_min;
+    d.log2_max = c->lut_log2_max;
+    d.exposure_ev = exposure_ev;
+    d.out = ptr<float4>(c->display);
+    GS_HIP(launch_layers_display(d, c->stream));
+    c->disp_w = w;
+    c->disp_h = h;
+    c->have_display = true;
+    return ok();
+}
+
 int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
     if (!c || !host) return fail(GS_ERR_INVALID, "bad arguments");
     GS_CHECK(set_device(c));
@@ -1355,6 +1525,10 @@ int gs_read_image(gs_ctx* c, int which, void* host, size_t bytes) {
         case GS_IMAGE_FRAME_AVG_RGBA32F:
             if (!c->have_avg) return fail(GS_ERR_STATE, "no frame average yet");
             GS_CHECK(copy_image(c, c->frame_avg.p, host, c->avg_w, c->avg_h, 16, bytes, gl_rows(c)));
+            return ok();
+        case GS_IMAGE_DISPLAY_RGBA32F:
+            if (!c->have_display) return fail(GS_ERR_STATE, "no display pass yet");
+            GS_CHECK(copy_image(c, c->display.p, host, c->disp_w, c->disp_h, 16, bytes, gl_rows(c)));
             return ok();
         case GS_IMAGE_TAA_RGBA32F:
         case GS_IMAGE_TAA_XYZ_RGBA32F: {

@@ -11,6 +11,8 @@
 // Float policy: the library is built with -ffp-contract=off and every
 // expression below keeps the operand order of the GLSL source, so each
 // value is the same IEEE binary32 result the CPU restatement computes.
+#include <hip/hip_fp16.h>
+
 #include "gs_internal.h"
 #include "gs_math.h"
 #include "gs_raster.h"
@@ -57,12 +59,77 @@ constexpr float kShC3_4 = -0.4570457994644658f;
 constexpr float kShC3_5 = 1.445305721320277f;
 constexpr float kShC3_6 = -0.5900435899266435f;
 
+// SH result of one channel before the +0.5 offset (cull.glsl:354-408); sh
+// holds the rest coefficients as RGB triples (coefficient k, channel c at
+// 3 k + c).
+__device__ __forceinline__ float sh_result(const float* dc, const float* sh, int ch, int eff, float dx, float dy,
+                                           float dz) {
+    float res = kShC0 * dc[ch];
+    if (eff >= 1) {
+        res += -kShC1 * dy * sh[0 * 3 + ch];
+        res += kShC1 * dz * sh[1 * 3 + ch];
+        res += -kShC1 * dx * sh[2 * 3 + ch];
+        if (eff >= 2) {
+            float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+            float xy = dx * dy, yz = dy * dz, xz = dz * dx;
+            res += kShC2_0 * xy * sh[3 * 3 + ch];
+            res += kShC2_1 * yz * sh[4 * 3 + ch];
+            res += kShC2_2 * (2.0f * zz - xx - yy) * sh[5 * 3 + ch];
+            res += kShC2_3 * xz * sh[6 * 3 + ch];
+            res += kShC2_4 * (xx - yy) * sh[7 * 3 + ch];
+            if (eff >= 3) {
+                res += kShC3_0 * dy * (3.0f * xx - yy) * sh[8 * 3 + ch];
+                res += kShC3_1 * xy * dz * sh[9 * 3 + ch];
+                res += kShC3_2 * dy * (4.0f * zz - xx - yy) * sh[10 * 3 + ch];
+                res += kShC3_3 * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[11 * 3 + ch];
+                res += kShC3_4 * dx * (4.0f * zz - xx - yy) * sh[12 * 3 + ch];
+                res += kShC3_5 * dz * (xx - yy) * sh[13 * 3 + ch];
+                res += kShC3_6 * dx * (xx - 3.0f * yy) * sh[14 * 3 + ch];
+            }
+        }
+    }
+    return res;
+}
+
+// Channel-major GSAnim coefficient index of rest value r = 3 k + c:
+// 3 + 15 c + k (cull.glsl:141); DC channel c is coefficient c.
+__host__ __device__ constexpr int rest_coef(int r) { return 3 + (r % 3) * 15 + r / 3; }
+
+// Signed byte j of a 48-byte delta row held in three uint4.
+__device__ __forceinline__ int delta_byte(const uint4* q, int j) {
+    const uint4 v = q[j / 16];
+    const int w = (j % 16) / 4;
+    const uint32_t word = w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
+    return (int)(int8_t)(uint8_t)((word >> ((j % 4) * 8)) & 255u);
+}
+
+// Half coefficient j of a 96-byte layer row held in six uint4.
+__device__ __forceinline__ float layer_half(const uint4* h, int j) {
+    const uint4 v = h[j / 8];
+    const int w = (j % 8) / 2;
+    const uint32_t word = w == 0 ? v.x : (w == 1 ? v.y : (w == 2 ? v.z : v.w));
+    return __half2float(__ushort_as_half((unsigned short)((word >> ((j % 2) * 16)) & 65535u)));
+}
+
+// animation_coefficient + the state lerp (cull.glsl:125-132,150-157).
+__device__ __forceinline__ float anim_coef(const CullLaunch& a, float base, const uint4* qa, const uint4* qb, int j) {
+    float va = a.anim_state_a != 0 ? base + (float)delta_byte(qa, j) * a.anim_scales[j] : base;
+    if (a.anim_weight > 0.0f) {
+        const float vb = a.anim_state_b != 0 ? base + (float)delta_byte(qb, j) * a.anim_scales[j] : base;
+        va = (1.0f - a.anim_weight) * va + a.anim_weight * vb;
+    }
+    return va;
+}
+
 // One thread per Gaussian, projected into every view of the launch.  The
 // Gaussian's planes are read once: P0 (and P1, P2 under the effect) for the
 // visibility tests of all views, then P1-P3 and the SH planes the deepest
 // visible view needs.  The 3D covariance does not depend on the view and is
 // formed once; projection, SH colour, record, pixel rectangle and dense key
-// are per view, with the operations of the single-view cull.
+// are per view, with the operations of the single-view cull.  ANIM selects
+// the GSAnim appearance branch: 1 applies the state deltas to every
+// coefficient once, 2 sums the clamped SH of each coefficient layer per view.
+template <int ANIM>
 __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
     __shared__ uint32_t s_cnt[kMaxViews][4];
     __shared__ uint32_t s_diag[8];
@@ -192,7 +259,7 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             }
             need = eff > need ? eff : need;
         }
-        const int planes_needed = need <= 0 ? 0 : (need == 1 ? 2 : (need == 2 ? 6 : 11));
+        const int planes_needed = ANIM == 2 ? 0 : (need <= 0 ? 0 : (need == 1 ? 2 : (need == 2 ? 6 : 11)));
         sh[0] = p3.z; sh[1] = p3.w;
 #pragma unroll
         for (int j = 0; j < 11; ++j) {
@@ -202,6 +269,28 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             sh[3 + 4 * j] = t.y;
             sh[4 + 4 * j] = t.z;
             sh[5 + 4 * j] = t.w;
+        }
+        if constexpr (ANIM == 1) {
+            // the 48 deltas of this row in state_a and state_b
+            uint4 qa[3], qb[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) qa[k] = qb[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (a.anim_state_a != 0) {
+                const uint4* r = a.anim_data + ((size_t)(a.anim_state_a - 1) * n + idx) * 3u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) qa[k] = r[k];
+            }
+            if (a.anim_weight > 0.0f && a.anim_state_b != 0) {
+                const uint4* r = a.anim_data + ((size_t)(a.anim_state_b - 1) * n + idx) * 3u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) qb[k] = r[k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dc[c] = anim_coef(a, dc[c], qa, qb, c);
+            const int loaded = 2 + 4 * planes_needed;
+#pragma unroll
+            for (int r = 0; r < 45; ++r)
+                if (r < loaded) sh[r] = anim_coef(a, sh[r], qa, qb, rest_coef(r));
         }
     }
 
@@ -288,38 +377,38 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             int eff = a.sh_degree;
             if (a.sh_lod_distance > 0.0f && (-z) > a.sh_lod_distance) eff = 0;
             float color[3];
+            if constexpr (ANIM == 2) {
+                // coefficient layers: sum of max(SH, 0) x RGB weight, no offset
+                color[0] = color[1] = color[2] = 0.0f;
+                for (int l = 0; l < a.anim_layers; ++l) {
+                    const uint4* row = a.anim_data + ((size_t)l * n + idx) * 6u;
+                    uint4 h[6];
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                float res = kShC0 * dc[ch];
-                if (eff >= 1) {
-                    res += -kShC1 * dy * sh[0 * 3 + ch];
-                    res += kShC1 * dz * sh[1 * 3 + ch];
-                    res += -kShC1 * dx * sh[2 * 3 + ch];
-                    if (eff >= 2) {
-                        float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-                        float xy = dx * dy, yz = dy * dz, xz = dz * dx;
-                        res += kShC2_0 * xy * sh[3 * 3 + ch];
-                        res += kShC2_1 * yz * sh[4 * 3 + ch];
-                        res += kShC2_2 * (2.0f * zz - xx - yy) * sh[5 * 3 + ch];
-                        res += kShC2_3 * xz * sh[6 * 3 + ch];
-                        res += kShC2_4 * (xx - yy) * sh[7 * 3 + ch];
-                        if (eff >= 3) {
-                            res += kShC3_0 * dy * (3.0f * xx - yy) * sh[8 * 3 + ch];
-                            res += kShC3_1 * xy * dz * sh[9 * 3 + ch];
-                            res += kShC3_2 * dy * (4.0f * zz - xx - yy) * sh[10 * 3 + ch];
-                            res += kShC3_3 * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[11 * 3 + ch];
-                            res += kShC3_4 * dx * (4.0f * zz - xx - yy) * sh[12 * 3 + ch];
-                            res += kShC3_5 * dz * (xx - yy) * sh[13 * 3 + ch];
-                            res += kShC3_6 * dx * (xx - 3.0f * yy) * sh[14 * 3 + ch];
-                        }
+                    for (int k = 0; k < 6; ++k) h[k] = row[k];
+                    float ldc[3], lsh[45];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) ldc[c] = layer_half(h, c);
+#pragma unroll
+                    for (int r = 0; r < 45; ++r) lsh[r] = layer_half(h, rest_coef(r));
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        const float res = sh_result(ldc, lsh, ch, eff, dx, dy, dz);
+                        color[ch] = color[ch] + maxf(res, 0.0f) * a.layer_weights[l * 3 + ch];
                     }
                 }
-                res += 0.5f;
-                float col = maxf(res, 0.0f) * 1.0f;
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    float res = sh_result(dc, sh, ch, eff, dx, dy, dz);
+                    res += 0.5f;
+                    color[ch] = maxf(res, 0.0f) * 1.0f;
+                }
+            }
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
                 float ecol = a.effect_active ? a.effect.color[ch] : 0.0f;
                 float ea = a.effect_active ? a.effect.color[3] : 0.0f;
-                col += ecol * (effect_wave * ea);
-                color[ch] = col;
+                color[ch] += ecol * (effect_wave * ea);
             }
 
             float4* rec = a.records + slot * kRecordVec4;
@@ -415,7 +504,12 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
     if (a.n == 0 || a.views == 0) return hipSuccess;
     if (a.views > (uint32_t)kMaxViews) return hipErrorInvalidValue;
     uint32_t blocks = (a.n + 255u) / 256u;
-    hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, s, a);
+    if (a.anim == 1)
+        hipLaunchKernelGGL(cull_kernel<1>, dim3(blocks), dim3(256), 0, s, a);
+    else if (a.anim == 2)
+        hipLaunchKernelGGL(cull_kernel<2>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(cull_kernel<0>, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -154,6 +154,15 @@ struct CullLaunch {
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats, one view)
     uint2* rect_px;          // [view][n]: covered pixel rectangle of each visible splat
     int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
+    // GSAnim appearance: 0 none, 1 int8 deltas (48 per row and state, state
+    // 0 excluded), 2 half coefficients (48 per row and layer)
+    int anim;
+    const uint4* anim_data;       // 16-byte aligned rows: 3 (deltas) or 6 (layers) uint4
+    int anim_state_a, anim_state_b;
+    float anim_weight;
+    int anim_layers;              // layers summed (anim 2)
+    const float* layer_weights;   // device, 3 per layer
+    float anim_scales[48];
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
@@ -285,6 +294,18 @@ struct TaaLaunch {
     float4* out_xyz;
 };
 hipError_t launch_taa(const TaaLaunch& a, hipStream_t s);
+
+// gs_display.hip: layers_display_fs over count pixels.
+struct LayersDisplayLaunch {
+    const void* source;       // RGBA16F (half_source) or RGBA32F pixels
+    int half_source;
+    uint32_t count;
+    const uint16_t* lut;      // lut_size^3 RGB uint16, red fastest
+    int lut_size;
+    float log2_min, log2_max, exposure_ev;
+    float4* out;
+};
+hipError_t launch_layers_display(const LayersDisplayLaunch& a, hipStream_t s);
 
 // gs_unet.hip: the denoiser's kernels.  Tensors are NHWC bf16 (raw uint16
 // bits) viewed through a channel stride and a channel offset.

@@ -106,7 +106,8 @@ enum {
     GS_IMAGE_ACCUM_RGBA32F = 3,   /* running mean of GPS samples               */
     GS_IMAGE_FRAME_AVG_RGBA32F = 4, /* mean of this frame's StochasticSplats samples */
     GS_IMAGE_TAA_RGBA32F = 5,     /* TAA history: rgb, alpha = history length  */
-    GS_IMAGE_TAA_XYZ_RGBA32F = 6  /* TAA history: world position, 1            */
+    GS_IMAGE_TAA_XYZ_RGBA32F = 6, /* TAA history: world position, 1            */
+    GS_IMAGE_DISPLAY_RGBA32F = 7  /* GSAnim 0.3 LUT display of a linear image  */
 };
 
 /* Buffers that gs_read_buffer can return (element counts in elements). */
@@ -158,6 +159,41 @@ int gs_set_memory_budget(gs_ctx* ctx, size_t bytes);
  * Refuses (GS_ERR_LIMIT) a scene that exceeds the memory budget and keeps the
  * previous scene in that case. */
 int gs_upload_gaussians(gs_ctx* ctx, const GsGaussian* gaussians, uint32_t count);
+
+/* GSAnim appearance tracks of the uploaded scene (GSAnim, src/gsanim.h:27-48),
+ * as renderer_upload_gaussians packs them (src/renderer.cpp:751-758,848-893):
+ * either int8 coefficient deltas per state (asset versions 1 and 2) or IEEE
+ * half coefficients per layer (version 3), 48 per Gaussian row in
+ * channel-major SH order (DC r, g, b, then rest coefficient k of channel c at
+ * 3 + 15 c + k).  rows must equal the uploaded Gaussian count. */
+typedef struct GsAnimation {
+    uint32_t rows;
+    uint32_t state_count;         /* delta states, state 0 (the scene itself) excluded */
+    const int8_t* deltas;         /* state_count * rows * 48, state/row/channel-major */
+    float scales[48];             /* per-coefficient delta scales */
+    uint32_t layer_count;         /* version 3: coefficient layers (then no deltas) */
+    const uint16_t* layer_data;   /* layer_count * rows * 48 half bits, layer/row/channel-major */
+    uint32_t lut_size;            /* version 3 display LUT edge (0 = none) */
+    const uint16_t* display_lut;  /* lut_size^3 RGB uint16 UNORM triples, red fastest */
+    float log2_min, log2_max;     /* LUT input range in log2 radiance */
+} GsAnimation;
+
+/* Uploads (or with NULL drops) the scene's animation.  Deltas or layers, not
+ * both; at most 256 states or layers (the loader's limit).  A new
+ * gs_upload_gaussians drops the animation.  Counts against the memory
+ * budget like the scene. */
+int gs_upload_animation(gs_ctx* ctx, const GsAnimation* animation);
+
+/* AnimationUBO of the next culls (shaders/cull.glsl:78-85, filled by
+ * renderer_cull_gaussians_gpu, src/renderer.cpp:1823-1835, from
+ * gsanim_sample / gsanim_layer_weights): with deltas, every SH coefficient
+ * (DC included) is base + q * scale in state_a, lerped by weight towards
+ * state_b (state 0 = the base itself); with layers, the colour is the sum
+ * over the first layer_count layers of max(SH(layer coefficients), 0) times
+ * the layer's RGB weight (layer_weights: 3 per layer), with no +0.5 offset.
+ * layer_count 0 with layered data leaves the scene's own coefficients. */
+int gs_set_animation_frame(gs_ctx* ctx, int32_t state_a, int32_t state_b, float weight, uint32_t layer_count,
+                           const float* layer_weights);
 
 /* Replaces renderer_set_gps_work_budget (src/renderer.cpp:1076-1123):
  * max_work_items in [1, 2^28]; refuses 0 or an over-budget value and keeps
@@ -251,6 +287,15 @@ int gs_taa_uniforms(const GsCameraUniforms* cam, const GsCameraUniforms* prev_ca
  * The history is double buffered: this call reads one buffer and writes the
  * other, which GS_IMAGE_TAA_RGBA32F / GS_IMAGE_TAA_XYZ_RGBA32F then read. */
 int gs_taa_accumulate(gs_ctx* ctx, const GsTaaUniforms* uniforms, int use_frame_average);
+
+/* Replaces layers_display_fs (shaders/accum.glsl:197-240; bound by
+ * renderer_display_appearance, src/renderer.cpp:2120-2137): the linear
+ * image `source` (GS_IMAGE_ALPHA_RGBA16F, GS_IMAGE_GPS_SAMPLE_RGBA32F,
+ * GS_IMAGE_ACCUM_RGBA32F or GS_IMAGE_TAA_RGBA32F) times 2^exposure_ev
+ * through the animation's log2-domain RGB LUT (exact trilinear lookup)
+ * into GS_IMAGE_DISPLAY_RGBA32F (alpha 1).  Needs a LUT from
+ * gs_upload_animation. */
+int gs_display_layers(gs_ctx* ctx, int source, float exposure_ev);
 
 /* Replaces accum_fs (shaders/accum.glsl:19-45): accum = h + (s - h) / n with
  * n = sample_count (1 = first sample, history ignored). */

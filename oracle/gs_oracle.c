@@ -299,10 +299,47 @@ static const float SH_C3_4 = -0.4570457994644658f;
 static const float SH_C3_5 = 1.445305721320277f;
 static const float SH_C3_6 = -0.5900435899266435f;
 
+/* animation_coefficient (cull.glsl:125-132): the int8 delta of coefficient j
+ * of row `row` in `state`, from the words as the renderer packs them (four
+ * signed bytes per word, src/renderer.cpp:852-855). */
+static float or_anim_state(const OrAnimation* an, float base, uint32_t row, int j, int state) {
+    if (state == 0) return base;
+    uint32_t offset = ((uint32_t)(state - 1) * an->count + row) * 48u + (uint32_t)j;
+    uint32_t word = an->words[offset / 4u];
+    int q = (int)((word >> ((offset % 4u) * 8u)) & 255u);
+    if (q >= 128) q -= 256;
+    return base + (float)q * an->scales[j];
+}
+
+/* fetch_vec4's delta branch (cull.glsl:150-157): state_a, lerped to state_b. */
+static float or_anim_coef(const OrAnimation* an, float base, uint32_t row, int j) {
+    float a = or_anim_state(an, base, row, j, an->state_a);
+    if (an->weight > 0.0f) {
+        float b = or_anim_state(an, base, row, j, an->state_b);
+        a = (1.0f - an->weight) * a + an->weight * b;
+    }
+    return a;
+}
+
+/* fetch_vec4's layer branch (cull.glsl:143-149): half coefficient j of row
+ * `row` in `layer`, two halves per word (src/renderer.cpp:849-851). */
+static float or_layer_coef(const OrAnimation* an, int layer, uint32_t row, int j) {
+    uint32_t offset = ((uint32_t)layer * an->count + row) * 48u + (uint32_t)j;
+    uint32_t pair = an->words[offset / 2u];
+    return or_f16_to_f32((uint16_t)((pair >> ((offset % 2u) * 16u)) & 65535u));
+}
+
+/* Channel-major coefficient index of rest value r (RGB triples, r = 3 k + c):
+ * j = 3 + 15 c + k (cull.glsl:141). */
+static inline int or_rest_j(int r) { return 3 + (r % 3) * 15 + r / 3; }
+
+/* SH result of one channel before the +0.5 offset (cull.glsl:354-408). */
+static float or_sh_result(const float* dc, const float* sh, int ch, int eff, float dx, float dy, float dz);
+
 static void or_cull_one(const GsGaussian* gs, uint32_t idx, const GsCameraUniforms* cam,
                         const GsSplatEffectParams* fx, int sh_degree, float sh_lod_distance,
-                        int collect_stats, uint32_t* keys, uint32_t* ids, float* projected,
-                        uint32_t* visible_count, uint32_t diag[8]) {
+                        int collect_stats, const OrAnimation* an, uint32_t* keys, uint32_t* ids,
+                        float* projected, uint32_t* visible_count, uint32_t diag[8]) {
     const float* V = cam->view;   /* V[c*4 + r] = view[c][r] */
     const float* P = cam->proj;
     float position[3] = {gs->position[0], gs->position[1], gs->position[2]};
@@ -467,39 +504,31 @@ static void or_cull_one(const GsGaussian* gs, uint32_t idx, const GsCameraUnifor
     /* SH (cull.glsl:348-416) */
     int eff = sh_degree;
     if (sh_lod_distance > 0.0f && (-z) > sh_lod_distance) eff = 0;
-    const float* sh = gs->sh_rest;
-    float color[3];
-    for (int ch = 0; ch < 3; ++ch) {
-        float res = SH_C0 * dc[ch];
-        if (eff >= 1) {
-            res += -SH_C1 * dy * sh[0 * 3 + ch];
-            res += SH_C1 * dz * sh[1 * 3 + ch];
-            res += -SH_C1 * dx * sh[2 * 3 + ch];
-            if (eff >= 2) {
-                float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-                float xy = dx * dy, yz = dy * dz, xz = dz * dx;
-                res += SH_C2_0 * xy * sh[3 * 3 + ch];
-                res += SH_C2_1 * yz * sh[4 * 3 + ch];
-                res += SH_C2_2 * (2.0f * zz - xx - yy) * sh[5 * 3 + ch];
-                res += SH_C2_3 * xz * sh[6 * 3 + ch];
-                res += SH_C2_4 * (xx - yy) * sh[7 * 3 + ch];
-                if (eff >= 3) {
-                    res += SH_C3_0 * dy * (3.0f * xx - yy) * sh[8 * 3 + ch];
-                    res += SH_C3_1 * xy * dz * sh[9 * 3 + ch];
-                    res += SH_C3_2 * dy * (4.0f * zz - xx - yy) * sh[10 * 3 + ch];
-                    res += SH_C3_3 * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[11 * 3 + ch];
-                    res += SH_C3_4 * dx * (4.0f * zz - xx - yy) * sh[12 * 3 + ch];
-                    res += SH_C3_5 * dz * (xx - yy) * sh[13 * 3 + ch];
-                    res += SH_C3_6 * dx * (xx - 3.0f * yy) * sh[14 * 3 + ch];
-                }
-            }
+    const int animated = an && an->count > 0;
+    const int layers = animated ? an->layer_count : 0;
+    float sh[45];
+    for (int r = 0; r < 45; ++r) sh[r] = gs->sh_rest[r];
+    if (animated && layers == 0) {
+        for (int ch = 0; ch < 3; ++ch) dc[ch] = or_anim_coef(an, dc[ch], idx, ch);
+        for (int r = 0; r < 45; ++r) sh[r] = or_anim_coef(an, sh[r], idx, or_rest_j(r));
+    }
+    float color[3] = {0.0f, 0.0f, 0.0f};
+    for (int layer = 0; layer < (layers > 1 ? layers : 1); ++layer) {
+        if (layers > 0) {
+            for (int ch = 0; ch < 3; ++ch) dc[ch] = or_layer_coef(an, layer, idx, ch);
+            for (int r = 0; r < 45; ++r) sh[r] = or_layer_coef(an, layer, idx, or_rest_j(r));
         }
-        res += 0.5f;
-        float col = fmaxf(res, 0.0f) * 1.0f;
+        for (int ch = 0; ch < 3; ++ch) {
+            float res = or_sh_result(dc, sh, ch, eff, dx, dy, dz);
+            if (layers == 0) res += 0.5f;
+            const float w = layers > 0 ? an->layer_weights[layer * 3 + ch] : 1.0f;
+            color[ch] = color[ch] + fmaxf(res, 0.0f) * w;
+        }
+    }
+    for (int ch = 0; ch < 3; ++ch) {
         float ecol = fx ? fx->color[ch] : 0.0f;
         float ea = fx ? fx->color[3] : 0.0f;
-        col += ecol * (effect_wave * ea);
-        color[ch] = col;
+        color[ch] += ecol * (effect_wave * ea);
     }
 
     /* projected record (cull.glsl:420-423) */
@@ -509,10 +538,46 @@ static void or_cull_one(const GsGaussian* gs, uint32_t idx, const GsCameraUnifor
     rec[12] = a; rec[13] = b * cam->clip_y_sign; rec[14] = c; rec[15] = det;
 }
 
+static float or_sh_result(const float* dc, const float* sh, int ch, int eff, float dx, float dy, float dz) {
+    float res = SH_C0 * dc[ch];
+    if (eff >= 1) {
+        res += -SH_C1 * dy * sh[0 * 3 + ch];
+        res += SH_C1 * dz * sh[1 * 3 + ch];
+        res += -SH_C1 * dx * sh[2 * 3 + ch];
+        if (eff >= 2) {
+            float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+            float xy = dx * dy, yz = dy * dz, xz = dz * dx;
+            res += SH_C2_0 * xy * sh[3 * 3 + ch];
+            res += SH_C2_1 * yz * sh[4 * 3 + ch];
+            res += SH_C2_2 * (2.0f * zz - xx - yy) * sh[5 * 3 + ch];
+            res += SH_C2_3 * xz * sh[6 * 3 + ch];
+            res += SH_C2_4 * (xx - yy) * sh[7 * 3 + ch];
+            if (eff >= 3) {
+                res += SH_C3_0 * dy * (3.0f * xx - yy) * sh[8 * 3 + ch];
+                res += SH_C3_1 * xy * dz * sh[9 * 3 + ch];
+                res += SH_C3_2 * dy * (4.0f * zz - xx - yy) * sh[10 * 3 + ch];
+                res += SH_C3_3 * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[11 * 3 + ch];
+                res += SH_C3_4 * dx * (4.0f * zz - xx - yy) * sh[12 * 3 + ch];
+                res += SH_C3_5 * dz * (xx - yy) * sh[13 * 3 + ch];
+                res += SH_C3_6 * dx * (xx - 3.0f * yy) * sh[14 * 3 + ch];
+            }
+        }
+    }
+    return res;
+}
+
 void or_cull(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
              const GsSplatEffectParams* effect, int sh_degree, float sh_lod_distance,
              int collect_stats, uint32_t* keys, uint32_t* ids, float* projected,
              uint32_t* visible_count, uint32_t diag[8], int threads) {
+    or_cull_anim(g, n, cam, effect, sh_degree, sh_lod_distance, collect_stats, NULL, keys, ids, projected,
+                 visible_count, diag, threads);
+}
+
+void or_cull_anim(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
+                  const GsSplatEffectParams* effect, int sh_degree, float sh_lod_distance,
+                  int collect_stats, const OrAnimation* animation, uint32_t* keys, uint32_t* ids,
+                  float* projected, uint32_t* visible_count, uint32_t diag[8], int threads) {
     /* cull_reset_cs (cull.glsl:456-473) */
     *visible_count = 0;
     for (int k = 0; k < 8; ++k) diag[k] = 0;
@@ -522,7 +587,7 @@ void or_cull(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
 #pragma omp parallel for schedule(static)
 #endif
     for (long long i = 0; i < (long long)n; ++i)
-        or_cull_one(g + i, (uint32_t)i, cam, effect, sh_degree, sh_lod_distance, collect_stats,
+        or_cull_one(g + i, (uint32_t)i, cam, effect, sh_degree, sh_lod_distance, collect_stats, animation,
                     keys, ids, projected, visible_count, diag);
     (void)threads;
 }
@@ -922,6 +987,41 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
         free(best);
     }
     (void)threads;
+}
+
+/* layers_display_fs (shaders/accum.glsl:197-240) over `count` pixels of
+ * linear RGBA floats; lut: size^3 RGB uint16 triples, red fastest. */
+static float or_lut_entry(const uint16_t* lut, int size, int x, int y, int z, int c) {
+    int offset = ((z * size + y) * size + x) * 3 + c;
+    return (float)lut[offset] / 65535.0f;
+}
+
+void or_layers_display(const float* src, uint32_t count, const uint16_t* lut, int size, float log2_min,
+                       float log2_max, float exposure_ev, float* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        int lo[3], hi[3];
+        float f[3];
+        for (int c = 0; c < 3; ++c) {
+            float rad = src[(size_t)i * 4 + c] * exp2f(exposure_ev);
+            float s = or_clampf((log2f(fmaxf(rad, exp2f(log2_min))) - log2_min) / (log2_max - log2_min), 0.0f, 1.0f);
+            float p = s * (float)(size - 1);
+            lo[c] = (int)floorf(p);
+            hi[c] = lo[c] + 1 < size - 1 ? lo[c] + 1 : size - 1;
+            f[c] = or_fract(p);
+        }
+        for (int c = 0; c < 3; ++c) {
+            float a = or_mix(or_mix(or_lut_entry(lut, size, lo[0], lo[1], lo[2], c),
+                                    or_lut_entry(lut, size, hi[0], lo[1], lo[2], c), f[0]),
+                             or_mix(or_lut_entry(lut, size, lo[0], hi[1], lo[2], c),
+                                    or_lut_entry(lut, size, hi[0], hi[1], lo[2], c), f[0]), f[1]);
+            float b = or_mix(or_mix(or_lut_entry(lut, size, lo[0], lo[1], hi[2], c),
+                                    or_lut_entry(lut, size, hi[0], lo[1], hi[2], c), f[0]),
+                             or_mix(or_lut_entry(lut, size, lo[0], hi[1], hi[2], c),
+                                    or_lut_entry(lut, size, hi[0], hi[1], hi[2], c), f[0]), f[1]);
+            out[(size_t)i * 4 + c] = or_mix(a, b, f[2]);
+        }
+        out[(size_t)i * 4 + 3] = 1.0f;
+    }
 }
 
 /* halton (src/renderer.cpp:246-255). */

@@ -48,6 +48,22 @@ float or_f16_to_f32(uint16_t h);
 /* cull: dense reference semantics.  keys/ids/projected are n-sized and
  * indexed by Gaussian; culled rows keep the 0xFFFFFFFF sentinel and the
  * reference's culled projected record.  effect may be NULL. */
+/* AnimationUBO + the packed animation buffer (cull.glsl:70-85): count = rows
+ * (0 = no animation); words: int8 deltas four per word, or half layer
+ * coefficients two per word; layer_weights: 3 per layer. */
+typedef struct OrAnimation {
+    uint32_t count;
+    int32_t state_a, state_b;
+    float weight;
+    float scales[48];
+    int32_t layer_count;
+    const uint32_t* words;
+    const float* layer_weights;
+} OrAnimation;
+void or_cull_anim(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
+                  const GsSplatEffectParams* effect, int sh_degree, float sh_lod_distance,
+                  int collect_stats, const OrAnimation* animation, uint32_t* keys, uint32_t* ids,
+                  float* projected, uint32_t* visible_count, uint32_t diag[8], int threads);
 void or_cull(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
              const GsSplatEffectParams* effect, int sh_degree, float sh_lod_distance,
              int collect_stats, uint32_t* keys, uint32_t* ids, float* projected,
@@ -92,6 +108,9 @@ uint32_t or_depth24(float ndc_z, float clip_z_01);
 void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,
                           float clip_z_01, float frame_seed, float background, float jitter_x, float jitter_y,
                           float* color, float* depth, int threads);
+/* layers_display_fs (shaders/accum.glsl:197-240). */
+void or_layers_display(const float* src, uint32_t count, const uint16_t* lut, int size, float log2_min,
+                       float log2_max, float exposure_ev, float* out);
 /* halton (src/renderer.cpp:246-255). */
 float or_halton(uint32_t index, uint32_t base);
 /* taa_accum_fs (shaders/accum.glsl:47-179); u = TaaAccumUBO as 52 floats. */

@@ -33,6 +33,8 @@ _SIGS = {
     "or_pcg2d": (None, [C.POINTER(C.c_uint32)]),
     "or_f32_to_f16": (C.c_uint16, [C.c_float]),
     "or_f16_to_f32": (C.c_float, [C.c_uint16]),
+    "or_cull_anim": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int,
+                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "or_cull": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int,
                        C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p, C.c_int]),
     "or_radix_sort": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
@@ -51,6 +53,8 @@ _SIGS = {
     "or_stochastic_splats": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_float,
                                     C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_int]),
     "or_halton": (C.c_float, [C.c_uint32, C.c_uint32]),
+    "or_layers_display": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float,
+                                 C.c_void_p]),
     "or_taa_accumulate": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
 }
@@ -84,9 +88,28 @@ def threads_default() -> int:
     return int(os.environ.get("GS_ORACLE_THREADS", "0"))
 
 
+class OrAnimation(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("state_a", C.c_int32), ("state_b", C.c_int32), ("weight", C.c_float),
+                ("scales", C.c_float * 48), ("layer_count", C.c_int32), ("words", C.c_void_p),
+                ("layer_weights", C.c_void_p)]
+
+
+def pack_animation_words(deltas=None, layers=None) -> np.ndarray:
+    """The animation buffer as the renderer packs it: int8 deltas four per
+    little-endian word, or half coefficients two per word (zero padded)."""
+    raw = (np.ascontiguousarray(deltas, np.int8).view(np.uint8).ravel() if layers is None
+           else np.ascontiguousarray(layers, np.uint16).view(np.uint8).ravel())
+    pad = (-raw.size) % 4
+    return np.concatenate([raw, np.zeros(pad, np.uint8)]).view("<u4").copy()
+
+
 def cull(gaussians: np.ndarray, cam, effect=None, sh_degree=3, sh_lod_distance=0.0, collect_stats=False,
-         threads=None):
-    """Returns (keys, ids, projected (n,16), visible_count, diag[8]) with reference semantics."""
+         threads=None, animation=None):
+    """Returns (keys, ids, projected (n,16), visible_count, diag[8]) with reference semantics.
+
+    animation: None, or a dict with "deltas" (states, rows, 48) int8, "scales"
+    (48,), "state_a", "state_b", "weight"; or with "layers" (layers, rows, 48)
+    half bits and "layer_weights" (layers, 3), optionally "layer_count"."""
     lib = load()
     g = np.ascontiguousarray(gaussians, dtype=np.float32)
     n = g.shape[0]
@@ -95,9 +118,28 @@ def cull(gaussians: np.ndarray, cam, effect=None, sh_degree=3, sh_lod_distance=0
     proj = np.empty((n, 16), np.float32)
     vis = C.c_uint32(0)
     diag = np.zeros(8, np.uint32)
-    lib.or_cull(_p(g), n, C.byref(cam), C.byref(effect) if effect is not None else None, sh_degree,
-                sh_lod_distance, int(collect_stats), _p(keys), _p(ids), _p(proj), C.byref(vis), _p(diag),
-                threads_default() if threads is None else threads)
+    an = None
+    keep = []
+    if animation is not None:
+        an = OrAnimation()
+        an.count = n
+        if "layers" in animation:
+            words = pack_animation_words(layers=animation["layers"])
+            lw = np.ascontiguousarray(animation["layer_weights"], np.float32).ravel()
+            an.layer_count = int(animation.get("layer_count", len(animation["layers"])))
+            an.layer_weights = lw.ctypes.data
+            keep.append(lw)
+        else:
+            words = pack_animation_words(deltas=animation["deltas"])
+            an.state_a = int(animation.get("state_a", 0))
+            an.state_b = int(animation.get("state_b", 0))
+            an.weight = float(animation.get("weight", 0.0))
+            an.scales[:] = [float(x) for x in np.asarray(animation["scales"], np.float32)]
+        an.words = words.ctypes.data
+        keep.append(words)
+    lib.or_cull_anim(_p(g), n, C.byref(cam), C.byref(effect) if effect is not None else None, sh_degree,
+                     sh_lod_distance, int(collect_stats), C.byref(an) if an is not None else None, _p(keys),
+                     _p(ids), _p(proj), C.byref(vis), _p(diag), threads_default() if threads is None else threads)
     return keys, ids, proj, vis.value, diag
 
 
@@ -171,6 +213,16 @@ def stochastic_splats(projected, ids, width, height, frame_seed, clip_z_01=0.0, 
                              float(jitter[0]), float(jitter[1]), _p(color), _p(depth),
                              threads_default() if threads is None else threads)
     return color, depth
+
+
+def layers_display(image, lut, log2_min, log2_max, exposure_ev=0.0):
+    """layers_display_fs of a linear RGBA image (..., 4); lut (size, size, size, 3) uint16 [b][g][r]."""
+    lib = load()
+    src = np.ascontiguousarray(image, dtype=np.float32)
+    t = np.ascontiguousarray(lut, dtype=np.uint16)
+    out = np.empty(src.shape[:-1] + (4,), np.float32)
+    lib.or_layers_display(_p(src), src.size // 4, _p(t), t.shape[0], log2_min, log2_max, exposure_ev, _p(out))
+    return out
 
 
 def taa_accumulate(cur_color, cur_depth, hist_color, hist_xyz, uniforms):
