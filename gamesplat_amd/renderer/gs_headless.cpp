@@ -16,6 +16,15 @@
 //   gpu_budget_mb=[0] diagnostics=[0]
 //   warmup=[3] frames=[20]     untimed then timed frame counts
 //   linear_capture=<path>      raw f32 RGBA of the last frame
+//   display_capture=<path>     raw f32 RGBA of what the frame displays (LUT for layered GSAnim)
+// GSAnim appearance from raw files (the loader's arrays):
+//   anim_deltas=<path> anim_scales=<path> anim_states=N       int8 deltas, 48 f32 scales
+//   anim_layers=<path> anim_layer_count=L                      half coefficients
+//   anim_keys=t:state:linear,...  anim_version=[2] anim_linear=[0] anim_duration=[1]
+//   anim_layer_keys=t:r:g:b:linear,...;... (one list per layer)
+//   anim_lut=<path> anim_lut_size=S anim_log2_min anim_log2_max
+//   anim_time=[0] anim_time_step=[0]  playback time of frame f: time + f * step
+//   first_layer_only=[0] exposure_ev=[0]
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -126,6 +135,67 @@ bool load_ply(const char* path, std::vector<Gaussian>& out) {
     return true;
 }
 
+template <typename T>
+bool read_raw(const char* path, std::vector<T>& out, size_t count) {
+    out.resize(count);
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    const bool ok = fread(out.data(), sizeof(T), count, f) == count;
+    fclose(f);
+    return ok;
+}
+
+// The GSAnim asset from the anim_* flags; false when a file is missing.
+bool load_animation(GSAnim& an, uint32_t rows) {
+    an.duration = argf("anim_duration", 1.0);
+    an.version = (int)argi("anim_version", 2);
+    an.linear = argi("anim_linear", 0) != 0;
+    an.gaussian_count = rows;
+    for (const char* p = arg("anim_keys", ""); *p;) {
+        GSAnimKey k{};
+        int linear = 0, used = 0;
+        if (sscanf(p, "%lf:%u:%d%n", &k.time_seconds, &k.state, &linear, &used) != 3) return false;
+        k.linear = linear != 0;
+        an.keys.push_back(k);
+        p += used;
+        if (*p == ',') ++p;
+    }
+    if (g_args.count("anim_layers")) {
+        an.version = 3;
+        const uint32_t layers = (uint32_t)argi("anim_layer_count", 1);
+        if (!read_raw(arg("anim_layers", ""), an.layer_data, (size_t)layers * rows * 48)) return false;
+        an.layers.resize(layers);
+        const char* p = arg("anim_layer_keys", "");
+        for (uint32_t l = 0; l < layers && *p; ++l) {
+            while (*p && *p != ';') {
+                GSAnimLayerKey k;
+                int linear = 0, used = 0;
+                if (sscanf(p, "%lf:%f:%f:%f:%d%n", &k.time_seconds, &k.weight[0], &k.weight[1], &k.weight[2],
+                           &linear, &used) != 5)
+                    return false;
+                k.linear = linear != 0;
+                an.layers[l].keys.push_back(k);
+                p += used;
+                if (*p == ',') ++p;
+            }
+            if (*p == ';') ++p;
+        }
+        an.lut_size = (uint32_t)argi("anim_lut_size", 0);
+        if (an.lut_size &&
+            !read_raw(arg("anim_lut", ""), an.display_lut, (size_t)an.lut_size * an.lut_size * an.lut_size * 3))
+            return false;
+        an.log2_min = (float)argf("anim_log2_min", -8.0);
+        an.log2_max = (float)argf("anim_log2_max", 4.0);
+    } else {
+        an.state_count = (uint32_t)argi("anim_states", 1);
+        if (!read_raw(arg("anim_deltas", ""), an.deltas, (size_t)an.state_count * rows * 48)) return false;
+        std::vector<float> scales;
+        if (!read_raw(arg("anim_scales", ""), scales, 48)) return false;
+        memcpy(an.scales, scales.data(), sizeof(an.scales));
+    }
+    return true;
+}
+
 double percentile(std::vector<double> v, double p) {
     if (v.empty()) return 0.0;
     std::sort(v.begin(), v.end());
@@ -162,6 +232,14 @@ int main(int argc, char** argv) {
     GaussianScene scene{};
     scene.gaussians = gaussians.data();
     scene.gaussian_count = (uint32_t)gaussians.size();
+    GSAnim animation;
+    if (g_args.count("anim_deltas") || g_args.count("anim_layers")) {
+        if (!load_animation(animation, scene.gaussian_count)) {
+            fprintf(stderr, "could not load the GSAnim flags\n");
+            return 1;
+        }
+        scene.animation = &animation;
+    }
 
     Renderer r;
     if (!renderer_init(&r)) return 1;
@@ -179,6 +257,8 @@ int main(int argc, char** argv) {
     r.gps_max_work_items = (uint32_t)(argf("gps_budget_m", 16.0) * 1048576.0);
     r.stochastic_accumulation_enabled = argi("gps_accumulation", 1) != 0;
     r.splat_diagnostics_enabled = argi("diagnostics", 0) != 0;
+    r.first_layer_only = argi("first_layer_only", 0) != 0;
+    r.exposure_ev = (float)argf("exposure_ev", 0.0);
     if (!renderer_upload_gaussians(&r, &scene)) return 1;
     double startup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
@@ -213,6 +293,7 @@ int main(int argc, char** argv) {
         float pose[16];
         gs_look_at_origin_pose(pos, pose);
         gs_camera_from_pose(pose, 0.8660254f, (float)width, (float)height, reinterpret_cast<GsCameraUniforms*>(&cam));
+        if (scene.animation) animation.playback_time = argf("anim_time", 0.0) + argf("anim_time_step", 0.0) * f;
         bool timed = f >= warmup;
         auto t0 = std::chrono::steady_clock::now();
         if (timed) hipEventRecord(ev[2 * (f - warmup)], stream);
@@ -235,6 +316,14 @@ int main(int argc, char** argv) {
         std::vector<float> px((size_t)width * height * 4);
         if (!renderer_read_linear(&r, px.data(), px.size())) return 1;
         FILE* f = fopen(cap, "wb");
+        if (!f || fwrite(px.data(), 4, px.size(), f) != px.size()) return 1;
+        fclose(f);
+    }
+    const char* dcap = arg("display_capture", "");
+    if (dcap[0]) {
+        std::vector<float> px((size_t)width * height * 4);
+        if (!renderer_read_display(&r, &scene, px.data(), px.size())) return 1;
+        FILE* f = fopen(dcap, "wb");
         if (!f || fwrite(px.data(), 4, px.size(), f) != px.size()) return 1;
         fclose(f);
     }

@@ -88,6 +88,45 @@ bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene) {
     r->gaussian_count = scene->gaussian_count;
     r->sort_capacity = scene->gaussian_count;
     renderer_reset_stochastic_accumulation(r);
+    if (scene->animation) {
+        // appearance tracks next to the scene (src/renderer.cpp:751-893)
+        const GSAnim& an = *scene->animation;
+        const size_t row_coefs = (size_t)scene->gaussian_count * 48u;
+        GsAnimation ga;
+        memset(&ga, 0, sizeof(ga));
+        ga.rows = scene->gaussian_count;
+        if (an.version == 3) {
+            ga.layer_count = (uint32_t)an.layers.size();
+            ga.layer_data = an.layer_data.data();
+            ga.lut_size = an.lut_size;
+            ga.display_lut = an.lut_size ? an.display_lut.data() : nullptr;
+            ga.log2_min = an.log2_min;
+            ga.log2_max = an.log2_max;
+            if (an.layer_data.size() != ga.layer_count * row_coefs ||
+                an.display_lut.size() != (size_t)an.lut_size * an.lut_size * an.lut_size * 3u) {
+                fprintf(stderr, "[renderer] GSAnim layer or LUT sizes do not match the scene\n");
+                return false;
+            }
+        } else {
+            ga.state_count = an.state_count;
+            ga.deltas = an.deltas.data();
+            memcpy(ga.scales, an.scales, sizeof(ga.scales));
+            if (an.deltas.size() != an.state_count * row_coefs) {
+                fprintf(stderr, "[renderer] GSAnim delta size does not match the scene\n");
+                return false;
+            }
+        }
+        if (ga.state_count || ga.layer_count) {
+            st = gs_upload_animation(r->ctx, &ga);
+            if (st != GS_OK) {
+                log_error("animation upload", st);
+                return false;
+            }
+        }
+        r->animation_state_a = r->animation_state_b = 0;
+        r->animation_weight = 0.0f;
+        r->animation_time = 0.0;
+    }
     return true;
 }
 
@@ -122,9 +161,8 @@ static uint32_t clamp_samples(uint32_t n) { return n < 1u ? 1u : (n > 16u ? 16u 
 // samples with consecutive frame seeds; with TAA each is Halton-jittered,
 // they are averaged, and the average is reprojected into the history; without
 // TAA they fold into the running mean, which a camera change resets.
-static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEffectParams* splat_effect) {
+static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEffectParams* splat_effect, float bg) {
     const GsCameraUniforms* gcam = reinterpret_cast<const GsCameraUniforms*>(cam);
-    const float bg = r->black_background ? 0.0f : 0.1f;
     // new target sizes start a new accumulation (renderer_ensure_stochastic_targets)
     if (r->stochastic_width != (uint32_t)cam->viewport[0] || r->stochastic_height != (uint32_t)cam->viewport[1])
         renderer_reset_stochastic_accumulation(r);
@@ -200,10 +238,37 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         r->last_frame_status = GS_ERR_STATE;
         return;
     }
+    const float bg = r->black_background || scene->animation ? 0.0f : 0.1f;
     if (scene->animation) {
-        fprintf(stderr, "[renderer] GSAnim appearance tracks are not supported by this build\n");
-        r->last_frame_status = GS_ERR_INVALID;
-        return;
+        // playback state (src/renderer.cpp:2148-2161) and AnimationUBO
+        // (src/renderer.cpp:1823-1835)
+        const GSAnim& an = *scene->animation;
+        GSAnimSample sample = gsanim_sample(an, an.playback_time);
+        if (an.selected_state >= 0 && (uint32_t)an.selected_state < an.state_count)
+            sample = {an.selected_state, an.selected_state, 0.0f};
+        if (sample.state_a != r->animation_state_a || sample.state_b != r->animation_state_b ||
+            sample.weight != r->animation_weight || (an.version == 3 && an.playback_time != r->animation_time))
+            renderer_reset_stochastic_accumulation(r);
+        r->animation_state_a = sample.state_a;
+        r->animation_state_b = sample.state_b;
+        r->animation_weight = sample.weight;
+        r->animation_time = an.playback_time;
+        std::vector<std::array<float, 3>> weights;
+        uint32_t layers = 0;
+        if (an.version == 3) {
+            weights = gsanim_layer_weights(an, an.playback_time);
+            layers = (uint32_t)weights.size();
+            if (r->first_layer_only && layers > 1u) layers = 1u;
+        }
+        const int st = gs_set_animation_frame(r->ctx, an.version == 3 ? 0 : sample.state_a,
+                                              an.version == 3 ? 0 : sample.state_b,
+                                              an.version == 3 ? 0.0f : sample.weight, layers,
+                                              layers ? weights[0].data() : nullptr);
+        if (st != GS_OK) {
+            log_error("animation frame", st);
+            r->last_frame_status = st;
+            return;
+        }
     }
     const bool gps = r->splat_render_mode == SplatRenderMode::GaussianPointSplatting;
     const bool stochastic = r->splat_render_mode == SplatRenderMode::StochasticSplats;
@@ -213,7 +278,9 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     }
     const GsCameraUniforms* gcam = reinterpret_cast<const GsCameraUniforms*>(cam);
     GsCullOptions opts = {r->sh_degree, r->sh_lod_distance, r->splat_diagnostics_enabled ? 1 : 0};
-    int st = gs_cull(r->ctx, gcam, reinterpret_cast<const GsSplatEffectParams*>(splat_effect), &opts);
+    // animated scenes draw without the shockwave (src/renderer.cpp:2168)
+    const SplatEffectParams* cull_effect = scene->animation ? nullptr : splat_effect;
+    int st = gs_cull(r->ctx, gcam, reinterpret_cast<const GsSplatEffectParams*>(cull_effect), &opts);
     if (st != GS_OK) {
         log_error("cull", st);
         r->last_frame_status = st;
@@ -244,7 +311,7 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     }
 
     if (stochastic) {
-        st = draw_stochastic(r, cam, splat_effect);
+        st = draw_stochastic(r, cam, splat_effect, bg);
         if (st != GS_OK) {
             log_error("stochastic splats", st);
             r->last_frame_status = st;
@@ -259,8 +326,8 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     r->stochastic_frame_seed++;
 
     uint32_t ss = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
-    const float bg = r->black_background ? 0.0f : 0.1f;
-    st = gs_gps_sample(r->ctx, r->stochastic_frame_seed, ss, 0, bg);
+    const int linear_color = scene->animation && scene->animation->version == 3 ? 1 : 0;
+    st = gs_gps_sample(r->ctx, r->stochastic_frame_seed, ss, linear_color, bg);
     if (st != GS_OK) {
         log_error("GPS sample", st);
         r->last_frame_status = st;
@@ -318,6 +385,29 @@ bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
     int st = gs_read_image(r->ctx, which, out, floats * 4);
     if (st != GS_OK) {
         log_error("read stochastic image", st);
+        return false;
+    }
+    return true;
+}
+
+bool renderer_read_display(Renderer* r, const GaussianScene* scene, float* out, size_t floats) {
+    const GSAnim* an = scene ? scene->animation : nullptr;
+    if (!an || an->version != 3 || !an->lut_size) return renderer_read_linear(r, out, floats);
+    if (!r->ctx || !out) return false;
+    int source = GS_IMAGE_ALPHA_RGBA16F;
+    size_t px = (size_t)r->appearance_width * r->appearance_height;
+    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) {
+        source = r->stochastic_display_image;
+        px = (size_t)r->stochastic_width * r->stochastic_height;
+    } else if (r->splat_render_mode == SplatRenderMode::GaussianPointSplatting) {
+        source = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
+        px = (size_t)r->stochastic_width * r->stochastic_height;
+    }
+    if (floats != px * 4) return false;
+    int st = gs_display_layers(r->ctx, source, r->exposure_ev);
+    if (st == GS_OK) st = gs_read_image(r->ctx, GS_IMAGE_DISPLAY_RGBA32F, out, floats * 4);
+    if (st != GS_OK) {
+        log_error("layer display", st);
         return false;
     }
     return true;

@@ -16,6 +16,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <array>
+#include <string>
+#include <vector>
+
 #include "../../include/gs_api.h"
 
 #ifndef GS_RENDERER_USE_REFERENCE_TYPES
@@ -28,7 +32,50 @@ struct Gaussian {
     float opacity;
     float sh_rest[GAUSSIAN_SH_REST_FLOATS];
 };
-struct GSAnim;
+// GSAnim as the loader fills it (src/gsanim.h:9-48): the renderer reads the
+// coefficient data, the keys and the playback state.
+struct GSAnimKey {
+    double time_seconds;
+    uint32_t state;
+    bool linear = false;  // outgoing interval interpolates (version 2)
+    bool has_interpolation = false;
+};
+struct GSAnimLayerKey {
+    double time_seconds = 0;
+    std::array<float, 3> weight{};
+    bool linear = false;
+};
+struct GSAnimLayer {
+    std::string name;
+    std::vector<GSAnimLayerKey> keys;
+};
+struct GSAnim {
+    int version = 1;
+    std::vector<GSAnimLayer> layers;
+    std::vector<uint16_t> layer_data;   // half bits, layer/row/channel-major, 48 per row
+    std::vector<uint16_t> display_lut;  // RGB uint16, red fastest
+    uint32_t lut_size = 0;
+    float log2_min = 0, log2_max = 0;
+    std::vector<int8_t> deltas;         // state/row/channel-major, state 0 excluded
+    float scales[48] = {};
+    uint32_t state_count = 0;
+    uint32_t gaussian_count = 0;
+    double duration = 0;
+    std::vector<GSAnimKey> keys;
+    bool linear = false;
+    double playback_time = 0;
+    bool playing = true;
+    int selected_state = -1;
+};
+struct GSAnimSample {
+    int state_a, state_b;
+    float weight;
+};
+// The states and blend weight at `seconds` (src/gsanim.h:60-62), and the
+// RGB weight of every layer of a version-3 asset (src/gsanim.h:64-65).
+GSAnimSample gsanim_sample(const GSAnim& animation, double seconds);
+std::vector<std::array<float, 3>> gsanim_layer_weights(const GSAnim& animation, double seconds);
+
 struct GaussianScene {
     GSAnim* animation;
     Gaussian* gaussians;
@@ -109,6 +156,9 @@ struct Renderer {
     float exposure_ev;
     bool first_layer_only;
     double animation_time;
+    int animation_state_a;
+    int animation_state_b;
+    float animation_weight;
     bool black_background;
     size_t gpu_budget_bytes;
     bool gps_work_budget_resize_failed;
@@ -155,3 +205,9 @@ void renderer_destroy(Renderer* r);
 // first, of the appearance image (sorted alpha) or the accumulation / GPS
 // sample image (stochastic modes).  out holds width*height*4 floats.
 bool renderer_read_linear(Renderer* r, float* out, size_t floats);
+
+// What renderer_display_appearance shows (src/renderer.cpp:2120-2137): for a
+// GSAnim version-3 scene the linear image through the asset's LUT at
+// exposure_ev, otherwise the linear image itself.  Same size and order as
+// renderer_read_linear.
+bool renderer_read_display(Renderer* r, const GaussianScene* scene, float* out, size_t floats);

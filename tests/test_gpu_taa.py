@@ -8,6 +8,7 @@ import pytest
 
 import gamesplat_amd as gsm
 from gamesplat_amd import abi, scenes
+from headless_util import headless_position
 
 pytestmark = pytest.mark.gpu
 
@@ -160,36 +161,6 @@ def test_taa_errors(gs):
         c.read_image(abi.GS_IMAGE_TAA_RGBA32F)
 
 
-def _headless_position(seed, frame, radius=15.0, orbit_deg=0.0):
-    """The headless driver's camera position: splitmix64 -> Box-Muller
-    direction, rotated about z by orbit_deg per frame (double, then float)."""
-    import math
-
-    mask = (1 << 64) - 1
-    state = [seed & mask]
-
-    def nxt():
-        state[0] = (state[0] + 0x9E3779B97F4A7C15) & mask
-        z = state[0]
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
-        return ((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0)
-
-    d = [0.0, 0.0, 0.0]
-    for k in (0, 2):
-        u1 = 1.0 - nxt()
-        u2 = nxt()
-        rr = math.sqrt(-2.0 * math.log(u1))
-        d[k] = rr * math.cos(6.283185307179586 * u2)
-        if k + 1 < 3:
-            d[k + 1] = rr * math.sin(6.283185307179586 * u2)
-    dn = math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
-    a = orbit_deg * math.pi / 180.0 * frame
-    c, s = math.cos(a), math.sin(a)
-    return np.array([(c * d[0] - s * d[1]) / dn * radius, (s * d[0] + c * d[1]) / dn * radius,
-                     d[2] / dn * radius], np.float32)
-
-
 def test_renderer_mirror_taa_matches_library_sequence(gs, tmp_path):
     """renderer_draw_frame's StochasticSplats + TAA glue (headless driver,
     taa=1): the same frame seeds, jitters, frame averages and TAA uniforms
@@ -214,7 +185,7 @@ def test_renderer_mirror_taa_matches_library_sequence(gs, tmp_path):
     with gs.Context(0) as c:
         c.upload(g)
         for f in range(warm + frames):
-            pose = gs.look_at_origin_pose(_headless_position(seed, f, orbit_deg=2.0))
+            pose = gs.look_at_origin_pose(headless_position(seed, f, orbit_deg=2.0))
             cam = gs.camera_from_pose(pose, np.float32(0.8660254), W, H)
             changed = prev is not None and bytes(prev) != bytes(cam)
             u = gs.taa_uniforms(cam, prev, changed)
