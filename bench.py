@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Stage whose HIP-event bracket holds exactly one launch of this kernel.
-STAGE_KERNEL = {"cull": "cull_kernel", "sort": "downsweep_kernel<8>", "composite": "composite_kernel<16,true>"}
+STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<8>", "composite": "composite_kernel<16,true>"}
 PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
 
 
