@@ -154,6 +154,8 @@ class GsFrameInfo(C.Structure):
         ("graph_captures", C.c_uint32),
         ("graph_replays", C.c_uint32),
         ("overflow_reruns", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("held_bytes", C.c_uint64),
     ]
 
 

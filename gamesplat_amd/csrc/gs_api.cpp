@@ -1210,6 +1210,7 @@ int gs_frame_info(gs_ctx* c, GsFrameInfo* out) {
     out->graph_captures = c->graph_captures;
     out->graph_replays = c->graph_replays;
     out->overflow_reruns = c->overflow_reruns;
+    out->held_bytes = c->held;
     return ok();
 }
 

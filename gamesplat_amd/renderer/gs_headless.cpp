@@ -10,13 +10,19 @@
 //   ply=<path>                 3DGS binary PLY; otherwise a synthetic scene:
 //   gaussians=[100000] clusters=[16] seed=[0x5EED0001] opacity_mean=[0]
 //   width=[1280] height=[720] render_mode=[alpha|gps|stochastic] stochastic_spp=[1]
-//   taa=[0] taa_samples=[4]    StochasticSplats with TAA (samples per frame)
 //   view_seed=[seed] radius=[15] orbit_deg=[0]   camera on a seeded direction
 //   gps_ss=[1] gps_budget_m=[16] gps_accumulation=[1] sh_degree=[3]
 //   gpu_budget_mb=[0] diagnostics=[0]
 //   warmup=[3] frames=[20]     untimed then timed frame counts
+//   taa=[off|on|0|1] taa_samples=[4]   StochasticSplats with TAA
 //   linear_capture=<path>      raw f32 RGBA of the last frame
 //   display_capture=<path>     raw f32 RGBA of what the frame displays (LUT for layered GSAnim)
+//   capture=<path>             binary PPM (P6) of the displayed frame, 8-bit, top row first
+//   csv=<path>                 per timed frame: wall_ms, cpu_submit_ms, gpu_elapsed_ms,
+//                              animation_time, state, selected_state, playing
+//   fixed_time=<s>             GSAnim playback pinned at s (not playing)
+//   exercise_changes=1         GSAnim: frames cycle selected_state, the last frame plays
+//   exercise_modes=1           the first three frames render alpha, stochastic, GPS, then render_mode
 // GSAnim appearance from raw files (the loader's arrays):
 //   anim_deltas=<path> anim_scales=<path> anim_states=N       int8 deltas, 48 f32 scales
 //   anim_layers=<path> anim_layer_count=L                      half coefficients
@@ -247,7 +253,8 @@ int main(int argc, char** argv) {
     r.splat_render_mode = mode == "gps"          ? SplatRenderMode::GaussianPointSplatting
                           : mode == "stochastic" ? SplatRenderMode::StochasticSplats
                                                  : SplatRenderMode::AlphaBlendSorted;
-    r.stochastic_taa_enabled = argi("taa", 0) != 0;
+    const std::string taa = arg("taa", "0");
+    r.stochastic_taa_enabled = taa == "on" || taa == "1";
     r.stochastic_taa_current_samples = (uint32_t)argi("taa_samples", 4);
     r.stochastic_samples_per_frame = (uint32_t)argi("stochastic_spp", 1);
     r.black_background = true;
@@ -259,6 +266,8 @@ int main(int argc, char** argv) {
     r.splat_diagnostics_enabled = argi("diagnostics", 0) != 0;
     r.first_layer_only = argi("first_layer_only", 0) != 0;
     r.exposure_ev = (float)argf("exposure_ev", 0.0);
+    size_t free0 = 0, total0 = 0;
+    hipMemGetInfo(&free0, &total0);
     if (!renderer_upload_gaussians(&r, &scene)) return 1;
     double startup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
@@ -281,35 +290,97 @@ int main(int argc, char** argv) {
     const double radius = argf("radius", 15.0), orbit = argf("orbit_deg", 0.0);
 
     const int warmup = (int)argi("warmup", 3), frames = (int)argi("frames", 20);
+    const SplatRenderMode final_mode = r.splat_render_mode;
     hipStream_t stream = (hipStream_t)gs_stream(r.ctx);
     std::vector<hipEvent_t> ev(2 * frames);
     for (auto& e : ev) hipEventCreate(&e);
-    std::vector<double> wall;
+    struct FrameRecord {
+        double wall_ms, cpu_ms, animation_time;
+        int state, selected_state, playing;
+    };
+    std::vector<FrameRecord> rec;
     CameraUniforms cam;
+    auto previous = std::chrono::steady_clock::now();
     for (int f = 0; f < warmup + frames; ++f) {
+        const auto begin = std::chrono::steady_clock::now();
         double a = orbit * M_PI / 180.0 * f, c = cos(a), sn = sin(a);
         float pos[3] = {(float)((c * d[0] - sn * d[1]) / dn * radius), (float)((sn * d[0] + c * d[1]) / dn * radius),
                         (float)(d[2] / dn * radius)};
         float pose[16];
         gs_look_at_origin_pose(pos, pose);
         gs_camera_from_pose(pose, 0.8660254f, (float)width, (float)height, reinterpret_cast<GsCameraUniforms*>(&cam));
-        if (scene.animation) animation.playback_time = argf("anim_time", 0.0) + argf("anim_time_step", 0.0) * f;
-        bool timed = f >= warmup;
-        auto t0 = std::chrono::steady_clock::now();
+        if (scene.animation) {
+            if (g_args.count("fixed_time")) {
+                animation.playing = false;
+                animation.playback_time = argf("fixed_time", 0.0);
+            } else {
+                animation.playback_time = argf("anim_time", 0.0) + argf("anim_time_step", 0.0) * f;
+            }
+            if (argi("exercise_changes", 0) && animation.state_count) {
+                const bool last = f == warmup + frames - 1;
+                animation.selected_state = last ? -1 : f % (int)animation.state_count;
+            }
+        }
+        if (argi("exercise_modes", 0)) {
+            // the harness's mode exercise: alpha, stochastic, GPS, then the requested mode
+            const SplatRenderMode m = f < 3 ? SplatRenderMode(f) : final_mode;
+            if (r.splat_render_mode != m) {
+                r.splat_render_mode = m;
+                renderer_reset_stochastic_accumulation(&r);
+            }
+        }
+        const bool timed = f >= warmup;
         if (timed) hipEventRecord(ev[2 * (f - warmup)], stream);
         renderer_draw_frame(&r, &scene, &cam, nullptr, nullptr);
         if (timed) hipEventRecord(ev[2 * (f - warmup) + 1], stream);
         if (r.last_frame_status != GS_OK) return 1;
-        if (timed) wall.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        const auto submitted = std::chrono::steady_clock::now();
+        if (timed)
+            rec.push_back({std::chrono::duration<double, std::milli>(begin - previous).count(),
+                           std::chrono::duration<double, std::milli>(submitted - begin).count(),
+                           scene.animation ? animation.playback_time : 0.0, r.animation_state_a,
+                           scene.animation ? animation.selected_state : -1,
+                           scene.animation && animation.playing ? 1 : 0});
+        previous = begin;
     }
     gs_synchronize(r.ctx);
-    std::vector<double> gpu;
+    std::vector<double> wall, cpu, gpu;
     for (int f = 0; f < frames; ++f) {
         float ms = 0.0f;
         hipEventElapsedTime(&ms, ev[2 * f], ev[2 * f + 1]);
         gpu.push_back(ms);
+        wall.push_back(rec[f].wall_ms);
+        cpu.push_back(rec[f].cpu_ms);
     }
     for (auto& e : ev) hipEventDestroy(e);
+    const char* csv_path = arg("csv", "");
+    if (csv_path[0]) {
+        FILE* f = fopen(csv_path, "w");
+        if (!f) return 1;
+        fprintf(f, "wall_ms,cpu_submit_ms,gpu_elapsed_ms,animation_time,state,selected_state,playing\n");
+        for (int k = 0; k < frames; ++k)
+            fprintf(f, "%.6f,%.6f,%.6f,%.9f,%d,%d,%d\n", rec[k].wall_ms, rec[k].cpu_ms, gpu[k], rec[k].animation_time,
+                    rec[k].state, rec[k].selected_state, rec[k].playing);
+        fclose(f);
+    }
+    const char* ppm = arg("capture", "");
+    if (ppm[0]) {
+        // displayed values, 8-bit UNORM (round to nearest), as glReadPixels of the swapchain
+        std::vector<float> px((size_t)width * height * 4);
+        if (!renderer_read_display(&r, &scene, px.data(), px.size())) return 1;
+        std::vector<unsigned char> rgb((size_t)width * height * 3);
+        for (size_t i = 0; i < (size_t)width * height; ++i)
+            for (int k = 0; k < 3; ++k) {
+                float v = px[i * 4 + k];
+                v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                rgb[i * 3 + k] = (unsigned char)lrintf(v * 255.0f);
+            }
+        FILE* f = fopen(ppm, "wb");
+        if (!f) return 1;
+        fprintf(f, "P6\n%d %d\n255\n", width, height);
+        if (fwrite(rgb.data(), 1, rgb.size(), f) != rgb.size()) return 1;
+        fclose(f);
+    }
 
     const char* cap = arg("linear_capture", "");
     if (cap[0]) {
@@ -329,12 +400,18 @@ int main(int argc, char** argv) {
     }
     GsStats st;
     gs_stats(r.ctx, &st);
-    printf("RESULT {\"frames\":%zu,\"wall_median_ms\":%.6f,\"wall_p95_ms\":%.6f,\"gpu_median_ms\":%.6f,"
-           "\"gpu_p95_ms\":%.6f,\"startup_ms\":%.3f,\"gaussians\":%u,\"visible\":%u,\"tile_entries\":%u,"
-           "\"gps_requested\":%u,\"gps_exhausted\":%u}\n",
-           wall.size(), percentile(wall, .5), percentile(wall, .95), percentile(gpu, .5), percentile(gpu, .95),
-           startup_ms, st.gaussian_count, st.visible_count, st.tile_entries, st.gps_requested_work_items,
-           st.gps_work_budget_exhausted);
+    GsFrameInfo fi;
+    gs_frame_info(r.ctx, &fi);
+    size_t free1 = 0, total1 = 0;
+    hipMemGetInfo(&free1, &total1);
+    printf("RESULT {\"frames\":%zu,\"wall_median_ms\":%.6f,\"wall_p95_ms\":%.6f,\"cpu_median_ms\":%.6f,"
+           "\"cpu_p95_ms\":%.6f,\"gpu_median_ms\":%.6f,\"gpu_p95_ms\":%.6f,\"startup_ms\":%.3f,"
+           "\"vram_free_delta_mib\":%.3f,\"scene_buffers_mib\":%.3f,\"gaussians\":%u,\"visible\":%u,"
+           "\"tile_entries\":%u,\"gps_requested\":%u,\"gps_exhausted\":%u}\n",
+           wall.size(), percentile(wall, .5), percentile(wall, .95), percentile(cpu, .5), percentile(cpu, .95),
+           percentile(gpu, .5), percentile(gpu, .95), startup_ms, ((double)free0 - (double)free1) / 1048576.0,
+           (double)fi.held_bytes / 1048576.0, st.gaussian_count, st.visible_count, st.tile_entries,
+           st.gps_requested_work_items, st.gps_work_budget_exhausted);
     renderer_destroy(&r);
     return 0;
 }

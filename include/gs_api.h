@@ -334,6 +334,8 @@ typedef struct GsFrameInfo {
     uint32_t graph_captures;   /* hipGraph captures so far   */
     uint32_t graph_replays;    /* replays of a captured frame */
     uint32_t overflow_reruns;  /* frames re-binned after a tile-entry overflow */
+    uint32_t pad;
+    uint64_t held_bytes;       /* device memory the context holds */
 } GsFrameInfo;
 int gs_frame_info(gs_ctx* ctx, GsFrameInfo* out);
 

@@ -1,6 +1,7 @@
 """C ABI behaviour on the GPU: error paths forced from the host (memory budget
 refusals, budget resize that keeps the old budget, calls out of order),
 repeatability, and the headless renderer shim's frame sequence."""
+import json
 import subprocess
 from pathlib import Path
 
@@ -100,3 +101,32 @@ def test_headless_driver_runs(tmp_path):
         assert line and '"frames":4' in line[0]
         img = np.fromfile(cap, np.float32).reshape(240, 320, 4)
         assert np.isfinite(img).all() and img[..., 3].max() > 0
+
+
+def test_headless_harness_outputs(tmp_path):
+    """The harness counterpart's PPM capture is the displayed frame at 8 bits
+    (here the linear image rounded), its CSV has one row per timed frame,
+    RESULT carries the harness's fields, and the mode exercise runs."""
+    exe = ROOT / "gamesplat_amd" / "gs_headless"
+    ppm, lin, csv = tmp_path / "f.ppm", tmp_path / "f.f32", tmp_path / "f.csv"
+    print("case: headless PPM / CSV / RESULT fields", flush=True)
+    r = subprocess.run([str(exe), "gaussians=20000", "width=200", "height=150", "warmup=1", "frames=5",
+                        f"capture={ppm}", f"linear_capture={lin}", f"csv={csv}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = json.loads([l for l in r.stdout.splitlines() if l.startswith("RESULT ")][0][7:])
+    for k in ("frames", "wall_median_ms", "cpu_median_ms", "gpu_median_ms", "gpu_p95_ms", "startup_ms",
+              "vram_free_delta_mib", "scene_buffers_mib"):
+        assert k in res, k
+    assert res["frames"] == 5 and res["scene_buffers_mib"] > 0
+    data = ppm.read_bytes()
+    assert data.startswith(b"P6\n200 150\n255\n")
+    img = np.frombuffer(data[len(b"P6\n200 150\n255\n"):], np.uint8).reshape(150, 200, 3)
+    want = np.rint(np.clip(np.fromfile(lin, np.float32).reshape(150, 200, 4)[..., :3], 0, 1) * 255)
+    np.testing.assert_array_equal(img, want.astype(np.uint8))
+    rows = csv.read_text().splitlines()
+    assert rows[0] == "wall_ms,cpu_submit_ms,gpu_elapsed_ms,animation_time,state,selected_state,playing"
+    assert len(rows) == 6
+    r = subprocess.run([str(exe), "gaussians=20000", "width=160", "height=120", "warmup=0", "frames=5",
+                        "exercise_modes=1", "render_mode=gps"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
