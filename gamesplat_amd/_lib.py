@@ -116,6 +116,21 @@ class GsAnimation(C.Structure):
     ]
 
 
+class GsPlyLayout(C.Structure):
+    _fields_ = [("stride", C.c_uint32), ("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32),
+                ("scale", C.c_int32 * 3), ("rot", C.c_int32 * 4), ("opacity", C.c_int32), ("dc", C.c_int32 * 3),
+                ("rest", C.c_int32 * 45)]
+
+
+class GsSogData(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("means_mins", C.c_float * 3), ("means_maxs", C.c_float * 3),
+                ("scales_codebook", C.c_float * 256), ("sh0_codebook", C.c_float * 256),
+                ("means_l", C.c_void_p), ("means_u", C.c_void_p), ("scales", C.c_void_p), ("quats", C.c_void_p),
+                ("sh0", C.c_void_p), ("shN_bands", C.c_int32), ("shN_count", C.c_int32),
+                ("shN_codebook", C.c_float * 256), ("shN_centroids", C.c_void_p), ("centroids_width", C.c_int32),
+                ("centroids_height", C.c_int32), ("shN_labels", C.c_void_p)]
+
+
 class GsSplatEffectParams(C.Structure):
     _fields_ = [("center_radius", C.c_float * 4), ("params", C.c_float * 4), ("color", C.c_float * 4)]
 
@@ -187,6 +202,8 @@ EXPORTS = {
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),
     "gs_stochastic_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "gs_taa_jitter": (None, [C.c_uint32, C.POINTER(C.c_float)]),
+    "gs_decode_ply": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(GsPlyLayout), C.c_void_p]),
+    "gs_decode_sog": (C.c_int, [C.POINTER(GsSogData), C.c_void_p]),
     "gs_upload_animation": (C.c_int, [C.c_void_p, C.POINTER(GsAnimation)]),
     "gs_set_animation_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
                                          C.POINTER(C.c_float)]),

@@ -63,8 +63,8 @@ long long argi(const char* k, long long dflt) {
     return it == g_args.end() ? dflt : strtoll(it->second.c_str(), nullptr, 0);
 }
 
-// Binary little-endian 3DGS PLY with float properties, activated as
-// src/gaussian_loader_ply.cpp:143-203 does.
+// Binary little-endian 3DGS PLY with float properties; the rows are
+// activated by gs_decode_ply as src/gaussian_loader_ply.cpp:143-203 does.
 bool load_ply(const char* path, std::vector<Gaussian>& out) {
     FILE* f = fopen(path, "rb");
     if (!f) return false;
@@ -100,41 +100,26 @@ bool load_ply(const char* path, std::vector<Gaussian>& out) {
         fclose(f);
         return false;
     }
-    int os[3] = {find("scale_0"), find("scale_1"), find("scale_2")};
-    int orot[4] = {find("rot_0"), find("rot_1"), find("rot_2"), find("rot_3")};
-    int oop = find("opacity");
-    int odc[3] = {find("f_dc_0"), find("f_dc_1"), find("f_dc_2")};
-    int orest[45];
-    for (int k = 0; k < 45; ++k) orest[k] = find("f_rest_" + std::to_string(k));
-    size_t stride = props.size();
+    GsPlyLayout lay;
+    memset(&lay, 0, sizeof(lay));
+    lay.stride = (uint32_t)props.size();
+    lay.x = ox;
+    lay.y = oy;
+    lay.z = oz;
+    for (int a = 0; a < 3; ++a) lay.scale[a] = find("scale_" + std::to_string(a));
+    for (int a = 0; a < 4; ++a) lay.rot[a] = find("rot_" + std::to_string(a));
+    lay.opacity = find("opacity");
+    for (int a = 0; a < 3; ++a) lay.dc[a] = find("f_dc_" + std::to_string(a));
+    for (int k = 0; k < 45; ++k) lay.rest[k] = find("f_rest_" + std::to_string(k));
+    const size_t stride = props.size();
     out.resize(count);
-    std::vector<float> row(stride * 4096);
-    for (uint32_t i = 0; i < count; i += 4096) {
-        uint32_t b = std::min(4096u, count - i);
-        if (fread(row.data(), stride * 4, b, f) != b) {
+    std::vector<float> rows(stride * 65536);
+    for (uint32_t i = 0; i < count; i += 65536) {
+        const uint32_t b = std::min(65536u, count - i);
+        if (fread(rows.data(), stride * 4, b, f) != b ||
+            gs_decode_ply(rows.data(), b, &lay, reinterpret_cast<GsGaussian*>(out.data() + i)) != GS_OK) {
             fclose(f);
             return false;
-        }
-        for (uint32_t k = 0; k < b; ++k) {
-            const float* v = row.data() + (size_t)k * stride;
-            Gaussian& g = out[i + k];
-            g.position[0] = v[ox]; g.position[1] = v[oy]; g.position[2] = v[oz];
-            for (int a = 0; a < 3; ++a) g.scale[a] = os[0] >= 0 ? expf(v[os[a]]) : 0.01f;
-            if (orot[0] >= 0) {
-                float rw = v[orot[0]], rx = v[orot[1]], ry = v[orot[2]], rz = v[orot[3]];
-                float len = sqrtf(rw * rw + rx * rx + ry * ry + rz * rz);
-                if (len > 1e-8f) { rw /= len; rx /= len; ry /= len; rz /= len; }
-                g.rotation[0] = rw; g.rotation[1] = rx; g.rotation[2] = ry; g.rotation[3] = rz;
-            } else {
-                g.rotation[0] = 1; g.rotation[1] = g.rotation[2] = g.rotation[3] = 0;
-            }
-            g.opacity = oop >= 0 ? 1.0f / (1.0f + expf(-v[oop])) : 1.0f;
-            for (int a = 0; a < 3; ++a) g.color[a] = odc[0] >= 0 ? v[odc[a]] : 0.0f;
-            for (int j = 0; j < 15; ++j)
-                for (int c = 0; c < 3; ++c) {
-                    int o = orest[c * 15 + j];
-                    g.sh_rest[j * 3 + c] = o >= 0 ? v[o] : 0.0f;
-                }
         }
     }
     fclose(f);

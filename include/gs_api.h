@@ -447,6 +447,57 @@ int gs_reset_stage_times(gs_ctx* ctx);
 /* Opaque hipStream_t of the context, for event timing by the caller. */
 void* gs_stream(gs_ctx* ctx);
 
+/* ---- scene decoding (host only) ------------------------------------------ */
+
+/* Float offsets of the 3DGS properties within one binary PLY vertex row
+ * (-1 = absent); rest[k] is f_rest_k (all red coefficients, then green, then
+ * blue). */
+typedef struct GsPlyLayout {
+    uint32_t stride;  /* floats per row */
+    int32_t x, y, z;
+    int32_t scale[3];
+    int32_t rot[4];
+    int32_t opacity;
+    int32_t dc[3];
+    int32_t rest[45];
+} GsPlyLayout;
+
+/* Activation of PLY rows into Gaussians as the reference's PLY loader does
+ * (src/gaussian_loader_ply.cpp:143-203): exp of the log scales (0.01 when
+ * absent), normalised quaternion (w, x, y, z; identity when absent), sigmoid
+ * opacity (1 when absent), raw DC, rest coefficients repacked from
+ * channel-major to RGB triples. */
+int gs_decode_ply(const float* rows, uint32_t count, const GsPlyLayout* layout, GsGaussian* out);
+
+/* A SOG v2 scene with its WebP images already decoded to RGBA8 (lossless:
+ * any decoder gives the same pixels) and meta.json parsed. */
+typedef struct GsSogData {
+    uint32_t count;
+    float means_mins[3], means_maxs[3];
+    float scales_codebook[256];
+    float sh0_codebook[256];
+    const uint8_t* means_l;   /* each RGBA8 with at least count pixels, row-major */
+    const uint8_t* means_u;
+    const uint8_t* scales;
+    const uint8_t* quats;
+    const uint8_t* sh0;
+    int32_t shN_bands;        /* 0 = no higher bands, else 1..3 */
+    int32_t shN_count;        /* palette entries, 1..65536 */
+    float shN_codebook[256];
+    const uint8_t* shN_centroids;  /* RGBA8, 64 * coefficients wide */
+    int32_t centroids_width, centroids_height;
+    const uint8_t* shN_labels;     /* RGBA8, at least count pixels */
+} GsSogData;
+
+/* Per-Gaussian decode of the reference's SOG loader
+ * (src/gaussian_loader_sog.cpp:403-510): 16-bit means lerped between the
+ * bounds and un-logged (sign(n) (e^|n| - 1)); scales exp(codebook); DC from
+ * its codebook; smallest-three quaternion (the mode byte 252..255 names the
+ * omitted component; stored x, y, z, w in rotation[0..3] as the reference
+ * keeps it); opacity = alpha / 255; palette SH by 16-bit label.  An invalid
+ * quaternion mode or label is GS_ERR_INVALID. */
+int gs_decode_sog(const GsSogData* sog, GsGaussian* out);
+
 /* ---- synthetic scenes and cameras (host only) --------------------------- */
 
 /* Seeded synthetic scene as the reference's PLY loader would activate it
