@@ -193,9 +193,13 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
 }
 
 // One front-to-back blend step of pixel (px, py), branch-free: coverage (the
-// pixel rectangle) and the fragment test (power in [cutoff, 0]) select the
-// alpha or 0, and a zero alpha leaves colour and transmittance bit-for-bit
-// unchanged.  A saturated pixel (T < 1e-4) keeps blending until its whole
+// pixel rectangle) and the fragment test (power >= cutoff) select the alpha
+// or 0, and a zero alpha leaves colour and transmittance bit-for-bit
+// unchanged.  The fragment shader's power > 0 discard cannot fire here: the
+// cull's +0.3 dilation keeps det >= 0.09 for every splat (the 1e-6 clamp is
+// never reached), so the conic is positive definite and power <= 0 in exact
+// arithmetic; where the fused evaluation rounds to a value just above 0 the
+// FP64 composite blends the splat too.  A saturated pixel (T < 1e-4) keeps blending until its whole
 // wave is saturated; what it adds is below 1e-4 of a colour, and it only
 // brings the result closer to blending every splat.  The quadratic form is evaluated with
 // fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy, in
@@ -205,7 +209,7 @@ __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, i
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, (par.x * dy) * dy);
     float alpha = minf(par.y * __builtin_amdgcn_exp2f(power), 0.99f);
-    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power <= 0.0f) & (power >= par.z);
+    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power >= par.z);
     alpha = take ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
