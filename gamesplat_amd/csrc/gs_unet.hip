@@ -59,21 +59,36 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     const size_t img = (size_t)n * a.h;
 
     // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the image);
-    // with x2, channels below split are read from the half-size tensor
+    // with x2, channels below split are read from the half-size tensor.  Each
+    // thread keeps kStageBatch loads in flight before it writes them to LDS.
+    constexpr int kStageBatch = 8;
     const size_t img2 = (size_t)n * (a.h / 2);
-    for (int i = tid; i < TR * TW * groups; i += 256) {
-        const int g = i % groups, pix = i / groups;
-        const int tx = pix % TW, ty = pix / TW;
-        const int yy = y0 + ty - 1, xx = x0 + tx - 1;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
-            if (a.x2 && g * 8 < a.split)
-                v = *reinterpret_cast<const uint4*>(a.x2 + ((img2 + yy / 2) * (a.w / 2) + xx / 2) * a.x2s + a.x2o +
-                                                    g * 8);
-            else
-                v = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
+    const int total = TR * TW * groups;
+    for (int i0 = tid; i0 < total; i0 += 256 * kStageBatch) {
+        uint4 v[kStageBatch];
+        int dst[kStageBatch];
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j) {
+            const int i = i0 + j * 256;
+            v[j] = make_uint4(0u, 0u, 0u, 0u);
+            dst[j] = -1;
+            if (i < total) {
+                const int g = i % groups, pix = i / groups;
+                const int tx = pix % TW, ty = pix / TW;
+                const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+                dst[j] = pix * cp + g * 8;
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+                    if (a.x2 && g * 8 < a.split)
+                        v[j] = *reinterpret_cast<const uint4*>(a.x2 + ((img2 + yy / 2) * (a.w / 2) + xx / 2) * a.x2s +
+                                                               a.x2o + g * 8);
+                    else
+                        v[j] = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
+                }
+            }
         }
-        *reinterpret_cast<uint4*>(s_in + (size_t)pix * cp + g * 8) = v;
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j)
+            if (dst[j] >= 0) *reinterpret_cast<uint4*>(s_in + dst[j]) = v[j];
     }
     __syncthreads();
 
