@@ -18,6 +18,8 @@
 // weights transposed per tap and the taps reversed.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -41,9 +43,61 @@ __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 u) { return __builtin_bit_ca
 // Block tile: 32 pixels wide, 4 * RG / 2 rows (RG 16-pixel row groups per
 // wave).  The input tile with a one-pixel halo is staged in LDS once (channel
 // pitch cin + 8 spreads the 16-byte reads of consecutive pixels over the
-// banks); all 9 taps then read their A operands from LDS, and the weights
-// (B operands) come through the L1/L2 caches.
+// banks), followed by a table that maps each group of 8 k to its LDS offset
+// (tap row * TW + tap column) * pitch + channel.  The MFMA's A operand is the
+// weights (M = output channel, through the L1/L2 caches) and its B operand
+// the input pixels (N = pixel, from LDS), so a lane's accumulator holds four
+// consecutive output channels of one pixel: one packed 8-byte store each.
+// k beyond 9 * cin (the zero padding of K) reads channel 0 of the tile: the
+// padded weights are zero, so those products add nothing.
 constexpr int kConvTileW = 32;
+
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t v = {(__bf16)lo, (__bf16)hi};  // round to nearest even
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// Value of lane ^ 1 (the neighbouring pixel of a row group).
+__device__ __forceinline__ uint32_t lane_xor1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+}
+
+// Stages the tile pixels p0, p0 + ppass, ... (below TR * TW) of one channel
+// group into LDS at pitch cp, eight loads in flight per thread.  src points at
+// the group's channels of the image; pixel (yy, xx) of the tile is read at
+// ((yy >> sh) * sw + (xx >> sh)) * ss.  With CHECK, pixels outside the h x w
+// image are skipped (the caller has zeroed the tile).  A skipped element
+// loads the image's first pixel and writes it to the 16-byte dummy slot
+// s_dummy: every load and store is unconditional, so all loads of a batch
+// are issued before the first store waits.
+template <int TR, int TW, bool CHECK>
+__device__ __forceinline__ void stage_tile(uint16_t* s_dst, int cp, uint16_t* s_dummy, const uint16_t* src, int sh,
+                                           int sw, int ss, int y0, int x0, int h, int w, int p0, int ppass) {
+    constexpr int kBatch = 8, kTilePix = TR * TW;
+    int tx = p0 % TW, ty = p0 / TW;
+    const int dtx = ppass % TW, dty = ppass / TW;
+    for (int pb = p0; pb < kTilePix; pb += ppass * kBatch) {
+        typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+        u32x4_t v[kBatch];
+        uint16_t* dst[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int yy = y0 + ty - 1, xx = x0 + tx - 1, p = pb + j * ppass;
+            bool ok = p < kTilePix;
+            if (CHECK) ok = ok && (unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w;
+            dst[j] = ok ? s_dst + p * cp : s_dummy;
+            v[j] = *reinterpret_cast<const u32x4_t*>(src + (ok ? ((yy >> sh) * sw + (xx >> sh)) * ss : 0));
+            tx += dtx;
+            ty += dty;
+            if (tx >= TW) { tx -= TW; ++ty; }
+        }
+        // keep the scheduler from pairing each load with its store
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) *reinterpret_cast<u32x4_t*>(dst[j]) = v[j];
+    }
+}
 
 template <int NT, int RG>
 __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
@@ -56,120 +110,164 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     const int x0 = (int)blockIdx.x * kConvTileW;
     const int n = (int)blockIdx.z;
     const int cin = a.cin, cp = cin + 8, groups = cin / 8;
-    const size_t img = (size_t)n * a.h;
+    uint16_t* s_dummy = s_in + TR * TW * cp;  // 16 bytes
+    int* s_tab = reinterpret_cast<int*>(s_dummy + 8);
 
-    // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the image);
-    // with x2, channels below split are read from the half-size tensor.  Each
-    // thread keeps kStageBatch loads in flight before it writes them to LDS.
-    constexpr int kStageBatch = 8;
-    const size_t img2 = (size_t)n * (a.h / 2);
-    const int total = TR * TW * groups;
-    for (int i0 = tid; i0 < total; i0 += 256 * kStageBatch) {
-        uint4 v[kStageBatch];
-        int dst[kStageBatch];
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j) {
-            const int i = i0 + j * 256;
-            v[j] = make_uint4(0u, 0u, 0u, 0u);
-            dst[j] = -1;
-            if (i < total) {
-                const int g = i % groups, pix = i / groups;
-                const int tx = pix % TW, ty = pix / TW;
-                const int yy = y0 + ty - 1, xx = x0 + tx - 1;
-                dst[j] = pix * cp + g * 8;
-                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
-                    if (a.x2 && g * 8 < a.split)
-                        v[j] = *reinterpret_cast<const uint4*>(a.x2 + ((img2 + yy / 2) * (a.w / 2) + xx / 2) * a.x2s +
-                                                               a.x2o + g * 8);
-                    else
-                        v[j] = *reinterpret_cast<const uint4*>(a.x + ((img + yy) * a.w + xx) * a.xs + a.xo + g * 8);
-                }
-            }
+    // k-group table (kpad / 8 <= 256 entries)
+    if (tid < a.kpad / 8) {
+        const int k = tid * 8;
+        int off = 0;
+        if (k < 9 * cin) {
+            const int tap = k / cin, ci = k - tap * cin;
+            off = ((tap / 3) * TW + tap % 3) * cp + ci;
         }
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j)
-            if (dst[j] >= 0) *reinterpret_cast<uint4*>(s_in + dst[j]) = v[j];
+        s_tab[tid] = off;
+    }
+
+    // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the
+    // image).  Thread tid owns channel group tid % groups of every ppass-th
+    // tile pixel from tid / groups on, so its source (x, or with x2 the
+    // half-size tensor for channels below split), stride and upsample shift
+    // are fixed.  Offsets are 32-bit within one image (the host checks
+    // h * w * stride < 2^31).  Blocks clear of the image border skip the
+    // bounds checks; the others zero the tile first.
+    {
+        const int g = tid % groups, p0 = tid / groups, ppass = 256 / groups;
+        const bool up = a.x2 && g * 8 < a.split;
+        const int sh = up ? 1 : 0;
+        const int sw = up ? a.w / 2 : a.w, ss = up ? a.x2s : a.xs;
+        const uint16_t* src = up ? a.x2 + (size_t)n * (a.h / 2) * (a.w / 2) * a.x2s + a.x2o + g * 8
+                                 : a.x + (size_t)n * a.h * a.w * a.xs + a.xo + g * 8;
+        const bool interior = y0 >= 1 && x0 >= 1 && y0 + BR < a.h && x0 + kConvTileW < a.w;
+        if (interior) {
+            if (p0 < ppass)
+                stage_tile<TR, TW, false>(s_in + g * 8, cp, s_dummy, src, sh, sw, ss, y0, x0, a.h, a.w, p0, ppass);
+        } else {
+            const int words = TR * TW * cp / 8;
+            for (int i = tid; i < words; i += 256) reinterpret_cast<uint4*>(s_in)[i] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+            if (p0 < ppass)
+                stage_tile<TR, TW, true>(s_in + g * 8, cp, s_dummy, src, sh, sw, ss, y0, x0, a.h, a.w, p0, ppass);
+        }
     }
     __syncthreads();
 
     const int row = lane & 15, kg = lane >> 4;
     f32x4_t acc[RG][NT];
 #pragma unroll
-    for (int g = 0; g < RG; ++g)
+    for (int r = 0; r < RG; ++r)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[g][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < NT; ++t) acc[r][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
-    // this lane's k = kc + 8 kg as (tap, channel), advanced by 32 per chunk
-    const int K = 9 * cin;
-    int tap = 0, ci = kg * 8;
-    while (ci >= cin) { ci -= cin; ++tap; }
-    // weights of the next chunk are loaded while this chunk's MFMAs run
-    const uint16_t* wrow = a.wt + (size_t)row * a.kpad + kg * 8;
-    uint4 bnext[NT];
+    // LDS element offset of this lane's pixel in each row group (tap 0)
+    int pix[RG];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) bnext[t] = *reinterpret_cast<const uint4*>(wrow + (size_t)t * 16 * a.kpad);
-    for (int kc = 0; kc < a.kpad; kc += 32) {
-        const int kk = kc + kg * 8;
-        uint4 bv[NT];
+    for (int r = 0; r < RG; ++r) pix[r] = ((wave * RPW + r / 2) * TW + (r & 1) * 16 + row) * cp;
+    const uint16_t* wl = a.wt + row * a.kpad + kg * 8;
+    const int wtile = 16 * a.kpad;
+    const int chunks = a.kpad / 32;
+    auto load_w = [&](uint4 (&w)[NT], int c) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) bv[t] = bnext[t];
-        if (kc + 32 < a.kpad) {
+        for (int t = 0; t < NT; ++t) w[t] = *reinterpret_cast<const uint4*>(wl + t * wtile + c * 32);
+    };
+    auto chunk = [&](const uint4 (&w)[NT], int off) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-                bnext[t] = *reinterpret_cast<const uint4*>(wrow + (size_t)t * 16 * a.kpad + kc + 32);
-        }
-        const bool live = kk < K;
-        const int dy = tap / 3, dx = tap - dy * 3;  // 0..2: the halo offsets
-#pragma unroll
-        for (int g = 0; g < RG; ++g) {
-            const int r = wave * RPW + g / 2, c = (g & 1) * 16 + row;
-            uint4 av = make_uint4(0u, 0u, 0u, 0u);
-            if (live) av = *reinterpret_cast<const uint4*>(s_in + (size_t)((r + dy) * TW + c + dx) * cp + ci);
+        for (int r = 0; r < RG; ++r) {
+            const uint4 pv = *reinterpret_cast<const uint4*>(s_in + pix[r] + off);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-                acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(av), as_bf16x8(bv[t]), acc[g][t], 0,
+                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(w[t]), as_bf16x8(pv), acc[r][t], 0,
                                                                      0, 0);
         }
-        ci += 32;
-        while (ci >= cin && tap < 9) { ci -= cin; ++tap; }
+    };
+    // two chunks per iteration with ping-pong weight registers: the loads of
+    // chunks c + 2 and c + 3 are in flight while chunks c and c + 1 run
+    uint4 w0[NT], w1[NT];
+    load_w(w0, 0);
+    int off0 = s_tab[kg], off1 = 0;
+    if (chunks > 1) {
+        load_w(w1, 1);
+        off1 = s_tab[4 + kg];
     }
-
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + row;
-        const float b = a.bias ? a.bias[co] : 0.0f;
-        uint16_t q[RG][4];
-#pragma unroll
-        for (int g = 0; g < RG; ++g) {
-            const int y = y0 + wave * RPW + g / 2;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int xx = x0 + (g & 1) * 16 + kg * 4 + i;
-                float v = acc[g][t][i] + b;
-                if (a.relu) v = v > 0.0f ? v : 0.0f;
-                q[g][i] = f2bf(v);
-                if (y < a.h && xx < a.w) a.y[((img + y) * a.w + xx) * a.ys + a.yo + co] = q[g][i];
+    for (int c = 0; c < chunks; c += 2) {
+        chunk(w0, off0);
+        if (c + 2 < chunks) {
+            load_w(w0, c + 2);
+            off0 = s_tab[(c + 2) * 4 + kg];
+        }
+        if (c + 1 < chunks) {
+            chunk(w1, off1);
+            if (c + 3 < chunks) {
+                load_w(w1, c + 3);
+                off1 = s_tab[(c + 3) * 4 + kg];
             }
         }
-        // 2x2 average pool: with two rows per wave a lane holds whole 2x2
-        // blocks (rows of groups g and g + 2, pixels i and i + 1); summed in
-        // the order of pool2_kernel, so the result is the same bits
-        if (RG == 4 && a.pool) {
-            const int yp = (y0 + wave * RPW) / 2;
+    }
+
+    // acc[r][t][i] = output channel t * 16 + 4 kg + i of pixel `row` of row group r
+    float bias[NT][4];
 #pragma unroll
-            for (int g = 0; g < 2; ++g)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const int xx = x0 + g * 16 + kg * 4 + 2 * j;
-                    if (2 * yp + 1 >= a.h || xx + 1 >= a.w) continue;
-                    float sum = 0.0f;
-                    sum += bf2f(q[g][2 * j]);
-                    sum += bf2f(q[g][2 * j + 1]);
-                    sum += bf2f(q[g + 2][2 * j]);
-                    sum += bf2f(q[g + 2][2 * j + 1]);
-                    a.pool[(((size_t)n * (a.h / 2) + yp) * (a.w / 2) + xx / 2) * a.ps + a.po + co] = f2bf(sum * 0.25f);
-                }
+        for (int i = 0; i < 4; ++i) bias[t][i] = a.bias ? a.bias[t * 16 + 4 * kg + i] : 0.0f;
+    uint16_t* yb = a.y + (size_t)n * a.h * a.w * a.ys + a.yo + 4 * kg;
+    uint32_t q[RG][NT][2];
+#pragma unroll
+    for (int r = 0; r < RG; ++r) {
+        const int y = y0 + wave * RPW + r / 2, xx = x0 + (r & 1) * 16 + row;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[r][t][i] + bias[t][i];
+                if (a.relu) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+            }
+            q[r][t][0] = pack_bf16x2(v[0], v[1]);
+            q[r][t][1] = pack_bf16x2(v[2], v[3]);
         }
+        if (y < a.h && xx < a.w) {
+            uint16_t* yp = yb + (y * a.w + xx) * a.ys;
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                *reinterpret_cast<uint2*>(yp + t * 16) = make_uint2(q[r][t][0], q[r][t][1]);
+        }
+    }
+    // 2x2 average pool: with two rows per wave, row groups r and r + 2 are
+    // rows y and y + 1 of the same pixels, and the pixel x + 1 is the
+    // neighbouring lane; summed in the order of pool2_kernel, so the result
+    // is the same bits
+    if constexpr (RG == 4) {
+      if (a.pool) {
+        const int yp = (y0 + wave * RPW) / 2;
+        uint16_t* pb = a.pool + (size_t)n * (a.h / 2) * (a.w / 2) * a.ps + a.po + 4 * kg;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int xx = x0 + r * 16 + row;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                uint32_t o[2];
+#pragma unroll
+                for (int h2 = 0; h2 < 2; ++h2) {
+                    const uint32_t a0 = q[r][t][h2], b0 = lane_xor1(a0);
+                    const uint32_t a1 = q[r + 2][t][h2], b1 = lane_xor1(a1);
+                    float s[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int sh = e ? 0 : 16;
+                        float sum = __uint_as_float((a0 << sh) & 0xFFFF0000u);
+                        sum += __uint_as_float((b0 << sh) & 0xFFFF0000u);
+                        sum += __uint_as_float((a1 << sh) & 0xFFFF0000u);
+                        sum += __uint_as_float((b1 << sh) & 0xFFFF0000u);
+                        s[e] = sum * 0.25f;
+                    }
+                    o[h2] = pack_bf16x2(s[0], s[1]);
+                }
+                if (!(row & 1) && 2 * yp + 1 < a.h && xx + 1 < a.w)
+                    *reinterpret_cast<uint2*>(pb + (yp * (a.w / 2) + xx / 2) * a.ps + t * 16) = make_uint2(o[0], o[1]);
+            }
+        }
+      }
     }
 }
 
@@ -561,7 +659,11 @@ hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
     constexpr int BR = 4 * (RG / 2);
     if (a.pool && (RG != 4 || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
     if (a.x2 && (a.split % 8 || a.split > a.cin || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(BR + 2) * (kConvTileW + 2) * (a.cin + 8) * 2;
+    if (a.kpad % 32 || a.kpad / 8 > 256 || a.kpad < 9 * a.cin) return hipErrorInvalidValue;
+    // element offsets inside one image are 32-bit
+    if ((size_t)a.h * a.w * std::max(std::max(a.xs, a.ys), a.x2 ? a.x2s : 0) >= (1u << 31)) return hipErrorInvalidValue;
+    if (a.pool && (size_t)a.h * a.w * a.ps >= (1u << 31)) return hipErrorInvalidValue;
+    const size_t lds = (size_t)(BR + 2) * (kConvTileW + 2) * (a.cin + 8) * 2 + 16 + (size_t)a.kpad / 8 * 4;
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
