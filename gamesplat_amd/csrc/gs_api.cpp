@@ -86,6 +86,7 @@ struct FramePlan {
     SortLaunch tiles;
     CompositeLaunch comp;
     uint32_t range_words;
+    int pack_tile;           // tile size the compact packs rectangles for (0: none)
 };
 
 }  // namespace
@@ -98,6 +99,7 @@ struct gs_ctx {
     size_t held = 0;
 
     uint32_t n = 0;
+    DevBuf vrects[3];  // packed tile rectangles moved by the depth sort
     DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
@@ -175,6 +177,7 @@ struct gs_ctx {
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
     bool stats_collected = false;
     int sort_slot = 0;
+    int rect_tile = 0;       // tile size of the packed rectangles the last cull wrote (0: none)
 
     bool profiling = false;
     std::vector<TimedLaunch> timed;
@@ -243,6 +246,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     for (int k = 0; k < 3; ++k) {
         GS_CHECK(ensure(c, c->keys[k], cap * 4, "sort keys"));
         GS_CHECK(ensure(c, c->ids[k], cap * 4, "sort ids"));
+        GS_CHECK(ensure(c, c->vrects[k], cap * 4, "sorted tile rectangles"));
     }
     const size_t blocks = (size_t)views * ((c->n + 255u) / 256u);
     GS_CHECK(ensure(c, c->dense_keys, cap * 4, "dense keys"));
@@ -436,7 +440,7 @@ void gs_destroy(gs_ctx* c) {
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
                       &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1],
-                      &c->anim_data, &c->lut, &c->display};
+                      &c->anim_data, &c->lut, &c->display, &c->vrects[0], &c->vrects[1], &c->vrects[2]};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -504,11 +508,12 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
         return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
     GS_CHECK(set_device(c));
     // Scene-resident bytes per Gaussian: planes 240 + record 48 + keys/ids 24
-    // + dense key 4 + rectangles 16 + GPS prepared 48 + item count/base 8,
-    // plus the GPS work list.
-    size_t required = (size_t)count * (240 + 48 + 24 + 4 + 16 + 48 + 8) + (size_t)c->gps_budget * 4;
+    // + sorted tile rectangles 12 + dense key 4 + rectangles 16 + GPS
+    // prepared 48 + item count/base 8, plus the GPS work list.
+    size_t required = (size_t)count * (240 + 48 + 24 + 12 + 4 + 16 + 48 + 8) + (size_t)c->gps_budget * 4;
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
                         c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
+                        c->vrects[0].bytes - c->vrects[1].bytes - c->vrects[2].bytes -
                         c->dense_keys.bytes - c->rects.bytes - c->rect_px.bytes - c->prepared.bytes - c->item_count.bytes -
                         c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
@@ -733,11 +738,24 @@ CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullO
     return a;
 }
 
+// Tile size the compact packs each visible splat's tile rectangle for (the
+// depth sort carries it to the binning, which then reads it in order instead
+// of gathering the pixel rectangle): `tile` when every view spans at most 255
+// tiles a side, else 0.
+int rect_pack_tile(const GsCameraUniforms* cams, uint32_t count, int tile) {
+    for (uint32_t v = 0; v < count; ++v) {
+        const long w = (long)cams[v].viewport[0], h = (long)cams[v].viewport[1];
+        if ((w + tile - 1) / tile > 255 || (h + tile - 1) / tile > 255) return 0;
+    }
+    return tile;
+}
+
 SortLaunch make_depth_sort(gs_ctx* c) {
     SortLaunch s{};
     for (int k = 0; k < 3; ++k) {
         s.keys[k] = ptr<uint32_t>(c->keys[k]);
         s.vals[k] = ptr<uint32_t>(c->ids[k]);
+        s.vals2[k] = c->rect_tile ? ptr<uint32_t>(c->vrects[k]) : nullptr;
     }
     s.count = &c->dc()->cull.visible_count;
     s.capacity = c->views * c->n;
@@ -765,6 +783,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.views = frame_dev(c)->views;
     b.counters = c->dc();
     b.rect_px = ptr<uint2>(c->rect_px);
+    b.sorted_rects = c->rect_tile == tile ? ptr<uint32_t>(c->vrects[c->sort_slot]) : nullptr;
     b.rects = ptr<uint2>(c->rects);
     b.block_counts = ptr<uint32_t>(c->block_counts);
     b.block_offsets = ptr<uint32_t>(c->block_offsets);
@@ -812,7 +831,8 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
     GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->block_offsets),
                               &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),
-                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), c->stream));
+                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
+                          ptr<uint32_t>(c->vrects[0]), c->stream));
     GS_CHECK(timed_end(c, &t));
     return GS_OK;
 }
@@ -893,6 +913,7 @@ int enqueue_stochastic(gs_ctx* c, bool* binned) {
             // which holds every jitter up to half a pixel, or moved exactly
             BinLaunch& b = c->plan.bin;
             b.jittered = 1;
+            b.sorted_rects = nullptr;
             b.records = ptr<float4>(c->records);
             b.tight = c->cull_tight;
             b.pad = mode == 1 ? 1 : 0;
@@ -1013,6 +1034,8 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     GS_CHECK(upload_cameras(c, cam, 1));
     c->plan.cull = make_cull(c, effect, opts);
     c->plan.cull_blocks = blocks;
+    c->plan.pack_tile = rect_pack_tile(cam, 1, c->default_tile);
+    c->rect_tile = c->plan.pack_tile;
     GS_CHECK(stage_cull(c, c->plan));
     c->have_cull = true;
     c->have_sort = c->have_composite = false;
@@ -1096,6 +1119,8 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     FramePlan& p = c->plan;
     p.cull = make_cull(c, effect, opts);
     p.cull_blocks = blocks;
+    p.pack_tile = rect_pack_tile(cams, count, tile);
+    c->rect_tile = p.pack_tile;
     p.depth = make_depth_sort(c);
     // the sorted list lands in buffer 2 for an even pass count, else 1
     c->sort_slot = p.depth.passes % 2 == 0 ? 2 : 1;

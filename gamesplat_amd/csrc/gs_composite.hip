@@ -11,8 +11,9 @@
 // fragment test and alpha are the fragment shader's.
 //
 //   bin_count   per front-to-back position: tiles touched by the splat's
-//               covered pixels (from the pixel rectangle the cull wrote),
-//               pair count per block;
+//               covered pixels (the tile rectangle the depth sort carried
+//               along, else from the pixel rectangle the cull wrote), pair
+//               count per block;
 //   (scan)      block offsets;
 //   bin_emit    (tile id, splat id) pairs in front-to-back order;
 //   (sort)      stable LSD sort by tile id keeps the depth order in a tile;
@@ -41,7 +42,10 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
     const uint32_t V = *a.count;
     const uint32_t q = blockIdx.x * 256u + tid;
     uint32_t cnt = 0u;
-    if (q < V) {
+    if (q < V && a.sorted_rects) {
+        const uint32_t r = a.sorted_rects[V - 1u - q];
+        cnt = (((r >> 8) & 0xFFu) - (r & 0xFFu)) * ((r >> 24) - ((r >> 16) & 0xFFu));
+    } else if (q < V) {
         const uint32_t val = a.sorted_ids[V - 1u - q];
         uint2 px;
         if (a.jittered) {
@@ -85,7 +89,13 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     const uint32_t q = blockIdx.x * 256u + tid;
     uint32_t cnt = 0u;
     if (q < V) {
-        uint2 r = a.rects[q];
+        uint2 r;
+        if (a.sorted_rects) {
+            const uint32_t p = a.sorted_rects[V - 1u - q];
+            r = make_uint2((p & 0xFFu) | (((p >> 8) & 0xFFu) << 16), ((p >> 16) & 0xFFu) | ((p >> 24) << 16));
+        } else {
+            r = a.rects[q];
+        }
         cnt = ((r.x >> 16) - (r.x & 0xFFFFu)) * ((r.y >> 16) - (r.y & 0xFFFFu));
         s_rect[tid] = r;
         const uint32_t val = a.sorted_ids[V - 1u - q];
@@ -547,6 +557,7 @@ __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads)) com
 hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
     const uint32_t blocks = (a.capacity + 255u) / 256u;
     if (!blocks) return hipSuccess;
+    if (a.sorted_rects && a.jittered) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bin_count_kernel, dim3(blocks), dim3(256), 0, s, a);
     hipError_t e = launch_scan_counts(a.block_counts, blocks, a.block_offsets, &a.counters->bin.dup_total, nullptr,
                                       nullptr, s);

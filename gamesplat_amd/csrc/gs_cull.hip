@@ -434,9 +434,12 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
 // 256 Gaussians in view v to offsets[v * blocks + b] + rank.  The value is
 // v * n + index, which addresses the view's records and rectangles, and the
 // list holds view 0's visible Gaussians in index order, then view 1's, ...
+// With vis_rects, the splat's pixel rectangle goes along as its tile
+// rectangle at tile size 1 << tile_shift, packed into 8-bit fields.
 __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict__ dense_keys, uint32_t n,
                                                       const uint32_t* __restrict__ offsets, uint32_t* vis_keys,
-                                                      uint32_t* vis_ids) {
+                                                      uint32_t* vis_ids, const uint2* __restrict__ rect_px,
+                                                      int tile_shift, uint32_t* vis_rects) {
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t v = blockIdx.y;
@@ -456,6 +459,16 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
                              (uint32_t)__popcll(bal & lanemask_lt());
         vis_keys[pos] = key;
         vis_ids[pos] = (uint32_t)slot;
+        if (vis_rects) {
+            const uint2 px = rect_px[slot];
+            uint32_t packed = 0u;
+            if (px.x) {
+                const uint32_t x0 = (px.x & 0xFFFFu) >> tile_shift, x1 = (((px.x >> 16) - 1u) >> tile_shift) + 1u;
+                const uint32_t y0 = (px.y & 0xFFFFu) >> tile_shift, y1 = (((px.y >> 16) - 1u) >> tile_shift) + 1u;
+                packed = x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
+            }
+            vis_rects[pos] = packed;
+        }
     }
 }
 
@@ -514,10 +527,19 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
 }
 
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
-                          uint32_t* vis_keys, uint32_t* vis_ids, hipStream_t s) {
+                          uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
+                          uint32_t* vis_rects, hipStream_t s) {
     if (n == 0 || views == 0) return hipSuccess;
+    int shift = 0;
+    if (pack_tile) {
+        if (!rect_px || !vis_rects || (pack_tile != 8 && pack_tile != 16 && pack_tile != 32))
+            return hipErrorInvalidValue;
+        while ((1 << shift) < pack_tile) ++shift;
+    } else {
+        vis_rects = nullptr;
+    }
     hipLaunchKernelGGL(compact_kernel, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n, offsets,
-                       vis_keys, vis_ids);
+                       vis_keys, vis_ids, rect_px, shift, vis_rects);
     return hipGetLastError();
 }
 

@@ -165,8 +165,13 @@ struct CullLaunch {
     float anim_scales[48];
 };
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
+// With pack_tile (8, 16 or 32) it also writes, per visible splat, the tile
+// rectangle of its pixel rectangle rect_px at that tile size as
+// x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when empty; every view must span at
+// most 255 tiles a side), which the depth sort carries to the binning.
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
-                          uint32_t* vis_keys, uint32_t* vis_ids, hipStream_t s);
+                          uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
+                          uint32_t* vis_rects, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
@@ -176,6 +181,7 @@ hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t fi
 struct SortLaunch {
     uint32_t* keys[3];       // keys[0] input; passes ping-pong keys[1] <-> keys[2]
     uint32_t* vals[3];
+    uint32_t* vals2[3];      // optional second value moved with the same permutation (NULL: none)
     const uint32_t* count;   // device element count
     uint32_t capacity;       // grid bound
     uint32_t* scratch;       // 512 words
@@ -197,6 +203,9 @@ struct BinLaunch {
     int tile;
     const ViewInfo* views;   // device
     const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
+    // optional: per depth-sorted position, the packed tile rectangle at this
+    // bin's tile size, moved by the depth sort (replaces the rect_px gather)
+    const uint32_t* sorted_rects;
     // jittered StochasticSplats samples: the rectangle is recomputed from
     // the record with the quad moved by (jitter_x, jitter_y) and grown by
     // pad pixels on each side
@@ -205,7 +214,7 @@ struct BinLaunch {
     int tight;               // the cull's tight-rectangle setting
     int pad;
     float jitter_x, jitter_y;
-    uint2* rects;            // per front-to-back position: tile rectangle
+    uint2* rects;            // per front-to-back position: tile rectangle (without sorted_rects)
     uint32_t* block_counts;  // pairs per 256-block
     uint32_t* block_offsets;
     DevCounters* counters;

@@ -12,6 +12,8 @@
 //              stable ranking inside the tile with wave64 ballots (peers =
 //              lanes whose digit matches on every bit), reorder by digit in
 //              LDS, then contiguous runs of equal digits are written out.
+// An optional second 32-bit value array moves with the same permutation: the
+// downsweep reorders it through the value staging area in a second round.
 // No kernel waits on another block or takes a global ticket (device-scope
 // atomics on one address serialize across XCDs on MI355X).  The element
 // count lives in device memory; blocks past it exit at once.
@@ -31,6 +33,8 @@ struct PassArgs {
     const uint32_t* vals_in;
     uint32_t* keys_out;
     uint32_t* vals_out;
+    const uint32_t* vals2_in;   // second value array (P2 passes)
+    uint32_t* vals2_out;
     const uint32_t* count;
     uint32_t* table;        // [RADIX][tiles]: tile digit counts, then column offsets
     uint32_t tiles;         // table stride (grid size)
@@ -104,7 +108,7 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     if (tid == 0) a.col_total[d] = carry;
 }
 
-template <int BITS>
+template <int BITS, bool P2>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -129,12 +133,14 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     // 1. load, wave-contiguous (each load is a coalesced 256-byte access)
     const uint32_t wbase = base + wave * (64u * kItems);
     uint32_t keys[kItems], vals[kItems], ranks[kItems];
+    uint32_t vals2[P2 ? kItems : 1];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
         bool in = g < count;
         keys[i] = in ? a.keys_in[g] : 0xFFFFFFFFu;
         vals[i] = in ? a.vals_in[g] : 0u;
+        if constexpr (P2) vals2[i] = in ? a.vals2_in[g] : 0u;
     }
 
     // 2. stable rank within the wave: peers = lanes with the same digit,
@@ -206,6 +212,21 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
         a.keys_out[out] = k;
         a.vals_out[out] = s_vals[p];
     }
+    if constexpr (P2) {
+        // second value: same local positions, staged where the first was
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            uint32_t g = wbase + (uint32_t)i * 64u + lane;
+            uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
+            s_vals[s_local[d] + s_whist[wave][d] + ranks[i]] = vals2[i];
+        }
+        __syncthreads();
+        for (uint32_t p = tid; p < valid; p += kThreads) {
+            uint32_t d = (s_keys[p] >> shift) & MASK;
+            a.vals2_out[s_dest[d] + p] = s_vals[p];
+        }
+    }
 }
 
 template <int BITS>
@@ -219,6 +240,8 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.vals_in = a.vals[src];
         pa.keys_out = a.keys[dst];
         pa.vals_out = a.vals[dst];
+        pa.vals2_in = a.vals2[src];
+        pa.vals2_out = a.vals2[dst];
         pa.count = a.count;
         pa.table = a.table;
         pa.tiles = tiles;
@@ -226,7 +249,10 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.shift = p * BITS;
         hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
-        hipLaunchKernelGGL(downsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
+        if (a.vals2[0])
+            hipLaunchKernelGGL((downsweep_kernel<BITS, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        else
+            hipLaunchKernelGGL((downsweep_kernel<BITS, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src = dst;
@@ -245,6 +271,7 @@ uint32_t sort_table_words(uint32_t capacity) {
 
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {
     if (a.passes < 1 || a.passes * a.bits > 32) return hipErrorInvalidValue;
+    if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
     if (a.bits == 8) return run_passes<8>(a, result_slot, s);
     if (a.bits == 4) return run_passes<4>(a, result_slot, s);
     return hipErrorInvalidValue;
