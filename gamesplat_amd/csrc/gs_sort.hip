@@ -205,15 +205,22 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
-    for (uint32_t p = tid; p < valid; p += kThreads) {
-        uint32_t k = s_keys[p];
-        uint32_t d = (k >> shift) & MASK;
-        uint32_t out = s_dest[d] + p;
-        a.keys_out[out] = k;
-        a.vals_out[out] = s_vals[p];
+    uint32_t outs[P2 ? kItems : 1];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const uint32_t p = tid + (uint32_t)j * kThreads;
+        if (p < valid) {
+            uint32_t k = s_keys[p];
+            uint32_t d = (k >> shift) & MASK;
+            uint32_t out = s_dest[d] + p;
+            a.keys_out[out] = k;
+            a.vals_out[out] = s_vals[p];
+            if constexpr (P2) outs[j] = out;
+        }
     }
     if constexpr (P2) {
-        // second value: same local positions, staged where the first was
+        // second value: same local positions, staged where the first was,
+        // written to the output positions the first round found
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kItems; ++i) {
@@ -222,9 +229,10 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
             s_vals[s_local[d] + s_whist[wave][d] + ranks[i]] = vals2[i];
         }
         __syncthreads();
-        for (uint32_t p = tid; p < valid; p += kThreads) {
-            uint32_t d = (s_keys[p] >> shift) & MASK;
-            a.vals2_out[s_dest[d] + p] = s_vals[p];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const uint32_t p = tid + (uint32_t)j * kThreads;
+            if (p < valid) a.vals2_out[outs[j]] = s_vals[p];
         }
     }
 }
