@@ -11,9 +11,11 @@
 //   downsweep  digit bases from the column totals (every block scans them),
 //              stable ranking inside the tile with wave64 ballots (peers =
 //              lanes whose digit matches on every bit), reorder by digit in
-//              LDS, then contiguous runs of equal digits are written out.
-// An optional second 32-bit value array moves with the same permutation: the
-// downsweep reorders it through the value staging area in a second round.
+//              LDS, then contiguous runs of equal digits are written out:
+//              keys, values (and second values) in turn through one staging
+//              array, which keeps the block's LDS small enough for seven
+//              blocks per CU.
+// An optional second 32-bit value array moves with the same permutation.
 // No kernel waits on another block or takes a global ticket (device-scope
 // atomics on one address serialize across XCDs on MI355X).  The element
 // count lives in device memory; blocks past it exit at once.
@@ -112,8 +114,7 @@ template <int BITS, bool P2>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
-    __shared__ uint32_t s_keys[kTile];
-    __shared__ uint32_t s_vals[kTile];
+    __shared__ uint32_t s_buf[kTile];   // keys, then values, then second values
     __shared__ uint32_t s_whist[kWaves][RADIX];
     __shared__ uint32_t s_local[RADIX];
     __shared__ uint32_t s_dest[RADIX];
@@ -132,39 +133,34 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 
     // 1. load, wave-contiguous (each load is a coalesced 256-byte access)
     const uint32_t wbase = base + wave * (64u * kItems);
-    uint32_t keys[kItems], vals[kItems], ranks[kItems];
-    uint32_t vals2[P2 ? kItems : 1];
+    uint32_t keys[kItems], ranks[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        bool in = g < count;
-        keys[i] = in ? a.keys_in[g] : 0xFFFFFFFFu;
-        vals[i] = in ? a.vals_in[g] : 0u;
-        if constexpr (P2) vals2[i] = in ? a.vals2_in[g] : 0u;
+        keys[i] = g < count ? a.keys_in[g] : 0xFFFFFFFFu;
     }
 
     // 2. stable rank within the wave: peers = lanes with the same digit,
-    //    found with BITS ballots; the lowest peer updates the wave's counter
-    const uint64_t lt = lanemask_lt();
+    //    found with BITS ballots (a lane keeps the ballot where its bit is
+    //    set, the complement where it is clear); the peers below a lane give
+    //    its rank.  Every lane reads the wave's digit counter before the
+    //    lowest peer (no peers below it) advances it: LDS accesses of one
+    //    wave complete in program order, and no other wave uses the counter.
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
         uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
-        uint64_t peers = ~0ull;
+        uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+            const uint32_t t = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // 0 or ~0: bit b of d
+            const uint64_t bal = __ballot(t != 0u);
+            plo &= ~((uint32_t)bal ^ t);
+            phi &= ~((uint32_t)(bal >> 32) ^ t);
         }
-        uint32_t below = (uint32_t)__popcll(peers & lt);
-        uint32_t leader = (uint32_t)__ffsll((unsigned long long)peers) - 1u;
-        uint32_t old = 0u;
-        if (lane == leader) {
-            old = s_whist[wave][d];
-            s_whist[wave][d] = old + (uint32_t)__popcll(peers);
-        }
-        old = __shfl(old, (int)leader, 64);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+        const uint32_t old = s_whist[wave][d];
+        if (below == 0u) s_whist[wave][d] = old + (uint32_t)(__popc(plo) + __popc(phi));
         ranks[i] = old + below;
     }
     __syncthreads();
@@ -193,46 +189,62 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     }
     __syncthreads();
 
-    // 4. reorder in LDS by digit, then write runs out
+    // 4. reorder in LDS by digit, then write runs out: the keys first (which
+    //    also gives every local position its output position), then the
+    //    values, then the second values, through one staging array.  Each
+    //    value array is loaded while the previous round is written out, so
+    //    only one array's items are held in registers at a time.
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
         uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
-        uint32_t p = s_local[d] + s_whist[wave][d] + ranks[i];
-        s_keys[p] = keys[i];
-        s_vals[p] = vals[i];
+        ranks[i] += s_local[d] + s_whist[wave][d];
+        s_buf[ranks[i]] = keys[i];
+    }
+    uint32_t v[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        uint32_t g = wbase + (uint32_t)i * 64u + lane;
+        v[i] = g < count ? a.vals_in[g] : 0u;
     }
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
-    uint32_t outs[P2 ? kItems : 1];
+    uint32_t outs[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
         if (p < valid) {
-            uint32_t k = s_keys[p];
-            uint32_t d = (k >> shift) & MASK;
-            uint32_t out = s_dest[d] + p;
-            a.keys_out[out] = k;
-            a.vals_out[out] = s_vals[p];
-            if constexpr (P2) outs[j] = out;
+            uint32_t k = s_buf[p];
+            outs[j] = s_dest[(k >> shift) & MASK] + p;
+            a.keys_out[outs[j]] = k;
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
     if constexpr (P2) {
-        // second value: same local positions, staged where the first was,
-        // written to the output positions the first round found
-        __syncthreads();
 #pragma unroll
         for (int i = 0; i < kItems; ++i) {
             uint32_t g = wbase + (uint32_t)i * 64u + lane;
-            uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
-            s_vals[s_local[d] + s_whist[wave][d] + ranks[i]] = vals2[i];
+            v[i] = g < count ? a.vals2_in[g] : 0u;
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const uint32_t p = tid + (uint32_t)j * kThreads;
+        if (p < valid) a.vals_out[outs[j]] = s_buf[p];
+    }
+    if constexpr (P2) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t p = tid + (uint32_t)j * kThreads;
-            if (p < valid) a.vals2_out[outs[j]] = s_vals[p];
+            if (p < valid) a.vals2_out[outs[j]] = s_buf[p];
         }
     }
 }
