@@ -44,18 +44,23 @@ struct PassArgs {
     int shift;
 };
 
+// Digit counts of one tile.  Each wave counts into kCopies histograms, lane
+// l into copy l % kCopies, so a digit shared by many lanes (the top digit of
+// clustered keys) serializes kCopies times fewer LDS atomics.
 template <int BITS>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
-    __shared__ uint32_t s_hist[kWaves][RADIX];
+    constexpr int kCopies = 4;
+    __shared__ uint32_t s_hist[kWaves * kCopies][RADIX];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t count = *a.count;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     if (base >= count) return;
-    for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
+    for (uint32_t i = tid; i < (uint32_t)(kWaves * kCopies * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
+    uint32_t* hist = s_hist[wave * kCopies + lane % kCopies];
     const uint32_t wbase = base + wave * (64u * kItems);
     uint32_t keys[kItems];
 #pragma unroll
@@ -66,13 +71,13 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        if (g < count) atomicAdd(&s_hist[wave][(keys[i] >> a.shift) & MASK], 1u);
+        if (g < count) atomicAdd(&hist[(keys[i] >> a.shift) & MASK], 1u);
     }
     __syncthreads();
     if (tid < (uint32_t)RADIX) {
         uint32_t c = 0u;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) c += s_hist[w][tid];
+        for (int w = 0; w < kWaves * kCopies; ++w) c += s_hist[w][tid];
         a.table[(size_t)tid * a.tiles + tile] = c;
     }
 }
@@ -131,13 +136,17 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     const uint32_t valid = min((uint32_t)kTile, count - base);
     const int shift = a.shift;
 
-    // 1. load, wave-contiguous (each load is a coalesced 256-byte access)
+    // 1. load, wave-contiguous (each load is a coalesced 256-byte access).
+    //    Positions past the count read the last element (no branch) and take
+    //    the key 0xFFFFFFFF: the last digit, ranked after every valid key.
     const uint32_t wbase = base + wave * (64u * kItems);
+    const uint32_t last = count - 1u;
     uint32_t keys[kItems], ranks[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        keys[i] = g < count ? a.keys_in[g] : 0xFFFFFFFFu;
+        const uint32_t k = a.keys_in[min(g, last)];
+        keys[i] = g < count ? k : 0xFFFFFFFFu;
     }
 
     // 2. stable rank within the wave: peers = lanes with the same digit,
@@ -148,8 +157,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     //    wave complete in program order, and no other wave uses the counter.
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
+        const uint32_t d = (keys[i] >> shift) & MASK;
         uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
@@ -196,17 +204,14 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     //    only one array's items are held in registers at a time.
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        uint32_t d = g < count ? (keys[i] >> shift) & MASK : MASK;
+        const uint32_t d = (keys[i] >> shift) & MASK;
         ranks[i] += s_local[d] + s_whist[wave][d];
         s_buf[ranks[i]] = keys[i];
     }
+    // values of positions past the count are staged but never written out
     uint32_t v[kItems];
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-        uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        v[i] = g < count ? a.vals_in[g] : 0u;
-    }
+    for (int i = 0; i < kItems; ++i) v[i] = a.vals_in[min(wbase + (uint32_t)i * 64u + lane, last)];
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
@@ -225,10 +230,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
     if constexpr (P2) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
-            uint32_t g = wbase + (uint32_t)i * 64u + lane;
-            v[i] = g < count ? a.vals2_in[g] : 0u;
-        }
+        for (int i = 0; i < kItems; ++i) v[i] = a.vals2_in[min(wbase + (uint32_t)i * 64u + lane, last)];
     }
     __syncthreads();
 #pragma unroll
