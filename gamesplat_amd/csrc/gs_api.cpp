@@ -85,7 +85,6 @@ struct FramePlan {
     BinLaunch bin;
     SortLaunch tiles;
     CompositeLaunch comp;
-    uint32_t range_words;
     int pack_tile;           // tile size the compact packs rectangles for (0: none)
 };
 
@@ -787,6 +786,8 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.rects = ptr<uint2>(c->rects);
     b.block_counts = ptr<uint32_t>(c->block_counts);
     b.block_offsets = ptr<uint32_t>(c->block_offsets);
+    b.clear = ptr<uint32_t>(c->ranges);
+    b.clear_words = ntiles * 2u;
     b.dup_keys = ptr<uint32_t>(c->dup_keys[0]);
     b.dup_vals = ptr<uint32_t>(c->dup_vals[0]);
     b.dup_capacity = c->dup_cap;
@@ -820,7 +821,6 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     cl.tight_rects = c->tight_rects;
     cl.persistent_blocks = c->blocks_per_cu > 0 ? (uint32_t)(c->blocks_per_cu * c->props.multiProcessorCount) : 0u;
     cl.queue = c->dc()->composite_queue;
-    p->range_words = ntiles * 2u;
 }
 
 int stage_cull(gs_ctx* c, const FramePlan& p) {
@@ -849,8 +849,7 @@ int stage_sort(gs_ctx* c, const FramePlan& p, int* slot) {
 // (tile, splat) pairs, their stable sort by tile and the tile ranges.
 int stage_bin(gs_ctx* c, const FramePlan& p) {
     c->bin_epoch++;
-    GS_HIP(launch_fill_u32(const_cast<uint32_t*>(reinterpret_cast<const uint32_t*>(p.comp.ranges)), 0u,
-                           p.range_words, c->stream));
+    // the count pass of the bin clears the tile ranges
     if (p.comp.persistent_blocks) GS_HIP(launch_fill_u32(p.comp.queue, 0u, 8, c->stream));
     TimedLaunch t{};
     GS_CHECK(timed_begin(c, GS_STAGE_BIN, &t));

@@ -41,6 +41,7 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t V = *a.count;
     const uint32_t q = blockIdx.x * 256u + tid;
+    for (uint32_t w = q; w < a.clear_words; w += gridDim.x * 256u) a.clear[w] = 0u;
     uint32_t cnt = 0u;
     if (q < V && a.sorted_rects) {
         const uint32_t r = a.sorted_rects[V - 1u - q];
@@ -122,26 +123,36 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     }
 }
 
-// Entry count the tile sort works on: min(total, capacity).
-__global__ void clamp_count_kernel(const uint32_t* total, uint32_t capacity, uint32_t* clamped) {
-    if (threadIdx.x == 0) *clamped = min(*total, capacity);
-}
-
 // [begin, end) of every tile's entries in the tile-sorted keys: each run
-// start writes its own begin and the previous run's end; one key load per
-// entry, the previous key comes from the neighbouring lane.
+// start writes its own begin and the previous run's end.  A thread takes four
+// consecutive keys (one 16-byte load when they are all in range); the key
+// before them is the last of the neighbouring lane's four.
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint32_t* count_ptr, uint2* ranges) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *count_ptr;
-    const uint32_t k = i < count ? keys[i] : 0xFFFFFFFFu;
-    uint32_t prev = __shfl_up(k, 1, 64);
-    if (__lane_id() == 0) prev = (i > 0 && i - 1 < count) ? keys[i - 1] : 0xFFFFFFFFu;
-    if (i >= count) return;
-    if (i == 0 || prev != k) {
-        ranges[k].x = i;
-        if (i > 0) ranges[prev].y = i;
+    const uint32_t i0 = t * 4u;
+    uint32_t k[4];
+    if (i0 + 3u < count) {
+        const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+        k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < count ? keys[i0 + j] : 0xFFFFFFFFu;
     }
-    if (i == count - 1) ranges[k].y = count;
+    uint32_t prev = __shfl_up(k[3], 1, 64);
+    if (__lane_id() == 0) prev = (i0 > 0 && i0 - 1u < count) ? keys[i0 - 1u] : 0xFFFFFFFFu;
+    if (i0 >= count) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = i0 + (uint32_t)j;
+        if (i >= count) break;
+        const uint32_t pk = j ? k[j - 1] : prev;
+        if (i == 0 || pk != k[j]) {
+            ranges[k[j]].x = i;
+            if (i > 0) ranges[pk].y = i;
+        }
+        if (i == count - 1u) ranges[k[j]].y = count;
+    }
 }
 
 // One splat as the compositors keep it in LDS (4 x 16 bytes).  The
@@ -558,20 +569,22 @@ hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
     const uint32_t blocks = (a.capacity + 255u) / 256u;
     if (!blocks) return hipSuccess;
     if (a.sorted_rects && a.jittered) return hipErrorInvalidValue;
+    if (a.clear_words && !a.clear) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bin_count_kernel, dim3(blocks), dim3(256), 0, s, a);
+    // the tile sort works on min(total, capacity) entries
     hipError_t e = launch_scan_counts(a.block_counts, blocks, a.block_offsets, &a.counters->bin.dup_total, nullptr,
-                                      nullptr, s);
+                                      nullptr, s, a.dup_capacity, &a.counters->bin.dup_count);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bin_emit_kernel, dim3(blocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(clamp_count_kernel, dim3(1), dim3(64), 0, s, &a.counters->bin.dup_total, a.dup_capacity,
-                       &a.counters->bin.dup_count);
     return hipGetLastError();
 }
 
 hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint32_t capacity, uint2* ranges,
                               hipStream_t s) {
     if (!capacity) return hipSuccess;
-    hipLaunchKernelGGL(tile_ranges_kernel, dim3((capacity + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+    if (reinterpret_cast<uintptr_t>(keys) % 16) return hipErrorInvalidValue;
+    const uint32_t threads = (capacity + 3u) / 4u;
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
     return hipGetLastError();
 }
 

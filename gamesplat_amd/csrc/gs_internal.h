@@ -133,9 +133,11 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
 // Kernel launch interfaces.
 
 // gs_scan.hip: exclusive scan of nb block counts into offsets; *total gets
-// the sum.  block_diag (nb x 8, may be NULL) is reduced into diag_out.
+// the sum, and *clamped (when given) min(sum, clamp_cap).  block_diag
+// (nb x 8, may be NULL) is reduced into diag_out.
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
-                              const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s);
+                              const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s,
+                              uint32_t clamp_cap = 0, uint32_t* clamped = nullptr);
 
 // gs_cull.hip
 struct CullLaunch {
@@ -217,6 +219,8 @@ struct BinLaunch {
     uint2* rects;            // per front-to-back position: tile rectangle (without sorted_rects)
     uint32_t* block_counts;  // pairs per 256-block
     uint32_t* block_offsets;
+    uint32_t* clear;         // optional: clear_words words zeroed by the count pass (the tile ranges)
+    uint32_t clear_words;
     DevCounters* counters;
     uint32_t* dup_keys;
     uint32_t* dup_vals;

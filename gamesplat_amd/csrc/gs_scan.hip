@@ -40,12 +40,28 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
                                                                        uint32_t nb, uint32_t* offsets,
                                                                        uint32_t* total,
                                                                        const uint32_t* __restrict__ block_diag,
-                                                                       uint32_t* diag_out) {
+                                                                       uint32_t* diag_out, uint32_t clamp_cap,
+                                                                       uint32_t* clamped) {
     extern __shared__ uint32_t s_c[];
     __shared__ uint32_t s_scan[kScanThreads / 64];
     __shared__ uint32_t s_diag[8];
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < nb; i += kScanThreads) s_c[i] = counts[i];
+    // eight loads in flight per thread before their LDS stores
+    constexpr int kBatch = 8;
+    for (uint32_t i0 = tid; i0 < nb; i0 += kScanThreads * kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
+            v[j] = counts[min(i, nb - 1u)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
+            if (i < nb) s_c[i] = v[j];
+        }
+    }
     __syncthreads();
     const uint32_t run = (nb + kScanThreads - 1) / kScanThreads;
     const uint32_t r0 = tid * run, r1 = min(r0 + run, nb);
@@ -60,7 +76,10 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
     }
     __syncthreads();
     for (uint32_t i = tid; i < nb; i += kScanThreads) offsets[i] = s_c[i];
-    if (tid == 0) *total = all;
+    if (tid == 0) {
+        *total = all;
+        if (clamped) *clamped = min(all, clamp_cap);
+    }
     if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
 }
 
@@ -68,7 +87,8 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
                                                                    uint32_t nb, uint32_t* offsets,
                                                                    uint32_t* total,
                                                                    const uint32_t* __restrict__ block_diag,
-                                                                   uint32_t* diag_out) {
+                                                                   uint32_t* diag_out, uint32_t clamp_cap,
+                                                                   uint32_t* clamped) {
     __shared__ uint32_t s_scan[kScanThreads / 64];
     __shared__ uint32_t s_diag[8];
     const uint32_t tid = threadIdx.x;
@@ -94,14 +114,18 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
         }
         carry += chunk;
     }
-    if (tid == 0) *total = carry;
+    if (tid == 0) {
+        *total = carry;
+        if (clamped) *clamped = min(carry, clamp_cap);
+    }
     if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
 }
 
 }  // namespace
 
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
-                              const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s) {
+                              const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s, uint32_t clamp_cap,
+                              uint32_t* clamped) {
     if (nb <= kLdsCounts) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -111,10 +135,10 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* off
             attr_set = true;
         }
         hipLaunchKernelGGL(scan_counts_lds_kernel, dim3(1), dim3(kScanThreads), (size_t)(nb ? nb : 1) * 4, s, counts,
-                           nb, offsets, total, block_diag, diag_out);
+                           nb, offsets, total, block_diag, diag_out, clamp_cap, clamped);
     } else {
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, offsets, total,
-                           block_diag, diag_out);
+                           block_diag, diag_out, clamp_cap, clamped);
     }
     return hipGetLastError();
 }
