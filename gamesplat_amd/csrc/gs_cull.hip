@@ -146,19 +146,37 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
     float position[3] = {0.0f, 0.0f, 0.0f};
     float opacity = 0.0f;
     float effect_wave = 0.0f;
-    float4 p1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p2 = p1;
+    float4 p1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p2 = p1, p3 = p1;
+    typedef float f32x4v_t __attribute__((ext_vector_type(4)));
+    f32x4v_t shp[11];
     float scale[3] = {0.0f, 0.0f, 0.0f};
     bool loaded = false;
+    // Without an SH LOD distance (or layer appearance) every visible
+    // Gaussian needs the planes of the cull's SH degree: they are loaded
+    // together with P0, before the visibility test, so a Gaussian costs one
+    // memory round trip (the few culled ones read planes they do not use).
+    const bool prefetch = ANIM != 2 && !(a.sh_lod_distance > 0.0f);
+    const int prefetch_planes = a.sh_degree <= 0 ? 0 : (a.sh_degree == 1 ? 2 : (a.sh_degree == 2 ? 6 : 11));
     if (idx < n) {
         float4 p0 = a.planes[idx];
+        if (prefetch) {
+            p1 = a.planes[(size_t)1 * n + idx];
+            p2 = a.planes[(size_t)2 * n + idx];
+            p3 = a.planes[(size_t)3 * n + idx];
+#pragma unroll
+            for (int j = 0; j < 11; ++j)
+                if (j < prefetch_planes) shp[j] = reinterpret_cast<const f32x4v_t*>(a.planes)[(size_t)(4 + j) * n + idx];
+        }
         position[0] = p0.x; position[1] = p0.y; position[2] = p0.z;
         opacity = p0.w;
 
         if (a.effect_active) {
             // shockwave (cull.glsl:183-207)
             const GsSplatEffectParams& fx = a.effect;
-            p1 = a.planes[(size_t)1 * n + idx];
-            p2 = a.planes[(size_t)2 * n + idx];
+            if (!prefetch) {
+                p1 = a.planes[(size_t)1 * n + idx];
+                p2 = a.planes[(size_t)2 * n + idx];
+            }
             loaded = true;
             scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
             float duration = maxf(fx.params[1], 0.001f);
@@ -222,11 +240,13 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
     float sh[46];
     if (vis) {
         if (!loaded) {
-            p1 = a.planes[(size_t)1 * n + idx];
-            p2 = a.planes[(size_t)2 * n + idx];
+            if (!prefetch) {
+                p1 = a.planes[(size_t)1 * n + idx];
+                p2 = a.planes[(size_t)2 * n + idx];
+            }
             scale[0] = p2.x; scale[1] = p2.y; scale[2] = p2.z;
         }
-        float4 p3 = a.planes[(size_t)3 * n + idx];
+        if (!prefetch) p3 = a.planes[(size_t)3 * n + idx];
         float qw = p1.x, qx = p1.y, qy = p1.z, qz = p1.w;
         dc[0] = p2.w; dc[1] = p3.x; dc[2] = p3.y;
         M3 R;
@@ -264,7 +284,9 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
 #pragma unroll
         for (int j = 0; j < 11; ++j) {
             float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (j < planes_needed) t = a.planes[(size_t)(4 + j) * n + idx];
+            // with prefetch, planes_needed == prefetch_planes
+            if (j < planes_needed)
+                t = prefetch ? make_float4(shp[j].x, shp[j].y, shp[j].z, shp[j].w) : a.planes[(size_t)(4 + j) * n + idx];
             sh[2 + 4 * j] = t.x;
             sh[3 + 4 * j] = t.y;
             sh[4 + 4 * j] = t.z;
