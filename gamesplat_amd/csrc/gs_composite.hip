@@ -156,15 +156,17 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
 }
 
 // One splat as the compositors keep it in LDS (4 x 16 bytes).  The
-// quadratic form and the cutoff are in log2 units (scaled by log2 e), so the
-// blend step's alpha is one v_exp_f32.
+// quadratic form and the cutoff are in log2 units (scaled by log2 e), and the
+// opacity enters as log2 o added to the form, so the blend step's alpha is
+// one v_exp_f32 and its fragment test a comparison with log2(1/255).
 struct StagedSplats {
     int4* rect;     // covered pixels [i0, i1) x [j0, j1)
     float4* geo;    // cx, cy, -k0/2 log2e, -k1 log2e
-    float4* par;    // -k2/2 log2e, opacity, log2((1/255) / o), -
+    float4* par;    // -k2/2 log2e, opacity, log2((1/255) / o), log2 o
     float4* col;    // r, g, b, -
 };
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLog2AlphaMin = -7.99435343685886f;  // log2(1/255)
 
 // The view and tile of a global tile id.
 struct TileRef {
@@ -205,11 +207,12 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
     // hardware log2: the composite's fragment test is held to a tolerance
     // (its quadratic form is FMA-evaluated), and the pixel rectangle's margin
     // covers the difference
+    const float log2o = __log2f(maxf(r0.z, 1e-8f));
     const float cutoff = __log2f((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
     s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width, tr.height,
                                  a.tight_rects != 0);
     s.geo[t] = make_float4(r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1);
-    s.par[t] = make_float4(-0.5f * kLog2e * k2, r0.z, cutoff, 0.0f);
+    s.par[t] = make_float4(-0.5f * kLog2e * k2, r0.z, cutoff, log2o);
     s.col[t] = make_float4(r2.x, r2.y, r2.z, 0.0f);
 }
 
@@ -222,15 +225,17 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
 // arithmetic; where the fused evaluation rounds to a value just above 0 the
 // FP64 composite blends the splat too.  A saturated pixel (T < 1e-4) keeps blending until its whole
 // wave is saturated; what it adds is below 1e-4 of a colour, and it only
-// brings the result closer to blending every splat.  The quadratic form is evaluated with
-// fused multiply-adds: power = dx * (-k0/2 dx - k1 dy) + (-k2/2 dy) dy, in
-// log2 units.
+// brings the result closer to blending every splat.  The quadratic form and
+// the opacity are evaluated with fused multiply-adds in log2 units:
+// power = dx * (-k0/2 dx - k1 dy) + ((-k2/2 dy) dy + log2 o), so alpha is
+// min(2^power, 0.99) and the fragment test alpha >= 1/255 is
+// power >= log2(1/255).
 __device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, int4 r, float4 geo, float4 par,
                                            float4 col, float& cr, float& cg, float& cb, float& tr) {
     const float dx = fx - geo.x, dy = fy - geo.y;
-    const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, (par.x * dy) * dy);
-    float alpha = minf(par.y * __builtin_amdgcn_exp2f(power), 0.99f);
-    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power >= par.z);
+    const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, fmaf(par.x * dy, dy, par.w));
+    float alpha = minf(__builtin_amdgcn_exp2f(power), 0.99f);
+    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power >= kLog2AlphaMin);
     alpha = take ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
