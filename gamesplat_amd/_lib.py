@@ -48,7 +48,7 @@ GS_OPT_BLOCKS_PER_CU = 5
 GS_OPT_GRAPH = 6
 MAX_VIEWS = 64
 
-STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve")
+STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve", "cull_kernel")
 
 GAUSSIAN_FLOATS = 59  # GsGaussian: pos 0-2, scale 3-5, rot 6-9 (w,x,y,z), dc 10-12, opacity 13, rest 14-58
 

@@ -825,9 +825,11 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
 
 int stage_cull(gs_ctx* c, const FramePlan& p) {
     c->bin_epoch++;
-    TimedLaunch t{};
+    TimedLaunch t{}, tk{};
     GS_CHECK(timed_begin(c, GS_STAGE_CULL, &t));
+    GS_CHECK(timed_begin(c, GS_STAGE_CULL_KERNEL, &tk));
     GS_HIP(launch_cull(p.cull, c->stream));
+    GS_CHECK(timed_end(c, &tk));
     GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->block_offsets),
                               &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),

@@ -122,7 +122,9 @@ enum {
     GS_BUFFER_GPS_PACKED = 5      /* u64 x (ss*W)*(ss*H): depth key << 32 | payload */
 };
 
-/* Pipeline stages timed by gs_stage_times when profiling is on. */
+/* Pipeline stages timed by gs_stage_times when profiling is on.  CULL is the
+ * whole cull stage (cull kernel, visible-count scan, compaction);
+ * CULL_KERNEL the cull kernel alone, inside it. */
 enum {
     GS_STAGE_CULL = 0,
     GS_STAGE_SORT = 1,
@@ -131,7 +133,8 @@ enum {
     GS_STAGE_GPS_EXPAND = 4,
     GS_STAGE_GPS_SPLAT = 5,
     GS_STAGE_GPS_RESOLVE = 6,
-    GS_STAGE_COUNT = 7
+    GS_STAGE_CULL_KERNEL = 7,
+    GS_STAGE_COUNT = 8
 };
 
 typedef struct gs_ctx gs_ctx;
