@@ -468,6 +468,11 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     const uint32_t idx = blockIdx.x * 256u + tid;
     const size_t slot = (size_t)v * n + idx;
     const uint32_t key = idx < n ? dense_keys[slot] : 0xFFFFFFFFu;
+    // the rectangle is loaded with the key (one memory round trip), and used
+    // only for visible splats
+    uint2 px = make_uint2(0u, 0u);
+    if (vis_rects && idx < n) px = rect_px[slot];
+    const uint32_t block_base = offsets[(size_t)v * gridDim.x + blockIdx.x];
     const bool visible = key != 0xFFFFFFFFu;
     const uint64_t bal = __ballot(visible);
     if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
@@ -477,12 +482,11 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     for (int w = 0; w < 4; ++w)
         if ((uint32_t)w < wave) before += s_scan[w];
     if (visible) {
-        const uint32_t pos = offsets[(size_t)v * gridDim.x + blockIdx.x] + before +
+        const uint32_t pos = block_base + before +
                              (uint32_t)__popcll(bal & lanemask_lt());
         vis_keys[pos] = key;
         vis_ids[pos] = (uint32_t)slot;
         if (vis_rects) {
-            const uint2 px = rect_px[slot];
             uint32_t packed = 0u;
             if (px.x) {
                 const uint32_t x0 = (px.x & 0xFFFFu) >> tile_shift, x1 = (((px.x >> 16) - 1u) >> tile_shift) + 1u;
