@@ -75,11 +75,17 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
 }
 
 // Pass 2 (after the block counts are scanned): each block writes the
-// (tile id, splat id) pairs of its 256 splats as one contiguous, coalesced
-// range.  Pairs past the capacity are not written.
+// (tile id, splat id) pairs of its 256 splats as one contiguous range.  A
+// splat with at most kOwnPairs pairs writes them itself, row by row of its
+// tile rectangle; the pairs of larger splats are written by the whole block,
+// each output slot finding its splat by a binary search over the large
+// splats' offsets.  Pairs past the capacity are not written.
+constexpr uint32_t kOwnPairs = 16u;
+
 __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     __shared__ uint32_t s_scan[4];
-    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_lstart[256];   // large splats: first pair (block-relative) ...
+    __shared__ uint32_t s_lrel[256];     // ... and their offset among the large splats' pairs
     __shared__ uint32_t s_id[256];
     __shared__ uint32_t s_tbase[256];
     __shared__ uint32_t s_tx[256];
@@ -88,9 +94,9 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     const uint32_t V = *a.count;
     if (blockIdx.x * 256u >= V) return;
     const uint32_t q = blockIdx.x * 256u + tid;
-    uint32_t cnt = 0u;
+    uint32_t cnt = 0u, val = 0u, tbase = 0u, tx = 0u;
+    uint2 r = make_uint2(0u, 0u);
     if (q < V) {
-        uint2 r;
         if (a.sorted_rects) {
             const uint32_t p = a.sorted_rects[V - 1u - q];
             r = make_uint2((p & 0xFFu) | (((p >> 8) & 0xFFu) << 16), ((p >> 16) & 0xFFu) | ((p >> 24) << 16));
@@ -98,26 +104,54 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
             r = a.rects[q];
         }
         cnt = ((r.x >> 16) - (r.x & 0xFFFFu)) * ((r.y >> 16) - (r.y & 0xFFFFu));
-        s_rect[tid] = r;
-        const uint32_t val = a.sorted_ids[V - 1u - q];
-        s_id[tid] = val;
+        val = a.sorted_ids[V - 1u - q];
         if (cnt) {
             const ViewInfo& vi = a.views[val / a.n];
-            s_tbase[tid] = vi.tile_base;
-            s_tx[tid] = (uint32_t)vi.tiles_x;
+            tbase = vi.tile_base;
+            tx = (uint32_t)vi.tiles_x;
         }
     }
     uint32_t total;
-    s_start[tid] = block_exclusive_sum<4>(cnt, s_scan, &total);
-    __syncthreads();
+    const uint32_t start = block_exclusive_sum<4>(cnt, s_scan, &total);
     const uint32_t base = a.block_offsets[blockIdx.x];
     const uint32_t room = base < a.dup_capacity ? a.dup_capacity - base : 0u;
     const uint32_t emit = min(total, room);
-    for (uint32_t i = tid; i < emit; i += 256u) {
-        const uint32_t o = owner_of(s_start, 256u, i);
-        const uint2 r = s_rect[o];
-        const uint32_t x0 = r.x & 0xFFFFu, w = (r.x >> 16) - x0, y0 = r.y & 0xFFFFu;
-        const uint32_t k = i - s_start[o];
+    const bool large = cnt > kOwnPairs;
+    if (cnt && !large) {
+        const uint32_t x0 = r.x & 0xFFFFu, x1 = r.x >> 16;
+        uint32_t xx = x0, row = tbase + (r.y & 0xFFFFu) * tx;
+        for (uint32_t i = start; i < start + cnt && i < emit; ++i) {
+            a.dup_keys[base + i] = row + xx;
+            a.dup_vals[base + i] = val;
+            if (++xx == x1) {
+                xx = x0;
+                row += tx;
+            }
+        }
+    }
+    // the large splats, compacted in order: their pair offsets among
+    // themselves and their first pair in the block's range
+    uint32_t lpairs;
+    const uint32_t lrel = block_exclusive_sum<4>(large ? cnt : 0u, s_scan, &lpairs);
+    if (lpairs == 0u) return;  // uniform: no splat of the block is large
+    uint32_t nlarge;
+    const uint32_t li = block_exclusive_sum<4>(large ? 1u : 0u, s_scan, &nlarge);
+    if (large) {
+        s_lstart[li] = start;
+        s_lrel[li] = lrel;
+        s_id[li] = val;
+        s_tbase[li] = tbase;
+        s_tx[li] = tx;
+        s_rect[li] = r;
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < lpairs; j += 256u) {
+        const uint32_t o = owner_of(s_lrel, nlarge, j);
+        const uint32_t k = j - s_lrel[o];
+        const uint32_t i = s_lstart[o] + k;
+        if (i >= emit) continue;
+        const uint2 rr = s_rect[o];
+        const uint32_t x0 = rr.x & 0xFFFFu, w = (rr.x >> 16) - x0, y0 = rr.y & 0xFFFFu;
         a.dup_keys[base + i] = s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w;
         a.dup_vals[base + i] = s_id[o];
     }
