@@ -771,6 +771,12 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     int bits_needed = 1;
     while (bits_needed < 31 && ((ntiles - 1u) >> bits_needed)) ++bits_needed;
     const int passes = (bits_needed + c->radix_bits - 1) / c->radix_bits;
+    // the pass count of the configured width, with the digits balanced over
+    // the passes (13 tile bits: two 7-bit passes, not 8 + 5): fewer digits
+    // per pass mean fewer ballots and longer runs of equal digits to write
+    int tile_bits = (bits_needed + passes - 1) / passes;
+    tile_bits = tile_bits <= 4 ? 4 : (tile_bits <= 6 ? 6 : (tile_bits <= 7 ? 7 : 8));
+    if (tile_bits > c->radix_bits) tile_bits = c->radix_bits;
 
     BinLaunch& b = p->bin;
     b = BinLaunch{};
@@ -803,7 +809,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     s.scratch = c->dc()->tile_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
     s.passes = passes;
-    s.bits = c->radix_bits;
+    s.bits = tile_bits;
 
     // the sorted pairs land in buffer 2 for an even pass count, else 1
     const int slot = passes % 2 == 0 ? 2 : 1;

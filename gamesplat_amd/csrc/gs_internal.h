@@ -189,7 +189,7 @@ struct SortLaunch {
     uint32_t* scratch;       // 512 words
     uint32_t* table;         // sort_table_words(capacity) words
     int passes;
-    int bits;                // digit width (8 or 4)
+    int bits;                // digit width (8, 7, 6 or 4)
 };
 // Stable LSD radix sort.  Returns the index (1 or 2) of the buffer holding the
 // result through *result_slot.

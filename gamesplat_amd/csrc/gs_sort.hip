@@ -295,6 +295,8 @@ hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t 
     if (a.passes < 1 || a.passes * a.bits > 32) return hipErrorInvalidValue;
     if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
     if (a.bits == 8) return run_passes<8>(a, result_slot, s);
+    if (a.bits == 7) return run_passes<7>(a, result_slot, s);
+    if (a.bits == 6) return run_passes<6>(a, result_slot, s);
     if (a.bits == 4) return run_passes<4>(a, result_slot, s);
     return hipErrorInvalidValue;
 }
