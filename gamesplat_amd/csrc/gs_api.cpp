@@ -760,8 +760,11 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     s.capacity = c->views * c->n;
     s.scratch = c->dc()->depth_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
-    s.bits = c->radix_bits;
-    s.passes = 32 / c->radix_bits;
+    // 32-bit depth keys: 8-bit digits take 4 passes, 4-bit ones 8; with
+    // 8-bit digits the keys go in five 7-bit passes (shorter digit tables,
+    // longer runs of equal digits)
+    s.bits = c->radix_bits == 8 ? 7 : c->radix_bits;
+    s.passes = (32 + s.bits - 1) / s.bits;
     return s;
 }
 

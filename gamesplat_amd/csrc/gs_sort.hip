@@ -292,7 +292,8 @@ uint32_t sort_table_words(uint32_t capacity) {
 }
 
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {
-    if (a.passes < 1 || a.passes * a.bits > 32) return hipErrorInvalidValue;
+    // the last pass may reach past bit 31 (its top digits are then zero)
+    if (a.passes < 1 || (a.passes - 1) * a.bits >= 32) return hipErrorInvalidValue;
     if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
     if (a.bits == 8) return run_passes<8>(a, result_slot, s);
     if (a.bits == 7) return run_passes<7>(a, result_slot, s);
