@@ -86,6 +86,7 @@ struct FramePlan {
     SortLaunch tiles;
     CompositeLaunch comp;
     int pack_tile;           // tile size the compact packs rectangles for (0: none)
+    int pack_pad;            // 1: it moves the cull's padded rectangles instead
 };
 
 }  // namespace
@@ -99,6 +100,7 @@ struct gs_ctx {
 
     uint32_t n = 0;
     DevBuf vrects[3];  // packed tile rectangles moved by the depth sort
+    DevBuf pad_rects;  // [view][n]: packed tile rectangles grown by a pixel (StochasticSplats)
     DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
@@ -177,6 +179,7 @@ struct gs_ctx {
     bool stats_collected = false;
     int sort_slot = 0;
     int rect_tile = 0;       // tile size of the packed rectangles the last cull wrote (0: none)
+    int rect_pad = 0;        // 1: they are grown by a pixel (StochasticSplats lists)
 
     bool profiling = false;
     std::vector<TimedLaunch> timed;
@@ -253,6 +256,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
     GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
     GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
+    GS_CHECK(ensure(c, c->pad_rects, cap * 4, "padded tile rectangles"));
     GS_CHECK(ensure_table(c, (uint32_t)std::max<size_t>(cap, c->dup_cap)));
     c->view_cap = views;
     return GS_OK;
@@ -439,7 +443,8 @@ void gs_destroy(gs_ctx* c) {
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
                       &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1],
-                      &c->anim_data, &c->lut, &c->display, &c->vrects[0], &c->vrects[1], &c->vrects[2]};
+                      &c->anim_data, &c->lut, &c->display, &c->vrects[0], &c->vrects[1], &c->vrects[2],
+                      &c->pad_rects};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -507,12 +512,12 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
         return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
     GS_CHECK(set_device(c));
     // Scene-resident bytes per Gaussian: planes 240 + record 48 + keys/ids 24
-    // + sorted tile rectangles 12 + dense key 4 + rectangles 16 + GPS
-    // prepared 48 + item count/base 8, plus the GPS work list.
-    size_t required = (size_t)count * (240 + 48 + 24 + 12 + 4 + 16 + 48 + 8) + (size_t)c->gps_budget * 4;
+    // + sorted tile rectangles 12 + dense key 4 + rectangles 16 + padded tile
+    // rectangle 4 + GPS prepared 48 + item count/base 8, plus the GPS work list.
+    size_t required = (size_t)count * (240 + 48 + 24 + 12 + 4 + 16 + 4 + 48 + 8) + (size_t)c->gps_budget * 4;
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
                         c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
-                        c->vrects[0].bytes - c->vrects[1].bytes - c->vrects[2].bytes -
+                        c->vrects[0].bytes - c->vrects[1].bytes - c->vrects[2].bytes - c->pad_rects.bytes -
                         c->dense_keys.bytes - c->rects.bytes - c->rect_px.bytes - c->prepared.bytes - c->item_count.bytes -
                         c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
@@ -791,7 +796,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.views = frame_dev(c)->views;
     b.counters = c->dc();
     b.rect_px = ptr<uint2>(c->rect_px);
-    b.sorted_rects = c->rect_tile == tile ? ptr<uint32_t>(c->vrects[c->sort_slot]) : nullptr;
+    b.sorted_rects = c->rect_tile == tile && !c->rect_pad ? ptr<uint32_t>(c->vrects[c->sort_slot]) : nullptr;
     b.rects = ptr<uint2>(c->rects);
     b.block_counts = ptr<uint32_t>(c->block_counts);
     b.block_offsets = ptr<uint32_t>(c->block_offsets);
@@ -843,7 +848,7 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
                               &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),
                           ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
-                          ptr<uint32_t>(c->vrects[0]), c->stream));
+                          ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_pad ? p.cull.pad_rects : nullptr));
     GS_CHECK(timed_end(c, &t));
     return GS_OK;
 }
@@ -918,7 +923,12 @@ int enqueue_stochastic(gs_ctx* c, bool* binned) {
         GS_CHECK(upload_views(c, c->host_cams.data(), 16));
         GS_CHECK(prepare_composite(c, 16));
         make_composite(c, 16, &c->plan);
-        if (mode) {
+        // rectangles grown by a pixel came with the depth sort: their lists
+        // serve samples with and without jitter up to half a pixel
+        const bool padded = c->rect_pad && c->rect_tile == 16 && mode <= 1;
+        if (padded) {
+            c->plan.bin.sorted_rects = ptr<uint32_t>(c->vrects[c->sort_slot]);
+        } else if (mode) {
             // the tile rectangles come from the records: grown by a pixel,
             // which holds every jitter up to half a pixel, or moved exactly
             BinLaunch& b = c->plan.bin;
@@ -932,7 +942,7 @@ int enqueue_stochastic(gs_ctx* c, bool* binned) {
         }
         GS_CHECK(stage_bin(c, c->plan));
         c->stoch_bins_epoch = c->bin_epoch;
-        c->stoch_bins_mode = mode;
+        c->stoch_bins_mode = padded ? 1 : mode;
         c->stoch_bins_jitter[0] = jx;
         c->stoch_bins_jitter[1] = jy;
     }
@@ -1044,8 +1054,18 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     GS_CHECK(upload_cameras(c, cam, 1));
     c->plan.cull = make_cull(c, effect, opts);
     c->plan.cull_blocks = blocks;
-    c->plan.pack_tile = rect_pack_tile(cam, 1, c->default_tile);
+    // after a StochasticSplats frame the next bin is most likely another one:
+    // the depth sort then carries the rectangles grown by a pixel (16-pixel
+    // tiles), which serve its jittered samples; other bins gather instead
+    const int pad_tile = c->composite_kind == 1 ? rect_pack_tile(cam, 1, 16) : 0;
+    c->plan.pack_pad = pad_tile ? 1 : 0;
+    c->plan.pack_tile = pad_tile ? pad_tile : rect_pack_tile(cam, 1, c->default_tile);
+    if (c->plan.pack_pad) {
+        c->plan.cull.pad_rects = ptr<uint32_t>(c->pad_rects);
+        c->plan.cull.pad_tile_shift = 4;
+    }
     c->rect_tile = c->plan.pack_tile;
+    c->rect_pad = c->plan.pack_pad;
     GS_CHECK(stage_cull(c, c->plan));
     c->have_cull = true;
     c->have_sort = c->have_composite = false;
@@ -1130,7 +1150,9 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     p.cull = make_cull(c, effect, opts);
     p.cull_blocks = blocks;
     p.pack_tile = rect_pack_tile(cams, count, tile);
+    p.pack_pad = 0;
     c->rect_tile = p.pack_tile;
+    c->rect_pad = 0;
     p.depth = make_depth_sort(c);
     // the sorted list lands in buffer 2 for an even pass count, else 1
     c->sort_slot = p.depth.passes % 2 == 0 ? 2 : 1;

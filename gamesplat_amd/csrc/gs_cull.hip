@@ -439,6 +439,11 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             rec[2] = make_float4(color[0], color[1], color[2], radius_y);
             a.rect_px[slot] = pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
                                          (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0);
+            if (a.pad_rects)
+                a.pad_rects[slot] = pack_tile_rect(
+                    pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
+                               (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0, 0.0f, 0.0f, 1),
+                    a.pad_tile_shift);
         }
         // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
         // block's visible count; compaction runs after the counts are scanned
@@ -457,11 +462,14 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
 // v * n + index, which addresses the view's records and rectangles, and the
 // list holds view 0's visible Gaussians in index order, then view 1's, ...
 // With vis_rects, the splat's pixel rectangle goes along as its tile
-// rectangle at tile size 1 << tile_shift, packed into 8-bit fields.
+// rectangle at tile size 1 << tile_shift, packed into 8-bit fields (PACKED:
+// the cull's packed rectangle is moved as it is).
+template <bool PACKED>
 __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict__ dense_keys, uint32_t n,
                                                       const uint32_t* __restrict__ offsets, uint32_t* vis_keys,
                                                       uint32_t* vis_ids, const uint2* __restrict__ rect_px,
-                                                      int tile_shift, uint32_t* vis_rects) {
+                                                      int tile_shift, uint32_t* vis_rects,
+                                                      const uint32_t* __restrict__ packed) {
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t v = blockIdx.y;
@@ -471,7 +479,11 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     // the rectangle is loaded with the key (one memory round trip), and used
     // only for visible splats
     uint2 px = make_uint2(0u, 0u);
-    if (vis_rects && idx < n) px = rect_px[slot];
+    uint32_t pre = 0u;
+    if (vis_rects && idx < n) {
+        if constexpr (PACKED) pre = packed[slot];
+        else px = rect_px[slot];
+    }
     const uint32_t block_base = offsets[(size_t)v * gridDim.x + blockIdx.x];
     const bool visible = key != 0xFFFFFFFFu;
     const uint64_t bal = __ballot(visible);
@@ -486,15 +498,7 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
                              (uint32_t)__popcll(bal & lanemask_lt());
         vis_keys[pos] = key;
         vis_ids[pos] = (uint32_t)slot;
-        if (vis_rects) {
-            uint32_t packed = 0u;
-            if (px.x) {
-                const uint32_t x0 = (px.x & 0xFFFFu) >> tile_shift, x1 = (((px.x >> 16) - 1u) >> tile_shift) + 1u;
-                const uint32_t y0 = (px.y & 0xFFFFu) >> tile_shift, y1 = (((px.y >> 16) - 1u) >> tile_shift) + 1u;
-                packed = x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
-            }
-            vis_rects[pos] = packed;
-        }
+        if (vis_rects) vis_rects[pos] = PACKED ? pre : pack_tile_rect(px, tile_shift);
     }
 }
 
@@ -554,7 +558,7 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
 
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
                           uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
-                          uint32_t* vis_rects, hipStream_t s) {
+                          uint32_t* vis_rects, hipStream_t s, const uint32_t* packed) {
     if (n == 0 || views == 0) return hipSuccess;
     int shift = 0;
     if (pack_tile) {
@@ -563,9 +567,14 @@ hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views
         while ((1 << shift) < pack_tile) ++shift;
     } else {
         vis_rects = nullptr;
+        packed = nullptr;
     }
-    hipLaunchKernelGGL(compact_kernel, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n, offsets,
-                       vis_keys, vis_ids, rect_px, shift, vis_rects);
+    if (packed)
+        hipLaunchKernelGGL(compact_kernel<true>, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n,
+                           offsets, vis_keys, vis_ids, rect_px, shift, vis_rects, packed);
+    else
+        hipLaunchKernelGGL(compact_kernel<false>, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n,
+                           offsets, vis_keys, vis_ids, rect_px, shift, vis_rects, packed);
     return hipGetLastError();
 }
 

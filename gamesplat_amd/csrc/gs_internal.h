@@ -156,6 +156,10 @@ struct CullLaunch {
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats, one view)
     uint2* rect_px;          // [view][n]: covered pixel rectangle of each visible splat
     int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
+    // optional [view][n]: the rectangle grown by a pixel (every StochasticSplats
+    // jitter up to half a pixel), packed as tiles of 1 << pad_tile_shift
+    uint32_t* pad_rects;
+    int pad_tile_shift;
     // GSAnim appearance: 0 none, 1 int8 deltas (48 per row and state, state
     // 0 excluded), 2 half coefficients (48 per row and layer)
     int anim;
@@ -170,10 +174,11 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
 // With pack_tile (8, 16 or 32) it also writes, per visible splat, the tile
 // rectangle of its pixel rectangle rect_px at that tile size as
 // x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when empty; every view must span at
-// most 255 tiles a side), which the depth sort carries to the binning.
+// most 255 tiles a side), which the depth sort carries to the binning; with
+// packed (the cull's pad_rects) that packed rectangle is moved instead.
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
                           uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
-                          uint32_t* vis_rects, hipStream_t s);
+                          uint32_t* vis_rects, hipStream_t s, const uint32_t* packed = nullptr);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
