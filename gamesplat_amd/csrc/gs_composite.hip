@@ -512,8 +512,10 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
                 const bool covered = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & !(power > 0.0f) &
                                      !(power < par.z);
                 const bool nearer = q < best_q || (q == best_q && id < best_id);
-                bool keep = false;
-                if (covered && nearer) {
+                const bool cand = covered & nearer;
+                // the wave tests its candidate pixels together, without
+                // per-lane branches (the scalar unit is shared by the CU)
+                if (__ballot(cand)) {
                     // keep iff h < alpha = min(o exp(power), 0.99).  The
                     // hardware exp2 brackets alpha within 2e-5 relative: its
                     // 1-ulp error, the rounding of power * log2(e) (1.2e-7
@@ -524,22 +526,20 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
                     // exp, so keep decisions match the CPU restatement.
                     const float h = hash13(fx, fy, (float)id + sl.frame_seed);
                     const float oe = par.y * __builtin_amdgcn_exp2f(power * 1.44269504f);
-                    if (h < minf(oe * (1.0f - 2e-5f), 0.99f)) {
-                        keep = true;
-                    } else if (h < minf(oe * (1.0f + 2e-5f), 0.99f)) {
-                        keep = h < minf(par.y * expf_(power), 0.99f);
+                    bool keep = cand & (h < minf(oe * (1.0f - 2e-5f), 0.99f));
+                    const bool unsure = cand & !keep & (h < minf(oe * (1.0f + 2e-5f), 0.99f));
+                    if (__ballot(unsure)) {
+                        const bool exact = h < minf(par.y * expf_(power), 0.99f);
+                        keep = unsure ? exact : keep;
                     }
+                    best_q = keep ? q : best_q;
+                    best_id = keep ? id : best_id;
+                    cr = keep ? col.x : cr;
+                    cg = keep ? col.y : cg;
+                    cb = keep ? col.z : cb;
+                    kept = kept | keep;
                 }
-                const bool win = keep;
-                if (win) {
-                    best_q = q;
-                    best_id = id;
-                    cr = col.x;
-                    cg = col.y;
-                    cb = col.z;
-                    kept = true;
-                }
-                if (q > best_q + kDepthSlack) open = false;
+                open = open & !(q > best_q + kDepthSlack);
             }
             if (!__ballot(open)) break;
         }
