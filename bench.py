@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Kernel timed by each roofline candidate (the cull and composite brackets
 # hold exactly one launch of it; the sort bracket holds its 12 launches).
-STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<8,true>",
+STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<7,true>",
                 "composite": "composite_kernel<16,true>"}
 PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
 
@@ -417,12 +417,13 @@ def main():
     #   cull kernel  P0 of every Gaussian (16 B) and its dense depth key (4 B);
     #                P1-P3 and the 11 SH planes of a visible one (224 B), its
     #                record (48 B) and pixel rectangle (8 B)
-    #   depth sort   4 passes x (upsweep key 4 B + downsweep key, id and tile
-    #                rectangle read 12 B and written 12 B) per visible splat
+    #   depth sort   5 passes (7-bit digits) x (upsweep key 4 B + downsweep
+    #                key, id and tile rectangle read 12 B and written 12 B) per
+    #                visible splat
     #   composite    per tile entry its id and record (4 + 48 B, rounded to 56
     #                with the tile range), plus the RGBA16F image
     cull_bytes = 20 * N + 280 * V
-    sort_bytes = 112 * V
+    sort_bytes = 140 * V
     comp_bytes = 56 * D + 8 * W * H
     cands = {
         "cull": (stage_ms["cull_kernel"], cull_bytes),
