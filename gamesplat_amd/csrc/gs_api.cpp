@@ -99,9 +99,10 @@ struct gs_ctx {
     size_t held = 0;
 
     uint32_t n = 0;
-    DevBuf vrects[3];  // packed tile rectangles moved by the depth sort
+    DevBuf vrects[4];  // packed tile rectangles moved by the depth sort
+    DevBuf block_range;  // [view][block]: visible key range
     DevBuf pad_rects;  // [view][n]: packed tile rectangles grown by a pixel (StochasticSplats)
-    DevBuf planes, records, keys[3], ids[3], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
+    DevBuf planes, records, keys[4], ids[4], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
 
@@ -245,7 +246,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     const size_t cap = (size_t)views * c->n;
     if (cap > 0xFFFFFFFFull) return fail(GS_ERR_LIMIT, "%u views x %u Gaussians exceed 2^32 splats", views, c->n);
     GS_CHECK(ensure(c, c->records, cap * kRecordVec4 * 16, "projected splats"));
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         GS_CHECK(ensure(c, c->keys[k], cap * 4, "sort keys"));
         GS_CHECK(ensure(c, c->ids[k], cap * 4, "sort ids"));
         GS_CHECK(ensure(c, c->vrects[k], cap * 4, "sorted tile rectangles"));
@@ -253,6 +254,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     const size_t blocks = (size_t)views * ((c->n + 255u) / 256u);
     GS_CHECK(ensure(c, c->dense_keys, cap * 4, "dense keys"));
     GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
+    GS_CHECK(ensure(c, c->block_range, blocks * 8, "block key ranges"));
     GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
     GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
     GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
@@ -437,8 +439,8 @@ void gs_destroy(gs_ctx* c) {
         (void)hipEventDestroy(t.stop);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->ids[0], &c->ids[1],
-                      &c->ids[2], &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
+    DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->keys[3], &c->ids[0],
+                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
@@ -516,8 +518,9 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     // rectangle 4 + GPS prepared 48 + item count/base 8, plus the GPS work list.
     size_t required = (size_t)count * (240 + 48 + 24 + 12 + 4 + 16 + 4 + 48 + 8) + (size_t)c->gps_budget * 4;
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
-                        c->keys[2].bytes - c->ids[0].bytes - c->ids[1].bytes - c->ids[2].bytes -
-                        c->vrects[0].bytes - c->vrects[1].bytes - c->vrects[2].bytes - c->pad_rects.bytes -
+                        c->keys[2].bytes - c->keys[3].bytes - c->ids[0].bytes - c->ids[1].bytes -
+                        c->ids[2].bytes - c->ids[3].bytes - c->vrects[0].bytes - c->vrects[1].bytes -
+                        c->vrects[2].bytes - c->vrects[3].bytes - c->pad_rects.bytes - c->block_range.bytes -
                         c->dense_keys.bytes - c->rects.bytes - c->rect_px.bytes - c->prepared.bytes - c->item_count.bytes -
                         c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
@@ -722,6 +725,7 @@ CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullO
     a.records = ptr<float4>(c->records);
     a.dense_keys = ptr<uint32_t>(c->dense_keys);
     a.block_counts = ptr<uint32_t>(c->block_counts);
+    a.block_range = ptr<uint2>(c->block_range);
     a.block_diag = a.collect_stats ? ptr<uint32_t>(c->block_diag) : nullptr;
     a.rect_px = ptr<uint2>(c->rect_px);
     a.tight_rects = c->tight_rects;
@@ -761,13 +765,19 @@ SortLaunch make_depth_sort(gs_ctx* c) {
         s.vals[k] = ptr<uint32_t>(c->ids[k]);
         s.vals2[k] = c->rect_tile ? ptr<uint32_t>(c->vrects[k]) : nullptr;
     }
+    // the keys' range (reduced by the cull's scan) sets the passes that run;
+    // the result is in slot 3
+    s.keys[3] = ptr<uint32_t>(c->keys[3]);
+    s.vals[3] = ptr<uint32_t>(c->ids[3]);
+    s.vals2[3] = c->rect_tile ? ptr<uint32_t>(c->vrects[3]) : nullptr;
+    s.key_range = &c->dc()->cull.key_min;
     s.count = &c->dc()->cull.visible_count;
     s.capacity = c->views * c->n;
     s.scratch = c->dc()->depth_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
     // 32-bit depth keys: 8-bit digits take 4 passes, 4-bit ones 8; with
-    // 8-bit digits the keys go in five 7-bit passes (shorter digit tables,
-    // longer runs of equal digits)
+    // 8-bit digits the keys go in 7-bit passes (shorter digit tables, longer
+    // runs of equal digits), at most five, as many as the key range needs
     s.bits = c->radix_bits == 8 ? 7 : c->radix_bits;
     s.passes = (32 + s.bits - 1) / s.bits;
     return s;
@@ -845,7 +855,8 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
     GS_HIP(launch_cull(p.cull, c->stream));
     GS_CHECK(timed_end(c, &tk));
     GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->block_offsets),
-                              &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream));
+                              &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream, 0,
+                              nullptr, p.cull.block_range, &c->dc()->cull.key_min));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),
                           ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
                           ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_pad ? p.cull.pad_rects : nullptr));
@@ -1154,8 +1165,8 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     c->rect_tile = p.pack_tile;
     c->rect_pad = 0;
     p.depth = make_depth_sort(c);
-    // the sorted list lands in buffer 2 for an even pass count, else 1
-    c->sort_slot = p.depth.passes % 2 == 0 ? 2 : 1;
+    // the depth sort's last needed pass writes the sorted list to buffer 3
+    c->sort_slot = 3;
     make_composite(c, tile, &p);
     if (c->use_graph && !c->profiling) {
         GS_CHECK(launch_graph(c, p));

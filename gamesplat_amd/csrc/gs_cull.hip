@@ -133,6 +133,7 @@ template <int ANIM>
 __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
     __shared__ uint32_t s_cnt[kMaxViews][4];
     __shared__ uint32_t s_diag[8];
+    __shared__ uint32_t s_klo[kMaxViews][4], s_khi[kMaxViews][4];  // visible key range per view and wave
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid < 8) s_diag[tid] = 0u;
@@ -450,10 +451,27 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
         if (idx < n) a.dense_keys[slot] = key;
         const uint64_t bal = __ballot(visible);
         if (lane == 0) s_cnt[v][wave] = (uint32_t)__popcll(bal);
+        if (a.block_range) {
+            uint32_t lo = visible ? key : 0xFFFFFFFFu, hi = visible ? key : 0u;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
+                hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
+            }
+            if (lane == 0) {
+                s_klo[v][wave] = lo;
+                s_khi[v][wave] = hi;
+            }
+        }
     }
     __syncthreads();
-    for (uint32_t v = tid; v < views; v += 256u)
+    for (uint32_t v = tid; v < views; v += 256u) {
         a.block_counts[(size_t)v * gridDim.x + block] = s_cnt[v][0] + s_cnt[v][1] + s_cnt[v][2] + s_cnt[v][3];
+        if (a.block_range)
+            a.block_range[(size_t)v * gridDim.x + block] =
+                make_uint2(min(min(s_klo[v][0], s_klo[v][1]), min(s_klo[v][2], s_klo[v][3])),
+                           max(max(s_khi[v][0], s_khi[v][1]), max(s_khi[v][2], s_khi[v][3])));
+    }
     if (a.collect_stats && tid < 8) a.block_diag[(size_t)block * 8 + tid] = s_diag[tid];
 }
 

@@ -48,9 +48,12 @@ struct ViewInfo {
 // Totals are written by the scan kernels, never accumulated with atomics.
 struct CullCounters {
     uint32_t visible_count;
+    uint32_t pad0;
+    uint32_t key_min, key_max;   // range of the visible depth keys (all views), 8-byte aligned
     uint32_t diag[8];
-    uint32_t pad[7];
+    uint32_t pad[4];
 };
+static_assert(sizeof(CullCounters) == 64, "CullCounters layout");
 struct SortCounters {
     uint32_t scratch[512];   // per pass: digit column totals
 };
@@ -137,7 +140,9 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
 // (nb x 8, may be NULL) is reduced into diag_out.
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
                               const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s,
-                              uint32_t clamp_cap = 0, uint32_t* clamped = nullptr);
+                              uint32_t clamp_cap = 0, uint32_t* clamped = nullptr,
+                              const uint2* block_range = nullptr, uint32_t* range_out = nullptr);
+// block_range (nb x (min, max), may be NULL) is reduced into range_out[0..1].
 
 // gs_cull.hip
 struct CullLaunch {
@@ -153,6 +158,7 @@ struct CullLaunch {
     float4* records;         // [view][n] x 3 float4
     uint32_t* dense_keys;    // [view][n]: depth key or 0xFFFFFFFF when culled
     uint32_t* block_counts;  // [view][block]: visible Gaussians per 256-block
+    uint2* block_range;      // [view][block]: min and max of the visible depth keys (or NULL)
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats, one view)
     uint2* rect_px;          // [view][n]: covered pixel rectangle of each visible splat
     int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
@@ -186,9 +192,14 @@ hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t fi
 
 // gs_sort.hip
 struct SortLaunch {
-    uint32_t* keys[3];       // keys[0] input; passes ping-pong keys[1] <-> keys[2]
-    uint32_t* vals[3];
-    uint32_t* vals2[3];      // optional second value moved with the same permutation (NULL: none)
+    // keys[0] input; passes ping-pong keys[1] <-> keys[2]; with key_range the
+    // last pass the range needs writes keys[3] instead
+    uint32_t* keys[4];
+    uint32_t* vals[4];
+    uint32_t* vals2[4];      // optional second value moved with the same permutation (NULL: none)
+    // optional device (min, max) of the keys: the passes sort key - min, and
+    // those past the bits the range needs exit at once
+    const uint32_t* key_range;
     const uint32_t* count;   // device element count
     uint32_t capacity;       // grid bound
     uint32_t* scratch;       // 512 words
@@ -196,8 +207,8 @@ struct SortLaunch {
     int passes;
     int bits;                // digit width (8, 7, 6 or 4)
 };
-// Stable LSD radix sort.  Returns the index (1 or 2) of the buffer holding the
-// result through *result_slot.
+// Stable LSD radix sort.  Returns the index (1, 2, or 3 with key_range) of
+// the buffer holding the result through *result_slot.
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s);
 uint32_t sort_table_words(uint32_t capacity);
 

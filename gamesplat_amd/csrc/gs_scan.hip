@@ -17,6 +17,46 @@ constexpr int kScanThreads = 1024;
 constexpr int kRun = 32;             // consecutive counts per thread per round
 constexpr uint32_t kLdsCounts = 32768;  // counts staged in LDS (128 KiB)
 
+// Smallest min and largest max of nb (min, max) pairs into out[0], out[1].
+__device__ __forceinline__ void reduce_range(const uint2* __restrict__ block_range, uint32_t nb, uint32_t* out,
+                                             uint32_t* s_lo, uint32_t* s_hi) {
+    const uint32_t tid = threadIdx.x;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    // eight loads in flight per thread
+    constexpr int kBatch = 8;
+    for (uint32_t i0 = tid; i0 < nb; i0 += kScanThreads * kBatch) {
+        uint2 r[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
+            r[j] = i < nb ? block_range[i] : make_uint2(0xFFFFFFFFu, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            lo = min(lo, r[j].x);
+            hi = max(hi, r[j].y);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
+    }
+    if ((tid & 63u) == 0) {
+        s_lo[tid >> 6] = lo;
+        s_hi[tid >> 6] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < kScanThreads / 64; ++w) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        out[0] = lo;
+        out[1] = hi;
+    }
+}
+
 __device__ __forceinline__ void reduce_diag(const uint32_t* __restrict__ block_diag, uint32_t nb, uint32_t* diag_out,
                                             uint32_t* s_diag) {
     const uint32_t tid = threadIdx.x;
@@ -41,10 +81,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
                                                                        uint32_t* total,
                                                                        const uint32_t* __restrict__ block_diag,
                                                                        uint32_t* diag_out, uint32_t clamp_cap,
-                                                                       uint32_t* clamped) {
+                                                                       uint32_t* clamped,
+                                                                       const uint2* __restrict__ block_range,
+                                                                       uint32_t* range_out) {
     extern __shared__ uint32_t s_c[];
     __shared__ uint32_t s_scan[kScanThreads / 64];
     __shared__ uint32_t s_diag[8];
+    __shared__ uint32_t s_lo[kScanThreads / 64], s_hi[kScanThreads / 64];
     const uint32_t tid = threadIdx.x;
     // eight loads in flight per thread before their LDS stores
     constexpr int kBatch = 8;
@@ -81,6 +124,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
         if (clamped) *clamped = min(all, clamp_cap);
     }
     if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
+    if (block_range) reduce_range(block_range, nb, range_out, s_lo, s_hi);
 }
 
 __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_t* __restrict__ counts,
@@ -88,9 +132,12 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
                                                                    uint32_t* total,
                                                                    const uint32_t* __restrict__ block_diag,
                                                                    uint32_t* diag_out, uint32_t clamp_cap,
-                                                                   uint32_t* clamped) {
+                                                                   uint32_t* clamped,
+                                                                   const uint2* __restrict__ block_range,
+                                                                   uint32_t* range_out) {
     __shared__ uint32_t s_scan[kScanThreads / 64];
     __shared__ uint32_t s_diag[8];
+    __shared__ uint32_t s_lo[kScanThreads / 64], s_hi[kScanThreads / 64];
     const uint32_t tid = threadIdx.x;
     // Rounds of 1024 runs of kRun consecutive counts: every thread issues the
     // loads of its run together (one memory latency per round), sums it, one
@@ -119,13 +166,15 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
         if (clamped) *clamped = min(carry, clamp_cap);
     }
     if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
+    if (block_range) reduce_range(block_range, nb, range_out, s_lo, s_hi);
 }
 
 }  // namespace
 
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
                               const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s, uint32_t clamp_cap,
-                              uint32_t* clamped) {
+                              uint32_t* clamped, const uint2* block_range, uint32_t* range_out) {
+    if (block_range && !range_out) return hipErrorInvalidValue;
     if (nb <= kLdsCounts) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -135,10 +184,10 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* off
             attr_set = true;
         }
         hipLaunchKernelGGL(scan_counts_lds_kernel, dim3(1), dim3(kScanThreads), (size_t)(nb ? nb : 1) * 4, s, counts,
-                           nb, offsets, total, block_diag, diag_out, clamp_cap, clamped);
+                           nb, offsets, total, block_diag, diag_out, clamp_cap, clamped, block_range, range_out);
     } else {
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, offsets, total,
-                           block_diag, diag_out, clamp_cap, clamped);
+                           block_diag, diag_out, clamp_cap, clamped, block_range, range_out);
     }
     return hipGetLastError();
 }

@@ -33,16 +33,54 @@ constexpr int kTile = kThreads * kItems;
 struct PassArgs {
     const uint32_t* keys_in;
     const uint32_t* vals_in;
+    const uint32_t* vals2_in;   // second value array (P2 passes)
     uint32_t* keys_out;
     uint32_t* vals_out;
-    const uint32_t* vals2_in;   // second value array (P2 passes)
     uint32_t* vals2_out;
+    uint32_t* keys_last;        // output of the last pass the key range needs (adaptive)
+    uint32_t* vals_last;
+    uint32_t* vals2_last;
     const uint32_t* count;
-    uint32_t* table;        // [RADIX][tiles]: tile digit counts, then column offsets
-    uint32_t tiles;         // table stride (grid size)
-    uint32_t* col_total;    // [RADIX]
+    const uint32_t* key_range;  // device (min, max) of the keys (read when adaptive)
+    uint32_t* table;            // [RADIX][tiles]: tile digit counts, then column offsets
+    uint32_t tiles;             // table stride (grid size)
+    uint32_t* col_total;        // [RADIX]
+    int adaptive;
+    int pass, passes;           // this pass, passes launched
     int shift;
 };
+
+// Whether pass `pass` runs and what it adds to or subtracts from the keys.
+// Fixed passes all run and the buffers alternate.  Adaptive passes sort
+// key - min: the first subtracts min as it loads the keys, the last one the
+// range needs adds it back and writes the *_last buffers, the later ones exit.
+struct PassPlan {
+    bool active, last;
+    uint32_t sub, add;
+};
+
+template <int BITS>
+__device__ __forceinline__ PassPlan pass_plan(const PassArgs& a, uint2 kr) {
+    int needed = a.passes;
+    if (a.adaptive) {
+        const uint32_t range = kr.y > kr.x ? kr.y - kr.x : 0u;
+        const int sig = range ? 32 - __clz((int)range) : 0;
+        needed = min(a.passes, max(1, (sig + BITS - 1) / BITS));
+    }
+    PassPlan p;
+    p.active = a.pass < needed;
+    p.last = a.adaptive && a.pass == needed - 1;
+    p.sub = a.adaptive && a.pass == 0 ? kr.x : 0u;
+    p.add = p.last ? kr.x : 0u;
+    return p;
+}
+
+// The element count and the key range, loaded together (one memory wait).
+__device__ __forceinline__ uint32_t load_count_range(const PassArgs& a, uint2* kr) {
+    const uint32_t count = *a.count;
+    *kr = *reinterpret_cast<const uint2*>(a.key_range);
+    return count;
+}
 
 // Digit counts of one tile.  Each wave counts into kCopies histograms, lane
 // l into copy l % kCopies, so a digit shared by many lanes (the top digit of
@@ -54,24 +92,28 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int kCopies = 4;
     __shared__ uint32_t s_hist[kWaves * kCopies][RADIX];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t count = *a.count;
+    uint2 kr;
+    const uint32_t count = load_count_range(a, &kr);
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     if (base >= count) return;
+    const PassPlan plan = pass_plan<BITS>(a, kr);
+    if (!plan.active) return;
     for (uint32_t i = tid; i < (uint32_t)(kWaves * kCopies * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
     uint32_t* hist = s_hist[wave * kCopies + lane % kCopies];
     const uint32_t wbase = base + wave * (64u * kItems);
+    const uint32_t* keys_in = a.keys_in;
     uint32_t keys[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        keys[i] = g < count ? a.keys_in[g] : 0u;
+        keys[i] = g < count ? keys_in[g] : 0u;
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        if (g < count) atomicAdd(&hist[(keys[i] >> a.shift) & MASK], 1u);
+        if (g < count) atomicAdd(&hist[((keys[i] - plan.sub) >> a.shift) & MASK], 1u);
     }
     __syncthreads();
     if (tid < (uint32_t)RADIX) {
@@ -90,7 +132,9 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     constexpr int kRun = 8;
     __shared__ uint32_t s_scan[kWaves];
     const uint32_t tid = threadIdx.x, d = blockIdx.x;
-    const uint32_t count = *a.count;
+    uint2 kr;
+    const uint32_t count = load_count_range(a, &kr);
+    if (!pass_plan<BITS>(a, kr).active) return;
     const uint32_t used = (count + kTile - 1) / kTile;
     uint32_t* col = a.table + (size_t)d * a.tiles;
     uint32_t carry = 0u;
@@ -127,10 +171,19 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
-    const uint32_t count = *a.count;
+    uint2 kr;
+    const uint32_t count = load_count_range(a, &kr);
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     if (base >= count) return;
+    const PassPlan plan = pass_plan<BITS>(a, kr);
+    if (!plan.active) return;
+    const uint32_t* keys_in = a.keys_in;
+    const uint32_t* vals_in = a.vals_in;
+    const uint32_t* vals2_in = a.vals2_in;
+    uint32_t* keys_out = plan.last ? a.keys_last : a.keys_out;
+    uint32_t* vals_out = plan.last ? a.vals_last : a.vals_out;
+    uint32_t* vals2_out = plan.last ? a.vals2_last : a.vals2_out;
     for (uint32_t i = tid; i < (uint32_t)(kWaves * RADIX); i += kThreads) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t valid = min((uint32_t)kTile, count - base);
@@ -145,8 +198,8 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        const uint32_t k = a.keys_in[min(g, last)];
-        keys[i] = g < count ? k : 0xFFFFFFFFu;
+        const uint32_t k = keys_in[min(g, last)];
+        keys[i] = g < count ? k - plan.sub : 0xFFFFFFFFu;
     }
 
     // 2. stable rank within the wave: peers = lanes with the same digit,
@@ -211,7 +264,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     // values of positions past the count are staged but never written out
     uint32_t v[kItems];
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) v[i] = a.vals_in[min(wbase + (uint32_t)i * 64u + lane, last)];
+    for (int i = 0; i < kItems; ++i) v[i] = vals_in[min(wbase + (uint32_t)i * 64u + lane, last)];
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
@@ -222,7 +275,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
         if (p < valid) {
             uint32_t k = s_buf[p];
             outs[j] = s_dest[(k >> shift) & MASK] + p;
-            a.keys_out[outs[j]] = k;
+            keys_out[outs[j]] = k + plan.add;
         }
     }
     __syncthreads();
@@ -230,13 +283,13 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
     if constexpr (P2) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) v[i] = a.vals2_in[min(wbase + (uint32_t)i * 64u + lane, last)];
+        for (int i = 0; i < kItems; ++i) v[i] = vals2_in[min(wbase + (uint32_t)i * 64u + lane, last)];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
-        if (p < valid) a.vals_out[outs[j]] = s_buf[p];
+        if (p < valid) vals_out[outs[j]] = s_buf[p];
     }
     if constexpr (P2) {
         __syncthreads();
@@ -246,7 +299,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t p = tid + (uint32_t)j * kThreads;
-            if (p < valid) a.vals2_out[outs[j]] = s_buf[p];
+            if (p < valid) vals2_out[outs[j]] = s_buf[p];
         }
     }
 }
@@ -255,19 +308,27 @@ template <int BITS>
 hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     uint32_t tiles = (a.capacity + kTile - 1) / kTile;
     if (tiles < 1) tiles = 1;
+    PassArgs pa{};
+    pa.count = a.count;
+    // a readable placeholder when the passes are fixed (not used for them)
+    pa.key_range = a.key_range ? a.key_range : a.scratch;
+    pa.adaptive = a.key_range ? 1 : 0;
+    pa.keys_last = a.keys[3];
+    pa.vals_last = a.vals[3];
+    pa.vals2_last = a.vals2[3];
+    pa.passes = a.passes;
+    pa.table = a.table;
+    pa.tiles = tiles;
+    pa.col_total = a.scratch;
     int src = 0, dst = 1;
     for (int p = 0; p < a.passes; ++p) {
-        PassArgs pa;
         pa.keys_in = a.keys[src];
         pa.vals_in = a.vals[src];
+        pa.vals2_in = a.vals2[src];
         pa.keys_out = a.keys[dst];
         pa.vals_out = a.vals[dst];
-        pa.vals2_in = a.vals2[src];
         pa.vals2_out = a.vals2[dst];
-        pa.count = a.count;
-        pa.table = a.table;
-        pa.tiles = tiles;
-        pa.col_total = a.scratch;
+        pa.pass = p;
         pa.shift = p * BITS;
         hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
         hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
@@ -280,7 +341,7 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         src = dst;
         dst = dst == 1 ? 2 : 1;
     }
-    *result_slot = src;
+    *result_slot = a.key_range ? 3 : src;
     return hipSuccess;
 }
 
@@ -295,6 +356,7 @@ hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t 
     // the last pass may reach past bit 31 (its top digits are then zero)
     if (a.passes < 1 || (a.passes - 1) * a.bits >= 32) return hipErrorInvalidValue;
     if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
+    if (a.key_range && (!a.keys[3] || !a.vals[3] || (a.vals2[0] && !a.vals2[3]))) return hipErrorInvalidValue;
     if (a.bits == 8) return run_passes<8>(a, result_slot, s);
     if (a.bits == 7) return run_passes<7>(a, result_slot, s);
     if (a.bits == 6) return run_passes<6>(a, result_slot, s);

@@ -82,6 +82,34 @@ def test_sort_ties_and_sizes(ctx, orc, gs):
         np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si, err_msg=f"n={n}")
 
 
+@pytest.mark.parametrize("spread,passes", [(0.0, 1), (1e-3, 2), (0.3, 3), (None, 4), (1e10, 5)])
+def test_sort_key_range_passes(ctx, orc, gs, spread, passes):
+    # The depth sort runs only the 7-bit passes that the visible keys' range
+    # (max - min) needs.  Splats on the view axis at depths 15 + U[0, spread)
+    # give a range of each pass count; 1e10 puts one splat far out (range
+    # above 2^28, all five passes).
+    print(f"case: sort key range spread={spread}", flush=True)
+    cam = scenes.random_view_camera(21, 640, 480)
+    d = scenes._direction(21).astype(np.float64)
+    g = gs.generate_scene(9, 20000, 4, 0.0)
+    if spread == 1e10:
+        g[0, 0:3] = d * (15.0 - 1e10)
+        g[1, 0:3] = d * (15.0 - 0.25)
+    elif spread is not None:
+        t = np.random.Generator(np.random.PCG64(5)).random(len(g)) * spread
+        g[:, 0:3] = (d[None, :] * -t[:, None]).astype(np.float32)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    vis = ok[oi != 0xFFFFFFFF].astype(np.int64)
+    assert vis.size > 1
+    span = int(vis.max() - vis.min())
+    need = max(1, -(-span.bit_length() // 7))
+    assert need == passes, f"key range {span:#x} needs {need} passes"
+    ctx.sort()
+    sk, si = orc.radix_sort(ok, oi)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_KEYS), sk)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si)
+
+
 def test_all_culled(ctx, orc, gs):
     print("case: all culled", flush=True)
     g = gs.generate_scene(3, 1000, 2, 0.0)
