@@ -27,7 +27,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # Kernel timed by each roofline candidate (the cull and composite brackets
-# hold exactly one launch of it; the sort bracket holds its 12 launches).
+# hold exactly one launch of it; the sort bracket holds its 15 launches, of
+# which those past the passes the key range needs exit at once).
 STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<7,true>",
                 "composite": "composite_kernel<16,true>"}
 PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
@@ -351,6 +352,9 @@ def main():
     ms = elapsed * 1000.0 / args.steps
     st = ctx.stats()
     V, D = st["visible_count"], st["tile_entries"]
+    # the depth sort runs the 7-bit passes the visible keys' range needs
+    sk = ctx.read_buffer(gs.abi.GS_BUFFER_SORTED_KEYS)[:V].astype(np.int64)
+    depth_passes = max(1, -(-int(sk.max() - sk.min()).bit_length() // 7)) if V else 1
 
     # per-frame GPU times from events on the context stream
     ft = sorted(frame_times(ctx, lambda k: ctx.render_alpha(cam, tile=args.tile), max(100, args.steps)))
@@ -417,13 +421,13 @@ def main():
     #   cull kernel  P0 of every Gaussian (16 B) and its dense depth key (4 B);
     #                P1-P3 and the 11 SH planes of a visible one (224 B), its
     #                record (48 B) and pixel rectangle (8 B)
-    #   depth sort   5 passes (7-bit digits) x (upsweep key 4 B + downsweep
-    #                key, id and tile rectangle read 12 B and written 12 B) per
-    #                visible splat
+    #   depth sort   per 7-bit pass the key range needs (upsweep key 4 B +
+    #                downsweep key, id and tile rectangle read 12 B and
+    #                written 12 B) per visible splat
     #   composite    per tile entry its id and record (4 + 48 B, rounded to 56
     #                with the tile range), plus the RGBA16F image
     cull_bytes = 20 * N + 280 * V
-    sort_bytes = 140 * V
+    sort_bytes = 28 * depth_passes * V
     comp_bytes = 56 * D + 8 * W * H
     cands = {
         "cull": (stage_ms["cull_kernel"], cull_bytes),
@@ -481,7 +485,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"sorted-alpha frame, {N} Gaussians SH-3, {W}x{H}, RGBA16F target",
                    "gaussians": N, "width": W, "height": H, "visible": V, "tile_entries": D,
-                   "seed": cfg["seed"], "parallelism": "single GPU"},
+                   "depth_sort_passes": depth_passes, "seed": cfg["seed"], "parallelism": "single GPU"},
         "gaussians_per_s": round(N * 1000.0 / ms, 1),
         "tile_entries_per_s": round(D / (stage_ms["composite"] * 1e-3), 1) if stage_ms["composite"] else None,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
