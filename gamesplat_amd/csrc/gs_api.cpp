@@ -804,6 +804,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.n = c->n;
     b.tile = tile;
     b.views = frame_dev(c)->views;
+    b.single_tiles_x = c->views == 1 ? ((uint32_t)c->cam.viewport[0] + (uint32_t)tile - 1u) / (uint32_t)tile : 0u;
     b.counters = c->dc();
     b.rect_px = ptr<uint2>(c->rect_px);
     b.sorted_rects = c->rect_tile == tile && !c->rect_pad ? ptr<uint32_t>(c->vrects[c->sort_slot]) : nullptr;

@@ -94,7 +94,9 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     const uint32_t V = *a.count;
     if (blockIdx.x * 256u >= V) return;
     const uint32_t q = blockIdx.x * 256u + tid;
-    uint32_t cnt = 0u, val = 0u, tbase = 0u, tx = 0u;
+    // the block's first pair, loaded with the splats (no dependence on them)
+    const uint32_t base = a.block_offsets[blockIdx.x];
+    uint32_t cnt = 0u, val = 0u, tbase = 0u, tx = a.single_tiles_x;
     uint2 r = make_uint2(0u, 0u);
     if (q < V) {
         if (a.sorted_rects) {
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         }
         cnt = ((r.x >> 16) - (r.x & 0xFFFFu)) * ((r.y >> 16) - (r.y & 0xFFFFu));
         val = a.sorted_ids[V - 1u - q];
-        if (cnt) {
+        if (cnt && !a.single_tiles_x) {
             const ViewInfo& vi = a.views[val / a.n];
             tbase = vi.tile_base;
             tx = (uint32_t)vi.tiles_x;
@@ -113,7 +115,6 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     }
     uint32_t total;
     const uint32_t start = block_exclusive_sum<4>(cnt, s_scan, &total);
-    const uint32_t base = a.block_offsets[blockIdx.x];
     const uint32_t room = base < a.dup_capacity ? a.dup_capacity - base : 0u;
     const uint32_t emit = min(total, room);
     const bool large = cnt > kOwnPairs;

@@ -220,6 +220,7 @@ struct BinLaunch {
     uint32_t n;              // per-view stride of the [view][n] arrays
     int tile;
     const ViewInfo* views;   // device
+    uint32_t single_tiles_x; // one view: its tiles per row (tile base 0); 0: look the view up
     const uint2* rect_px;    // per Gaussian: covered pixel rectangle (from the cull)
     // optional: per depth-sorted position, the packed tile rectangle at this
     // bin's tile size, moved by the depth sort (replaces the rect_px gather)
