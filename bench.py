@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Kernel timed by each roofline candidate (the cull and composite brackets
 # hold exactly one launch of it; the sort bracket holds its 15 launches, of
 # which those past the passes the key range needs exit at once).
-STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<7,true>",
+STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<7,true,true>",
                 "composite": "composite_kernel<16,true>"}
 PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
 

@@ -30,6 +30,18 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTile = kThreads * kItems;
 
+// a[i]; with O32 (arrays under 4 GiB) the byte offset is formed in 32 bits,
+// so a load or store takes the array base in scalar registers and one offset
+// register rather than a 64-bit address per element.
+template <bool O32, typename T>
+__device__ __forceinline__ T& at(T* a, uint32_t i) {
+    if constexpr (O32)
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<void*>(static_cast<const void*>(a))) +
+                                     (uint32_t)(i * (uint32_t)sizeof(T)));
+    else
+        return a[i];
+}
+
 struct PassArgs {
     const uint32_t* keys_in;
     const uint32_t* vals_in;
@@ -85,7 +97,7 @@ __device__ __forceinline__ uint32_t load_count_range(const PassArgs& a, uint2* k
 // Digit counts of one tile.  Each wave counts into kCopies histograms, lane
 // l into copy l % kCopies, so a digit shared by many lanes (the top digit of
 // clustered keys) serializes kCopies times fewer LDS atomics.
-template <int BITS>
+template <int BITS, bool O32>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -108,7 +120,7 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        keys[i] = g < count ? keys_in[g] : 0u;
+        keys[i] = g < count ? at<O32>(keys_in, g) : 0u;
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
@@ -159,7 +171,7 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     if (tid == 0) a.col_total[d] = carry;
 }
 
-template <int BITS, bool P2>
+template <int BITS, bool P2, bool O32>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -194,11 +206,15 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     //    the key 0xFFFFFFFF: the last digit, ranked after every valid key.
     const uint32_t wbase = base + wave * (64u * kItems);
     const uint32_t last = count - 1u;
-    uint32_t keys[kItems], ranks[kItems];
+    // ranks (< kTile) two to a register and output digits four to a
+    // register: fewer registers hold more waves in flight
+    constexpr int kPairs = (kItems + 1) / 2, kQuads = (kItems + 3) / 4;
+    static_assert(kTile <= 65536 && BITS <= 8, "packed ranks and digits");
+    uint32_t keys[kItems], ranks[kPairs];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        const uint32_t k = keys_in[min(g, last)];
+        const uint32_t k = at<O32>(keys_in, min(g, last));
         keys[i] = g < count ? k - plan.sub : 0xFFFFFFFFu;
     }
 
@@ -222,9 +238,15 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
         const uint32_t old = s_whist[wave][d];
         if (below == 0u) s_whist[wave][d] = old + (uint32_t)(__popc(plo) + __popc(phi));
-        ranks[i] = old + below;
+        const uint32_t rank = old + below;
+        ranks[i >> 1] = (i & 1) ? ranks[i >> 1] | (rank << 16) : rank;
     }
     __syncthreads();
+    // the digits and their counter addresses are recomputed after the
+    // barrier rather than held in registers across it (an empty asm
+    // statement that the compiler must assume changes the keys)
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) asm volatile("" : "+v"(keys[i]));
 
     // 3. per-digit offsets: within the tile (over waves, then digits) and global
     uint32_t bin_total = 0u;
@@ -258,48 +280,51 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const uint32_t d = (keys[i] >> shift) & MASK;
-        ranks[i] += s_local[d] + s_whist[wave][d];
-        s_buf[ranks[i]] = keys[i];
+        const uint32_t rank = ((ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) + s_local[d] + s_whist[wave][d];
+        ranks[i >> 1] = (i & 1) ? (ranks[i >> 1] & 0xFFFFu) | (rank << 16) : (ranks[i >> 1] & 0xFFFF0000u) | rank;
+        s_buf[rank] = keys[i];
     }
     // values of positions past the count are staged but never written out
     uint32_t v[kItems];
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) v[i] = vals_in[min(wbase + (uint32_t)i * 64u + lane, last)];
+    for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals_in, min(wbase + (uint32_t)i * 64u + lane, last));
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
-    uint32_t outs[kItems];
+    // the output position of local position p is s_dest[digit] + p
+    uint32_t digits[kQuads] = {};
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
         if (p < valid) {
             uint32_t k = s_buf[p];
-            outs[j] = s_dest[(k >> shift) & MASK] + p;
-            keys_out[outs[j]] = k + plan.add;
+            const uint32_t d = (k >> shift) & MASK;
+            digits[j >> 2] |= d << ((j & 3) * 8);
+            at<O32>(keys_out, s_dest[d] + p) = k + plan.add;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
+    for (int i = 0; i < kItems; ++i) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
     if constexpr (P2) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) v[i] = vals2_in[min(wbase + (uint32_t)i * 64u + lane, last)];
+        for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals2_in, min(wbase + (uint32_t)i * 64u + lane, last));
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
-        if (p < valid) vals_out[outs[j]] = s_buf[p];
+        if (p < valid) at<O32>(vals_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
     }
     if constexpr (P2) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) s_buf[ranks[i]] = v[i];
+        for (int i = 0; i < kItems; ++i) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t p = tid + (uint32_t)j * kThreads;
-            if (p < valid) vals2_out[outs[j]] = s_buf[p];
+            if (p < valid) at<O32>(vals2_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
         }
     }
 }
@@ -320,6 +345,8 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     pa.table = a.table;
     pa.tiles = tiles;
     pa.col_total = a.scratch;
+    // every array holds at most capacity 4-byte elements
+    const bool o32 = (uint64_t)a.capacity * 4u <= 0xFFFFFFFFull;
     int src = 0, dst = 1;
     for (int p = 0; p < a.passes; ++p) {
         pa.keys_in = a.keys[src];
@@ -330,12 +357,19 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.vals2_out = a.vals2[dst];
         pa.pass = p;
         pa.shift = p * BITS;
-        hipLaunchKernelGGL(upsweep_kernel<BITS>, dim3(tiles), dim3(kThreads), 0, s, pa);
-        hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
-        if (a.vals2[0])
-            hipLaunchKernelGGL((downsweep_kernel<BITS, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        if (o32)
+            hipLaunchKernelGGL((upsweep_kernel<BITS, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
         else
-            hipLaunchKernelGGL((downsweep_kernel<BITS, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
+            hipLaunchKernelGGL((upsweep_kernel<BITS, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
+        if (a.vals2[0] && o32)
+            hipLaunchKernelGGL((downsweep_kernel<BITS, true, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        else if (a.vals2[0])
+            hipLaunchKernelGGL((downsweep_kernel<BITS, true, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        else if (o32)
+            hipLaunchKernelGGL((downsweep_kernel<BITS, false, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        else
+            hipLaunchKernelGGL((downsweep_kernel<BITS, false, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src = dst;
