@@ -194,12 +194,40 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
 // quadratic form and the cutoff are in log2 units (scaled by log2 e), and the
 // opacity enters as log2 o added to the form, so the blend step's alpha is
 // one v_exp_f32 and its fragment test a comparison with log2(1/255).
+// The covered pixels [i0, i1) x [j0, j1) ride in par.y and col.w as the
+// packed corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so one step
+// reads three 16-byte records at one address.
+// (A native 4-float vector, so each record is one 128-bit LDS access.)
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 struct StagedSplats {
-    int4* rect;     // covered pixels [i0, i1) x [j0, j1)
-    float4* geo;    // cx, cy, -k0/2 log2e, -k1 log2e
-    float4* par;    // -k2/2 log2e, opacity, log2((1/255) / o), log2 o
-    float4* col;    // r, g, b, -
+    f32x4_t* geo;   // cx, cy, -k0/2 log2e, -k1 log2e
+    f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), log2((1/255) / o), log2 o
+    f32x4_t* col;   // r, g, b, rect (high corner bits)
 };
+// Inclusive 16-bit corners of a pixel rectangle [i0, i1) x [j0, j1) (image
+// coordinates are below 2^16); an empty one becomes (0xFFFF, 0xFFFF) twice,
+// which no pixel of an image equals.
+__device__ __forceinline__ uint2 pack_rect16(int4 r) {
+    if (r.x >= r.y || r.z >= r.w) return make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    return make_uint2((uint32_t)r.x | ((uint32_t)r.z << 16), (uint32_t)(r.y - 1) | ((uint32_t)(r.w - 1) << 16));
+}
+
+__device__ __forceinline__ int4 unpack_rect16(uint2 r) {
+    return make_int4((int)(r.x & 0xFFFFu), (int)(r.y & 0xFFFFu) + 1, (int)(r.x >> 16), (int)(r.y >> 16) + 1);
+}
+
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
+// Whether the pixel p = x | y << 16 lies in the packed rectangle r: clamping
+// both coordinates into it at once (packed 16-bit max and min) leaves p
+// unchanged exactly when it does.
+__device__ __forceinline__ bool in_rect16(uint32_t p, uint2 r) {
+    const u16x2_t c = __builtin_elementwise_min(
+        __builtin_elementwise_max(__builtin_bit_cast(u16x2_t, p), __builtin_bit_cast(u16x2_t, r.x)),
+        __builtin_bit_cast(u16x2_t, r.y));
+    return __builtin_bit_cast(uint32_t, c) == p;
+}
+
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2AlphaMin = -7.99435343685886f;  // log2(1/255)
 
@@ -244,14 +272,14 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
     // covers the difference
     const float log2o = __log2f(maxf(r0.z, 1e-8f));
     const float cutoff = __log2f((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
-    s.rect[t] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width, tr.height,
-                                 a.tight_rects != 0);
-    s.geo[t] = make_float4(r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1);
-    s.par[t] = make_float4(-0.5f * kLog2e * k2, r0.z, cutoff, log2o);
-    s.col[t] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+    const uint2 rect = pack_rect16(splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width,
+                                                    tr.height, a.tight_rects != 0));
+    s.geo[t] = f32x4_t{r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1};
+    s.par[t] = f32x4_t{-0.5f * kLog2e * k2, __uint_as_float(rect.x), cutoff, log2o};
+    s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
 }
 
-// One front-to-back blend step of pixel (px, py), branch-free: coverage (the
+// One front-to-back blend step of pixel p = x | y << 16, branch-free: coverage (the
 // pixel rectangle) and the fragment test (power >= cutoff) select the alpha
 // or 0, and a zero alpha leaves colour and transmittance bit-for-bit
 // unchanged.  The fragment shader's power > 0 discard cannot fire here: the
@@ -265,12 +293,13 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
 // power = dx * (-k0/2 dx - k1 dy) + ((-k2/2 dy) dy + log2 o), so alpha is
 // min(2^power, 0.99) and the fragment test alpha >= 1/255 is
 // power >= log2(1/255).
-__device__ __forceinline__ void blend_step(int px, int py, float fx, float fy, int4 r, float4 geo, float4 par,
-                                           float4 col, float& cr, float& cg, float& cb, float& tr) {
+__device__ __forceinline__ void blend_step(uint32_t pxy, float fx, float fy, f32x4_t geo, f32x4_t par, f32x4_t col,
+                                           float& cr, float& cg, float& cb, float& tr) {
+    const uint2 r = make_uint2(__float_as_uint(par.y), __float_as_uint(col.w));
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, fmaf(par.x * dy, dy, par.w));
     float alpha = minf(__builtin_amdgcn_exp2f(power), 0.99f);
-    const bool take = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & (power >= kLog2AlphaMin);
+    const bool take = in_rect16(pxy, r) & (power >= kLog2AlphaMin);
     alpha = take ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
@@ -333,11 +362,11 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
         const int cnt = (int)min((uint32_t)BATCH, t.range.y - base);
         if (__ballot(live)) {
             for (int j = 0; j < cnt; ++j) {
-                const int4 r = st.rect[j];
-                const float4 geo = st.geo[j], par = st.par[j], col = st.col[j];
+                const f32x4_t geo = st.geo[j], par = st.par[j], col = st.col[j];
 #pragma unroll
                 for (int k = 0; k < PPT; ++k)
-                    blend_step(px[k], py[k], fx[k], fy[k], r, geo, par, col, cr[k], cg[k], cb[k], tr[k]);
+                    blend_step((uint32_t)px[k] | ((uint32_t)py[k] << 16), fx[k], fy[k], geo, par, col, cr[k], cg[k],
+                               cb[k], tr[k]);
             }
         }
     }
@@ -357,7 +386,7 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
 // blend step (about 10 eps |power| / (1 - |rho|)^2 each, with 1 - rho^2 =
 // r2 = 1 - B^2 / (4 A C)); splats with r2 < 0.01 or a clamped, indefinite
 // conic are always walked.
-__device__ __forceinline__ bool ellipse_meets_box(int4 r, float4 geo, float4 par, int qx0, int qy0) {
+__device__ __forceinline__ bool ellipse_meets_box(int4 r, f32x4_t geo, f32x4_t par, int qx0, int qy0) {
     const float A = geo.z, B = geo.w, C = par.x;
     if (!(A < 0.0f && C < 0.0f)) return true;
     const float r2 = 1.0f - (B * B) / (4.0f * A * C);
@@ -392,6 +421,7 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
     const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const uint32_t pxy = (uint32_t)px | ((uint32_t)py << 16);
     const bool valid = px < t.width && py < t.height;
     // pixels outside the image start saturated and are never written
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? 1.0f : 0.0f;
@@ -408,7 +438,7 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             uint32_t j = chunk + lane;
             bool hit = false;
             if (j < cnt) {
-                int4 r = st.rect[j];
+                const int4 r = unpack_rect16(make_uint2(__float_as_uint(st.par[j].y), __float_as_uint(st.col[j].w)));
                 hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
                 if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, st.geo[j], st.par[j], qx0, qy0);
             }
@@ -416,7 +446,7 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             while (mask) {
                 const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
-                blend_step(px, py, fx, fy, st.rect[k], st.geo[k], st.par[k], st.col[k], cr, cg, cb, tr);
+                blend_step(pxy, fx, fy, st.geo[k], st.par[k], st.col[k], cr, cg, cb, tr);
             }
             if (!__ballot(tr >= kTransmittanceMin)) break;
         }
@@ -581,12 +611,11 @@ struct CompositeKernelShape {
 template <int T, bool QUAD>
 __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads)) composite_kernel(CompositeLaunch a) {
     constexpr int BATCH = CompositeKernelShape<T, QUAD>::kThreads;
-    __shared__ int4 s_rect[BATCH];
-    __shared__ float4 s_geo[BATCH];
-    __shared__ float4 s_par[BATCH];
-    __shared__ float4 s_col[BATCH];
+    __shared__ f32x4_t s_geo[BATCH];
+    __shared__ f32x4_t s_par[BATCH];
+    __shared__ f32x4_t s_col[BATCH];
     __shared__ uint32_t s_item;
-    const StagedSplats st{s_rect, s_geo, s_par, s_col};
+    const StagedSplats st{s_geo, s_par, s_col};
     if (a.persistent_blocks == 0u) {
         const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
         if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
