@@ -199,6 +199,7 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
 // reads three 16-byte records at one address.
 // (A native 4-float vector, so each record is one 128-bit LDS access.)
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 struct StagedSplats {
     f32x4_t* geo;   // cx, cy, -k0/2 log2e, -k1 log2e
     f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), log2((1/255) / o), log2 o
@@ -485,10 +486,12 @@ __device__ __forceinline__ uint32_t depth24(float ndc_z, float clip_z_01) {
 __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
     constexpr int BATCH = 256;
     constexpr uint32_t kDepthSlack = 4u;  // D24 steps: float ndc_z is monotone in depth up to rounding
-    __shared__ int4 s_rect[BATCH];
-    __shared__ float4 s_geo[BATCH];   // cx, cy, k0, 2 k1
-    __shared__ float4 s_par[BATCH];   // k2, opacity, cutoff, D24 depth (bits)
-    __shared__ float4 s_col[BATCH];   // r, g, b (half precision), id (bits)
+    // covered pixels as packed inclusive 16-bit corners (x, y; z, w unused:
+    // the same 16-byte stride as the other records, so one address serves all)
+    __shared__ u32x4_t s_rect[BATCH];
+    __shared__ f32x4_t s_geo[BATCH];  // cx, cy, k0, 2 k1
+    __shared__ f32x4_t s_par[BATCH];  // k2, opacity, cutoff, D24 depth (bits)
+    __shared__ f32x4_t s_col[BATCH];  // r, g, b (half precision), id (bits)
     const CompositeLaunch& a = sl.tiles;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
@@ -497,6 +500,7 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
     const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const uint32_t pxy = (uint32_t)px | ((uint32_t)py << 16);
     const bool valid = px < t.width && py < t.height;
     uint32_t best_q = 16777215u, best_id = 0xFFFFFFFFu;
     float cr = sl.background, cg = sl.background, cb = sl.background;
@@ -513,13 +517,14 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
             float det = r1.x * r1.z - r1.y * r1.y;
             if (det < 1e-6f) det = 1e-6f;
             const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
-            s_rect[tid] = splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width, t.height, false,
-                                           sl.jitter_x, sl.jitter_y);
-            s_geo[tid] = make_float4(r0.x + sl.jitter_x, r0.y + sl.jitter_y, k0, 2.0f * k1);
-            s_par[tid] = make_float4(k2, r0.z, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
-                                     __uint_as_float(depth24(r0.w, sl.clip_z_01)));
-            s_col[tid] = make_float4(__half2float(__float2half_rn(r2.x)), __half2float(__float2half_rn(r2.y)),
-                                     __half2float(__float2half_rn(r2.z)), __uint_as_float(g));
+            const uint2 rect = pack_rect16(splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width,
+                                                            t.height, false, sl.jitter_x, sl.jitter_y));
+            s_rect[tid] = u32x4_t{rect.x, rect.y, 0u, 0u};
+            s_geo[tid] = f32x4_t{r0.x + sl.jitter_x, r0.y + sl.jitter_y, k0, 2.0f * k1};
+            s_par[tid] = f32x4_t{k2, r0.z, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
+                                 __uint_as_float(depth24(r0.w, sl.clip_z_01))};
+            s_col[tid] = f32x4_t{__half2float(__float2half_rn(r2.x)), __half2float(__float2half_rn(r2.y)),
+                                 __half2float(__float2half_rn(r2.z)), __uint_as_float(g)};
         }
         __syncthreads();
         const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
@@ -528,20 +533,19 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
             const uint32_t j = chunk + lane;
             bool hit = false;
             if (j < cnt) {
-                const int4 r = s_rect[j];
+                const int4 r = unpack_rect16(make_uint2(s_rect[j].x, s_rect[j].y));
                 hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
             }
             uint64_t mask = __ballot(hit);
             while (mask) {
                 const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
-                const int4 r = s_rect[k];
-                const float4 geo = s_geo[k], par = s_par[k], col = s_col[k];
+                const uint2 r = make_uint2(s_rect[k].x, s_rect[k].y);
+                const f32x4_t geo = s_geo[k], par = s_par[k], col = s_col[k];
                 const uint32_t q = __float_as_uint(par.w), id = __float_as_uint(col.w);
                 const float dx = fx - geo.x, dy = fy - geo.y;
                 const float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + par.x * dy * dy);
-                const bool covered = (px >= r.x) & (px < r.y) & (py >= r.z) & (py < r.w) & !(power > 0.0f) &
-                                     !(power < par.z);
+                const bool covered = in_rect16(pxy, r) & !(power > 0.0f) & !(power < par.z);
                 const bool nearer = q < best_q || (q == best_q && id < best_id);
                 const bool cand = covered & nearer;
                 // the wave tests its candidate pixels together, without
