@@ -5,7 +5,7 @@
 // No counterpart in the reference: gamesplat shows the accumulated GPS
 // samples directly.  The network maps the running mean (RGBA) and the GPS
 // depth of 1-4 accumulated frames to the sorted-alpha image; sorted alpha
-// stays the fallback and the judge of its output.
+// stays the fallback and the image its output is scored against.
 //
 // Network (4 levels, 32/48/64/96 channels):
 //   e0a 8->32, e0b 32->32 -> skip0 | pool | e1a 32->48, e1b 48->48 -> skip1
