@@ -204,6 +204,7 @@ struct StagedSplats {
     f32x4_t* geo;   // cx, cy, -k0/2 log2e, -k1 log2e
     f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), log2((1/255) / o), log2 o
     f32x4_t* col;   // r, g, b, rect (high corner bits)
+    uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q
 };
 // Inclusive 16-bit corners of a pixel rectangle [i0, i1) x [j0, j1) (image
 // coordinates are below 2^16); an empty one becomes (0xFFFF, 0xFFFF) twice,
@@ -430,19 +431,29 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
     for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
         if (!__syncthreads_or(tr >= kTransmittanceMin)) break;
         uint32_t e = base + tid;
-        if (e < t.range.y) stage_splat(a, t, a.tile_vals[e], tid, st);
+        if (e < t.range.y) {
+            stage_splat(a, t, a.tile_vals[e], tid, st);
+            // the quadrants the splat can touch, tested once here for all
+            // four waves (the tests share the splat's terms)
+            const f32x4_t geo = st.geo[tid], par = st.par[tid];
+            const int4 r = unpack_rect16(make_uint2(__float_as_uint(par.y), __float_as_uint(st.col[tid].w)));
+            uint32_t quads = 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int x0 = t.tx * 16 + (q & 1) * 8, y0 = t.ty * 16 + (q >> 1) * 8;
+                bool hit = r.x < x0 + 8 && r.y > x0 && r.z < y0 + 8 && r.w > y0;
+                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, geo, par, x0, y0);
+                quads |= hit ? 1u << q : 0u;
+            }
+            st.quads[tid] = quads;
+        }
         __syncthreads();
         const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
         if (!__ballot(tr >= kTransmittanceMin)) continue;
         for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
             // which of these 64 splats touch this wave's quadrant
-            uint32_t j = chunk + lane;
-            bool hit = false;
-            if (j < cnt) {
-                const int4 r = unpack_rect16(make_uint2(__float_as_uint(st.par[j].y), __float_as_uint(st.col[j].w)));
-                hit = r.x < qx0 + 8 && r.y > qx0 && r.z < qy0 + 8 && r.w > qy0;
-                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, st.geo[j], st.par[j], qx0, qy0);
-            }
+            const uint32_t j = chunk + lane;
+            const bool hit = j < cnt && ((st.quads[j] >> wave) & 1u);
             uint64_t mask = __ballot(hit);
             while (mask) {
                 const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
@@ -618,8 +629,9 @@ __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads)) com
     __shared__ f32x4_t s_geo[BATCH];
     __shared__ f32x4_t s_par[BATCH];
     __shared__ f32x4_t s_col[BATCH];
+    __shared__ uint32_t s_quads[QUAD ? BATCH : 1];
     __shared__ uint32_t s_item;
-    const StagedSplats st{s_geo, s_par, s_col};
+    const StagedSplats st{s_geo, s_par, s_col, s_quads};
     if (a.persistent_blocks == 0u) {
         const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
         if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
