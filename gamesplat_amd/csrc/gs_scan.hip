@@ -17,26 +17,9 @@ constexpr int kScanThreads = 1024;
 constexpr int kRun = 32;             // consecutive counts per thread per round
 constexpr uint32_t kLdsCounts = 32768;  // counts staged in LDS (128 KiB)
 
-// Smallest min and largest max of nb (min, max) pairs into out[0], out[1].
-__device__ __forceinline__ void reduce_range(const uint2* __restrict__ block_range, uint32_t nb, uint32_t* out,
-                                             uint32_t* s_lo, uint32_t* s_hi) {
+// The block's smallest lo and largest hi into out[0], out[1].
+__device__ __forceinline__ void finish_range(uint32_t lo, uint32_t hi, uint32_t* out, uint32_t* s_lo, uint32_t* s_hi) {
     const uint32_t tid = threadIdx.x;
-    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    // eight loads in flight per thread
-    constexpr int kBatch = 8;
-    for (uint32_t i0 = tid; i0 < nb; i0 += kScanThreads * kBatch) {
-        uint2 r[kBatch];
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
-            r[j] = i < nb ? block_range[i] : make_uint2(0xFFFFFFFFu, 0u);
-        }
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            lo = min(lo, r[j].x);
-            hi = max(hi, r[j].y);
-        }
-    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
@@ -55,6 +38,28 @@ __device__ __forceinline__ void reduce_range(const uint2* __restrict__ block_ran
         out[0] = lo;
         out[1] = hi;
     }
+}
+
+// Smallest min and largest max of nb (min, max) pairs into out[0], out[1].
+__device__ __forceinline__ void reduce_range(const uint2* __restrict__ block_range, uint32_t nb, uint32_t* out,
+                                             uint32_t* s_lo, uint32_t* s_hi) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    // eight loads in flight per thread
+    constexpr int kBatch = 8;
+    for (uint32_t i0 = threadIdx.x; i0 < nb; i0 += kScanThreads * kBatch) {
+        uint2 r[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
+            r[j] = i < nb ? block_range[i] : make_uint2(0xFFFFFFFFu, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            lo = min(lo, r[j].x);
+            hi = max(hi, r[j].y);
+        }
+    }
+    finish_range(lo, hi, out, s_lo, s_hi);
 }
 
 __device__ __forceinline__ void reduce_diag(const uint32_t* __restrict__ block_diag, uint32_t nb, uint32_t* diag_out,
@@ -89,20 +94,35 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
     __shared__ uint32_t s_diag[8];
     __shared__ uint32_t s_lo[kScanThreads / 64], s_hi[kScanThreads / 64];
     const uint32_t tid = threadIdx.x;
-    // eight loads in flight per thread before their LDS stores
+    // eight loads in flight per thread before their LDS stores; the blocks'
+    // key ranges (when given) are loaded with their counts (a clamped
+    // duplicate of the last one leaves the min and max unchanged)
     constexpr int kBatch = 8;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
     for (uint32_t i0 = tid; i0 < nb; i0 += kScanThreads * kBatch) {
         uint32_t v[kBatch];
+        uint2 rg[kBatch];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const uint32_t i = i0 + (uint32_t)j * kScanThreads;
             v[j] = counts[min(i, nb - 1u)];
+        }
+        if (block_range) {
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) rg[j] = block_range[min(i0 + (uint32_t)j * kScanThreads, nb - 1u)];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const uint32_t i = i0 + (uint32_t)j * kScanThreads;
             if (i < nb) s_c[i] = v[j];
+        }
+        if (block_range) {
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                lo = min(lo, rg[j].x);
+                hi = max(hi, rg[j].y);
+            }
         }
     }
     __syncthreads();
@@ -124,7 +144,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_lds_kernel(const uin
         if (clamped) *clamped = min(all, clamp_cap);
     }
     if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
-    if (block_range) reduce_range(block_range, nb, range_out, s_lo, s_hi);
+    if (block_range) finish_range(lo, hi, range_out, s_lo, s_hi);
 }
 
 __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_t* __restrict__ counts,
