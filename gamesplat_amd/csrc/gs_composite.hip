@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
 // splats' offsets.  Pairs past the capacity are not written.
 constexpr uint32_t kOwnPairs = 16u;
 
+template <bool O32>
 __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_lstart[256];   // large splats: first pair (block-relative) ...
@@ -122,8 +123,8 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         const uint32_t x0 = r.x & 0xFFFFu, x1 = r.x >> 16;
         uint32_t xx = x0, row = tbase + (r.y & 0xFFFFu) * tx;
         for (uint32_t i = start; i < start + cnt && i < emit; ++i) {
-            a.dup_keys[base + i] = row + xx;
-            a.dup_vals[base + i] = val;
+            at<O32>(a.dup_keys, base + i) = row + xx;
+            at<O32>(a.dup_vals, base + i) = val;
             if (++xx == x1) {
                 xx = x0;
                 row += tx;
@@ -153,8 +154,8 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         if (i >= emit) continue;
         const uint2 rr = s_rect[o];
         const uint32_t x0 = rr.x & 0xFFFFu, w = (rr.x >> 16) - x0, y0 = rr.y & 0xFFFFu;
-        a.dup_keys[base + i] = s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w;
-        a.dup_vals[base + i] = s_id[o];
+        at<O32>(a.dup_keys, base + i) = s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w;
+        at<O32>(a.dup_vals, base + i) = s_id[o];
     }
 }
 
@@ -660,7 +661,11 @@ hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
     hipError_t e = launch_scan_counts(a.block_counts, blocks, a.block_offsets, &a.counters->bin.dup_total, nullptr,
                                       nullptr, s, a.dup_capacity, &a.counters->bin.dup_count);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bin_emit_kernel, dim3(blocks), dim3(256), 0, s, a);
+    // pair positions stay below the capacity
+    if ((uint64_t)a.dup_capacity * 4u <= 0xFFFFFFFFull)
+        hipLaunchKernelGGL(bin_emit_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(bin_emit_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -30,18 +30,6 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTile = kThreads * kItems;
 
-// a[i]; with O32 (arrays under 4 GiB) the byte offset is formed in 32 bits,
-// so a load or store takes the array base in scalar registers and one offset
-// register rather than a 64-bit address per element.
-template <bool O32, typename T>
-__device__ __forceinline__ T& at(T* a, uint32_t i) {
-    if constexpr (O32)
-        return *reinterpret_cast<T*>(reinterpret_cast<char*>(const_cast<void*>(static_cast<const void*>(a))) +
-                                     (uint32_t)(i * (uint32_t)sizeof(T)));
-    else
-        return a[i];
-}
-
 struct PassArgs {
     const uint32_t* keys_in;
     const uint32_t* vals_in;
