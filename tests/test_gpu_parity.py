@@ -147,6 +147,38 @@ def test_composite_vs_fp64(ctx, orc, c1_scene, tile, quad):
     assert st["tile_entries"] > 0
 
 
+@pytest.mark.parametrize("path,tile", [("calls", 16), ("frame", 16), ("frame", 32)])
+def test_composite_wide_view(ctx, orc, gs, path, tile):
+    """A 4352-pixel-wide view is 272 tiles of 16: more than a packed 8-bit
+    tile coordinate holds, so the bin gathers pixel rectangles; at tile 32
+    (136 tiles) the depth sort carries packed tile rectangles.  Splats on a
+    shell around the camera cover the whole width."""
+    W, H = 4352, 256
+    print(f"case: composite wide view {W}x{H} path={path} tile={tile}", flush=True)
+    g = gs.generate_scene(13, 60000, 4, 0.0)
+    pos = scenes._direction(13) * 15.0
+    rng = np.random.Generator(np.random.PCG64(3))
+    d = rng.standard_normal((len(g), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    g[:, 0:3] = (pos[None, :] + d * rng.uniform(2.0, 10.0, (len(g), 1))).astype(np.float32)
+    cam = scenes.random_view_camera(13, W, H)
+    if path == "calls":
+        ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+        ctx.sort()
+        ctx.composite(tile)
+    else:
+        ctx.upload(g)
+        ctx.render_alpha(cam, tile=tile)
+        ok, oi, op, ov, od = orc.cull(g, cam, sh_degree=3, sh_lod_distance=0.0, collect_stats=False, effect=None)
+    img = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+    sk, si = orc.radix_sort(ok, oi)
+    ref = orc.composite(op, si, W, H, mode=0)
+    assert (ref[:, 16 * 255:, 3] > 0).mean() > 0.5  # the columns past tile 255 are covered
+    err = np.abs(img - ref)
+    print(f"max abs err vs fp64 {err.max():.3e}, mean {err.mean():.3e}")
+    assert err.max() <= TOL_RGBA16F
+
+
 def _needle_scene(gs, seed=21, n=20000):
     """Elongated, rotated splats with opacities down to just above 1/255:
     the cases where the opacity-aware rectangle cut has the least slack."""
