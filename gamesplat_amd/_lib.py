@@ -46,6 +46,8 @@ GS_OPT_TIGHT_RECTS = 3
 GS_OPT_RADIX_BITS = 4
 GS_OPT_BLOCKS_PER_CU = 5
 GS_OPT_GRAPH = 6
+GS_OPT_REFERENCE_BLEND = 7
+GS_OPT_WIDE_INDEX = 8
 MAX_VIEWS = 64
 
 STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve", "cull_kernel")

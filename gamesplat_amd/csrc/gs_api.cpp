@@ -152,6 +152,8 @@ struct gs_ctx {
     int keep_cells = 0;
     int default_tile = 16;
     int quadrant_cull = 1;
+    int reference_blend = 0;  // the composite's blend arithmetic (GS_OPT_REFERENCE_BLEND)
+    int wide_index = 0;       // GS_OPT_WIDE_INDEX
     int tight_rects = 1;
     int cull_tight = 1;      // tight_rects as the last cull used it
 
@@ -411,6 +413,13 @@ int gs_create(gs_ctx** out, int device) {
         delete c;
         return fail(GS_ERR_HIP, "device setup failed: %s", hipGetErrorString(e));
     }
+    // kernel attributes are set here, never while a frame is being captured
+    e = init_kernel_attributes();
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(GS_ERR_HIP, "kernel attribute setup failed: %s", hipGetErrorString(e));
+    }
     int r = ensure(c, c->counters, sizeof(DevCounters), "counters");
     if (r == GS_OK) r = ensure(c, c->frame_dev, sizeof(FrameConst), "frame constants");
     if (r == GS_OK) {
@@ -502,6 +511,12 @@ int gs_set_option(gs_ctx* c, int option, int value) {
             return ok();
         case GS_OPT_GRAPH:
             c->use_graph = value ? 1 : 0;
+            return ok();
+        case GS_OPT_REFERENCE_BLEND:
+            c->reference_blend = value ? 1 : 0;
+            return ok();
+        case GS_OPT_WIDE_INDEX:
+            c->wide_index = value ? 1 : 0;
             return ok();
         default:
             return fail(GS_ERR_INVALID, "unknown option %d", option);
@@ -780,6 +795,7 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     // runs of equal digits), at most five, as many as the key range needs
     s.bits = c->radix_bits == 8 ? 7 : c->radix_bits;
     s.passes = (32 + s.bits - 1) / s.bits;
+    s.wide_index = c->wide_index;
     return s;
 }
 
@@ -816,6 +832,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.dup_keys = ptr<uint32_t>(c->dup_keys[0]);
     b.dup_vals = ptr<uint32_t>(c->dup_vals[0]);
     b.dup_capacity = c->dup_cap;
+    b.wide_index = c->wide_index;
 
     SortLaunch& s = p->tiles;
     s = SortLaunch{};
@@ -829,6 +846,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     s.table = ptr<uint32_t>(c->table);
     s.passes = passes;
     s.bits = tile_bits;
+    s.wide_index = c->wide_index;
 
     // the sorted pairs land in buffer 2 for an even pass count, else 1
     const int slot = passes % 2 == 0 ? 2 : 1;
@@ -844,6 +862,7 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     cl.image = ptr<uint16_t>(c->image);
     cl.quadrant_cull = c->quadrant_cull;
     cl.tight_rects = c->tight_rects;
+    cl.reference_blend = tile == 16 ? c->reference_blend : 0;
     cl.persistent_blocks = c->blocks_per_cu > 0 ? (uint32_t)(c->blocks_per_cu * c->props.multiProcessorCount) : 0u;
     cl.queue = c->dc()->composite_queue;
 }
@@ -903,7 +922,12 @@ int stage_composite(gs_ctx* c, const FramePlan& p) {
 int prepare_composite(gs_ctx* c, int tile) {
     GS_CHECK(ensure(c, c->image, (size_t)c->total_pixels * 8, "RGBA16F images"));
     GS_CHECK(ensure(c, c->ranges, (size_t)c->total_tiles * 8, "tile ranges"));
-    if (!c->dup_cap) GS_CHECK(grow_tile_buffers(c, std::max(3u * c->views * c->n, 65536u)));
+    if (!c->dup_cap) {
+        // three pairs per splat of every view to start with, formed in 64
+        // bits and capped at the pair limit; an overflow grows them
+        const size_t want = std::min<size_t>(std::max<size_t>((size_t)3 * c->views * c->n, 65536), 0x7FFFFFFF);
+        GS_CHECK(grow_tile_buffers(c, (uint32_t)want));
+    }
     (void)tile;
     return GS_OK;
 }
@@ -951,6 +975,7 @@ int enqueue_stochastic(gs_ctx* c, bool* binned) {
             b.pad = mode == 1 ? 1 : 0;
             b.jitter_x = mode == 2 ? jx : 0.0f;
             b.jitter_y = mode == 2 ? jy : 0.0f;
+            b.clip_z_01 = c->cam.clip_z_01;
         }
         GS_CHECK(stage_bin(c, c->plan));
         c->stoch_bins_epoch = c->bin_epoch;
@@ -1022,7 +1047,12 @@ int launch_graph(gs_ctx* c, const FramePlan& p) {
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&p);
     if (!c->graph_exec || c->graph_key.size() != sizeof(FramePlan) ||
         memcmp(c->graph_key.data(), b, sizeof(FramePlan)) != 0) {
-        if (c->graph_exec) GS_HIP(hipGraphExecDestroy(c->graph_exec));
+        // the last replay of the old graph may still run on the stream: the
+        // executable graph is destroyed only once it has finished
+        if (c->graph_exec) {
+            GS_HIP(hipStreamSynchronize(c->stream));
+            GS_HIP(hipGraphExecDestroy(c->graph_exec));
+        }
         if (c->graph) GS_HIP(hipGraphDestroy(c->graph));
         c->graph_exec = nullptr;
         c->graph = nullptr;
@@ -1103,6 +1133,7 @@ int gs_composite_alpha(gs_ctx* c, int tile) {
     if (!c->have_sort) return fail(GS_ERR_STATE, "gs_composite_alpha needs gs_cull and gs_sort first");
     if (tile == 0) tile = c->default_tile;
     if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+    if (c->reference_blend) tile = 16;  // the reference-arithmetic compositor's tile
     GS_CHECK(set_device(c));
     GS_CHECK(settle_composite(c, false, false));
     c->have_composite = false;
@@ -1148,6 +1179,7 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     GS_CHECK(check_cull_options(opts, count));
     if (tile == 0) tile = c->default_tile;
     if (tile != 8 && tile != 16 && tile != 32) return fail(GS_ERR_INVALID, "tile size must be 8, 16 or 32");
+    if (c->reference_blend) tile = 16;  // the reference-arithmetic compositor's tile
     GS_CHECK(set_device(c));
     GS_CHECK(settle_composite(c, false, false));
     const uint32_t blocks = (c->n + 255u) / 256u;

@@ -33,6 +33,13 @@ namespace gs {
 namespace {
 
 constexpr float kTransmittanceMin = 1e-4f;
+// The compositors carry each pixel's transmittance T as u = T / 255 and a
+// splat's alpha as alpha' = 255 alpha = 2^(power - log2(1/255)), so the
+// fragment test alpha >= 1/255 is power' >= 0 and the blend weight
+// alpha T = alpha' u needs no extra scaling (u' = u - alpha' u / 255).
+constexpr float kInv255 = 1.0f / 255.0f;
+constexpr float kAlphaCap255 = 0.99f * 255.0f;  // alpha <= 0.99
+constexpr float kUMin = kTransmittanceMin / 255.0f;
 
 // Pass 1 over the depth-sorted splats, nearest first: tile rectangle and
 // pair count per splat, pair count per block.
@@ -54,8 +61,10 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
             const float4 r1 = a.records[(size_t)val * kRecordVec4 + 1];
             const float4 r2 = a.records[(size_t)val * kRecordVec4 + 2];
             const ViewInfo& vi = a.views[val / a.n];
-            px = pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, vi.width, vi.height, a.tight != 0,
-                            a.jitter_x, a.jitter_y, a.pad);
+            px = depth_clipped(r0.w, a.clip_z_01)
+                     ? make_uint2(0u, 0u)
+                     : pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, vi.width, vi.height, a.tight != 0,
+                                  a.jitter_x, a.jitter_y, a.pad);
         } else {
             px = a.rect_px[val];
         }
@@ -191,20 +200,23 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
     }
 }
 
-// One splat as the compositors keep it in LDS (4 x 16 bytes).  The
-// quadratic form and the cutoff are in log2 units (scaled by log2 e), and the
-// opacity enters as log2 o added to the form, so the blend step's alpha is
-// one v_exp_f32 and its fragment test a comparison with log2(1/255).
-// The covered pixels [i0, i1) x [j0, j1) ride in par.y and col.w as the
-// packed corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so one step
-// reads three 16-byte records at one address.
+// One splat as the compositors keep it in LDS (4 x 16 bytes, plus the
+// quadrant mask).  The quadratic form is in log2 units (scaled by log2 e),
+// and the opacity enters as log2 o added to the form, so the blend step's
+// alpha is one v_exp_f32 and its fragment test a comparison with
+// log2(1/255).  The covered pixels [i0, i1) x [j0, j1) ride in par.y and
+// col.w as the packed corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so
+// one step reads three 16-byte records at one address.  The fourth record
+// holds the reference's own conic and cutoff, read only for the fragments
+// whose fused log2 test lies within par.z of the cutoff.
 // (A native 4-float vector, so each record is one 128-bit LDS access.)
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 struct StagedSplats {
     f32x4_t* geo;   // cx, cy, -k0/2 log2e, -k1 log2e
-    f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), log2((1/255) / o), log2 o
+    f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), decision margin (log2 units), log2 o - log2(1/255)
     f32x4_t* col;   // r, g, b, rect (high corner bits)
+    f32x4_t* exact; // c / det, 2 (-b / det), a / det, log((1/255) / max(o, 1e-8)): the reference's operations
     uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q
 };
 // Inclusive 16-bit corners of a pixel rectangle [i0, i1) x [j0, j1) (image
@@ -260,9 +272,23 @@ __device__ __forceinline__ TileRef tile_ref(const CompositeLaunch& a, uint32_t i
 }
 
 // Stage splat g into slot t: conic from one reciprocal of the clamped
-// determinant, fragment cutoff log((1/255) / max(o, 1e-8)), pixel rectangle.
-__device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const TileRef& tr, uint32_t g, uint32_t t,
-                                            StagedSplats s) {
+// determinant in log2 units, pixel rectangle, the reference's conic
+// ((c, -b, a) / det, shaders/cull.glsl:273-275) and fragment cutoff
+// log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70), and the margin
+// within which the fused log2 test may decide differently from the
+// reference's float test.  Returns the cutoff in log2 units (for the
+// quadrant test).
+//
+// Margin: with K the conic and kappa = lambda_max / lambda_min its condition
+// number, |k0 dx^2| + |2 k1 dx dy| + |k2 dy^2| <= 2 kappa q at a pixel where
+// the form is q; at the cutoff, q log2(e) / 2 = |log2((1/255) / o)|.  The
+// fused log2 power and the reference's float power each carry about 8
+// roundings of those terms (staging, products, sums): together about
+// 2e-6 kappa |cutoff|, plus a few 1e-6 from the log2 of o and the rounded
+// cutoff.  The margin is four times that; a splat whose conic is not
+// positive definite in float always takes the reference's test.
+__device__ __forceinline__ float stage_splat(const CompositeLaunch& a, const TileRef& tr, uint32_t g, uint32_t t,
+                                             StagedSplats s) {
     const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
     const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
     const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
@@ -270,45 +296,59 @@ __device__ __forceinline__ void stage_splat(const CompositeLaunch& a, const Tile
     if (det < 1e-6f) det = 1e-6f;
     const float inv = 1.0f / det;
     const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
-    // hardware log2: the composite's fragment test is held to a tolerance
-    // (its quadratic form is FMA-evaluated), and the pixel rectangle's margin
-    // covers the difference
     const float log2o = __log2f(maxf(r0.z, 1e-8f));
     const float cutoff = __log2f((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
     const uint2 rect = pack_rect16(splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width,
                                                     tr.height, a.tight_rects != 0));
+    const float ht = 0.5f * (k0 + k2), hd = 0.5f * (k0 - k2);
+    const float sp = __builtin_sqrtf(hd * hd + k1 * k1);
+    const float kappa = (ht + sp) / (ht - sp);
+    const float margin = ht - sp > 0.0f ? 8e-6f * kappa * __builtin_fabsf(cutoff) + 3e-5f : __builtin_inff();
     s.geo[t] = f32x4_t{r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1};
-    s.par[t] = f32x4_t{-0.5f * kLog2e * k2, __uint_as_float(rect.x), cutoff, log2o};
+    s.par[t] = f32x4_t{-0.5f * kLog2e * k2, __uint_as_float(rect.x), margin, log2o - kLog2AlphaMin};
     s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
+    s.exact[t] = f32x4_t{r1.z / det, 2.0f * (-r1.y / det), r1.x / det, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f))};
+    return cutoff;
 }
 
-// One front-to-back blend step of pixel p = x | y << 16, branch-free: coverage (the
-// pixel rectangle) and the fragment test (power >= cutoff) select the alpha
-// or 0, and a zero alpha leaves colour and transmittance bit-for-bit
-// unchanged.  The fragment shader's power > 0 discard cannot fire here: the
-// cull's +0.3 dilation keeps det >= 0.09 for every splat (the 1e-6 clamp is
-// never reached), so the conic is positive definite and power <= 0 in exact
-// arithmetic; where the fused evaluation rounds to a value just above 0 the
-// FP64 composite blends the splat too.  A saturated pixel (T < 1e-4) keeps blending until its whole
-// wave is saturated; what it adds is below 1e-4 of a colour, and it only
-// brings the result closer to blending every splat.  The quadratic form and
-// the opacity are evaluated with fused multiply-adds in log2 units:
-// power = dx * (-k0/2 dx - k1 dy) + ((-k2/2 dy) dy + log2 o), so alpha is
-// min(2^power, 0.99) and the fragment test alpha >= 1/255 is
-// power >= log2(1/255).
+// One front-to-back blend step of pixel p = x | y << 16: coverage (the
+// pixel rectangle) and the fragment test select the alpha or 0, and a zero
+// alpha leaves colour and transmittance bit-for-bit unchanged.  A saturated
+// pixel (T < 1e-4) keeps blending until its whole wave is saturated; what it
+// adds is below 1e-4 of a colour, and it only brings the result closer to
+// blending every splat.  The quadratic form and the opacity are evaluated
+// with fused multiply-adds in log2 units relative to the cutoff:
+// power' = dx * (-k0/2 dx - k1 dy) + ((-k2/2 dy) dy + log2 o - log2(1/255)),
+// so 255 alpha = min(2^power', 255 * 0.99) and the fragment test
+// alpha >= 1/255 is power' >= 0; tr holds T / 255.  Where power' lies
+// within the splat's margin of 0 (a few fragments per thousand steps, on the rim of the alpha
+// ellipse), the test is redone with the reference's float operations in
+// source order (power = -0.5 (k0 dx dx + 2 k1 dx dy + k2 dy dy), discard if
+// power > 0 or power < log((1/255) / o); shaders/splat.glsl:84-96), so the
+// set of blended fragments is the reference's.  Elsewhere the power > 0
+// discard cannot fire: the cull's +0.3 dilation keeps det >= 0.09, so the
+// conic is positive definite, and its float evaluation stays within a
+// relative 1e-2 of the exact form for every conditioning the dilation
+// allows.
 __device__ __forceinline__ void blend_step(uint32_t pxy, float fx, float fy, f32x4_t geo, f32x4_t par, f32x4_t col,
-                                           float& cr, float& cg, float& cb, float& tr) {
+                                           const f32x4_t* exact, float& cr, float& cg, float& cb, float& tr) {
     const uint2 r = make_uint2(__float_as_uint(par.y), __float_as_uint(col.w));
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, fmaf(par.x * dy, dy, par.w));
-    float alpha = minf(__builtin_amdgcn_exp2f(power), 0.99f);
-    const bool take = in_rect16(pxy, r) & (power >= kLog2AlphaMin);
+    float alpha = minf(__builtin_amdgcn_exp2f(power), kAlphaCap255);
+    bool pass = power >= 0.0f;
+    if (__builtin_fabsf(power) < par.z) {
+        const f32x4_t e = *exact;
+        const float pr = -0.5f * (e.x * dx * dx + e.y * dx * dy + e.z * dy * dy);
+        pass = !(pr > 0.0f) & !(pr < e.w);
+    }
+    const bool take = in_rect16(pxy, r) & pass;
     alpha = take ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
     cg = fmaf(col.y, w, cg);
     cb = fmaf(col.z, w, cb);
-    tr = fmaf(-alpha, tr, tr);
+    tr = fmaf(w, -kInv255, tr);
 }
 
 __device__ __forceinline__ void store_pixel(const TileRef& t, int px, int py, float cr, float cg, float cb,
@@ -318,7 +358,7 @@ __device__ __forceinline__ void store_pixel(const TileRef& t, int px, int py, fl
     h.x = __half_as_ushort(__float2half_rn(cr));
     h.y = __half_as_ushort(__float2half_rn(cg));
     h.z = __half_as_ushort(__float2half_rn(cb));
-    h.w = __half_as_ushort(__float2half_rn(1.0f - tr));
+    h.w = __half_as_ushort(__float2half_rn(1.0f - tr * 255.0f));
     *reinterpret_cast<ushort4*>(t.image + o) = h;
 }
 
@@ -351,13 +391,13 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
         fy[k] = (float)py[k] + 0.5f;
         cr[k] = cg[k] = cb[k] = 0.0f;
         // pixels outside the image start saturated and are never written
-        tr[k] = px[k] < t.width && py[k] < t.height ? 1.0f : 0.0f;
+        tr[k] = px[k] < t.width && py[k] < t.height ? kInv255 : 0.0f;
     }
 
     for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
         bool live = false;
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) live |= tr[k] >= kTransmittanceMin;
+        for (int k = 0; k < PPT; ++k) live |= tr[k] >= kUMin;
         if (!__syncthreads_or(live)) break;
         uint32_t e = base + tid;
         if (e < t.range.y) stage_splat(a, t, a.tile_vals[e], tid, st);
@@ -368,8 +408,8 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
                 const f32x4_t geo = st.geo[j], par = st.par[j], col = st.col[j];
 #pragma unroll
                 for (int k = 0; k < PPT; ++k)
-                    blend_step((uint32_t)px[k] | ((uint32_t)py[k] << 16), fx[k], fy[k], geo, par, col, cr[k], cg[k],
-                               cb[k], tr[k]);
+                    blend_step((uint32_t)px[k] | ((uint32_t)py[k] << 16), fx[k], fy[k], geo, par, col, st.exact + j,
+                               cr[k], cg[k], cb[k], tr[k]);
             }
         }
     }
@@ -389,8 +429,9 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
 // blend step (about 10 eps |power| / (1 - |rho|)^2 each, with 1 - rho^2 =
 // r2 = 1 - B^2 / (4 A C)); splats with r2 < 0.01 or a clamped, indefinite
 // conic are always walked.
-__device__ __forceinline__ bool ellipse_meets_box(int4 r, f32x4_t geo, f32x4_t par, int qx0, int qy0) {
-    const float A = geo.z, B = geo.w, C = par.x;
+__device__ __forceinline__ bool ellipse_meets_box(int4 r, f32x4_t geo, float C, float cutoff, float decision,
+                                                  int qx0, int qy0) {
+    const float A = geo.z, B = geo.w;
     if (!(A < 0.0f && C < 0.0f)) return true;
     const float r2 = 1.0f - (B * B) / (4.0f * A * C);
     if (!(r2 >= 0.01f)) return true;
@@ -407,8 +448,9 @@ __device__ __forceinline__ bool ellipse_meets_box(int4 r, f32x4_t geo, f32x4_t p
         return fmaf(fmaf(A, t, B * u), t, (C * u) * u);
     };
     const float best = maxf(maxf(along_y(xl), along_y(xh)), maxf(along_x(yl), along_x(yh)));
-    const float margin = __builtin_fabsf(par.z) * (0.003f + 1e-5f / (r2 * r2)) + 1e-5f;
-    return best >= par.z - margin;
+    // the blend step's decision margin widens the band that may pass
+    const float margin = __builtin_fabsf(cutoff) * (0.003f + 1e-5f / (r2 * r2)) + 1e-5f + decision;
+    return best >= cutoff - margin;
 }
 
 // 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
@@ -427,13 +469,13 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
     const uint32_t pxy = (uint32_t)px | ((uint32_t)py << 16);
     const bool valid = px < t.width && py < t.height;
     // pixels outside the image start saturated and are never written
-    float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? 1.0f : 0.0f;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? kInv255 : 0.0f;  // tr: T / 255
 
     for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
-        if (!__syncthreads_or(tr >= kTransmittanceMin)) break;
+        if (!__syncthreads_or(tr >= kUMin)) break;
         uint32_t e = base + tid;
         if (e < t.range.y) {
-            stage_splat(a, t, a.tile_vals[e], tid, st);
+            const float cutoff = stage_splat(a, t, a.tile_vals[e], tid, st);
             // the quadrants the splat can touch, tested once here for all
             // four waves (the tests share the splat's terms)
             const f32x4_t geo = st.geo[tid], par = st.par[tid];
@@ -443,14 +485,14 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             for (int q = 0; q < 4; ++q) {
                 const int x0 = t.tx * 16 + (q & 1) * 8, y0 = t.ty * 16 + (q >> 1) * 8;
                 bool hit = r.x < x0 + 8 && r.y > x0 && r.z < y0 + 8 && r.w > y0;
-                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, geo, par, x0, y0);
+                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, geo, par.x, cutoff, par.z, x0, y0);
                 quads |= hit ? 1u << q : 0u;
             }
             st.quads[tid] = quads;
         }
         __syncthreads();
         const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
-        if (!__ballot(tr >= kTransmittanceMin)) continue;
+        if (!__ballot(tr >= kUMin)) continue;
         for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
             // which of these 64 splats touch this wave's quadrant
             const uint32_t j = chunk + lane;
@@ -459,14 +501,108 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             while (mask) {
                 const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
-                blend_step(pxy, fx, fy, st.geo[k], st.par[k], st.col[k], cr, cg, cb, tr);
+                blend_step(pxy, fx, fy, st.geo[k], st.par[k], st.col[k], st.exact + k, cr, cg, cb, tr);
             }
-            if (!__ballot(tr >= kTransmittanceMin)) break;
+            if (!__ballot(tr >= kUMin)) break;
         }
     }
     __syncthreads();  // the staged batch is free for the next tile
 
     if (valid) store_pixel(t, px, py, cr, cg, cb, tr);
+}
+
+// Reference blend arithmetic on 16x16 tiles.  The reference draws the
+// depth-sorted quads far to near into an RGBA16F target with ONE /
+// ONE_MINUS_SRC_ALPHA blending (src/renderer.cpp:292-299): each fragment's
+// (rgb alpha, alpha) is added to (1 - alpha) times the stored value, and the
+// sum is stored as half precision again (round to nearest even) before the
+// next fragment reads it.  Here each pixel walks its tile list from the back
+// (the list is front to back), evaluates the fragment shader
+// (shaders/splat.glsl:84-104) with the reference's operations in source
+// order (conic = (c, -b, a) / det from the record, power =
+// -0.5 (k0 dx dx + 2 k1 dx dy + k2 dy dy), the deterministic log and exp) and
+// rounds every channel to half after every blend.  No pixel stops early: a
+// fragment behind a saturated one still changes the rounding of the stored
+// value.  Staged per splat: geo (cx, cy, k0, 2 k1), par (k2, rect low
+// corners, cutoff, opacity), col (r, g, b, rect high corners).
+__device__ __forceinline__ float round_half(float v) { return __half2float(__float2half_rn(v)); }
+
+__device__ __forceinline__ void composite_tile_ref(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
+    constexpr int BATCH = 256;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const TileRef t = tile_ref(a, item);
+
+    const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
+    const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const uint32_t pxy = (uint32_t)px | ((uint32_t)py << 16);
+    const bool valid = px < t.width && py < t.height;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = 0.0f;  // the black (0, 0, 0, 0) clear
+
+    for (uint32_t hi = t.range.y; hi > t.range.x;) {
+        const uint32_t lo = hi - t.range.x > (uint32_t)BATCH ? hi - (uint32_t)BATCH : t.range.x;
+        const uint32_t cnt = hi - lo;
+        __syncthreads();  // the previous batch has been walked
+        if (tid < cnt) {
+            const uint32_t g = a.tile_vals[lo + tid];
+            const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
+            const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
+            const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
+            float det = r1.x * r1.z - r1.y * r1.y;
+            if (det < 1e-6f) det = 1e-6f;
+            const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+            const uint2 rect = pack_rect16(splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, t.width,
+                                                            t.height, a.tight_rects != 0));
+            st.geo[tid] = f32x4_t{r0.x, r0.y, k0, 2.0f * k1};
+            st.par[tid] = f32x4_t{k2, __uint_as_float(rect.x), logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)), r0.z};
+            st.col[tid] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
+            const int4 r = unpack_rect16(rect);
+            uint32_t quads = 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int x0 = t.tx * 16 + (q & 1) * 8, y0 = t.ty * 16 + (q >> 1) * 8;
+                quads |= (r.x < x0 + 8 && r.y > x0 && r.z < y0 + 8 && r.w > y0) ? 1u << q : 0u;
+            }
+            st.quads[tid] = quads;
+        }
+        __syncthreads();
+        // chunks of 64 staged splats, last first; within a chunk the highest
+        // set bit of the quadrant ballot first
+        for (int chunk = (int)((cnt - 1u) & ~63u); chunk >= 0; chunk -= 64) {
+            const uint32_t j = (uint32_t)chunk + lane;
+            uint64_t mask = __ballot(j < cnt && ((st.quads[j] >> wave) & 1u));
+            while (mask) {
+                const int bit = 63 - __builtin_clzll(mask);
+                mask &= ~(1ull << bit);
+                const uint32_t k = (uint32_t)chunk + (uint32_t)bit;
+                const f32x4_t geo = st.geo[k], par = st.par[k], col = st.col[k];
+                const uint2 r = make_uint2(__float_as_uint(par.y), __float_as_uint(col.w));
+                const float dx = fx - geo.x, dy = fy - geo.y;
+                const float power = -0.5f * (geo.z * dx * dx + geo.w * dx * dy + par.x * dy * dy);
+                const bool take = in_rect16(pxy, r) & !(power > 0.0f) & !(power < par.z);
+                if (take) {
+                    const float alpha = minf(par.w * expf_(power), 0.99f);
+                    const float keep = 1.0f - alpha;
+                    cr = round_half(col.x * alpha + cr * keep);
+                    cg = round_half(col.y * alpha + cg * keep);
+                    cb = round_half(col.z * alpha + cb * keep);
+                    ca = round_half(alpha + ca * keep);
+                }
+            }
+        }
+        hi = lo;
+    }
+    __syncthreads();  // the staged batch is free for the next tile
+
+    if (valid) {
+        const size_t o = ((size_t)py * t.width + px) * 4;
+        ushort4 h;
+        h.x = __half_as_ushort(__float2half_rn(cr));
+        h.y = __half_as_ushort(__float2half_rn(cg));
+        h.z = __half_as_ushort(__float2half_rn(cb));
+        h.w = __half_as_ushort(__float2half_rn(ca));
+        *reinterpret_cast<ushort4*>(t.image + o) = h;
+    }
 }
 
 // StochasticSplats sample (shaders/splat.glsl:107-146): every covered
@@ -622,17 +758,45 @@ struct CompositeKernelShape {
     static constexpr int kThreads = QUAD ? 256 : TileShape<T>::kThreads;
 };
 
+// The reference-arithmetic compositor over 16x16 tiles, launched like
+// composite_kernel below.
+__global__ void __launch_bounds__(256) composite_ref_kernel(CompositeLaunch a) {
+    __shared__ f32x4_t s_geo[256];
+    __shared__ f32x4_t s_par[256];
+    __shared__ f32x4_t s_col[256];
+    __shared__ uint32_t s_quads[256];
+    __shared__ uint32_t s_item;
+    const StagedSplats st{s_geo, s_par, s_col, nullptr, s_quads};
+    if (a.persistent_blocks == 0u) {
+        composite_tile_ref(a, xcd_remap(blockIdx.x, a.total_tiles), st);
+        return;
+    }
+    const uint32_t xcd = blockIdx.x & 7u;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = next_tile(a, xcd);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= a.total_tiles) break;
+        composite_tile_ref(a, item, st);
+    }
+}
+
 // One block per tile (persistent_blocks == 0, XCD-aware tile order) or a
 // persistent grid pulling tiles from the per-XCD queues.
 template <int T, bool QUAD>
-__global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads)) composite_kernel(CompositeLaunch a) {
+// The quadrant kernel at eight waves per SIMD (at most 64 VGPRs): its blend
+// loop is VALU-issue bound and the other waves hide its LDS reads.
+__global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads), (QUAD ? 8 : 1))
+    composite_kernel(CompositeLaunch a) {
     constexpr int BATCH = CompositeKernelShape<T, QUAD>::kThreads;
     __shared__ f32x4_t s_geo[BATCH];
     __shared__ f32x4_t s_par[BATCH];
     __shared__ f32x4_t s_col[BATCH];
+    __shared__ f32x4_t s_exact[BATCH];
     __shared__ uint32_t s_quads[QUAD ? BATCH : 1];
     __shared__ uint32_t s_item;
-    const StagedSplats st{s_geo, s_par, s_col, s_quads};
+    const StagedSplats st{s_geo, s_par, s_col, s_exact, s_quads};
     if (a.persistent_blocks == 0u) {
         const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
         if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
@@ -662,7 +826,7 @@ hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
                                       nullptr, s, a.dup_capacity, &a.counters->bin.dup_count);
     if (e != hipSuccess) return e;
     // pair positions stay below the capacity
-    if ((uint64_t)a.dup_capacity * 4u <= 0xFFFFFFFFull)
+    if ((uint64_t)a.dup_capacity * 4u <= 0xFFFFFFFFull && !a.wide_index)
         hipLaunchKernelGGL(bin_emit_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(bin_emit_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
@@ -688,6 +852,11 @@ hipError_t launch_stochastic(const StochasticLaunch& a, hipStream_t s) {
 hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s) {
     if (!a.total_tiles || !a.nviews) return hipSuccess;
     const uint32_t grid = a.persistent_blocks ? a.persistent_blocks : a.total_tiles;
+    if (a.reference_blend) {
+        if (a.tile != 16) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(composite_ref_kernel, dim3(grid), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     switch (a.tile) {
         case 8:
             hipLaunchKernelGGL((composite_kernel<8, false>), dim3(grid), dim3(TileShape<8>::kThreads), 0, s, a);

@@ -438,13 +438,17 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             rec[0] = make_float4(cx, rcy, opacity, ndc_z);
             rec[1] = make_float4(ca, cb * cam.clip_y_sign, cc, radius_x);
             rec[2] = make_float4(color[0], color[1], color[2], radius_y);
-            a.rect_px[slot] = pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
-                                         (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0);
+            // the rasteriser drops a quad outside the clip volume's depth
+            // range (far plane): it covers no pixel, so it gets no tile
+            const bool drawn = !depth_clipped(ndc_z, cam.clip_z_01);
+            a.rect_px[slot] = drawn ? pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
+                                                 (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0)
+                                    : make_uint2(0u, 0u);
             if (a.pad_rects)
-                a.pad_rects[slot] = pack_tile_rect(
+                a.pad_rects[slot] = drawn ? pack_tile_rect(
                     pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
                                (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0, 0.0f, 0.0f, 1),
-                    a.pad_tile_shift);
+                    a.pad_tile_shift) : 0u;
         }
         // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
         // block's visible count; compaction runs after the counts are scanned

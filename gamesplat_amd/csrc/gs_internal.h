@@ -147,6 +147,9 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
 // ---------------------------------------------------------------------------
 // Kernel launch interfaces.
 
+// gs_scan.hip: function attributes the launches below need (the dynamic
+// LDS size of the scan), set once per process before any frame is captured.
+hipError_t init_kernel_attributes();
 // gs_scan.hip: exclusive scan of nb block counts into offsets; *total gets
 // the sum, and *clamped (when given) min(sum, clamp_cap).  block_diag
 // (nb x 8, may be NULL) is reduced into diag_out.
@@ -218,6 +221,7 @@ struct SortLaunch {
     uint32_t* table;         // sort_table_words(capacity) words
     int passes;
     int bits;                // digit width (8, 7, 6 or 4)
+    int wide_index;          // 1: 64-bit element indexing even where 32-bit offsets suffice
 };
 // Stable LSD radix sort.  Returns the index (1, 2, or 3 with key_range) of
 // the buffer holding the result through *result_slot.
@@ -245,6 +249,7 @@ struct BinLaunch {
     int tight;               // the cull's tight-rectangle setting
     int pad;
     float jitter_x, jitter_y;
+    float clip_z_01;         // the view's depth range (quads outside it are clipped)
     uint2* rects;            // per front-to-back position: tile rectangle (without sorted_rects)
     uint32_t* block_counts;  // pairs per 256-block
     uint32_t* block_offsets;
@@ -254,6 +259,7 @@ struct BinLaunch {
     uint32_t* dup_keys;
     uint32_t* dup_vals;
     uint32_t dup_capacity;   // pairs that fit in dup_keys / dup_vals
+    int wide_index;          // 1: 64-bit pair indexing even where 32-bit offsets suffice
 };
 // Counts and emits (tile id, splat id) pairs; leaves the requested total in
 // counters->bin.dup_total and min(total, capacity) in counters->bin.dup_count.
@@ -272,6 +278,9 @@ struct CompositeLaunch {
     uint16_t* image;         // RGBA16F per view, framebuffer row order, packed by pixel_base
     int quadrant_cull;       // 16x16 tiles: 8x8 quadrant kernel; 1 = rectangle + ellipse test, 2 = rectangle only
     int tight_rects;         // per-splat pixel rectangle cut to the opacity-aware ellipse box
+    // 1: the reference's arithmetic (back to front, half-precision storage
+    // after every blend; tile 16).  0: front to back in fp32.
+    int reference_blend;
     // 0: one block per tile.  >0: that many persistent blocks pull tiles from
     // 8 per-XCD queues (queue: 8 counters, cleared before the launch).
     uint32_t persistent_blocks;

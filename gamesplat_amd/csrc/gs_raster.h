@@ -25,6 +25,17 @@ __device__ __forceinline__ int32_t snap256(float w) {
 
 __device__ __forceinline__ int32_t ceil_div256(int32_t a) { return -((-a) >> 8); }
 
+// splat_vs emits gl_Position = (ndc, ndc_z, 1) (shaders/splat.glsl:63-67) and
+// no pipeline enables depth clamping (src/renderer.cpp:277-320), so the
+// rasteriser clips a quad (its four corners share ndc_z) unless
+// -w <= z <= w: -1 <= ndc_z <= 1 in GL clip space, 0 <= ndc_z <= 1 when
+// clip_z_01 selects the [0, 1] depth range.  With the camera's far plane at
+// 100 (src/camera.cpp:12) this drops every splat deeper than 100.
+__device__ __forceinline__ bool depth_clipped(float ndc_z, float clip_z_01) {
+    const float lo = clip_z_01 > 0.5f ? 0.0f : -1.0f;
+    return !(ndc_z <= 1.0f && ndc_z >= lo);
+}
+
 // Quad edges x0, x1, y0, y1 in 24.8 fixed point window coordinates; the
 // corners are centre + corner * radius + jitter, in that order (TAA jitter
 // of StochasticSplats samples, zero otherwise: adding 0 changes no value).

@@ -7,6 +7,8 @@
 // across the XCDs, which made a per-block ticket or total counter cost
 // ~25 ns per block.  One 1024-thread block scans the counts instead.  The
 // total goes to *total.
+#include <mutex>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -191,18 +193,24 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
 
 }  // namespace
 
+hipError_t init_kernel_attributes() {
+    // once per process; the attribute belongs to the kernel on every device
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [] {
+        status = hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_counts_lds_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCounts * 4);
+    });
+    return status;
+}
+
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* offsets, uint32_t* total,
                               const uint32_t* block_diag, uint32_t* diag_out, hipStream_t s, uint32_t clamp_cap,
                               uint32_t* clamped, const uint2* block_range, uint32_t* range_out) {
     if (block_range && !range_out) return hipErrorInvalidValue;
     if (nb <= kLdsCounts) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&scan_counts_lds_kernel),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, kLdsCounts * 4);
-            if (e != hipSuccess) return e;
-            attr_set = true;
-        }
+        hipError_t e = init_kernel_attributes();
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(scan_counts_lds_kernel, dim3(1), dim3(kScanThreads), (size_t)(nb ? nb : 1) * 4, s, counts,
                            nb, offsets, total, block_diag, diag_out, clamp_cap, clamped, block_range, range_out);
     } else {

@@ -334,7 +334,7 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     pa.tiles = tiles;
     pa.col_total = a.scratch;
     // every array holds at most capacity 4-byte elements
-    const bool o32 = (uint64_t)a.capacity * 4u <= 0xFFFFFFFFull;
+    const bool o32 = (uint64_t)a.capacity * 4u <= 0xFFFFFFFFull && !a.wide_index;
     int src = 0, dst = 1;
     for (int p = 0; p < a.passes; ++p) {
         pa.keys_in = a.keys[src];

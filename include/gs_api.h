@@ -427,7 +427,21 @@ enum {
      * (cameras and view sizes live in device memory, so moving cameras
      * replay; new buffer sizes, options or effect parameters re-capture).
      * Ignored while profiling is on.  0 (default): direct launches. */
-    GS_OPT_GRAPH = 6
+    GS_OPT_GRAPH = 6,
+    /* 1: sorted-alpha composites use the reference's blend arithmetic
+     * (ONE / ONE_MINUS_SRC_ALPHA into RGBA16F, src/renderer.cpp:292-299):
+     * each tile list is walked back to front, the fragment shader's
+     * operations run in source order (shaders/splat.glsl:84-104, with the
+     * deterministic log and exp), and every channel is rounded to half
+     * after every blend, as the target stores it; on 16x16 tiles whatever
+     * tile size is asked.  0 (default): front to back in fp32 with early
+     * termination, rounded to half once. */
+    GS_OPT_REFERENCE_BLEND = 7,
+    /* 1: the radix sorts and the bin address their arrays with 64-bit
+     * element indices even where 32-bit byte offsets suffice (the variants
+     * arrays of 2^30 or more elements take); results are identical.  For
+     * testing those variants at small sizes.  0 (default): off. */
+    GS_OPT_WIDE_INDEX = 8
 };
 int gs_set_option(gs_ctx* ctx, int option, int value);
 

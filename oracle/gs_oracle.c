@@ -677,8 +677,19 @@ void or_quad_fixed(const float* rec, float width, float height, float jitter_x, 
 
 static inline int32_t or_ceil_div256(int32_t a) { return -((-a) >> 8); }
 
+/* splat_vs emits gl_Position = (ndc, ndc_z, 1) (shaders/splat.glsl:63-67) and
+ * the pipelines enable no depth clamp (src/renderer.cpp:277-320), so the
+ * rasteriser clips the whole quad (all four corners share ndc_z) unless
+ * -w <= z <= w, i.e. -1 <= ndc_z <= 1 for GL clip space, 0 <= ndc_z <= 1 when
+ * clip_z_01 selects the [0, 1] depth range.  With the camera's far plane at
+ * 100 (src/camera.cpp:12) this drops splats whose view depth exceeds 100. */
+int or_clipped(float ndc_z, float clip_z_01) {
+    const float lo = clip_z_01 > 0.5f ? 0.0f : -1.0f;
+    return !(ndc_z <= 1.0f && ndc_z >= lo);
+}
+
 void or_composite(const float* projected, const uint32_t* sorted_ids, uint32_t n,
-                  int width, int height, float clip_y_sign, int mode, double* out,
+                  int width, int height, float clip_y_sign, float clip_z_01, int mode, double* out,
                   int threads) {
     const int band = 16;
     const int nbands = (height + band - 1) / band;
@@ -697,6 +708,7 @@ void or_composite(const float* projected, const uint32_t* sorted_ids, uint32_t n
             uint32_t id = sorted_ids[mode == 1 ? n - 1 - s : s];
             if (id == 0xFFFFFFFFu) continue;
             const float* rec = projected + (size_t)id * 16;
+            if (or_clipped(rec[11], clip_z_01)) continue;
             int32_t q[4];
             or_quad_fixed(rec, (float)width, (float)height, 0.0f, 0.0f, q);
             int32_t i_lo = or_ceil_div256(q[0] - 128), i_hi = or_ceil_div256(q[1] - 128);
@@ -952,6 +964,7 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
             const uint32_t id = ids[s];
             if (id == 0xFFFFFFFFu) continue;
             const float* rec = projected + (size_t)id * 16;
+            if (or_clipped(rec[11], clip_z_01)) continue;
             /* the quad corners and the fragment centre move by the TAA jitter
              * (shaders/splat.glsl:61,72) */
             int32_t q[4];

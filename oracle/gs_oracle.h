@@ -72,11 +72,15 @@ void or_cull(const GsGaussian* g, uint32_t n, const GsCameraUniforms* cam,
 /* 4-pass 8-bit LSD radix sort of all n (key, id) pairs, in place. */
 void or_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t n);
 
+/* Whether the rasteriser clips a splat quad at this ndc depth (outside
+ * [-1, 1], or [0, 1] with clip_z_01). */
+int or_clipped(float ndc_z, float clip_z_01);
 /* Sorted-alpha composite.  mode 0: fp64 back-to-front, 1: fp64 front-to-back,
- * 2: fp32 shader + fp16 storage after every blend (RGBA16F target).
+ * 2: fp32 shader + fp16 storage after every blend (RGBA16F target).  Quads
+ * outside the clip volume's depth range are not drawn.
  * out: W*H*4 doubles, top row first (clip_y_sign > 0 flips rows). */
 void or_composite(const float* projected, const uint32_t* sorted_ids, uint32_t n,
-                  int width, int height, float clip_y_sign, int mode, double* out,
+                  int width, int height, float clip_y_sign, float clip_z_01, int mode, double* out,
                   int threads);
 
 /* Quad coverage of one projected splat in 24.8 fixed point window
@@ -102,7 +106,7 @@ void or_gps_resolve(const uint64_t* cells, int width, int height, int ss, float 
 float or_hash13(float x, float y, float z);
 uint32_t or_depth24(float ndc_z, float clip_z_01);
 /* StochasticSplats sample (shaders/splat.glsl:107-146): splats drawn in index
- * order, kept when hash13 < alpha, nearest 24-bit depth wins (LESS), colour
+ * order (quads outside the clip volume's depth range are not drawn), kept when hash13 < alpha, nearest 24-bit depth wins (LESS), colour
  * stored as half floats; clear (bg, bg, bg, 0).  color W*H*4, depth W*H,
  * framebuffer row order. */
 void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t n, int width, int height,

@@ -38,7 +38,8 @@ _SIGS = {
     "or_cull": (None, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int,
                        C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p, C.c_int]),
     "or_radix_sort": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
-    "or_composite": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_int,
+    "or_clipped": (C.c_int, [C.c_float, C.c_float]),
+    "or_composite": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
                             C.c_void_p, C.c_int]),
     "or_quad_fixed": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "or_gps_expand": (None, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_int,
@@ -152,13 +153,14 @@ def radix_sort(keys: np.ndarray, ids: np.ndarray):
 
 
 def composite(projected: np.ndarray, sorted_ids: np.ndarray, width: int, height: int, clip_y_sign: float = 1.0,
-              mode: int = 0, threads=None) -> np.ndarray:
-    """mode 0 fp64 back-to-front, 1 fp64 front-to-back, 2 fp16-per-blend. (H, W, 4) float64, top row first."""
+              mode: int = 0, threads=None, clip_z_01: float = 0.0) -> np.ndarray:
+    """mode 0 fp64 back-to-front, 1 fp64 front-to-back, 2 fp16-per-blend. (H, W, 4) float64, top row first.
+    Splats whose ndc depth lies outside the clip volume ([-1, 1], [0, 1] with clip_z_01) are not drawn."""
     lib = load()
     p = np.ascontiguousarray(projected, dtype=np.float32)
     s = np.ascontiguousarray(sorted_ids, dtype=np.uint32)
     out = np.empty((height, width, 4), np.float64)
-    lib.or_composite(_p(p), _p(s), s.size, width, height, clip_y_sign, mode, _p(out),
+    lib.or_composite(_p(p), _p(s), s.size, width, height, clip_y_sign, clip_z_01, mode, _p(out),
                      threads_default() if threads is None else threads)
     return out
 

@@ -323,3 +323,32 @@ def test_gps_accumulate(ctx, orc, c1_scene):
         hist = orc.accumulate(color if hist is None else hist, color, f)
     acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
     np.testing.assert_array_equal(acc, hist[::-1])
+
+
+def test_wide_index_variants(ctx, orc, c1_scene):
+    """The 64-bit-index variants of the radix sorts and the bin (taken by
+    arrays of 2^30 or more elements) give the same sorted order and image as
+    the 32-bit-offset ones; forced on at the C1 size."""
+    g, cfg, cam = c1_scene
+    print("case: wide index variants c1", flush=True)
+    ctx.upload(g)
+    out = {}
+    try:
+        for wide in (0, 1):
+            ctx.set_option(abi.GS_OPT_WIDE_INDEX, wide)
+            ctx.cull(cam)
+            ctx.sort()
+            ids = ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS)
+            ctx.composite(16)
+            img = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy()
+            ctx.render_alpha(cam, tile=8)
+            img8 = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy()
+            out[wide] = (ids, img, img8)
+    finally:
+        ctx.set_option(abi.GS_OPT_WIDE_INDEX, 0)
+    ok, oi, op, ov, od = orc.cull(g, cam)
+    sk, si = orc.radix_sort(ok, oi)
+    np.testing.assert_array_equal(out[1][0], si)
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+    assert out[1][1].any()
