@@ -11,7 +11,7 @@
 //     next key's time or at duration for the last key;
 //   - invalid input (no keys, non-finite times, duration <= 0) gives state
 //     0 and, for layers, no weights.
-#ifndef GS_RENDERER_USE_REFERENCE_TYPES
+#ifndef GS_RENDERER_IN_TREE
 
 #include <math.h>
 

@@ -276,7 +276,7 @@ int main(int argc, char** argv) {
 
     const int warmup = (int)argi("warmup", 3), frames = (int)argi("frames", 20);
     const SplatRenderMode final_mode = r.splat_render_mode;
-    hipStream_t stream = (hipStream_t)gs_stream(r.ctx);
+    hipStream_t stream = (hipStream_t)gs_stream(renderer_gs_context(&r));
     std::vector<hipEvent_t> ev(2 * frames);
     for (auto& e : ev) hipEventCreate(&e);
     struct FrameRecord {
@@ -318,7 +318,7 @@ int main(int argc, char** argv) {
         if (timed) hipEventRecord(ev[2 * (f - warmup)], stream);
         renderer_draw_frame(&r, &scene, &cam, nullptr, nullptr);
         if (timed) hipEventRecord(ev[2 * (f - warmup) + 1], stream);
-        if (r.last_frame_status != GS_OK) return 1;
+        if (renderer_last_status(&r) != GS_OK) return 1;
         const auto submitted = std::chrono::steady_clock::now();
         if (timed)
             rec.push_back({std::chrono::duration<double, std::milli>(begin - previous).count(),
@@ -328,7 +328,7 @@ int main(int argc, char** argv) {
                            scene.animation && animation.playing ? 1 : 0});
         previous = begin;
     }
-    gs_synchronize(r.ctx);
+    gs_synchronize(renderer_gs_context(&r));
     std::vector<double> wall, cpu, gpu;
     for (int f = 0; f < frames; ++f) {
         float ms = 0.0f;
@@ -384,9 +384,9 @@ int main(int argc, char** argv) {
         fclose(f);
     }
     GsStats st;
-    gs_stats(r.ctx, &st);
+    gs_stats(renderer_gs_context(&r), &st);
     GsFrameInfo fi;
-    gs_frame_info(r.ctx, &fi);
+    gs_frame_info(renderer_gs_context(&r), &fi);
     size_t free1 = 0, total1 = 0;
     hipMemGetInfo(&free1, &total1);
     printf("RESULT {\"frames\":%zu,\"wall_median_ms\":%.6f,\"wall_p95_ms\":%.6f,\"cpu_median_ms\":%.6f,"

@@ -13,9 +13,33 @@
 #include <string.h>
 
 #include <chrono>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace {
+
+// What the splat path keeps per Renderer beyond the reference's fields: the
+// gs context, the status of the last frame and the time of the last GPS
+// work-statistics readback.  A side table keyed by the Renderer's address,
+// so the reference's struct stays as it is.
+struct SplatState {
+    gs_ctx* ctx = nullptr;
+    int last_status = GS_OK;
+    double last_stats_time = 0.0;
+};
+std::mutex g_state_mutex;
+std::unordered_map<const Renderer*, SplatState> g_states;
+
+SplatState& state(const Renderer* r) {
+    std::lock_guard<std::mutex> lock(g_state_mutex);
+    return g_states[r];  // element references stay valid across rehashing
+}
+
+void drop_state(const Renderer* r) {
+    std::lock_guard<std::mutex> lock(g_state_mutex);
+    g_states.erase(r);
+}
 
 void log_error(const char* what, int status) {
     fprintf(stderr, "[renderer] %s failed (%d): %s\n", what, status, gs_last_error());
@@ -47,9 +71,11 @@ float half_to_float(uint16_t h) {
     return f;
 }
 
-double g_last_stats_time = 0.0;
-
 }  // namespace
+
+gs_ctx* renderer_gs_context(const Renderer* r) { return state(r).ctx; }
+
+int renderer_last_status(const Renderer* r) { return state(r).last_status; }
 
 bool renderer_init(Renderer* r) {
     memset(r, 0, sizeof(*r));
@@ -63,10 +89,13 @@ bool renderer_init(Renderer* r) {
     r->stochastic_taa_enabled = true;
     r->sh_degree = 3;
     r->sh_lod_distance = 0.0f;
-    int st = gs_create(&r->ctx, 0);
+    SplatState& ss = state(r);
+    if (ss.ctx) gs_destroy(ss.ctx);  // a re-initialised Renderer at the same address
+    ss = SplatState{};
+    int st = gs_create(&ss.ctx, 0);
     if (st != GS_OK) {
         log_error("gs_create", st);
-        r->ctx = nullptr;
+        ss.ctx = nullptr;
         return false;
     }
     r->gps_capabilities = {true, true, true, true, true};
@@ -76,11 +105,12 @@ bool renderer_init(Renderer* r) {
 }
 
 bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene) {
-    if (!r->ctx || !scene) return false;
-    gs_set_memory_budget(r->ctx, r->gpu_budget_bytes);
-    int st = gs_set_gps_budget(r->ctx, r->gps_max_work_items ? r->gps_max_work_items : 1u);
+    gs_ctx* ctx = state(r).ctx;
+    if (!ctx || !scene) return false;
+    gs_set_memory_budget(ctx, r->gpu_budget_bytes);
+    int st = gs_set_gps_budget(ctx, r->gps_max_work_items ? r->gps_max_work_items : 1u);
     if (st == GS_OK)
-        st = gs_upload_gaussians(r->ctx, reinterpret_cast<const GsGaussian*>(scene->gaussians), scene->gaussian_count);
+        st = gs_upload_gaussians(ctx, reinterpret_cast<const GsGaussian*>(scene->gaussians), scene->gaussian_count);
     if (st != GS_OK) {
         log_error("upload", st);
         return false;
@@ -117,7 +147,7 @@ bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene) {
             }
         }
         if (ga.state_count || ga.layer_count) {
-            st = gs_upload_animation(r->ctx, &ga);
+            st = gs_upload_animation(ctx, &ga);
             if (st != GS_OK) {
                 log_error("animation upload", st);
                 return false;
@@ -133,7 +163,8 @@ bool renderer_upload_gaussians(Renderer* r, const GaussianScene* scene) {
 bool renderer_set_gps_work_budget(Renderer* r, uint32_t max_work_items) {
     r->gps_work_budget_resize_failed = false;
     if (max_work_items == r->gps_gpu.max_work_items) return true;
-    int st = r->ctx ? gs_set_gps_budget(r->ctx, max_work_items) : GS_ERR_STATE;
+    gs_ctx* ctx = state(r).ctx;
+    int st = ctx ? gs_set_gps_budget(ctx, max_work_items) : GS_ERR_STATE;
     if (st != GS_OK) {
         log_error("GPS work budget resize", st);
         r->gps_work_budget_resize_failed = true;
@@ -161,7 +192,8 @@ static uint32_t clamp_samples(uint32_t n) { return n < 1u ? 1u : (n > 16u ? 16u 
 // samples with consecutive frame seeds; with TAA each is Halton-jittered,
 // they are averaged, and the average is reprojected into the history; without
 // TAA they fold into the running mean, which a camera change resets.
-static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEffectParams* splat_effect, float bg) {
+static int draw_stochastic(Renderer* r, gs_ctx* ctx, const CameraUniforms* cam, const SplatEffectParams* splat_effect,
+                           float bg) {
     const GsCameraUniforms* gcam = reinterpret_cast<const GsCameraUniforms*>(cam);
     // new target sizes start a new accumulation (renderer_ensure_stochastic_targets)
     if (r->stochastic_width != (uint32_t)cam->viewport[0] || r->stochastic_height != (uint32_t)cam->viewport[1])
@@ -200,13 +232,13 @@ static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEf
         r->stochastic_frame_seed++;
         float jitter[2] = {0.0f, 0.0f};
         if (use_taa) gs_taa_jitter(r->stochastic_frame_seed, jitter);
-        st = gs_stochastic_sample(r->ctx, r->stochastic_frame_seed, bg, jitter);
+        st = gs_stochastic_sample(ctx, r->stochastic_frame_seed, bg, jitter);
         if (st != GS_OK) break;
         if (use_taa && samples > 1u) {
-            st = gs_frame_average(r->ctx, i);
+            st = gs_frame_average(ctx, i);
         } else if (!use_taa && (r->stochastic_accumulation_enabled || samples > 1u)) {
             const uint32_t next = r->stochastic_accumulation_enabled ? r->stochastic_sample_count + 1u : i + 1u;
-            st = gs_accumulate(r->ctx, next);
+            st = gs_accumulate(ctx, next);
             if (st == GS_OK && r->stochastic_accumulation_enabled) {
                 r->stochastic_sample_count = next;
                 r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
@@ -214,7 +246,7 @@ static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEf
         }
     }
     if (st == GS_OK && use_taa) {
-        st = gs_taa_accumulate(r->ctx, &taa, samples > 1u ? 1 : 0);
+        st = gs_taa_accumulate(ctx, &taa, samples > 1u ? 1 : 0);
         if (st == GS_OK) {
             r->stochastic_taa_prev_view_proj_valid = true;
             r->stochastic_sample_count++;
@@ -224,18 +256,16 @@ static int draw_stochastic(Renderer* r, const CameraUniforms* cam, const SplatEf
     if (st != GS_OK) return st;
     r->stochastic_width = (uint32_t)cam->viewport[0];
     r->stochastic_height = (uint32_t)cam->viewport[1];
-    r->stochastic_display_image =
-        use_taa ? GS_IMAGE_TAA_RGBA32F
-                : (r->stochastic_accumulation_enabled || samples > 1u ? GS_IMAGE_ACCUM_RGBA32F
-                                                                      : GS_IMAGE_GPS_SAMPLE_RGBA32F);
     return GS_OK;
 }
 
 void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms* cam, const OverlayParams*,
                          const NodeRenderParams*, const SplatEffectParams* splat_effect, float, const CameraUniforms*) {
-    r->last_frame_status = GS_OK;
-    if (!r->ctx || !scene || !cam) {
-        r->last_frame_status = GS_ERR_STATE;
+    SplatState& ss = state(r);
+    gs_ctx* ctx = ss.ctx;
+    ss.last_status = GS_OK;
+    if (!ctx || !scene || !cam) {
+        ss.last_status = GS_ERR_STATE;
         return;
     }
     const float bg = r->black_background || scene->animation ? 0.0f : 0.1f;
@@ -260,13 +290,13 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
             layers = (uint32_t)weights.size();
             if (r->first_layer_only && layers > 1u) layers = 1u;
         }
-        const int st = gs_set_animation_frame(r->ctx, an.version == 3 ? 0 : sample.state_a,
+        const int st = gs_set_animation_frame(ctx, an.version == 3 ? 0 : sample.state_a,
                                               an.version == 3 ? 0 : sample.state_b,
                                               an.version == 3 ? 0.0f : sample.weight, layers,
                                               layers ? weights[0].data() : nullptr);
         if (st != GS_OK) {
             log_error("animation frame", st);
-            r->last_frame_status = st;
+            ss.last_status = st;
             return;
         }
     }
@@ -280,16 +310,16 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     GsCullOptions opts = {r->sh_degree, r->sh_lod_distance, r->splat_diagnostics_enabled ? 1 : 0};
     // animated scenes draw without the shockwave (src/renderer.cpp:2168)
     const SplatEffectParams* cull_effect = scene->animation ? nullptr : splat_effect;
-    int st = gs_cull(r->ctx, gcam, reinterpret_cast<const GsSplatEffectParams*>(cull_effect), &opts);
+    int st = gs_cull(ctx, gcam, reinterpret_cast<const GsSplatEffectParams*>(cull_effect), &opts);
     if (st != GS_OK) {
         log_error("cull", st);
-        r->last_frame_status = st;
+        ss.last_status = st;
         return;
     }
     r->splat_diagnostics.valid = false;
     if (r->splat_diagnostics_enabled) {
         GsStats s;
-        if (gs_stats(r->ctx, &s) == GS_OK && s.diagnostics_valid) {
+        if (gs_stats(ctx, &s) == GS_OK && s.diagnostics_valid) {
             memcpy(&r->splat_diagnostics, s.diagnostics, sizeof(s.diagnostics));
             r->splat_diagnostics.valid = true;
         }
@@ -298,11 +328,11 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     scene->visible_count = scene->gaussian_count;
 
     if (!gps && !stochastic) {
-        st = gs_sort(r->ctx);
-        if (st == GS_OK) st = gs_composite_alpha(r->ctx, 0);
+        st = gs_sort(ctx);
+        if (st == GS_OK) st = gs_composite_alpha(ctx, 0);
         if (st != GS_OK) {
             log_error("sorted-alpha frame", st);
-            r->last_frame_status = st;
+            ss.last_status = st;
             return;
         }
         r->appearance_width = (int)cam->viewport[0];
@@ -311,10 +341,10 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     }
 
     if (stochastic) {
-        st = draw_stochastic(r, cam, splat_effect, bg);
+        st = draw_stochastic(r, ctx, cam, splat_effect, bg);
         if (st != GS_OK) {
             log_error("stochastic splats", st);
-            r->last_frame_status = st;
+            ss.last_status = st;
         }
         return;
     }
@@ -325,28 +355,28 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     r->stochastic_prev_cam_valid = true;
     r->stochastic_frame_seed++;
 
-    uint32_t ss = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
+    const uint32_t sf = r->gps_supersample_factor < 1u ? 1u : (r->gps_supersample_factor > 4u ? 4u : r->gps_supersample_factor);
     const int linear_color = scene->animation && scene->animation->version == 3 ? 1 : 0;
-    st = gs_gps_sample(r->ctx, r->stochastic_frame_seed, ss, linear_color, bg);
+    st = gs_gps_sample(ctx, r->stochastic_frame_seed, sf, linear_color, bg);
     if (st != GS_OK) {
         log_error("GPS sample", st);
-        r->last_frame_status = st;
+        ss.last_status = st;
         return;
     }
     r->gps_gpu.width = (uint32_t)cam->viewport[0];
     r->gps_gpu.height = (uint32_t)cam->viewport[1];
-    r->gps_gpu.supersample_factor = ss;
+    r->gps_gpu.supersample_factor = sf;
     r->stochastic_width = r->gps_gpu.width;
     r->stochastic_height = r->gps_gpu.height;
     double t = now_s();
-    if (!r->gps_gpu.work_stats_valid || t - g_last_stats_time >= 0.5) {
+    if (!r->gps_gpu.work_stats_valid || t - ss.last_stats_time >= 0.5) {
         GsStats s;
-        if (gs_stats(r->ctx, &s) == GS_OK) {
+        if (gs_stats(ctx, &s) == GS_OK) {
             r->gps_gpu.requested_work_items = s.gps_requested_work_items;
             r->gps_gpu.work_budget_exhausted = s.gps_work_budget_exhausted != 0;
             r->gps_gpu.work_stats_valid = true;
             r->gps_gpu.work_stats_elapsed = 0.0f;
-            g_last_stats_time = t;
+            ss.last_stats_time = t;
         }
     }
     if (!r->stochastic_accumulation_enabled) {
@@ -354,23 +384,34 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
         return;
     }
     uint32_t next = r->stochastic_sample_count + 1u;
-    st = gs_accumulate(r->ctx, next);
+    st = gs_accumulate(ctx, next);
     if (st != GS_OK) {
         log_error("accumulate", st);
-        r->last_frame_status = st;
+        ss.last_status = st;
         return;
     }
     r->stochastic_sample_count = next;
     r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
 }
 
+// The image the reference harness captures (tests/gsanim_render.cpp:148-156).
+static int linear_image(const Renderer* r) {
+    if (r->splat_render_mode == SplatRenderMode::AlphaBlendSorted) return GS_IMAGE_ALPHA_RGBA16F;
+    const bool stochastic = r->splat_render_mode == SplatRenderMode::StochasticSplats;
+    if (r->stochastic_accumulation_enabled || (stochastic && r->stochastic_taa_enabled))
+        return stochastic && r->stochastic_taa_enabled ? GS_IMAGE_TAA_RGBA32F : GS_IMAGE_ACCUM_RGBA32F;
+    return GS_IMAGE_GPS_SAMPLE_RGBA32F;  // the GPS resolved image or the StochasticSplats sample
+}
+
 bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
-    if (!r->ctx || !out) return false;
-    if (r->splat_render_mode == SplatRenderMode::AlphaBlendSorted) {
+    gs_ctx* ctx = state(r).ctx;
+    if (!ctx || !out) return false;
+    const int which = linear_image(r);
+    if (which == GS_IMAGE_ALPHA_RGBA16F) {
         size_t px = (size_t)r->appearance_width * r->appearance_height;
         if (floats != px * 4) return false;
         std::vector<uint16_t> h(px * 4);
-        int st = gs_read_image(r->ctx, GS_IMAGE_ALPHA_RGBA16F, h.data(), h.size() * 2);
+        int st = gs_read_image(ctx, GS_IMAGE_ALPHA_RGBA16F, h.data(), h.size() * 2);
         if (st != GS_OK) {
             log_error("read appearance image", st);
             return false;
@@ -380,9 +421,7 @@ bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
     }
     size_t px = (size_t)r->stochastic_width * r->stochastic_height;
     if (floats != px * 4) return false;
-    int which = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
-    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) which = r->stochastic_display_image;
-    int st = gs_read_image(r->ctx, which, out, floats * 4);
+    int st = gs_read_image(ctx, which, out, floats * 4);
     if (st != GS_OK) {
         log_error("read stochastic image", st);
         return false;
@@ -393,19 +432,14 @@ bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
 bool renderer_read_display(Renderer* r, const GaussianScene* scene, float* out, size_t floats) {
     const GSAnim* an = scene ? scene->animation : nullptr;
     if (!an || an->version != 3 || !an->lut_size) return renderer_read_linear(r, out, floats);
-    if (!r->ctx || !out) return false;
-    int source = GS_IMAGE_ALPHA_RGBA16F;
-    size_t px = (size_t)r->appearance_width * r->appearance_height;
-    if (r->splat_render_mode == SplatRenderMode::StochasticSplats) {
-        source = r->stochastic_display_image;
-        px = (size_t)r->stochastic_width * r->stochastic_height;
-    } else if (r->splat_render_mode == SplatRenderMode::GaussianPointSplatting) {
-        source = r->stochastic_accumulation_enabled ? GS_IMAGE_ACCUM_RGBA32F : GS_IMAGE_GPS_SAMPLE_RGBA32F;
-        px = (size_t)r->stochastic_width * r->stochastic_height;
-    }
+    gs_ctx* ctx = state(r).ctx;
+    if (!ctx || !out) return false;
+    const int source = linear_image(r);
+    const size_t px = source == GS_IMAGE_ALPHA_RGBA16F ? (size_t)r->appearance_width * r->appearance_height
+                                                       : (size_t)r->stochastic_width * r->stochastic_height;
     if (floats != px * 4) return false;
-    int st = gs_display_layers(r->ctx, source, r->exposure_ev);
-    if (st == GS_OK) st = gs_read_image(r->ctx, GS_IMAGE_DISPLAY_RGBA32F, out, floats * 4);
+    int st = gs_display_layers(ctx, source, r->exposure_ev);
+    if (st == GS_OK) st = gs_read_image(ctx, GS_IMAGE_DISPLAY_RGBA32F, out, floats * 4);
     if (st != GS_OK) {
         log_error("layer display", st);
         return false;
@@ -414,6 +448,7 @@ bool renderer_read_display(Renderer* r, const GaussianScene* scene, float* out, 
 }
 
 void renderer_destroy(Renderer* r) {
-    if (r->ctx) gs_destroy(r->ctx);
-    r->ctx = nullptr;
+    gs_ctx* ctx = state(r).ctx;
+    if (ctx) gs_destroy(ctx);
+    drop_state(r);
 }

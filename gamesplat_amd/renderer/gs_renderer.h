@@ -1,16 +1,19 @@
 // gs_renderer.h - the splat path of gamesplat's renderer.h (src/renderer.h:249-266)
-// implemented over the gs_* C ABI, for headless use.
+// implemented over the gs_* C ABI.
 //
-// Function names, argument meaning and return conventions are the
-// reference's; the Renderer struct keeps the fields src/main.cpp and the
-// harness read or write (src/main.cpp:341-470,1344-1470,
-// tests/gsanim_render.cpp:46,97-113,148-156,186).  sokol handles
-// (pipelines, images, views) are replaced by one gs_ctx; mesh, overlay and
-// ImGui passes are not part of this build.
-//
-// When compiled inside the reference tree, define GS_RENDERER_USE_REFERENCE_TYPES
-// and include the reference's gaussian.h / camera.h first: the data types
-// below are then taken from there (they are byte-identical).
+// Two builds of one source (gs_renderer.cpp):
+//   - in the reference tree (-DGS_RENDERER_IN_TREE, with the reference's
+//     src/ and vendor/ on the include path): every type and every renderer_*
+//     declaration comes from the reference's own renderer.h, gaussian.h,
+//     camera.h and gsanim.h, so src/main.cpp and the harness
+//     (tests/gsanim_render.cpp) compile against their own Renderer; the gs
+//     context of each Renderer lives in a side table, not in the struct;
+//   - standalone (default, used by gs_headless): the same types are declared
+//     here with the reference's names and layouts, keeping the Renderer
+//     fields src/main.cpp and the harness read or write
+//     (src/main.cpp:341-470,1344-1470, tests/gsanim_render.cpp:46,97-113,
+//     148-156,186); sokol handles are left out.
+// Mesh, overlay and ImGui passes are not part of this build in either mode.
 #pragma once
 
 #include <stddef.h>
@@ -22,7 +25,10 @@
 
 #include "../../include/gs_api.h"
 
-#ifndef GS_RENDERER_USE_REFERENCE_TYPES
+#ifdef GS_RENDERER_IN_TREE
+#include "gsanim.h"
+#include "renderer.h"
+#else
 #define GAUSSIAN_SH_REST_FLOATS 45
 struct Gaussian {
     float position[3];
@@ -104,13 +110,19 @@ struct SplatEffectParams {
     float params[4];
     float color[4];
 };
-struct OverlayParams;
-struct NodeRenderParams;
-#endif
-
-static_assert(sizeof(Gaussian) == sizeof(GsGaussian), "Gaussian layout");
-static_assert(sizeof(CameraUniforms) == sizeof(GsCameraUniforms), "CameraUniforms layout");
-static_assert(sizeof(SplatEffectParams) == sizeof(GsSplatEffectParams), "SplatEffectParams layout");
+// Passed through renderer_draw_frame and not read by the splat path
+// (src/renderer.h:7-26): the overlay's sokol texture handles are left out.
+struct OverlayParams {
+    float camera_ray_basis[16];
+    float camera_tan_half_fov[2];
+    float ref_rotation[16];
+    float alpha;
+};
+struct NodeRenderParams {
+    const float* positions;
+    uint32_t count;
+    float half_size;
+};
 
 struct SplatDiagnostics {
     uint32_t total_quad_kpix;
@@ -151,7 +163,6 @@ struct GaussianPointSplatGpu {
 
 struct Renderer {
     SplatRenderMode splat_render_mode;
-    gs_ctx* ctx;                        // replaces every sokol resource of the splat path
     int appearance_width, appearance_height;
     float exposure_ev;
     bool first_layer_only;
@@ -172,7 +183,6 @@ struct Renderer {
     GaussianPointSplatGpu gps_gpu;
     uint32_t stochastic_width;
     uint32_t stochastic_height;
-    int stochastic_display_image;    // StochasticSplats: the GS_IMAGE_* shown (TAA history, running mean or sample)
     uint32_t stochastic_sample_count;
     uint32_t stochastic_samples_per_frame;
     uint32_t stochastic_taa_current_samples;
@@ -187,7 +197,6 @@ struct Renderer {
     bool stochastic_taa_prev_view_proj_valid;
     bool stochastic_accumulation_enabled;
     bool stochastic_taa_enabled;
-    int last_frame_status;              // gs status of the last renderer_draw_frame (0 = ok)
 };
 
 bool renderer_init(Renderer* r);
@@ -199,11 +208,27 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
                          const SplatEffectParams* splat_effect = NULL, float wireframe_occlusion = 1.0f,
                          const CameraUniforms* map_cam = NULL);
 void renderer_destroy(Renderer* r);
+#endif  // GS_RENDERER_IN_TREE
+
+static_assert(sizeof(Gaussian) == sizeof(GsGaussian), "Gaussian layout");
+static_assert(sizeof(CameraUniforms) == sizeof(GsCameraUniforms), "CameraUniforms layout");
+static_assert(sizeof(SplatEffectParams) == sizeof(GsSplatEffectParams), "SplatEffectParams layout");
+static_assert(sizeof(SplatDiagnostics) >= 8 * sizeof(uint32_t), "SplatDiagnostics layout");
+
+// The gs context renderer_init created for r (NULL before, or after
+// renderer_destroy), and the gs status of its last renderer_draw_frame
+// (GS_OK when it succeeded).
+gs_ctx* renderer_gs_context(const Renderer* r);
+int renderer_last_status(const Renderer* r);
 
 // Headless readback of what the reference harness captures with
-// linear_capture (tests/gsanim_render.cpp:144-165): raw float32 RGBA, top row
-// first, of the appearance image (sorted alpha) or the accumulation / GPS
-// sample image (stochastic modes).  out holds width*height*4 floats.
+// linear_capture (tests/gsanim_render.cpp:144-165), chosen by its rule: the
+// appearance image in sorted-alpha mode (GS_IMAGE_ALPHA_RGBA16F as floats);
+// with accumulation on (or StochasticSplats with TAA) the accumulation image
+// (GS_IMAGE_TAA_RGBA32F for TAA, else GS_IMAGE_ACCUM_RGBA32F); otherwise the
+// GPS resolved image or the StochasticSplats sample image (both
+// GS_IMAGE_GPS_SAMPLE_RGBA32F).  Raw float32 RGBA, top row first; out holds
+// width*height*4 floats.
 bool renderer_read_linear(Renderer* r, float* out, size_t floats);
 
 // What renderer_display_appearance shows (src/renderer.cpp:2120-2137): for a
