@@ -1,0 +1,139 @@
+"""Parity at the full benchmark configurations (C1 and C2 are covered with the
+composite in test_gpu_reference_blend.py):
+
+- C3: 6M Gaussians (256 clusters, opacity logit mean 1), 1920x1080, GPS at
+  1 spp with a 16 Mi work budget: requested and dropped work and every
+  depth|payload cell bit-exact against the oracle for one frame; 256
+  accumulated frames whose running mean equals the oracle's accumulation of
+  the device's per-frame samples bit for bit, with frames 1, 128 and 256
+  themselves bit-exact against the oracle's splat and resolve.
+- C4: the U-Net's 1920x1080 inference of the C3 frame against the oracle's
+  forward on crops whose interior lies beyond the network's receptive field
+  from the crop border (and on a crop at the image corner).
+- C5: one resident 3M scene, 64 cameras of mixed sizes in one batched frame:
+  four views equal their single-view frames and one view's depth order is
+  the oracle's.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gamesplat_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+BUDGET = 16 * 1024 * 1024
+WEIGHTS = Path(__file__).resolve().parents[1] / "gamesplat_amd" / "weights" / "unet_denoiser.f32"
+
+
+@pytest.fixture(scope="module")
+def c3(gs):
+    g, cfg = scenes.scene_config("c3")
+    cam = scenes.random_view_camera(cfg["seed"], cfg["width"], cfg["height"])
+    return g, cfg, cam
+
+
+def test_c3_gps_frame_and_256_frame_mean(gs, orc, c3):
+    g, cfg, cam = c3
+    W, H = cfg["width"], cfg["height"]
+    print(f"case: c3 gps n={len(g)} {W}x{H} budget {BUDGET}", flush=True)
+    keys, ids, proj, vis, _ = orc.cull(g, cam)
+    prepared, work, requested, overflow = orc.gps_expand(proj, ids, BUDGET, 1.0, 0.0)
+    with gs.Context(0) as ctx:
+        ctx.upload(g)
+        ctx.set_gps_budget(BUDGET)
+        ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 1)
+        ctx.cull(cam)
+        st = ctx.stats()
+        assert st["visible_count"] == vis
+        history = None
+        for f in range(1, 257):
+            ctx.gps_sample(frame_seed=f, supersample=1)
+            ctx.accumulate(f)
+            sample = ctx.read_image(abi.GS_IMAGE_GPS_SAMPLE_RGBA32F)
+            if f in (1, 128, 256):
+                cells = orc.gps_splat(prepared, work, min(requested, BUDGET), W, H, f, 1.0)
+                if f == 1:
+                    st = ctx.stats()
+                    print(f"requested {st['gps_requested_work_items']} dropped {st['gps_dropped_work_items']}",
+                          flush=True)
+                    assert st["gps_requested_work_items"] == requested
+                    assert st["gps_work_budget_exhausted"] == overflow == 1
+                    assert st["gps_dropped_work_items"] == requested - BUDGET
+                    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * W,)), cells)
+                color, _ = orc.gps_resolve(cells, W, H, 1, 0.0)
+                np.testing.assert_array_equal(sample, color[::-1], err_msg=f"frame {f}")
+                print(f"frame {f}: sample bit-exact", flush=True)
+            history = sample.copy() if history is None else orc.accumulate(history, sample, f)
+        acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
+        ctx.render_alpha(cam)
+        alpha = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+    np.testing.assert_array_equal(acc, history)
+    mse = float(np.mean((acc.astype(np.float64) - alpha) ** 2))
+    print(f"256-frame GPS mean vs sorted alpha: {10 * np.log10(1 / mse):.2f} dB", flush=True)
+
+
+def test_c4_denoiser_inference_1080p_vs_oracle_crops(gs, orc, c3):
+    from oracle import unet
+
+    g, cfg, cam = c3
+    W, H = cfg["width"], cfg["height"]
+    print(f"case: c4 U-Net inference {W}x{H} vs oracle crops", flush=True)
+    weights = np.fromfile(str(WEIGHTS), dtype=np.float32)
+    with gs.Context(0) as ctx, gs.Denoiser(ctx, seed=1) as d:
+        d.set_params(weights)
+        ctx.upload(g)
+        ctx.set_gps_budget(BUDGET)
+        ctx.cull(cam)
+        ctx.gps_sample(frame_seed=9, supersample=1)
+        ctx.accumulate(1)
+        acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
+        dep = ctx.read_image(abi.GS_IMAGE_GPS_DEPTH_R32F)
+        d.denoise()
+        out = d.read_image(W, H).astype(np.float64)[::-1]  # device row order
+    # the network sees the device's row order (GL: bottom row first)
+    inp = np.zeros((H, W, 8), np.float32)
+    inp[..., :4] = acc[::-1]
+    inp[..., 4] = dep[::-1]
+    # receptive field: two 3x3 convolutions per level at strides 1, 2, 4, 8 on
+    # the way down and 4, 2, 1 on the way up, plus the pooling footprint:
+    # under 64 pixels, so a crop's interior 64 pixels in from its border sees
+    # exactly what the full image does (crop origins on the 8-pixel pool grid)
+    margin, size = 64, 192
+    crops = [(256, 512, True), (600, 1400, True), (0, 0, False)]
+    for y0, x0, interior in crops:
+        x = inp[None, y0:y0 + size, x0:x0 + size]
+        _, rout, _ = unet.forward_backward(weights, x, np.zeros((1, size, size, 4), np.float32), want_grad=False)
+        if interior:
+            ref = rout[0, margin:size - margin, margin:size - margin]
+            got = out[y0 + margin:y0 + size - margin, x0 + margin:x0 + size - margin]
+        else:  # at the image corner the zero padding is the image's own
+            ref = rout[0, :size - margin, :size - margin]
+            got = out[:size - margin, :size - margin]
+        err = float(np.abs(got - ref).max())
+        print(f"crop ({y0}, {x0}): max abs err vs oracle {err:.3e}", flush=True)
+        assert err < 2e-2
+
+
+def test_c5_batched_views_equal_single_views(gs, orc):
+    g, cfg = scenes.scene_config("c5")
+    cams = scenes.multiview_cameras(cfg["seed"])
+    picks = [3, 21, 40, 63]  # one of each size class
+    print(f"case: c5 batched 64 views, n={len(g)}", flush=True)
+    with gs.Context(0) as ctx:
+        ctx.upload(g)
+        ctx.render_views(cams)
+        batched = {v: ctx.read_view_image(v).view(np.uint16).copy() for v in picks}
+        fi = ctx.frame_info()
+        assert fi["views"] == 64
+        for v in picks:
+            ctx.render_alpha(cams[v])
+            single = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16)
+            np.testing.assert_array_equal(batched[v], single, err_msg=f"view {v}")
+            assert single.any()
+            if v == 63:
+                ids = ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS)
+    keys, oids, proj, vis, _ = orc.cull(g, cams[63])
+    sk, si = orc.radix_sort(keys, oids)
+    np.testing.assert_array_equal(ids, si)
