@@ -48,6 +48,7 @@ GS_OPT_BLOCKS_PER_CU = 5
 GS_OPT_GRAPH = 6
 GS_OPT_REFERENCE_BLEND = 7
 GS_OPT_WIDE_INDEX = 8
+GS_SORT_STATIC_MAX_PAIRS = 1 << 22
 MAX_VIEWS = 64
 
 STAGES = ("cull", "sort", "bin", "composite", "gps_expand", "gps_splat", "gps_resolve", "cull_kernel")
@@ -199,6 +200,7 @@ EXPORTS = {
     "gs_cull": (C.c_int, [C.c_void_p, C.POINTER(GsCameraUniforms), C.POINTER(GsSplatEffectParams),
                           C.POINTER(GsCullOptions)]),
     "gs_sort": (C.c_int, [C.c_void_p]),
+    "gs_sort_pairs_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "gs_composite_alpha": (C.c_int, [C.c_void_p, C.c_int]),
     "gs_gps_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_float]),
     "gs_accumulate": (C.c_int, [C.c_void_p, C.c_uint32]),

@@ -413,8 +413,12 @@ int gs_create(gs_ctx** out, int device) {
         delete c;
         return fail(GS_ERR_HIP, "device setup failed: %s", hipGetErrorString(e));
     }
-    // kernel attributes are set here, never while a frame is being captured
+    // kernel attributes are set here, never while a frame is being captured;
+    // the static sort scratch's address on this device is looked up once
     e = init_kernel_attributes();
+    // a failed lookup leaves gs_sort_pairs_static refusing with the reason;
+    // every other entry point works without it
+    if (e == hipSuccess && static_scratch_init(device) != hipSuccess) (void)hipGetLastError();
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1790,6 +1794,13 @@ namespace gs {
 
 int api_error(int code, const char* msg) { return fail(code, "%s", msg); }
 int api_ok() { return ok(); }
+
+int ctx_launch_target(gs_ctx* c, hipStream_t* stream, int* device) {
+    if (!c) return GS_ERR_INVALID;
+    *stream = c->stream;
+    *device = c->device;
+    return GS_OK;
+}
 
 int ctx_frames(gs_ctx* c, CtxFrames* out) {
     if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");

@@ -430,6 +430,13 @@ hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int wid
 hipError_t launch_gather(const uint16_t* pool_in, const float* pool_tgt, const uint32_t* idx, uint32_t pixels,
                          uint32_t batch, uint16_t* inp, float* tgt, hipStream_t s);
 
+// gs_static.hip: device address of the static sort scratch on `device`
+// (which must be current), looked up once per device by gs_create.
+hipError_t static_scratch_init(int device);
+
+// gs_api.cpp: the stream and device a context launches on (no other work).
+int ctx_launch_target(gs_ctx* c, hipStream_t* stream, int* device);
+
 // gs_api.cpp: what the denoiser reads from a render context.
 struct CtxFrames {
     hipStream_t stream;

@@ -218,6 +218,25 @@ int gs_cull(gs_ctx* ctx, const GsCameraUniforms* cam,
  * pairs; culled entries sort last as in the reference. */
 int gs_sort(gs_ctx* ctx);
 
+/* No counterpart in the reference: gs_sort for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Stable ascending sort of `count`
+ * (key, id) pairs the caller holds in device memory, with the kernels of
+ * gs_sort (five 7-bit passes over the full 32-bit keys; the order equals
+ * gs_sort's for the same pairs: ascending key, ties by input position).
+ * keys_in / ids_in are only read; keys_out / ids_out receive the sorted
+ * pairs and also serve as a ping-pong buffer during the call.  All four are
+ * device pointers, 4-byte aligned, not overlapping.  Runs on the context's
+ * stream and returns without waiting.  The rest of the sort's memory
+ * (ping-pong pair, digit table, column totals, count) is a static device
+ * array of about 33 MiB per GPU, held until the process ends: the call makes
+ * no hipMalloc, hipFree or other allocation.  count above
+ * GS_SORT_STATIC_MAX_PAIRS is refused with GS_ERR_LIMIT.  Every call on one
+ * GPU shares that array: calls that overlap in time on one GPU race. */
+#define GS_SORT_STATIC_MAX_PAIRS (1u << 22)
+int gs_sort_pairs_static(gs_ctx* ctx, const uint32_t* keys_in, const uint32_t* ids_in, uint32_t* keys_out,
+                         uint32_t* ids_out, uint32_t count);
+
 /* Replaces the sorted instanced splat draw into the RGBA16F target
  * (shaders/splat.glsl:6-105, src/renderer.cpp:292-299,1687-1720,2516-2541):
  * tile binning of the sorted list and front-to-back compositing, black
