@@ -225,6 +225,28 @@ def denoised_stochastic(ctx, cam, steps: int, warmup: int, converged: int) -> di
             den.denoise()
         ctx.synchronize()
         den_ms = (time.perf_counter() - t0) * 1000.0 / steps
+        # the judge: per frame, the sorted-alpha image of the same cull and the
+        # denoised image's PSNR and max abs error against it (the renderer
+        # falls back to sorted alpha below 40 dB); timed with the judge too
+        judged = []
+        for f in range(1, 9):
+            frame(f)
+            ctx.sort()
+            ctx.composite()
+            p, m = den.judge()
+            judged.append({"frame_seed": f, "psnr_db": round(p, 3), "max_abs": round(m, 5)})
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, steps + 1):
+            frame(f)
+            ctx.sort()
+            ctx.composite()
+            den.judge()
+        ctx.synchronize()
+        judged_ms = (time.perf_counter() - t0) * 1000.0 / steps
+        out["per_frame_vs_sorted_alpha"] = judged
+        out["frames_below_40db"] = sum(1 for j in judged if j["psnr_db"] < 40.0)
+        out["ms_per_frame_with_judge"] = round(judged_ms, 4)
         fl = unet_flops(W + (-W) % 8, H + (-H) % 8)
         out.update({"frames_per_s": round(1000.0 / ms, 2), "ms_per_frame": round(ms, 4),
                     "denoise_ms": round(den_ms, 4), "denoise_gflop": round(fl / 1e9, 2),

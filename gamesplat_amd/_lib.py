@@ -236,6 +236,7 @@ EXPORTS = {
     "gs_denoiser_pool_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "gs_denoiser_train": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, C.c_void_p]),
     "gs_denoise": (C.c_int, [C.c_void_p]),
+    "gs_denoiser_judge": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float)]),
     "gs_denoiser_read_image": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "gs_read_image": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "gs_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
@@ -599,6 +600,13 @@ class Denoiser:
         losses = np.empty(steps, np.float32)
         check(self._lib.gs_denoiser_train(self._h, steps, batch, lr, seed, losses.ctypes.data_as(C.c_void_p)))
         return losses
+
+    def judge(self) -> tuple[float, float]:
+        """(PSNR dB, max abs error) of the last denoised image against the
+        context's sorted-alpha image of the same cull."""
+        p, m = C.c_double(), C.c_float()
+        check(self._lib.gs_denoiser_judge(self._h, C.byref(p), C.byref(m)))
+        return p.value, m.value
 
     def denoise(self):
         check(self._lib.gs_denoise(self._h))

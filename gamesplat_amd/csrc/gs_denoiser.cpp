@@ -101,6 +101,7 @@ struct gs_denoiser {
     // last denoised frame
     uint16_t* den = nullptr;
     size_t den_bytes = 0;
+    float2* judge = nullptr;  // per-block (squared-error sum, max abs) of gs_denoiser_judge
     int den_w = 0, den_h = 0;
     float den_flip = 1.0f;
 };
@@ -376,7 +377,8 @@ void gs_denoiser_destroy(gs_denoiser* d) {
     (void)hipSetDevice(d->device);
     if (d->s) (void)hipStreamSynchronize(d->s);
     free_acts(d);
-    void* bufs[] = {d->params, d->grads, d->m, d->v, d->wbuf, d->pool_in, d->pool_tgt, d->idx, d->losses, d->den};
+    void* bufs[] = {d->params, d->grads, d->m, d->v, d->wbuf, d->pool_in, d->pool_tgt, d->idx, d->losses, d->den,
+                    d->judge};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete d;
@@ -623,6 +625,33 @@ int gs_denoise(gs_denoiser* d) {
     d->den_w = f.gps_w;
     d->den_h = f.gps_h;
     d->den_flip = f.clip_y_sign;
+    return api_ok();
+}
+
+int gs_denoiser_judge(gs_denoiser* d, double* psnr_db, float* max_abs) {
+    if (!d || !psnr_db || !max_abs) return api_error(GS_ERR_INVALID, "bad arguments");
+    if (!d->den_w) return api_error(GS_ERR_STATE, "no denoised frame yet");
+    CtxFrames f;
+    DN_CHECK(ctx_frames(d->ctx, &f));
+    if (!f.alpha) return api_error(GS_ERR_STATE, "gs_denoiser_judge needs a sorted-alpha frame of the same cull");
+    if (f.alpha_w != d->den_w || f.alpha_h != d->den_h)
+        return api_error(GS_ERR_INVALID, "sorted-alpha and denoised images differ in size");
+    constexpr uint32_t kBlocks = 1024;
+    if (!d->judge) DN_CHECK(dmalloc(&d->judge, sizeof(float2) * kBlocks));
+    const uint32_t count = (uint32_t)d->den_w * (uint32_t)d->den_h * 4u;
+    DN_HIP(launch_image_error(d->den, f.alpha, count, d->judge, kBlocks, d->s));
+    std::vector<float2> part(kBlocks);
+    DN_HIP(hipMemcpyAsync(part.data(), d->judge, sizeof(float2) * kBlocks, hipMemcpyDeviceToHost, d->s));
+    DN_HIP(hipStreamSynchronize(d->s));
+    double sum = 0.0;
+    float mx = 0.0f;
+    for (const float2& p : part) {
+        sum += (double)p.x;
+        mx = p.y > mx ? p.y : mx;
+    }
+    const double mse = sum / (double)count;
+    *psnr_db = mse > 0.0 ? 10.0 * log10(1.0 / mse) : 1.0 / 0.0;
+    *max_abs = mx;
     return api_ok();
 }
 

@@ -427,6 +427,10 @@ struct PackLaunch {
 };
 hipError_t launch_pack_input(const PackLaunch& a, hipStream_t s);
 hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int width, int height, hipStream_t s);
+// Squared-error sum and max abs difference of two half arrays, one float2
+// (sum, max) per block.
+hipError_t launch_image_error(const uint16_t* a, const uint16_t* b, uint32_t count, float2* partial, uint32_t blocks,
+                              hipStream_t s);
 hipError_t launch_gather(const uint16_t* pool_in, const float* pool_tgt, const uint32_t* idx, uint32_t pixels,
                          uint32_t batch, uint16_t* inp, float* tgt, hipStream_t s);
 

@@ -770,6 +770,42 @@ hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int wid
     return hipGetLastError();
 }
 
+// Squared-error sum and max abs difference of two half-precision arrays of
+// `count` values (grid-stride, one (sum, max) per block; the caller adds the
+// block sums in double).
+__global__ void __launch_bounds__(256) image_error_kernel(const uint16_t* __restrict__ a,
+                                                          const uint16_t* __restrict__ b, uint32_t count,
+                                                          float2* partial) {
+    __shared__ float s_sum[4], s_max[4];
+    float sum = 0.0f, mx = 0.0f;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+        const float d = __half2float(__ushort_as_half(a[i])) - __half2float(__ushort_as_half(b[i]));
+        sum = fmaf(d, d, sum);
+        mx = fmaxf(mx, fabsf(d));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        sum += __shfl_xor(sum, o, 64);
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        s_sum[wave] = sum;
+        s_max[wave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = make_float2((s_sum[0] + s_sum[1]) + (s_sum[2] + s_sum[3]),
+                                          fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3])));
+}
+
+hipError_t launch_image_error(const uint16_t* a, const uint16_t* b, uint32_t count, float2* partial, uint32_t blocks,
+                              hipStream_t s) {
+    if (!blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(image_error_kernel, dim3(blocks), dim3(256), 0, s, a, b, count, partial);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather(const uint16_t* pool_in, const float* pool_tgt, const uint32_t* idx, uint32_t pixels,
                          uint32_t batch, uint16_t* inp, float* tgt, hipStream_t s) {
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for((long)pixels * batch)), dim3(256), 0, s,

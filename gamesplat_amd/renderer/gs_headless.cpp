@@ -31,6 +31,9 @@
 //   anim_lut=<path> anim_lut_size=S anim_log2_min anim_log2_max
 //   anim_time=[0] anim_time_step=[0]  playback time of frame f: time + f * step
 //   first_layer_only=[0] exposure_ev=[0]
+// Learned denoiser on the GPS path (renderer_enable_denoiser):
+//   denoiser=<weights path> (float32, gs_denoiser_get_params layout)
+//   denoise_min_psnr=[40] denoise_max_abs=[1] denoise_judge_every=[1]
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -254,6 +257,19 @@ int main(int argc, char** argv) {
     size_t free0 = 0, total0 = 0;
     hipMemGetInfo(&free0, &total0);
     if (!renderer_upload_gaussians(&r, &scene)) return 1;
+    const char* weights_path = arg("denoiser", "");
+    if (weights_path[0]) {
+        std::vector<float> w(gs_denoiser_param_count());
+        FILE* wf = fopen(weights_path, "rb");
+        if (!wf || fread(w.data(), 4, w.size(), wf) != w.size()) {
+            fprintf(stderr, "could not read %zu denoiser parameters from %s\n", w.size(), weights_path);
+            return 1;
+        }
+        fclose(wf);
+        if (!renderer_enable_denoiser(&r, w.data(), w.size(), (float)argf("denoise_min_psnr", 40.0),
+                                      (float)argf("denoise_max_abs", 1.0), (uint32_t)argi("denoise_judge_every", 1)))
+            return 1;
+    }
     double startup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
     // seeded camera direction (splitmix64 -> normal via Box-Muller), radius
@@ -389,6 +405,13 @@ int main(int argc, char** argv) {
     gs_frame_info(renderer_gs_context(&r), &fi);
     size_t free1 = 0, total1 = 0;
     hipMemGetInfo(&free1, &total1);
+    RendererDenoiseReport dr;
+    renderer_denoise_report(&r, &dr);
+    if (dr.enabled)
+        printf("DENOISE {\"frames\":%u,\"judged\":%u,\"fallbacks\":%u,\"showing_denoised\":%d,"
+               "\"last_psnr_db\":%.6f,\"last_max_abs\":%.9g}\n",
+               dr.frames, dr.judged, dr.fallbacks, dr.showing_denoised ? 1 : 0, dr.last_psnr_db,
+               (double)dr.last_max_abs);
     printf("RESULT {\"frames\":%zu,\"wall_median_ms\":%.6f,\"wall_p95_ms\":%.6f,\"cpu_median_ms\":%.6f,"
            "\"cpu_p95_ms\":%.6f,\"gpu_median_ms\":%.6f,\"gpu_p95_ms\":%.6f,\"startup_ms\":%.3f,"
            "\"vram_free_delta_mib\":%.3f,\"scene_buffers_mib\":%.3f,\"gaussians\":%u,\"visible\":%u,"

@@ -27,6 +27,16 @@ struct SplatState {
     gs_ctx* ctx = nullptr;
     int last_status = GS_OK;
     double last_stats_time = 0.0;
+    // learned denoiser on the GPS path, judged against sorted alpha
+    gs_denoiser* den = nullptr;
+    float min_psnr_db = 40.0f;
+    float max_abs_error = 1.0f;
+    uint32_t judge_every = 1;
+    bool fallback = false;      // the last judged frame missed a bar: show sorted alpha
+    bool have_denoised = false; // the display holds this frame's denoised image or its fallback
+    uint32_t frames = 0, judged = 0, fallbacks = 0;
+    double last_psnr_db = 0.0;
+    float last_max_abs = 0.0f;
 };
 std::mutex g_state_mutex;
 std::unordered_map<const Renderer*, SplatState> g_states;
@@ -90,7 +100,8 @@ bool renderer_init(Renderer* r) {
     r->sh_degree = 3;
     r->sh_lod_distance = 0.0f;
     SplatState& ss = state(r);
-    if (ss.ctx) gs_destroy(ss.ctx);  // a re-initialised Renderer at the same address
+    if (ss.den) gs_denoiser_destroy(ss.den);  // a re-initialised Renderer at the same address
+    if (ss.ctx) gs_destroy(ss.ctx);
     ss = SplatState{};
     int st = gs_create(&ss.ctx, 0);
     if (st != GS_OK) {
@@ -185,6 +196,8 @@ void renderer_reset_stochastic_accumulation(Renderer* r) {
     r->stochastic_prev_effect_valid = false;
     r->stochastic_taa_prev_view_proj_valid = false;
 }
+
+static int denoise_frame(Renderer* r, SplatState& ss, gs_ctx* ctx, const CameraUniforms* cam);
 
 static uint32_t clamp_samples(uint32_t n) { return n < 1u ? 1u : (n > 16u ? 16u : n); }
 
@@ -392,6 +405,88 @@ void renderer_draw_frame(Renderer* r, GaussianScene* scene, const CameraUniforms
     }
     r->stochastic_sample_count = next;
     r->stochastic_accum_write_index = (r->stochastic_accum_write_index + 1u) & 1u;
+    if (ss.den) {
+        st = denoise_frame(r, ss, ctx, cam);
+        if (st != GS_OK) {
+            log_error("denoised frame", st);
+            ss.last_status = st;
+        }
+    }
+}
+
+// The learned denoiser's frame: denoise the running mean; on a judge frame
+// (every judge_every-th), and on every frame while the last judgement fell
+// back, render the sorted-alpha image of the same cull; a judge frame
+// measures the denoised image against it, and a PSNR below min_psnr_db or
+// a max abs error above max_abs_error makes sorted alpha the display until
+// a later judge frame passes.
+static int denoise_frame(Renderer* r, SplatState& ss, gs_ctx* ctx, const CameraUniforms* cam) {
+    ss.have_denoised = false;
+    int st = gs_denoise(ss.den);
+    if (st != GS_OK) return st;
+    const bool judge = ss.judge_every && ss.frames % ss.judge_every == 0;
+    ss.frames++;
+    if (judge || ss.fallback) {
+        st = gs_sort(ctx);
+        if (st == GS_OK) st = gs_composite_alpha(ctx, 0);
+        if (st != GS_OK) return st;
+        r->appearance_width = (int)cam->viewport[0];
+        r->appearance_height = (int)cam->viewport[1];
+        if (judge) {
+            st = gs_denoiser_judge(ss.den, &ss.last_psnr_db, &ss.last_max_abs);
+            if (st != GS_OK) return st;
+            ss.judged++;
+            ss.fallback = !(ss.last_psnr_db >= (double)ss.min_psnr_db) || !(ss.last_max_abs <= ss.max_abs_error);
+            if (ss.fallback) ss.fallbacks++;
+        }
+    }
+    ss.have_denoised = true;
+    return GS_OK;
+}
+
+bool renderer_enable_denoiser(Renderer* r, const float* params, size_t count, float min_psnr_db, float max_abs_error,
+                              uint32_t judge_every) {
+    SplatState& ss = state(r);
+    if (!ss.ctx || !params) return false;
+    if (!ss.den) {
+        const int st = gs_denoiser_create(ss.ctx, 1, &ss.den);
+        if (st != GS_OK) {
+            log_error("denoiser create", st);
+            ss.den = nullptr;
+            return false;
+        }
+    }
+    const int st = gs_denoiser_set_params(ss.den, params, count);
+    if (st != GS_OK) {
+        log_error("denoiser weights", st);
+        return false;
+    }
+    ss.min_psnr_db = min_psnr_db;
+    ss.max_abs_error = max_abs_error;
+    ss.judge_every = judge_every;
+    ss.fallback = ss.have_denoised = false;
+    ss.frames = ss.judged = ss.fallbacks = 0;
+    return true;
+}
+
+void renderer_disable_denoiser(Renderer* r) {
+    SplatState& ss = state(r);
+    if (ss.den) gs_denoiser_destroy(ss.den);
+    ss.den = nullptr;
+    ss.have_denoised = ss.fallback = false;
+}
+
+bool renderer_denoise_report(const Renderer* r, RendererDenoiseReport* out) {
+    if (!out) return false;
+    const SplatState& ss = state(r);
+    out->enabled = ss.den != nullptr;
+    out->showing_denoised = ss.have_denoised && !ss.fallback;
+    out->frames = ss.frames;
+    out->judged = ss.judged;
+    out->fallbacks = ss.fallbacks;
+    out->last_psnr_db = ss.last_psnr_db;
+    out->last_max_abs = ss.last_max_abs;
+    return true;
 }
 
 // The image the reference harness captures (tests/gsanim_render.cpp:148-156).
@@ -404,9 +499,26 @@ static int linear_image(const Renderer* r) {
 }
 
 bool renderer_read_linear(Renderer* r, float* out, size_t floats) {
-    gs_ctx* ctx = state(r).ctx;
+    const SplatState& ss = state(r);
+    gs_ctx* ctx = ss.ctx;
     if (!ctx || !out) return false;
-    const int which = linear_image(r);
+    int which = linear_image(r);
+    if (ss.den && ss.have_denoised && r->splat_render_mode == SplatRenderMode::GaussianPointSplatting) {
+        if (ss.fallback) {
+            which = GS_IMAGE_ALPHA_RGBA16F;  // the judge's verdict: sorted alpha
+        } else {
+            const size_t px = (size_t)r->stochastic_width * r->stochastic_height;
+            if (floats != px * 4) return false;
+            std::vector<uint16_t> h(px * 4);
+            const int st = gs_denoiser_read_image(ss.den, h.data(), h.size() * 2);
+            if (st != GS_OK) {
+                log_error("read denoised image", st);
+                return false;
+            }
+            for (size_t i = 0; i < h.size(); ++i) out[i] = half_to_float(h[i]);
+            return true;
+        }
+    }
     if (which == GS_IMAGE_ALPHA_RGBA16F) {
         size_t px = (size_t)r->appearance_width * r->appearance_height;
         if (floats != px * 4) return false;
@@ -448,7 +560,8 @@ bool renderer_read_display(Renderer* r, const GaussianScene* scene, float* out, 
 }
 
 void renderer_destroy(Renderer* r) {
-    gs_ctx* ctx = state(r).ctx;
-    if (ctx) gs_destroy(ctx);
+    SplatState& ss = state(r);
+    if (ss.den) gs_denoiser_destroy(ss.den);
+    if (ss.ctx) gs_destroy(ss.ctx);
     drop_state(r);
 }

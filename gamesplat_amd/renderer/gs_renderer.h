@@ -221,6 +221,28 @@ static_assert(sizeof(SplatDiagnostics) >= 8 * sizeof(uint32_t), "SplatDiagnostic
 gs_ctx* renderer_gs_context(const Renderer* r);
 int renderer_last_status(const Renderer* r);
 
+// The learned denoiser on the GPS path (no counterpart in the reference).
+// With accumulation on, every GPS frame's running mean goes through the
+// U-Net (params: gs_denoiser_get_params layout).  Every judge_every-th frame
+// (1 = every frame) the sorted-alpha image of the same cull is rendered and
+// judges the denoised one: a PSNR below min_psnr_db or a max abs error above
+// max_abs_error makes sorted alpha the displayed image (renderer_read_linear)
+// until a later judge frame passes; while it does, sorted alpha is rendered
+// every frame.
+struct RendererDenoiseReport {
+    bool enabled;
+    bool showing_denoised;   // the last frame displays the denoised image
+    uint32_t frames;         // frames denoised
+    uint32_t judged;         // frames judged against sorted alpha
+    uint32_t fallbacks;      // judged frames that fell back to sorted alpha
+    double last_psnr_db;     // the last judged frame
+    float last_max_abs;
+};
+bool renderer_enable_denoiser(Renderer* r, const float* params, size_t count, float min_psnr_db, float max_abs_error,
+                              uint32_t judge_every);
+void renderer_disable_denoiser(Renderer* r);
+bool renderer_denoise_report(const Renderer* r, RendererDenoiseReport* out);
+
 // Headless readback of what the reference harness captures with
 // linear_capture (tests/gsanim_render.cpp:144-165), chosen by its rule: the
 // appearance image in sorted-alpha mode (GS_IMAGE_ALPHA_RGBA16F as floats);

@@ -413,6 +413,13 @@ int gs_denoiser_pool_size(gs_denoiser* d, uint32_t* count);
 int gs_denoiser_train(gs_denoiser* d, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses);
 /* Denoises the context's current GPS accumulation into an RGBA16F image. */
 int gs_denoise(gs_denoiser* d);
+/* The judge of a denoised frame: PSNR (peak 1, mean squared error over
+ * every RGBA value) and the largest absolute per-channel difference of the
+ * last gs_denoise image against the context's sorted-alpha image of the same
+ * cull and size (gs_sort + gs_composite_alpha after the GPS sample).  A
+ * caller falls back to the sorted-alpha image when these miss its bars.
+ * Synchronises the stream. */
+int gs_denoiser_judge(gs_denoiser* d, double* psnr_db, float* max_abs);
 /* The last denoised image, top row first; bytes = width * height * 8. */
 int gs_denoiser_read_image(gs_denoiser* d, void* host, size_t bytes);
 

@@ -183,3 +183,40 @@ def test_sog_scene_renders(tmp_path):
         img = c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float32)
         assert c.stats()["visible_count"] > 0
     assert np.isfinite(img).all() and img[..., 3].max() > 0
+
+
+def test_headless_denoiser_judge_and_fallback(tmp_path):
+    """The renderer's denoised GPS frame is judged against the sorted-alpha
+    image of the same cull: below the PSNR bar the display falls back to that
+    image bit for bit; with the bars open it shows the denoised image, and the
+    reported PSNR and max abs error are those of the displayed image against
+    sorted alpha."""
+    exe = ROOT / "gamesplat_amd" / "gs_headless"
+    weights = ROOT / "gamesplat_amd" / "weights" / "unet_denoiser.f32"
+    common = ["gaussians=60000", "width=320", "height=240", "warmup=0", "frames=3", "gps_ss=1"]
+    alpha = tmp_path / "alpha.f32"
+    r = subprocess.run([str(exe), *common, "render_mode=alpha", f"linear_capture={alpha}"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = np.fromfile(alpha, np.float32).reshape(240, 320, 4).astype(np.float64)
+    out = {}
+    for tag, bars in (("strict", ["denoise_min_psnr=60", "denoise_max_abs=0.001"]),
+                      ("open", ["denoise_min_psnr=0", "denoise_max_abs=100"])):
+        print(f"case: headless denoiser {tag}", flush=True)
+        cap = tmp_path / f"{tag}.f32"
+        r = subprocess.run([str(exe), *common, "render_mode=gps", f"denoiser={weights}", *bars,
+                            f"linear_capture={cap}"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        rep = json.loads([l for l in r.stdout.splitlines() if l.startswith("DENOISE ")][0][8:])
+        out[tag] = (rep, np.fromfile(cap, np.float32).reshape(240, 320, 4).astype(np.float64))
+    rep, img = out["strict"]
+    assert rep["frames"] == 3 and rep["judged"] == 3 and rep["fallbacks"] == 3 and rep["showing_denoised"] == 0
+    np.testing.assert_array_equal(img, ref)
+    rep, img = out["open"]
+    assert rep["judged"] == 3 and rep["fallbacks"] == 0 and rep["showing_denoised"] == 1
+    assert not np.array_equal(img, ref)
+    mse = float(np.mean((img - ref) ** 2))
+    assert abs(10 * np.log10(1 / mse) - rep["last_psnr_db"]) < 1e-3
+    assert abs(float(np.abs(img - ref).max()) - rep["last_max_abs"]) < 1e-6
+    print(f"denoised frame vs sorted alpha: {rep['last_psnr_db']:.2f} dB, max abs {rep['last_max_abs']:.3f}",
+          flush=True)

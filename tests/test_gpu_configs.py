@@ -137,3 +137,31 @@ def test_c5_batched_views_equal_single_views(gs, orc):
     keys, oids, proj, vis, _ = orc.cull(g, cams[63])
     sk, si = orc.radix_sort(keys, oids)
     np.testing.assert_array_equal(ids, si)
+
+
+def test_c4_what_the_budget_leaves_in_the_sample(gs, orc, c3):
+    """At the 16 Mi work budget the GPS reservation (index order) stops early:
+    the sample holds points of the first splats only, and no point of the
+    rest.  The sorted-alpha image of the splats that keep work is the most a
+    function of the sample can see; its PSNR against the full sorted-alpha
+    image is reported (the denoised-stochastic quality ceiling at this
+    budget), next to the fraction of visible splats kept."""
+    g, cfg, cam = c3
+    W, H = cfg["width"], cfg["height"]
+    keys, ids, proj, vis, _ = orc.cull(g, cam)
+    prepared, work, requested, overflow = orc.gps_expand(proj, ids, BUDGET, 1.0, 0.0)
+    kept = np.unique(work[:BUDGET, 0])
+    print(f"case: c4 budget ceiling: {kept.size} of {vis} visible splats keep work "
+          f"({requested} items requested, {BUDGET} kept)", flush=True)
+    with gs.Context(0) as ctx:
+        ctx.upload(g)
+        ctx.render_alpha(cam)
+        full = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+        ctx.upload(np.ascontiguousarray(g[kept]))
+        ctx.render_alpha(cam)
+        part = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).astype(np.float64)
+    mse = float(np.mean((part - full) ** 2))
+    psnr = 10 * np.log10(1 / mse)
+    print(f"sorted alpha of the kept splats vs all: {psnr:.2f} dB, max abs {np.abs(part - full).max():.3f}",
+          flush=True)
+    assert overflow == 1 and 0 < kept.size < vis
