@@ -336,8 +336,10 @@ __device__ __forceinline__ void blend_step(uint32_t pxy, float fx, float fy, f32
     const float dx = fx - geo.x, dy = fy - geo.y;
     const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, fmaf(par.x * dy, dy, par.w));
     float alpha = minf(__builtin_amdgcn_exp2f(power), kAlphaCap255);
-    bool pass = power >= 0.0f;
-    if (__builtin_fabsf(power) < par.z) {
+    // power' >= margin: passes; below -margin: fails; between: the reference's test
+    const bool lo = power >= -par.z;
+    bool pass = lo;
+    if (lo & !(power >= par.z)) {
         const f32x4_t e = *exact;
         const float pr = -0.5f * (e.x * dx * dx + e.y * dx * dy + e.z * dy * dy);
         pass = !(pr > 0.0f) & !(pr < e.w);
