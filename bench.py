@@ -87,6 +87,48 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
+def upload_rates(ctx, g, repeats: int = 3) -> dict:
+    """Scene upload of the benchmark scene (host memory to the device planes),
+    best of `repeats`, each to a synchronised stream: Gaussians from pageable
+    memory (the library repacks them into page-locked plane chunks while the
+    previous chunk copies), planes from pageable memory (the same pipeline,
+    memcpy instead of repack), and planes from page-locked memory (one copy
+    per plane).  Rates are host bytes over wall time (236 bytes a Gaussian,
+    240 a plane entry)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from gamesplat_amd import abi
+
+    lib = abi.load()
+    n = g.shape[0]
+    planes = np.empty((15, n, 4), np.float32)
+    abi.check(lib.gs_planes_from_gaussians(g.ctypes.data, n, planes.ctypes.data))
+    pin = C.c_void_p()
+    abi.check(lib.gs_alloc_pinned(planes.nbytes, C.byref(pin)))
+    out = {}
+    try:
+        C.memmove(pin, planes.ctypes.data, planes.nbytes)
+        cases = (("gaussians_pageable", lambda: ctx.upload(g), g.nbytes),
+                 ("planes_pageable", lambda: abi.check(lib.gs_upload_planes(ctx.handle, planes.ctypes.data, n)),
+                  planes.nbytes),
+                 ("planes_pinned", lambda: abi.check(lib.gs_upload_planes(ctx.handle, pin, n)), planes.nbytes))
+        for name, fn, nbytes in cases:
+            best = float("inf")
+            for _ in range(repeats):
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                fn()
+                ctx.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            out[name] = {"ms": round(best * 1e3, 2), "host_GB/s": round(nbytes / best / 1e9, 2)}
+    finally:
+        lib.gs_free_pinned(pin)
+    ctx.n = n
+    return out
+
+
 def frame_times(ctx, frame, count: int):
     """Per-frame GPU times (ms) from HIP events recorded on the context's
     stream around each frame, frames issued back to back."""
@@ -392,6 +434,8 @@ def main():
     cam = scenes.random_view_camera(cfg["seed"], W, H)
     ctx = gs.Context(0)
     ctx.upload(g)
+    upload = upload_rates(ctx, g)
+    print(f"# upload: {json.dumps(upload)}", file=sys.stderr, flush=True)
     if args.graph:
         ctx.set_option(gs.abi.GS_OPT_GRAPH, 1)
     print(f"# {ctx.device_info()}", file=sys.stderr, flush=True)
@@ -590,6 +634,7 @@ def main():
         "tile_entries_per_s": round(D / (stage_ms["composite"] * 1e-3), 1) if stage_ms["composite"] else None,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items() if v},
         "reference_blend": reference_blend,
+        "upload": upload,
         "stochastic": stoch,
         "roofline": roofline,
         "cpu_baseline": cpu,

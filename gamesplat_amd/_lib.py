@@ -208,6 +208,13 @@ EXPORTS = {
     "gs_taa_jitter": (None, [C.c_uint32, C.POINTER(C.c_float)]),
     "gs_decode_ply": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(GsPlyLayout), C.c_void_p]),
     "gs_decode_sog": (C.c_int, [C.POINTER(GsSogData), C.c_void_p]),
+    "gs_planes_from_gaussians": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gs_decode_ply_planes": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(GsPlyLayout), C.c_void_p]),
+    "gs_upload_planes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "gs_upload_ply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(GsPlyLayout)]),
+    "gs_upload_sog": (C.c_int, [C.c_void_p, C.POINTER(GsSogData)]),
+    "gs_alloc_pinned": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "gs_free_pinned": (None, [C.c_void_p]),
     "gs_upload_animation": (C.c_int, [C.c_void_p, C.POINTER(GsAnimation)]),
     "gs_set_animation_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint32,
                                          C.POINTER(C.c_float)]),

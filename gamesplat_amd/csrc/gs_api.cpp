@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "gs_internal.h"
+#include "gs_planes.h"
 
 using namespace gs;
 
@@ -139,6 +140,11 @@ struct gs_ctx {
     int stoch_bins_mode = 0;
     float stoch_bins_jitter[2] = {0.0f, 0.0f};
     int img_w = 0, img_h = 0;
+
+    // page-locked chunk buffers of the pipelined uploads
+    float* up_pin[2] = {nullptr, nullptr};
+    hipEvent_t up_ev[2] = {nullptr, nullptr};
+    bool up_used[2] = {false, false};
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
     DevBuf work, prepared, item_count, item_base, cells, sample, depth, accum;
@@ -469,6 +475,10 @@ void gs_destroy(gs_ctx* c) {
     if (c->ring) (void)hipHostFree(c->ring);
     if (c->dup_event) (void)hipEventDestroy(c->dup_event);
     if (c->dup_host) (void)hipHostFree(c->dup_host);
+    for (int b = 0; b < 2; ++b) {
+        if (c->up_ev[b]) (void)hipEventDestroy(c->up_ev[b]);
+        if (c->up_pin[b]) (void)hipHostFree(c->up_pin[b]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -527,11 +537,15 @@ int gs_set_option(gs_ctx* c, int option, int value) {
     }
 }
 
-int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
-    if (!c || (!g && count)) return fail(GS_ERR_INVALID, "bad arguments");
+}  // extern "C"
+
+namespace {
+
+// Device planes for a scene of `count` Gaussians, after the memory-budget
+// checks of renderer_upload_gaussians (src/renderer.cpp:709-734).
+int reserve_planes(gs_ctx* c, uint32_t count, DevBuf* planes) {
     if (count == 0 || count > kMaxGaussians)
         return fail(GS_ERR_LIMIT, "gaussian count %u outside [1, %u]", count, kMaxGaussians);
-    GS_CHECK(set_device(c));
     // Scene-resident bytes per Gaussian: planes 240 + record 48 + keys/ids 24
     // + sorted tile rectangles 12 + dense key 4 + rectangles 16 + padded tile
     // rectangle 4 + GPS prepared 48 + item count/base 8, plus the GPS work list.
@@ -545,53 +559,35 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     if (c->budget && held_other + required > c->budget)
         return fail(GS_ERR_LIMIT, "scene of %u Gaussians needs %.1f MiB; memory budget is %.1f MiB", count,
                     required / 1048576.0, c->budget / 1048576.0);
+    const size_t bytes = (size_t)count * kPlanes * 16;
+    if (c->budget && c->held + bytes > c->budget) {
+        // the previous scene cannot stay resident next to the new one
+        release(c, c->planes);
+        c->n = 0;
+        c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GS_ERR_OOM, "hipMalloc(%zu) for Gaussian planes failed: %s", bytes, hipGetErrorString(e));
+    }
+    planes->p = p;
+    planes->bytes = bytes;
+    c->held += bytes;
+    return GS_OK;
+}
 
-    DevBuf planes;
-    {
-        size_t bytes = (size_t)count * kPlanes * 16;
-        if (c->budget && c->held + bytes > c->budget) {
-            // the previous scene cannot stay resident next to the new one
-            release(c, c->planes);
-            c->n = 0;
-            c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
-        }
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(GS_ERR_OOM, "hipMalloc(%zu) for Gaussian planes failed: %s", bytes, hipGetErrorString(e));
-        }
-        planes.p = p;
-        planes.bytes = bytes;
-        c->held += bytes;
-    }
-    DevBuf staging;
-    int r = ensure(c, staging, (size_t)std::min(count, kUploadChunk) * sizeof(GsGaussian), "upload staging");
-    if (r != GS_OK) {
-        release(c, planes);
-        return r;
-    }
-    for (uint32_t first = 0; first < count && r == GS_OK; first += kUploadChunk) {
-        uint32_t chunk = std::min(kUploadChunk, count - first);
-        hipError_t e = hipMemcpyAsync(staging.p, g + first, (size_t)chunk * sizeof(GsGaussian), hipMemcpyHostToDevice,
-                                      c->stream);
-        if (e == hipSuccess)
-            e = launch_transpose_upload(ptr<float>(staging), chunk, first, count, ptr<float4>(planes), c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) r = fail(GS_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
-    }
-    release(c, staging);
-    if (r != GS_OK) {
-        release(c, planes);
-        return r;
-    }
+// The new planes become the scene: per-view buffers are sized for it, and
+// every frame state and the animation of the previous scene are dropped.
+int install_planes(gs_ctx* c, DevBuf& planes, uint32_t count) {
     release(c, c->planes);
     c->planes = planes;
+    planes = DevBuf{};
     c->n = count;
     c->view_cap = 0;
     GS_CHECK(ensure_views(c, 1));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
-    // a new scene starts without animation
     release(c, c->anim_data);
     release(c, c->lut);
     c->anim_kind = 0;
@@ -600,7 +596,125 @@ int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
     c->anim_state_a = c->anim_state_b = 0;
     c->anim_weight = 0.0f;
     c->anim_layer_count = 0;
+    return GS_OK;
+}
+
+// Two page-locked chunk buffers in the plane layout (kUploadChunk entries,
+// plane stride = the chunk's entry count), kept for later uploads.
+int ensure_upload_staging(gs_ctx* c) {
+    if (c->up_pin[0]) return GS_OK;
+    for (int b = 0; b < 2; ++b) {
+        GS_HIP(hipHostMalloc((void**)&c->up_pin[b], (size_t)kUploadChunk * kPlanes * 16, hipHostMallocDefault));
+        GS_HIP(hipEventCreateWithFlags(&c->up_ev[b], hipEventDisableTiming));
+        c->up_used[b] = false;
+    }
+    return GS_OK;
+}
+
+// Pipelined upload: the host fills chunk k of the planes into one pinned
+// buffer (produce(first, n, planes) with plane stride n, on the host's cores)
+// while the copies of chunk k - 1 run from the other; each chunk is 15
+// asynchronous copies, one per plane, straight into the device planes.
+template <typename Produce>
+int upload_pipelined(gs_ctx* c, uint32_t count, Produce produce) {
+    GS_CHECK(set_device(c));
+    DevBuf planes;
+    GS_CHECK(reserve_planes(c, count, &planes));
+    int r = ensure_upload_staging(c);
+    for (uint32_t first = 0, k = 0; first < count && r == GS_OK; first += kUploadChunk, ++k) {
+        const uint32_t n = std::min(kUploadChunk, count - first);
+        const int b = (int)(k & 1u);
+        if (c->up_used[b] && hipEventSynchronize(c->up_ev[b]) != hipSuccess) {
+            r = fail(GS_ERR_HIP, "upload staging wait failed");
+            break;
+        }
+        r = produce(first, n, c->up_pin[b]);
+        if (r != GS_OK) {
+            r = fail(r, "scene decode failed at Gaussian %u", first);
+            break;
+        }
+        hipError_t e = hipSuccess;
+        for (int p = 0; p < kPlanes && e == hipSuccess; ++p)
+            e = hipMemcpyAsync(static_cast<float4*>(planes.p) + (size_t)p * count + first, c->up_pin[b] + (size_t)p * n * 4,
+                               (size_t)n * 16, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->up_ev[b], c->stream);
+        if (e != hipSuccess) r = fail(GS_ERR_HIP, "upload copy failed: %s", hipGetErrorString(e));
+        c->up_used[b] = true;
+    }
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (r == GS_OK && e != hipSuccess) r = fail(GS_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+    if (r != GS_OK) {
+        release(c, planes);
+        return r;
+    }
+    GS_CHECK(install_planes(c, planes, count));
     return ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_upload_gaussians(gs_ctx* c, const GsGaussian* g, uint32_t count) {
+    if (!c || (!g && count)) return fail(GS_ERR_INVALID, "bad arguments");
+    return upload_pipelined(c, count, [g](uint32_t first, uint32_t n, float* buf) {
+        return fill_planes_gaussians(g + first, n, n, buf);
+    });
+}
+
+int gs_upload_ply(gs_ctx* c, const float* rows, uint32_t count, const GsPlyLayout* layout) {
+    if (!c || !layout || (!rows && count)) return fail(GS_ERR_INVALID, "bad arguments");
+    return upload_pipelined(c, count, [rows, layout](uint32_t first, uint32_t n, float* buf) {
+        return fill_planes_ply(rows + (size_t)first * layout->stride, n, layout, n, buf);
+    });
+}
+
+int gs_upload_sog(gs_ctx* c, const GsSogData* sog) {
+    if (!c || !sog) return fail(GS_ERR_INVALID, "bad arguments");
+    return upload_pipelined(c, sog->count, [sog](uint32_t first, uint32_t n, float* buf) {
+        return fill_planes_sog(sog, first, n, n, buf);
+    });
+}
+
+int gs_upload_planes(gs_ctx* c, const float* host_planes, uint32_t count) {
+    if (!c || (!host_planes && count)) return fail(GS_ERR_INVALID, "bad arguments");
+    hipPointerAttribute_t attr{};
+    const bool pinned = hipPointerGetAttributes(&attr, host_planes) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();
+    if (!pinned) {
+        // pageable memory: through the pinned chunk buffers
+        return upload_pipelined(c, count, [host_planes, count](uint32_t first, uint32_t n, float* buf) {
+            return fill_planes_copy(host_planes, count, first, n, n, buf);
+        });
+    }
+    // page-locked: one asynchronous copy of the whole plane array
+    GS_CHECK(set_device(c));
+    DevBuf planes;
+    GS_CHECK(reserve_planes(c, count, &planes));
+    hipError_t e = hipMemcpyAsync(planes.p, host_planes, planes.bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        release(c, planes);
+        return fail(GS_ERR_HIP, "upload copy failed: %s", hipGetErrorString(e));
+    }
+    GS_CHECK(install_planes(c, planes, count));
+    return ok();
+}
+
+int gs_alloc_pinned(size_t bytes, void** out) {
+    if (!out || !bytes) return fail(GS_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(GS_ERR_OOM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return ok();
+}
+
+void gs_free_pinned(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 int gs_upload_animation(gs_ctx* c, const GsAnimation* an) {

@@ -536,33 +536,6 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
     for (; i < count; i += stride) p[i] = v;
 }
 
-// AoS GsGaussian (59 floats) -> 15 float4 planes.  128 Gaussians per block
-// are staged through LDS so both the read and the plane writes coalesce.
-constexpr int kUploadBlock = 128;
-constexpr int kGaussianFloats = 59;
-
-__global__ void __launch_bounds__(kUploadBlock) transpose_upload_kernel(
-        const float* aos, uint32_t count, uint32_t first, uint32_t n, float4* planes) {
-    __shared__ float s[kUploadBlock * kGaussianFloats];
-    const uint32_t base = blockIdx.x * kUploadBlock;
-    const uint32_t here = min((uint32_t)kUploadBlock, count - base);
-    const float* src = aos + (size_t)base * kGaussianFloats;
-    for (uint32_t i = threadIdx.x; i < here * kGaussianFloats; i += kUploadBlock) s[i] = src[i];
-    __syncthreads();
-    if (threadIdx.x >= here) return;
-    const float* g = s + threadIdx.x * kGaussianFloats;
-    // GsGaussian: position 0-2, scale 3-5, rotation 6-9, color 10-12, opacity 13, rest 14-58
-    const float* rest = g + 14;
-    size_t i = (size_t)first + base + threadIdx.x;
-    planes[0 * (size_t)n + i] = make_float4(g[0], g[1], g[2], g[13]);
-    planes[1 * (size_t)n + i] = make_float4(g[6], g[7], g[8], g[9]);
-    planes[2 * (size_t)n + i] = make_float4(g[3], g[4], g[5], g[10]);
-    planes[3 * (size_t)n + i] = make_float4(g[11], g[12], rest[0], rest[1]);
-    for (int j = 0; j < 10; ++j)
-        planes[(size_t)(4 + j) * n + i] = make_float4(rest[2 + 4 * j], rest[3 + 4 * j], rest[4 + 4 * j], rest[5 + 4 * j]);
-    planes[14 * (size_t)n + i] = make_float4(rest[42], rest[43], rest[44], 0.0f);
-}
-
 }  // namespace
 
 hipError_t launch_cull(const CullLaunch& a, hipStream_t s) {
@@ -613,15 +586,6 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_
     size_t blocks = (count + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_u64_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
-                                   float4* planes, hipStream_t s) {
-    if (!count) return hipSuccess;
-    uint32_t blocks = (count + kUploadBlock - 1) / kUploadBlock;
-    hipLaunchKernelGGL(transpose_upload_kernel, dim3(blocks), dim3(kUploadBlock), 0, s, aos, count, first, n,
-                       planes);
     return hipGetLastError();
 }
 

@@ -202,8 +202,6 @@ hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views
                           uint32_t* vis_rects, hipStream_t s, const uint32_t* packed = nullptr);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
-hipError_t launch_transpose_upload(const float* aos, uint32_t count, uint32_t first, uint32_t n,
-                                   float4* planes, hipStream_t s);
 
 // gs_sort.hip
 struct SortLaunch {

@@ -59,8 +59,9 @@ def _ply_header(f):
             return props, count, binary_le, f.tell()
 
 
-def load_ply(path) -> np.ndarray:
-    """Binary little-endian 3DGS PLY with float properties."""
+def read_ply(path):
+    """Binary little-endian 3DGS PLY with float properties: its rows (count,
+    properties) float32 and their GsPlyLayout."""
     with open(path, "rb") as f:
         props, count, binary_le, offset = _ply_header(f)
     if not binary_le:
@@ -84,12 +85,27 @@ def load_ply(path) -> np.ndarray:
     lay.dc[:] = [get(f"f_dc_{c}") for c in range(3)] if all(f"f_dc_{c}" in index for c in range(3)) \
         else [-1, -1, -1]
     lay.rest[:] = [get(f"f_rest_{k}") for k in range(45)]
-    out = np.empty((count, GAUSSIAN_FLOATS), np.float32)
-    data = np.ascontiguousarray(rows)
-    st = _lib_handle().gs_decode_ply(data.ctypes.data, count, C.byref(lay), out.ctypes.data)
+    return np.ascontiguousarray(rows), lay
+
+
+def load_ply(path) -> np.ndarray:
+    """Binary little-endian 3DGS PLY with float properties, decoded to Gaussians."""
+    rows, lay = read_ply(path)
+    out = np.empty((rows.shape[0], GAUSSIAN_FLOATS), np.float32)
+    st = _lib_handle().gs_decode_ply(rows.ctypes.data, rows.shape[0], C.byref(lay), out.ctypes.data)
     if st != 0:
         raise LoaderError(f"PLY decode failed ({st})")
     return out
+
+
+def upload_ply(ctx, path) -> int:
+    """Decodes a PLY straight into the device scene (gs_upload_ply: the rows
+    are decoded chunk by chunk into page-locked plane buffers while the
+    previous chunk copies).  Returns the Gaussian count."""
+    rows, lay = read_ply(path)
+    _lib.check(_lib_handle().gs_upload_ply(ctx.handle, rows.ctypes.data, rows.shape[0], C.byref(lay)))
+    ctx.n = rows.shape[0]
+    return ctx.n
 
 
 def write_ply(path, fields: dict) -> None:
@@ -141,8 +157,9 @@ def read_archive(source) -> dict:
         return {i.filename: z.read(i) for i in z.infolist() if not i.is_dir()}
 
 
-def load_sog(source) -> np.ndarray:
-    """SOG v2 archive (a path, bytes or a file object)."""
+def read_sog(source):
+    """SOG v2 archive (a path, bytes or a file object) as GsSogData, with the
+    decoded images it points to (keep them alive while it is used)."""
     archive = read_archive(source if not isinstance(source, (str, Path)) else str(source))
     if "meta.json" not in archive:
         raise LoaderError("SOG archive is missing meta.json")
@@ -199,12 +216,30 @@ def load_sog(source) -> np.ndarray:
         d.shN_centroids, d.centroids_width, d.centroids_height = cen.ctypes.data, cen.shape[1], cen.shape[0]
         d.shN_labels = lab.ctypes.data
         keep += [cen, lab]
-    out = np.empty((count, GAUSSIAN_FLOATS), np.float32)
+    return d, keep
+
+
+def load_sog(source) -> np.ndarray:
+    """SOG v2 archive (a path, bytes or a file object), decoded to Gaussians."""
+    d, keep = read_sog(source)
+    out = np.empty((d.count, GAUSSIAN_FLOATS), np.float32)
     st = _lib_handle().gs_decode_sog(C.byref(d), out.ctypes.data)
     del keep
     if st != 0:
         raise LoaderError("SOG decode failed: invalid quaternion mode or palette label")
     return out
+
+
+def upload_sog(ctx, source) -> int:
+    """Decodes a SOG archive straight into the device scene (gs_upload_sog,
+    pipelined like upload_ply).  Returns the Gaussian count."""
+    d, keep = read_sog(source)
+    code = _lib_handle().gs_upload_sog(ctx.handle, C.byref(d))
+    del keep
+    if code != 0:
+        raise LoaderError(f"SOG upload failed: {_lib_handle().gs_last_error().decode()}")
+    ctx.n = d.count
+    return ctx.n
 
 
 def load_scene(path) -> np.ndarray:

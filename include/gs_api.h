@@ -157,11 +157,32 @@ int gs_set_memory_budget(gs_ctx* ctx, size_t bytes);
 
 /* ---- upload -------------------------------------------------------------- */
 
-/* Replaces renderer_upload_gaussians (src/renderer.cpp:736-1074): copies the
- * host AoS Gaussians to HBM and repacks them into 15 float4 attribute planes.
- * Refuses (GS_ERR_LIMIT) a scene that exceeds the memory budget and keeps the
+/* Replaces renderer_upload_gaussians (src/renderer.cpp:736-1074), whose
+ * 64 MiB chunked copies of the packed AoS records it follows: the host
+ * repacks each chunk of 262,144 Gaussians into the device's attribute planes
+ * (GS_SCENE_PLANES below) in one of two page-locked buffers, on its cores,
+ * while the copies of the previous chunk run from the other.  Refuses
+ * (GS_ERR_LIMIT) a scene that exceeds the memory budget and keeps the
  * previous scene in that case. */
 int gs_upload_gaussians(gs_ctx* ctx, const GsGaussian* gaussians, uint32_t count);
+
+/* The scene's layout in HBM: GS_SCENE_PLANES planes of `count` float4 each,
+ * plane p of Gaussian i at floats (p * count + i) * 4:
+ *   P0 position x y z, opacity      P1 rotation w x y z
+ *   P2 scale x y z, DC r            P3 DC g b, rest[0], rest[1]
+ *   P4+j rest[2+4j .. 5+4j] (j = 0..9)      P14 rest[42..44], 0
+ * (rest as GsGaussian.sh_rest: RGB triples per coefficient). */
+#define GS_SCENE_PLANES 15
+/* Uploads a scene already in that layout.  From page-locked memory (e.g.
+ * gs_alloc_pinned) it is one asynchronous copy; from pageable memory it goes
+ * through the upload's page-locked chunk buffers.  Same budget rules as
+ * gs_upload_gaussians.  No counterpart in the reference (loaders that
+ * decode straight into the planes). */
+int gs_upload_planes(gs_ctx* ctx, const float* planes, uint32_t count);
+/* Page-locked host memory for gs_upload_planes / gs_decode_ply_planes
+ * (hipHostMalloc); gs_free_pinned(NULL) is accepted. */
+int gs_alloc_pinned(size_t bytes, void** out);
+void gs_free_pinned(void* p);
 
 /* GSAnim appearance tracks of the uploaded scene (GSAnim, src/gsanim.h:27-48),
  * as renderer_upload_gaussians packs them (src/renderer.cpp:751-758,848-893):
@@ -540,6 +561,19 @@ typedef struct GsSogData {
  * keeps it); opacity = alpha / 255; palette SH by 16-bit label.  An invalid
  * quaternion mode or label is GS_ERR_INVALID. */
 int gs_decode_sog(const GsSogData* sog, GsGaussian* out);
+
+/* The decoders straight into the scene's plane layout (GS_SCENE_PLANES,
+ * plane stride count), with the same arithmetic: a loader's fast path
+ * (src/gaussian_loader_ply.cpp:116-203 reads 4096-vertex batches, then
+ * renderer_upload_gaussians repacks and copies them).  Host only. */
+int gs_planes_from_gaussians(const GsGaussian* gaussians, uint32_t count, float* planes);
+int gs_decode_ply_planes(const float* rows, uint32_t count, const GsPlyLayout* layout, float* planes);
+/* Decode and upload in one pipeline: the host decodes chunk k of the rows
+ * (or SOG texels) into page-locked plane buffers while the copies of chunk
+ * k - 1 run; the result equals gs_decode_ply / gs_decode_sog followed by
+ * gs_upload_gaussians.  Same budget rules as gs_upload_gaussians. */
+int gs_upload_ply(gs_ctx* ctx, const float* rows, uint32_t count, const GsPlyLayout* layout);
+int gs_upload_sog(gs_ctx* ctx, const GsSogData* sog);
 
 /* ---- synthetic scenes and cameras (host only) --------------------------- */
 

@@ -220,3 +220,67 @@ def test_headless_denoiser_judge_and_fallback(tmp_path):
     assert abs(float(np.abs(img - ref).max()) - rep["last_max_abs"]) < 1e-6
     print(f"denoised frame vs sorted alpha: {rep['last_psnr_db']:.2f} dB, max abs {rep['last_max_abs']:.3f}",
           flush=True)
+
+
+def test_upload_paths_give_the_same_scene(gs, tmp_path):
+    """Every way into the device scene - Gaussians (pipelined through pinned
+    plane chunks), host planes from pageable and from page-locked memory, a
+    PLY decoded chunk by chunk straight into the planes, a SOG archive the
+    same way - gives the same projected splats, bit for bit, as uploading
+    the decoded Gaussians; the scene spans several 262,144-Gaussian chunks."""
+    import ctypes as C
+    import importlib.util
+
+    from gamesplat_amd import loaders
+
+    n = 600_000
+    rng = np.random.default_rng(43)
+    f = {"x": rng.normal(0, 3, n), "y": rng.normal(0, 3, n), "z": rng.normal(0, 3, n)}
+    f.update({f"f_dc_{c}": rng.normal(0, .5, n) for c in range(3)})
+    f.update({f"f_rest_{k}": rng.normal(0, .05, n) for k in range(45)})
+    f["opacity"] = rng.normal(0.5, 2, n)
+    f.update({f"scale_{a}": rng.normal(-3.5, .6, n) for a in range(3)})
+    f.update({f"rot_{a}": rng.normal(0, 1, n) for a in range(4)})
+    loaders.write_ply(tmp_path / "s.ply", f)
+    g = loaders.load_ply(tmp_path / "s.ply")
+    cam = scenes.random_view_camera(43, 320, 240, radius=12.0)
+    lib = abi.load()
+    planes = np.empty((15, n, 4), np.float32)
+    abi.check(lib.gs_planes_from_gaussians(g.ctypes.data, n, planes.ctypes.data))
+    outs = {}
+    with gs.Context(0) as c:
+        def projected(view=cam):
+            c.cull(view)
+            assert c.stats()["visible_count"] > 1000
+            return c.read_buffer(abi.GS_BUFFER_PROJECTED).view(np.uint32).copy()
+
+        print("case: upload paths, Gaussians", flush=True)
+        c.upload(g)
+        outs["gaussians"] = projected()
+        print("case: upload paths, pageable planes", flush=True)
+        abi.check(lib.gs_upload_planes(c.handle, planes.ctypes.data, n))
+        outs["planes"] = projected()
+        print("case: upload paths, pinned planes", flush=True)
+        pin = C.c_void_p()
+        abi.check(lib.gs_alloc_pinned(planes.nbytes, C.byref(pin)))
+        try:
+            C.memmove(pin, planes.ctypes.data, planes.nbytes)
+            abi.check(lib.gs_upload_planes(c.handle, pin, n))
+        finally:
+            lib.gs_free_pinned(pin)
+        outs["pinned"] = projected()
+        print("case: upload paths, PLY straight to planes", flush=True)
+        loaders.upload_ply(c, tmp_path / "s.ply")
+        outs["ply"] = projected()
+        spec = importlib.util.spec_from_file_location("tl", Path(__file__).parent / "test_loaders.py")
+        tl = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(tl)
+        data, _, _ = tl._sog(300_000, 640, np.random.default_rng(6))
+        print("case: upload paths, SOG", flush=True)
+        sog_cam = scenes.random_view_camera(5, 320, 240, radius=6.0)
+        c.upload(loaders.load_sog(data))
+        sog_ref = projected(sog_cam)
+        loaders.upload_sog(c, data)
+        np.testing.assert_array_equal(projected(sog_cam), sog_ref)
+    for k in ("planes", "pinned", "ply"):
+        np.testing.assert_array_equal(outs[k], outs["gaussians"], err_msg=k)

@@ -178,3 +178,48 @@ def test_real_scene_from_reference_bundle():
     np.testing.assert_allclose(np.linalg.norm(g[:, 6:10].astype(np.float64), axis=1), 1.0, atol=1e-5)
     assert (g[:, 13] >= 0).all() and (g[:, 13] <= 1).all() and (g[:, 3:6] > 0).all()
     assert np.abs(g[:, 14 + 24:]).max() > 0  # third-band coefficients present
+
+
+def _planes_reference(g):
+    """The documented plane layout (include/gs_api.h GS_SCENE_PLANES) built
+    with numpy: P0 position + opacity, P1 rotation, P2 scale + DC r, P3 DC g b
+    + rest 0..1, P4+j rest 2+4j..5+4j, P14 rest 42..44 and 0."""
+    n = g.shape[0]
+    rest = g[:, 14:59]
+    planes = np.zeros((15, n, 4), np.float32)
+    planes[0] = np.c_[g[:, 0:3], g[:, 13]]
+    planes[1] = g[:, 6:10]
+    planes[2] = np.c_[g[:, 3:6], g[:, 10]]
+    planes[3] = np.c_[g[:, 11:13], rest[:, 0:2]]
+    for j in range(10):
+        planes[4 + j] = rest[:, 2 + 4 * j:6 + 4 * j]
+    planes[14, :, 0:3] = rest[:, 42:45]
+    return planes
+
+
+def test_planes_layout_and_ply_straight_to_planes(tmp_path):
+    """gs_planes_from_gaussians writes the documented layout, and decoding a
+    PLY straight into it (gs_decode_ply_planes) equals decoding to Gaussians
+    and repacking, bit for bit."""
+    import ctypes as C
+
+    from gamesplat_amd import abi, loaders
+
+    rng = np.random.default_rng(41)
+    n = 5003
+    f = {"x": rng.normal(0, 3, n), "y": rng.normal(0, 3, n), "z": rng.normal(0, 3, n)}
+    f.update({f"f_dc_{c}": rng.normal(0, .5, n) for c in range(3)})
+    f.update({f"f_rest_{k}": rng.normal(0, .05, n) for k in range(45)})
+    f["opacity"] = rng.normal(0.5, 2, n)
+    f.update({f"scale_{a}": rng.normal(-3.5, .6, n) for a in range(3)})
+    f.update({f"rot_{a}": rng.normal(0, 1, n) for a in range(4)})
+    loaders.write_ply(tmp_path / "s.ply", f)
+    g = loaders.load_ply(tmp_path / "s.ply")
+    lib = abi.load()
+    planes = np.empty((15, n, 4), np.float32)
+    abi.check(lib.gs_planes_from_gaussians(g.ctypes.data, n, planes.ctypes.data))
+    np.testing.assert_array_equal(planes, _planes_reference(g))
+    rows, lay = loaders.read_ply(tmp_path / "s.ply")
+    direct = np.empty((15, n, 4), np.float32)
+    abi.check(lib.gs_decode_ply_planes(rows.ctypes.data, n, C.byref(lay), direct.ctypes.data))
+    np.testing.assert_array_equal(direct.view(np.uint32), planes.view(np.uint32))
