@@ -200,24 +200,31 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
     }
 }
 
-// One splat as the compositors keep it in LDS (4 x 16 bytes, plus the
-// quadrant mask).  The quadratic form is in log2 units (scaled by log2 e),
-// and the opacity enters as log2 o added to the form, so the blend step's
-// alpha is one v_exp_f32 and its fragment test a comparison with
-// log2(1/255).  The covered pixels [i0, i1) x [j0, j1) ride in par.y and
-// col.w as the packed corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so
-// one step reads three 16-byte records at one address.  The fourth record
-// holds the reference's own conic and cutoff, read only for the fragments
-// whose fused log2 test lies within par.z of the cutoff.
+// One splat as the compositors keep it in LDS (4 x 16 bytes and 8 bytes,
+// plus the quadrant mask).  The fragment power is kept as a quadratic
+// polynomial in the pixel centre relative to the tile centre, (X, Y) = (x +
+// 0.5 - xc, y + 0.5 - yc), in log2 units (scaled by log2 e) with the opacity
+// folded in:
+//   power' = A X^2 + B X Y + C Y^2 + E X + F Y + G
+// (A, B, C from the conic; E, F, G from the splat centre relative to the
+// tile centre and log2 o - log2(1/255)), so a pixel's blend step is five
+// fused multiply-adds on per-pixel constants X^2, X Y, Y^2, X, Y, one
+// v_exp_f32 for 255 alpha, and its fragment test a comparison with 0.  The
+// covered pixels [i0, i1) x [j0, j1) ride in par.w and col.w as the packed
+// corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so one step reads
+// three 16-byte records at one address.  The fourth record and the centre
+// hold the reference's own conic, cutoff and centre, read only for the
+// fragments whose fused test lies within par.z of the cutoff.
 // (A native 4-float vector, so each record is one 128-bit LDS access.)
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 struct StagedSplats {
-    f32x4_t* geo;   // cx, cy, -k0/2 log2e, -k1 log2e
-    f32x4_t* par;   // -k2/2 log2e, rect (low corner bits), decision margin (log2 units), log2 o - log2(1/255)
+    f32x4_t* geo;   // A, B, C, E
+    f32x4_t* par;   // F, G, decision margin (log2 units), rect (low corner bits)
     f32x4_t* col;   // r, g, b, rect (high corner bits)
     f32x4_t* exact; // c / det, 2 (-b / det), a / det, log((1/255) / max(o, 1e-8)): the reference's operations
-    uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q
+    uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q; bit 4: slow step
+    float2* ctr;    // cx, cy
 };
 // Inclusive 16-bit corners of a pixel rectangle [i0, i1) x [j0, j1) (image
 // coordinates are below 2^16); an empty one becomes (0xFFFF, 0xFFFF) twice,
@@ -245,6 +252,7 @@ __device__ __forceinline__ bool in_rect16(uint32_t p, uint2 r) {
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2AlphaMin = -7.99435343685886f;  // log2(1/255)
+constexpr float kLog2AlphaCap255 = 7.97985386716375f;  // log2(0.99 * 255)
 
 // The view and tile of a global tile id.
 struct TileRef {
@@ -271,81 +279,183 @@ __device__ __forceinline__ TileRef tile_ref(const CompositeLaunch& a, uint32_t i
     return t;
 }
 
-// Stage splat g into slot t: conic from one reciprocal of the clamped
-// determinant in log2 units, pixel rectangle, the reference's conic
-// ((c, -b, a) / det, shaders/cull.glsl:273-275) and fragment cutoff
-// log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70), and the margin
-// within which the fused log2 test may decide differently from the
-// reference's float test.  Returns the cutoff in log2 units (for the
-// quadrant test).
+// What the quadrant test needs of a staged splat: centre, the form's
+// coefficients relative to the centre (log2 units), log2 cutoff, decision
+// margin and covered pixels.
+struct StagedTerms {
+    float cx, cy, A, B, C, cutoff, margin;
+    int4 rect;
+    bool slow;
+};
+
+// Per-splat constants of the quadrant test (ellipse_meets_box): whether it
+// applies (a concave form with 1 - rho^2 >= 0.01), the argmax slopes of the
+// form along y and x, and the band below the cutoff that may still pass.
+struct QuadCullTerms {
+    bool applies;
+    float sy, sx, band;
+};
+
+// Stage splat g into slot t of a T x T tile: conic from one reciprocal of the
+// clamped determinant in log2 units, pixel rectangle, the reference's conic
+// ((c, -b, a) / det, shaders/cull.glsl:273-275), fragment cutoff
+// log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70) and centre, and the
+// margin within which the fused test may decide differently from the
+// reference's float test.
 //
 // Margin: with K the conic and kappa = lambda_max / lambda_min its condition
 // number, |k0 dx^2| + |2 k1 dx dy| + |k2 dy^2| <= 2 kappa q at a pixel where
 // the form is q; at the cutoff, q log2(e) / 2 = |log2((1/255) / o)|.  The
-// fused log2 power and the reference's float power each carry about 8
-// roundings of those terms (staging, products, sums): together about
-// 2e-6 kappa |cutoff|, plus a few 1e-6 from the log2 of o and the rounded
-// cutoff.  The margin is four times that; a splat whose conic is not
-// positive definite in float always takes the reference's test.
-__device__ __forceinline__ float stage_splat(const CompositeLaunch& a, const TileRef& tr, uint32_t g, uint32_t t,
-                                             StagedSplats s) {
-    const float4 r0 = a.records[(size_t)g * kRecordVec4 + 0];
-    const float4 r1 = a.records[(size_t)g * kRecordVec4 + 1];
-    const float4 r2 = a.records[(size_t)g * kRecordVec4 + 2];
-    float det = r1.x * r1.z - r1.y * r1.y;
-    if (det < 1e-6f) det = 1e-6f;
-    const float inv = 1.0f / det;
-    const float k0 = r1.z * inv, k1 = -r1.y * inv, k2 = r1.x * inv;
-    const float log2o = __log2f(maxf(r0.z, 1e-8f));
-    const float cutoff = __log2f((1.0f / 255.0f) / maxf(r0.z, 1e-8f));
-    const uint2 rect = pack_rect16(splat_pixel_rect(r0.x, r0.y, r1.x, r1.y, r1.z, r1.w, r2.w, r0.z, tr.width,
-                                                    tr.height, a.tight_rects != 0));
-    const float ht = 0.5f * (k0 + k2), hd = 0.5f * (k0 - k2);
-    const float sp = __builtin_sqrtf(hd * hd + k1 * k1);
-    const float kappa = (ht + sp) / (ht - sp);
-    const float margin = ht - sp > 0.0f ? 8e-6f * kappa * __builtin_fabsf(cutoff) + 3e-5f : __builtin_inff();
-    s.geo[t] = f32x4_t{r0.x, r0.y, -0.5f * kLog2e * k0, -kLog2e * k1};
-    s.par[t] = f32x4_t{-0.5f * kLog2e * k2, __uint_as_float(rect.x), margin, log2o - kLog2AlphaMin};
-    s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
-    s.exact[t] = f32x4_t{r1.z / det, 2.0f * (-r1.y / det), r1.x / det, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f))};
-    return cutoff;
+// float form with the staged coefficients A, B, C (log2 o folded in) and the
+// reference's float power each carry about 8 roundings of those terms
+// (staging, products, sums): together about 2e-6 kappa |cutoff|, plus a few
+// 1e-6 from the log2 of o and the rounded cutoff; four times that is the
+// first part.  The expanded polynomial adds its own rounding: forming E, F
+// and G takes at most four roundings of a value no larger than S, the sum of
+// the magnitudes of every product and partial sum involved (over the tile's
+// pixel range), and its five fused multiply-adds one each, so it differs
+// from the form above by at most about 9 eps S; 16 eps S is the second part.
+// A splat whose conic is not positive definite in float always takes the
+// reference's test.
+//
+// Slow step: the rectangle test and the 0.99 alpha cap can be left out of a
+// splat's blend steps when neither can change a result.  Pixels outside the
+// alpha ellipse's bounds fail the reference's fragment test, and the fused
+// test decides as the reference does wherever it lies outside the margin
+// (and takes the reference's test inside it), so when the quad's pixel
+// bounds hold the ellipse's bounds, the rectangle excludes nothing the
+// fragment test keeps.  The cap binds only where 255 alpha = 2^power'
+// exceeds 0.99 * 255; power' is at most G' = log2 o - log2(1/255) at the
+// centre, plus the margin.  Everything else takes the slow step.
+struct SplatRecord {
+    float4 r0, r1, r2;
+};
+
+__device__ __forceinline__ SplatRecord load_record(const CompositeLaunch& a, uint32_t g) {
+    SplatRecord r;
+    r.r0 = a.records[(size_t)g * kRecordVec4 + 0];
+    r.r1 = a.records[(size_t)g * kRecordVec4 + 1];
+    r.r2 = a.records[(size_t)g * kRecordVec4 + 2];
+    return r;
 }
 
-// One front-to-back blend step of pixel p = x | y << 16: coverage (the
-// pixel rectangle) and the fragment test select the alpha or 0, and a zero
-// alpha leaves colour and transmittance bit-for-bit unchanged.  A saturated
-// pixel (T < 1e-4) keeps blending until its whole wave is saturated; what it
-// adds is below 1e-4 of a colour, and it only brings the result closer to
-// blending every splat.  The quadratic form and the opacity are evaluated
-// with fused multiply-adds in log2 units relative to the cutoff:
-// power' = dx * (-k0/2 dx - k1 dy) + ((-k2/2 dy) dy + log2 o - log2(1/255)),
-// so 255 alpha = min(2^power', 255 * 0.99) and the fragment test
-// alpha >= 1/255 is power' >= 0; tr holds T / 255.  Where power' lies
-// within the splat's margin of 0 (a few fragments per thousand steps, on the rim of the alpha
-// ellipse), the test is redone with the reference's float operations in
-// source order (power = -0.5 (k0 dx dx + 2 k1 dx dy + k2 dy dy), discard if
-// power > 0 or power < log((1/255) / o); shaders/splat.glsl:84-96), so the
-// set of blended fragments is the reference's.  Elsewhere the power > 0
-// discard cannot fire: the cull's +0.3 dilation keeps det >= 0.09, so the
-// conic is positive definite, and its float evaluation stays within a
-// relative 1e-2 of the exact form for every conditioning the dilation
-// allows.
-__device__ __forceinline__ void blend_step(uint32_t pxy, float fx, float fy, f32x4_t geo, f32x4_t par, f32x4_t col,
-                                           const f32x4_t* exact, float& cr, float& cg, float& cb, float& tr) {
-    const uint2 r = make_uint2(__float_as_uint(par.y), __float_as_uint(col.w));
-    const float dx = fx - geo.x, dy = fy - geo.y;
-    const float power = fmaf(fmaf(geo.z, dx, geo.w * dy), dx, fmaf(par.x * dy, dy, par.w));
-    float alpha = minf(__builtin_amdgcn_exp2f(power), kAlphaCap255);
+template <int T>
+__device__ __forceinline__ StagedTerms stage_splat(const CompositeLaunch& a, const TileRef& tr, const SplatRecord& rec,
+                                                   uint32_t t, StagedSplats s) {
+    const float4 r0 = rec.r0, r1 = rec.r1, r2 = rec.r2;
+    float det = r1.x * r1.z - r1.y * r1.y;
+    if (det < 1e-6f) det = 1e-6f;
+    // the reference's conic (IEEE divisions), which the fused form scales
+    const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+    const float log2o = __log2f(maxf(r0.z, 1e-8f));
+    StagedTerms st;
+    st.cx = r0.x;
+    st.cy = r0.y;
+    st.A = -0.5f * kLog2e * k0;
+    st.B = -kLog2e * k1;
+    st.C = -0.5f * kLog2e * k2;
+    const float Dc = log2o - kLog2AlphaMin;
+    st.cutoff = -Dc;  // log2((1/255) / o)
+    const int4 quad = quad_pixel_bounds(r0.x, r0.y, r1.w, r2.w, tr.width, tr.height);
+    int4 ell;
+    const bool bounded = ellipse_pixel_bounds(r0.x, r0.y, r1.x, r1.y, r1.z, r0.z, &ell);
+    st.rect = quad;
+    if (bounded && a.tight_rects != 0) {
+        st.rect.x = max(quad.x, ell.x);
+        st.rect.y = min(quad.y, ell.y);
+        st.rect.z = max(quad.z, ell.z);
+        st.rect.w = min(quad.w, ell.w);
+    }
+    st.rect = make_int4(max(st.rect.x, 0), min(st.rect.y, tr.width), max(st.rect.z, 0), min(st.rect.w, tr.height));
+    // the expanded polynomial about the tile centre; S bounds the magnitudes
+    // of every product and sum formed in its coefficients and its evaluation
+    const float u = r0.x - (float)(tr.tx * T + T / 2), v = r0.y - (float)(tr.ty * T + T / 2);
+    const float A = st.A, B = st.B, C = st.C;
+    const float Au = A * u, Bv = B * v, Bu = B * u, Cv = C * v;
+    const float E = -2.0f * Au - Bv, F = -2.0f * Cv - Bu;
+    const float G = (Au + Bv) * u + Cv * v + Dc;
+    constexpr float h = T / 2 - 0.5f;
+    const float S = h * h * (fabsf(A) + fabsf(B) + fabsf(C)) +
+                    h * (2.0f * fabsf(Au) + fabsf(Bv) + 2.0f * fabsf(Cv) + fabsf(Bu)) +
+                    (fabsf(Au * u) + fabsf(Bv * u) + fabsf(Cv * v) + fabsf(Dc));
+    const float ht = 0.5f * (k0 + k2), hd = 0.5f * (k0 - k2);
+    const float sp = __builtin_sqrtf(hd * hd + k1 * k1);
+    const float kappa = (ht + sp) * __builtin_amdgcn_rcpf(ht - sp);
+    st.margin = ht - sp > 0.0f
+                    ? 8e-6f * kappa * __builtin_fabsf(st.cutoff) + 3e-5f + 16.0f * 5.9604645e-8f * S
+                    : __builtin_inff();
+    const bool rect_free = bounded && ell.x >= quad.x && ell.y <= quad.y && ell.z >= quad.z && ell.w <= quad.w;
+    const bool cap_free = Dc + st.margin + 1e-4f < kLog2AlphaCap255;
+    st.slow = !(rect_free && cap_free);
+    const uint2 rect = pack_rect16(st.rect);
+    s.geo[t] = f32x4_t{A, B, C, E};
+    s.par[t] = f32x4_t{F, G, st.margin, __uint_as_float(rect.x)};
+    s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
+    s.exact[t] = f32x4_t{k0, 2.0f * k1, k2, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f))};
+    s.ctr[t] = make_float2(r0.x, r0.y);
+    return st;
+}
+
+// A pixel's constants for the blend steps: its centre (x + 0.5, y + 0.5),
+// its position x | y << 16, and the monomials of (X, Y) relative to the tile
+// centre (all exact in float).
+struct PixelTerms {
+    float fx, fy, X, Y, XX, XY, YY;
+    uint32_t pxy;
+};
+
+template <int T>
+__device__ __forceinline__ PixelTerms pixel_terms(int px, int py, int tx, int ty) {
+    PixelTerms p;
+    p.fx = (float)px + 0.5f;
+    p.fy = (float)py + 0.5f;
+    p.X = (float)(px - tx * T) + (0.5f - (float)(T / 2));
+    p.Y = (float)(py - ty * T) + (0.5f - (float)(T / 2));
+    p.XX = p.X * p.X;
+    p.XY = p.X * p.Y;
+    p.YY = p.Y * p.Y;
+    p.pxy = (uint32_t)px | ((uint32_t)py << 16);
+    return p;
+}
+
+// One front-to-back blend step of a pixel: coverage (the pixel rectangle)
+// and the fragment test select the alpha or 0, and a zero alpha leaves colour
+// and transmittance bit-for-bit unchanged.  A saturated pixel (T < 1e-4)
+// keeps blending until its whole wave is saturated; what it adds is below
+// 1e-4 of a colour, and it only brings the result closer to blending every
+// splat.  power' is the staged polynomial (log2 units relative to the
+// cutoff), so 255 alpha = min(2^power', 255 * 0.99) and the fragment test
+// alpha >= 1/255 is power' >= 0; tr holds T / 255.  Where power' lies within
+// the splat's margin of 0 (a few fragments per thousand steps, on the rim of
+// the alpha ellipse), the test is redone with the reference's float
+// operations in source order (d = pixel centre - splat centre, power =
+// -0.5 (k0 dx dx + 2 k1 dx dy + k2 dy dy), discard if power > 0 or power <
+// log((1/255) / o); shaders/splat.glsl:84-96), so the set of blended
+// fragments is the reference's.  Elsewhere the power > 0 discard cannot
+// fire: the cull's +0.3 dilation keeps det >= 0.09, so the conic is positive
+// definite, and its float evaluation stays within a relative 1e-2 of the
+// exact form for every conditioning the dilation allows.  SLOW = false
+// leaves out the rectangle test and the cap (see stage_splat).
+template <bool SLOW>
+__device__ __forceinline__ void blend_step(const PixelTerms& p, f32x4_t geo, f32x4_t par, f32x4_t col,
+                                           const f32x4_t* exact, const float2* ctr, float& cr, float& cg, float& cb,
+                                           float& tr) {
+    const float power =
+        fmaf(geo.x, p.XX, fmaf(geo.y, p.XY, fmaf(geo.z, p.YY, fmaf(geo.w, p.X, fmaf(par.x, p.Y, par.y)))));
+    float alpha = __builtin_amdgcn_exp2f(power);
+    if (SLOW) alpha = minf(alpha, kAlphaCap255);
     // power' >= margin: passes; below -margin: fails; between: the reference's test
     const bool lo = power >= -par.z;
     bool pass = lo;
     if (lo & !(power >= par.z)) {
         const f32x4_t e = *exact;
+        const float2 c = *ctr;
+        const float dx = p.fx - c.x, dy = p.fy - c.y;
         const float pr = -0.5f * (e.x * dx * dx + e.y * dx * dy + e.z * dy * dy);
         pass = !(pr > 0.0f) & !(pr < e.w);
     }
-    const bool take = in_rect16(pxy, r) & pass;
-    alpha = take ? alpha : 0.0f;
+    if (SLOW) pass &= in_rect16(p.pxy, make_uint2(__float_as_uint(par.w), __float_as_uint(col.w)));
+    alpha = pass ? alpha : 0.0f;
     const float w = alpha * tr;
     cr = fmaf(col.x, w, cr);
     cg = fmaf(col.y, w, cg);
@@ -373,7 +483,7 @@ struct TileShape {
 };
 
 // T x T tile, PPT pixels per thread; every pixel walks every staged splat of
-// its tile.
+// its tile (with the slow step).
 template <int T>
 __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
     using S = TileShape<T>;
@@ -382,15 +492,14 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
     const TileRef t = tile_ref(a, item);
 
     int px[PPT], py[PPT];
-    float fx[PPT], fy[PPT];
+    PixelTerms pt[PPT];
     float cr[PPT], cg[PPT], cb[PPT], tr[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         int p = (int)tid + k * NT;
         px[k] = t.tx * T + p % T;
         py[k] = t.ty * T + p / T;
-        fx[k] = (float)px[k] + 0.5f;
-        fy[k] = (float)py[k] + 0.5f;
+        pt[k] = pixel_terms<T>(px[k], py[k], t.tx, t.ty);
         cr[k] = cg[k] = cb[k] = 0.0f;
         // pixels outside the image start saturated and are never written
         tr[k] = px[k] < t.width && py[k] < t.height ? kInv255 : 0.0f;
@@ -402,7 +511,7 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
         for (int k = 0; k < PPT; ++k) live |= tr[k] >= kUMin;
         if (!__syncthreads_or(live)) break;
         uint32_t e = base + tid;
-        if (e < t.range.y) stage_splat(a, t, a.tile_vals[e], tid, st);
+        if (e < t.range.y) stage_splat<T>(a, t, load_record(a, a.tile_vals[e]), tid, st);
         __syncthreads();
         const int cnt = (int)min((uint32_t)BATCH, t.range.y - base);
         if (__ballot(live)) {
@@ -410,8 +519,7 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
                 const f32x4_t geo = st.geo[j], par = st.par[j], col = st.col[j];
 #pragma unroll
                 for (int k = 0; k < PPT; ++k)
-                    blend_step((uint32_t)px[k] | ((uint32_t)py[k] << 16), fx[k], fy[k], geo, par, col, st.exact + j,
-                               cr[k], cg[k], cb[k], tr[k]);
+                    blend_step<true>(pt[k], geo, par, col, st.exact + j, st.ctr + j, cr[k], cg[k], cb[k], tr[k]);
             }
         }
     }
@@ -427,39 +535,52 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
 // concave quadratic (log2 units: A dx^2 + B dx dy + C dy^2, A, C < 0), so its
 // maximum over the box of those centres is 0 when the splat centre lies in
 // the box and otherwise sits on an edge, at the clamped vertex of the 1D
-// parabola.  The margin covers the float evaluation of power here and in the
+// parabola.  The band covers the float evaluation of power here and in the
 // blend step (about 10 eps |power| / (1 - |rho|)^2 each, with 1 - rho^2 =
-// r2 = 1 - B^2 / (4 A C)); splats with r2 < 0.01 or a clamped, indefinite
-// conic are always walked.
-__device__ __forceinline__ bool ellipse_meets_box(int4 r, f32x4_t geo, float C, float cutoff, float decision,
-                                                  int qx0, int qy0) {
-    const float A = geo.z, B = geo.w;
-    if (!(A < 0.0f && C < 0.0f)) return true;
-    const float r2 = 1.0f - (B * B) / (4.0f * A * C);
-    if (!(r2 >= 0.01f)) return true;
-    const float xl = (float)max(qx0, r.x) + 0.5f - geo.x, xh = (float)min(qx0 + 8, r.y) - 0.5f - geo.x;
-    const float yl = (float)max(qy0, r.z) + 0.5f - geo.y, yh = (float)min(qy0 + 8, r.w) - 0.5f - geo.y;
+// r2 = 1 - B^2 / (4 A C)) and the blend step's decision margin; splats with
+// r2 < 0.01 or a clamped, indefinite conic are always walked.  The slopes
+// and the band are per-splat constants (quad_cull_terms); a vertex placed
+// a few ulps off by the approximate reciprocals lowers the edge maximum by
+// far less than the band.
+__device__ __forceinline__ QuadCullTerms quad_cull_terms(const StagedTerms& s) {
+    QuadCullTerms q;
+    q.applies = false;
+    q.sy = q.sx = q.band = 0.0f;
+    if (!(s.A < 0.0f && s.C < 0.0f)) return q;
+    const float r2 = 1.0f - (s.B * s.B) * __builtin_amdgcn_rcpf(4.0f * s.A * s.C);
+    if (!(r2 >= 0.01f)) return q;
+    q.applies = true;
+    q.sy = -s.B * __builtin_amdgcn_rcpf(2.0f * s.C);  // argmax of the parabola along y
+    q.sx = -s.B * __builtin_amdgcn_rcpf(2.0f * s.A);  // and along x
+    q.band = __builtin_fabsf(s.cutoff) * (0.003f + 1.001e-5f * __builtin_amdgcn_rcpf(r2 * r2)) + 1e-5f + s.margin;
+    return q;
+}
+
+__device__ __forceinline__ bool ellipse_meets_box(const StagedTerms& s, const QuadCullTerms& q, int qx0, int qy0) {
+    if (!q.applies) return true;
+    const float A = s.A, B = s.B, C = s.C;
+    const int4 r = s.rect;
+    const float xl = (float)max(qx0, r.x) + 0.5f - s.cx, xh = (float)min(qx0 + 8, r.y) - 0.5f - s.cx;
+    const float yl = (float)max(qy0, r.z) + 0.5f - s.cy, yh = (float)min(qy0 + 8, r.w) - 0.5f - s.cy;
     if (xl <= 0.0f && xh >= 0.0f && yl <= 0.0f && yh >= 0.0f) return true;
-    const float sy = -B / (2.0f * C), sx = -B / (2.0f * A);  // argmax of the parabola along y (x)
     auto along_y = [&](float u) {  // max over dy in [yl, yh] at dx = u
-        const float t = clampf(sy * u, yl, yh);
+        const float t = clampf(q.sy * u, yl, yh);
         return fmaf(fmaf(C, t, B * u), t, (A * u) * u);
     };
     auto along_x = [&](float u) {  // max over dx in [xl, xh] at dy = u
-        const float t = clampf(sx * u, xl, xh);
+        const float t = clampf(q.sx * u, xl, xh);
         return fmaf(fmaf(A, t, B * u), t, (C * u) * u);
     };
     const float best = maxf(maxf(along_y(xl), along_y(xh)), maxf(along_x(yl), along_x(yh)));
-    // the blend step's decision margin widens the band that may pass
-    const float margin = __builtin_fabsf(cutoff) * (0.003f + 1e-5f / (r2 * r2)) + 1e-5f + decision;
-    return best >= cutoff - margin;
+    return best >= s.cutoff - q.band;
 }
 
 // 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
 // of an LDS batch is tested against the quadrant's pixel range with one
 // ballot per 64 splats; the wave then walks only the overlapping splats
 // (scalar bit iteration) with one LDS wait per splat, and stops as soon as
-// all 64 of its pixels are saturated.
+// all 64 of its pixels are saturated.  Whether a splat takes the slow blend
+// step is a second ballot, so the choice is a scalar branch.
 __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
     constexpr int BATCH = 256;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -467,43 +588,55 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
 
     const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
-    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-    const uint32_t pxy = (uint32_t)px | ((uint32_t)py << 16);
+    PixelTerms pt = pixel_terms<16>(px, py, t.tx, t.ty);
+    // kept in registers through the walk rather than recomputed in it
+    asm volatile("" : "+v"(pt.XX), "+v"(pt.XY), "+v"(pt.YY), "+v"(pt.X), "+v"(pt.Y));
     const bool valid = px < t.width && py < t.height;
     // pixels outside the image start saturated and are never written
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = valid ? kInv255 : 0.0f;  // tr: T / 255
 
-    for (uint32_t base = t.range.x; base < t.range.y; base += BATCH) {
+    // a batch's list entries are loaded before the previous batch's walk, so
+    // its staging waits for the splat records only
+    const uint32_t end = t.range.y;
+    uint32_t g_next = t.range.x + tid < end ? a.tile_vals[t.range.x + tid] : 0u;
+    for (uint32_t base = t.range.x; base < end; base += BATCH) {
         if (!__syncthreads_or(tr >= kUMin)) break;
         uint32_t e = base + tid;
-        if (e < t.range.y) {
-            const float cutoff = stage_splat(a, t, a.tile_vals[e], tid, st);
+        if (e < end) {
+            const StagedTerms s = stage_splat<16>(a, t, load_record(a, g_next), tid, st);
             // the quadrants the splat can touch, tested once here for all
             // four waves (the tests share the splat's terms)
-            const f32x4_t geo = st.geo[tid], par = st.par[tid];
-            const int4 r = unpack_rect16(make_uint2(__float_as_uint(par.y), __float_as_uint(st.col[tid].w)));
-            uint32_t quads = 0u;
+            uint32_t quads = s.slow ? 16u : 0u;
+            const QuadCullTerms qc = quad_cull_terms(s);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int x0 = t.tx * 16 + (q & 1) * 8, y0 = t.ty * 16 + (q >> 1) * 8;
-                bool hit = r.x < x0 + 8 && r.y > x0 && r.z < y0 + 8 && r.w > y0;
-                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(r, geo, par.x, cutoff, par.z, x0, y0);
+                bool hit = s.rect.x < x0 + 8 && s.rect.y > x0 && s.rect.z < y0 + 8 && s.rect.w > y0;
+                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(s, qc, x0, y0);
                 quads |= hit ? 1u << q : 0u;
             }
             st.quads[tid] = quads;
         }
         __syncthreads();
-        const uint32_t cnt = min((uint32_t)BATCH, t.range.y - base);
+        if (base + BATCH + tid < end) g_next = a.tile_vals[base + BATCH + tid];
+        const uint32_t cnt = min((uint32_t)BATCH, end - base);
         if (!__ballot(tr >= kUMin)) continue;
         for (uint32_t chunk = 0; chunk < cnt; chunk += 64) {
-            // which of these 64 splats touch this wave's quadrant
+            // which of these 64 splats touch this wave's quadrant, and which
+            // of them take the slow step
             const uint32_t j = chunk + lane;
-            const bool hit = j < cnt && ((st.quads[j] >> wave) & 1u);
-            uint64_t mask = __ballot(hit);
+            const uint32_t q = j < cnt ? st.quads[j] : 0u;
+            uint64_t mask = __ballot((q >> wave) & 1u);
+            const uint64_t slow = __ballot((q >> 4) & 1u);
             while (mask) {
-                const uint32_t k = chunk + (uint32_t)__builtin_ctzll(mask);
+                const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+                const uint32_t k = chunk + b;
                 mask &= mask - 1ull;
-                blend_step(pxy, fx, fy, st.geo[k], st.par[k], st.col[k], st.exact + k, cr, cg, cb, tr);
+                const f32x4_t geo = st.geo[k], par = st.par[k], col = st.col[k];
+                if ((slow >> b) & 1ull)
+                    blend_step<true>(pt, geo, par, col, st.exact + k, st.ctr + k, cr, cg, cb, tr);
+                else
+                    blend_step<false>(pt, geo, par, col, st.exact + k, st.ctr + k, cr, cg, cb, tr);
             }
             if (!__ballot(tr >= kUMin)) break;
         }
@@ -768,7 +901,7 @@ __global__ void __launch_bounds__(256) composite_ref_kernel(CompositeLaunch a) {
     __shared__ f32x4_t s_col[256];
     __shared__ uint32_t s_quads[256];
     __shared__ uint32_t s_item;
-    const StagedSplats st{s_geo, s_par, s_col, nullptr, s_quads};
+    const StagedSplats st{s_geo, s_par, s_col, nullptr, s_quads, nullptr};
     if (a.persistent_blocks == 0u) {
         composite_tile_ref(a, xcd_remap(blockIdx.x, a.total_tiles), st);
         return;
@@ -796,9 +929,10 @@ __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads), (QU
     __shared__ f32x4_t s_par[BATCH];
     __shared__ f32x4_t s_col[BATCH];
     __shared__ f32x4_t s_exact[BATCH];
+    __shared__ float2 s_ctr[BATCH];
     __shared__ uint32_t s_quads[QUAD ? BATCH : 1];
     __shared__ uint32_t s_item;
-    const StagedSplats st{s_geo, s_par, s_col, s_exact, s_quads};
+    const StagedSplats st{s_geo, s_par, s_col, s_exact, s_quads, s_ctr};
     if (a.persistent_blocks == 0u) {
         const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
         if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);

@@ -52,8 +52,8 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
     return q;
 }
 
-// Pixels [i0, i1) x [j0, j1) inside the image whose centre the quad covers,
-// optionally cut down to those where the fragment test can pass.
+// Pixel bounds [i0, i1) x [j0, j1) (not cut to the image) outside which the
+// fragment test cannot pass; false when the bound does not apply.
 //
 // The fragment test keeps power = -q(d)/2 >= cutoff = log((1/255) / o), i.e.
 // q(d) <= c = -2 cutoff, with q(d) = d^T K d and K the conic.  When the
@@ -65,35 +65,50 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
 // so the box is grown by that relative margin (plus 0.3% and an absolute
 // 1e-5 on c, and 1e-3 px) and is only used for 1 - rho^2 >= 0.01.  Every
 // pixel that can pass the fragment test stays inside, so composites with and
-// without the cut are identical; it only shortens the tile lists.  A jitter
-// (jx, jy) moves the quad and the fragment centre (cx + jx, cy + jy).  pad
-// grows the rectangle by that many pixels on each side before it is cut to
-// the image: with pad 1 it holds the rectangle of every jitter up to half a
-// pixel (each bound moves by less than one pixel, rounding included).
+// without the cut are identical; it only shortens the tile lists.
+__device__ __forceinline__ bool ellipse_pixel_bounds(float cx, float cy, float sa, float sb, float sc, float opacity,
+                                                     int4* e) {
+    const float det = sa * sc - sb * sb;
+    const float r2 = det / (sa * sc);
+    if (!(det >= 1e-6f && r2 >= 0.01f)) return false;
+    const float c = maxf(-2.0f * __logf((1.0f / 255.0f) / maxf(opacity, 1e-8f)), 0.0f);
+    const float cg = c * (1.003f + 4.8e-6f / (r2 * r2)) + 1e-5f;
+    const float ex = __builtin_sqrtf(cg * sa) + 1e-3f;
+    const float ey = __builtin_sqrtf(cg * sc) + 1e-3f;
+    // pixel i can pass only if |i + 0.5 - cx| <= ex
+    e->x = (int)__builtin_ceilf(clampf(cx - ex - 0.5f, -2.0f, 65536.0f));
+    e->y = (int)__builtin_floorf(clampf(cx + ex - 0.5f, -2.0f, 65536.0f)) + 1;
+    e->z = (int)__builtin_ceilf(clampf(cy - ey - 0.5f, -2.0f, 65536.0f));
+    e->w = (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1;
+    return true;
+}
+
+// Pixels [i0, i1) x [j0, j1) (not cut to the image) whose centre the quad
+// covers.
+__device__ __forceinline__ int4 quad_pixel_bounds(float cx, float cy, float rx, float ry, int width, int height,
+                                                  float jx = 0.0f, float jy = 0.0f) {
+    const int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height, jx, jy);
+    return make_int4(ceil_div256(q.x - 128), ceil_div256(q.y - 128), ceil_div256(q.z - 128), ceil_div256(q.w - 128));
+}
+
+// Pixels [i0, i1) x [j0, j1) inside the image whose centre the quad covers,
+// optionally cut down to the ellipse bounds above.  A jitter (jx, jy) moves
+// the quad and the fragment centre (cx + jx, cy + jy).  pad grows the
+// rectangle by that many pixels on each side before it is cut to the image:
+// with pad 1 it holds the rectangle of every jitter up to half a pixel (each
+// bound moves by less than one pixel, rounding included).
 __device__ __forceinline__ int4 splat_pixel_rect(float cx, float cy, float sa, float sb, float sc, float rx,
                                                  float ry, float opacity, int width, int height, bool tight,
                                                  float jx = 0.0f, float jy = 0.0f, int pad = 0) {
-    int4 q = quad_fixed(cx, cy, rx, ry, (float)width, (float)height, jx, jy);
-    cx = cx + jx;
-    cy = cy + jy;
-    int i0 = ceil_div256(q.x - 128), i1 = ceil_div256(q.y - 128);
-    int j0 = ceil_div256(q.z - 128), j1 = ceil_div256(q.w - 128);
-    if (tight) {
-        const float det = sa * sc - sb * sb;
-        const float r2 = det / (sa * sc);
-        if (det >= 1e-6f && r2 >= 0.01f) {
-            const float c = maxf(-2.0f * __logf((1.0f / 255.0f) / maxf(opacity, 1e-8f)), 0.0f);
-            const float cg = c * (1.003f + 4.8e-6f / (r2 * r2)) + 1e-5f;
-            const float ex = __builtin_sqrtf(cg * sa) + 1e-3f;
-            const float ey = __builtin_sqrtf(cg * sc) + 1e-3f;
-            // pixel i can pass only if |i + 0.5 - cx| <= ex
-            i0 = max(i0, (int)__builtin_ceilf(clampf(cx - ex - 0.5f, -2.0f, 65536.0f)));
-            i1 = min(i1, (int)__builtin_floorf(clampf(cx + ex - 0.5f, -2.0f, 65536.0f)) + 1);
-            j0 = max(j0, (int)__builtin_ceilf(clampf(cy - ey - 0.5f, -2.0f, 65536.0f)));
-            j1 = min(j1, (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1);
-        }
+    int4 q = quad_pixel_bounds(cx, cy, rx, ry, width, height, jx, jy);
+    int4 e;
+    if (tight && ellipse_pixel_bounds(cx + jx, cy + jy, sa, sb, sc, opacity, &e)) {
+        q.x = max(q.x, e.x);
+        q.y = min(q.y, e.y);
+        q.z = max(q.z, e.z);
+        q.w = min(q.w, e.w);
     }
-    return make_int4(max(i0 - pad, 0), min(i1 + pad, width), max(j0 - pad, 0), min(j1 + pad, height));
+    return make_int4(max(q.x - pad, 0), min(q.y + pad, width), max(q.z - pad, 0), min(q.w + pad, height));
 }
 
 // The rectangle above packed (i0 | i1 << 16, j0 | j1 << 16); all zero when
