@@ -18,7 +18,10 @@
 // An optional second 32-bit value array moves with the same permutation.
 // No kernel waits on another block or takes a global ticket (device-scope
 // atomics on one address serialize across XCDs on MI355X).  The element
-// count lives in device memory; blocks past it exit at once.
+// count lives in device memory; blocks past the tiles it needs exit at once,
+// and the others take their tiles XCD by XCD (xcd_remap): the digit runs of
+// neighbouring tiles, which end and start in the same cache lines of the
+// output, are then written through the same L2.
 #include "gs_internal.h"
 
 namespace gs {
@@ -94,9 +97,10 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint2 kr;
     const uint32_t count = load_count_range(a, &kr);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t used = (count + (uint32_t)kTile - 1u) / (uint32_t)kTile;
+    if (blockIdx.x >= used) return;
+    const uint32_t tile = xcd_remap(blockIdx.x, used);
     const uint32_t base = tile * (uint32_t)kTile;
-    if (base >= count) return;
     const PassPlan plan = pass_plan<BITS>(a, kr);
     if (!plan.active) return;
     for (uint32_t i = tid; i < (uint32_t)(kWaves * kCopies * RADIX); i += kThreads) (&s_hist[0][0])[i] = 0u;
@@ -173,9 +177,10 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint2 kr;
     const uint32_t count = load_count_range(a, &kr);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t used = (count + (uint32_t)kTile - 1u) / (uint32_t)kTile;
+    if (blockIdx.x >= used) return;
+    const uint32_t tile = xcd_remap(blockIdx.x, used);
     const uint32_t base = tile * (uint32_t)kTile;
-    if (base >= count) return;
     const PassPlan plan = pass_plan<BITS>(a, kr);
     if (!plan.active) return;
     const uint32_t* keys_in = a.keys_in;
