@@ -64,6 +64,8 @@ def _parse():
     ap.add_argument("--converged-frames", type=int, default=256)
     ap.add_argument("--multiview-steps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--denoise-only", action="store_true",
+                    help="only time the learned denoiser on the workload's 1 spp stochastic frame and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (calibrates HBM byte counters) and exit")
     return ap.parse_args()
@@ -85,6 +87,39 @@ def copy_calibration(nbytes: int) -> None:
     hip.hipFree(src)
     hip.hipFree(dst)
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+
+
+def denoise_only(workload: str, steps: int, warmup: int) -> dict:
+    """The learned denoiser alone on the workload's 1 spp GPS frame (trained
+    weights): wall time per pass over `steps` passes after `warmup`, and its
+    MFMA rate against the bf16 peak."""
+    import numpy as np
+
+    import gamesplat_amd as gs
+    from gamesplat_amd import scenes
+
+    g, cfg = scenes.scene_config(workload)
+    W, H = cfg["width"], cfg["height"]
+    cam = scenes.random_view_camera(cfg["seed"], W, H)
+    with gs.Context(0) as ctx, gs.Denoiser(ctx, seed=1) as den:
+        if WEIGHTS.exists():
+            den.set_params(np.fromfile(WEIGHTS, dtype=np.float32))
+        ctx.upload(g)
+        ctx.set_gps_budget(16 * 1024 * 1024)
+        ctx.cull(cam)
+        ctx.gps_sample(1, 1)
+        ctx.accumulate(1)
+        for _ in range(warmup):
+            den.denoise()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            den.denoise()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+    fl = unet_flops(W + (-W) % 8, H + (-H) % 8)
+    return {"metric": "denoiser pass", "ms": round(ms, 4), "width": W, "height": H, "gflop": round(fl / 1e9, 2),
+            "mfma_tflops": round(fl / (ms * 1e-3) / 1e12, 2), "frac_of_bf16_peak": round(fl / (ms * 1e-3) / 2.5e15, 4)}
 
 
 def upload_rates(ctx, g, repeats: int = 3) -> dict:
@@ -418,6 +453,9 @@ def main():
     args = _parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return 0
+    if args.denoise_only:
+        print(json.dumps(denoise_only(args.workload, args.steps, args.warmup)), flush=True)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "

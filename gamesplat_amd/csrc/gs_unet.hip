@@ -180,36 +180,38 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
                                                                      0, 0);
         }
     };
-    // two chunks per iteration with ping-pong weight registers: the loads of
-    // chunks c + 2 and c + 3 are in flight while chunks c and c + 1 run
-    uint4 w0[NT], w1[NT];
-    load_w(w0, 0);
-    int off0 = s_tab[kg], off1 = 0;
-    if (chunks > 1) {
-        load_w(w1, 1);
-        off1 = s_tab[4 + kg];
-    }
-    for (int c = 0; c < chunks; c += 2) {
-        chunk(w0, off0);
-        if (c + 2 < chunks) {
-            load_w(w0, c + 2);
-            off0 = s_tab[(c + 2) * 4 + kg];
-        }
-        if (c + 1 < chunks) {
-            chunk(w1, off1);
-            if (c + 3 < chunks) {
-                load_w(w1, c + 3);
-                off1 = s_tab[(c + 3) * 4 + kg];
-            }
-        }
-    }
-
-    // acc[r][t][i] = output channel t * 16 + 4 kg + i of pixel `row` of row group r
+    // the bias is loaded before the main loop, which hides its latency
     float bias[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) bias[t][i] = a.bias ? a.bias[t * 16 + 4 * kg + i] : 0.0f;
+    // weights PD chunks ahead: a chunk's MFMAs take ~16 NT RG cycles, too few
+    // to cover an L2 load with the two-chunk lead of a ping-pong pair
+    constexpr int PD = NT <= 2 ? 4 : (NT <= 4 ? 3 : 2);
+    uint4 w[PD][NT];
+    int off[PD];
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+        off[j] = 0;
+        if (j < chunks) {
+            load_w(w[j], j);
+            off[j] = s_tab[j * 4 + kg];
+        }
+    }
+    for (int c = 0; c < chunks; c += PD) {
+#pragma unroll
+        for (int j = 0; j < PD; ++j) {
+            if (c + j < chunks) {
+                chunk(w[j], off[j]);
+                if (c + j + PD < chunks) {
+                    load_w(w[j], c + j + PD);
+                    off[j] = s_tab[(c + j + PD) * 4 + kg];
+                }
+            }
+        }
+    }
+    // acc[r][t][i] = output channel t * 16 + 4 kg + i of pixel `row` of row group r
     uint16_t* yb = a.y + (size_t)n * a.h * a.w * a.ys + a.yo + 4 * kg;
     uint32_t q[RG][NT][2];
 #pragma unroll
