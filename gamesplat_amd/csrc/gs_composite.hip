@@ -213,8 +213,10 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
 // covered pixels [i0, i1) x [j0, j1) ride in par.w and col.w as the packed
 // corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so one step reads
 // three 16-byte records at one address.  The fourth record and the centre
-// hold the reference's own conic, cutoff and centre, read only for the
-// fragments whose fused test lies within par.z of the cutoff.
+// hold the reference's own conic, the argument of its cutoff's log and the
+// centre, read only for the fragments whose fused test lies within par.z of
+// the cutoff (under one wave step in a hundred): the log itself is taken
+// there, not at staging.
 // (A native 4-float vector, so each record is one 128-bit LDS access.)
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
@@ -222,7 +224,7 @@ struct StagedSplats {
     f32x4_t* geo;   // A, B, C, E
     f32x4_t* par;   // F, G, decision margin (log2 units), rect (low corner bits)
     f32x4_t* col;   // r, g, b, rect (high corner bits)
-    f32x4_t* exact; // c / det, 2 (-b / det), a / det, log((1/255) / max(o, 1e-8)): the reference's operations
+    f32x4_t* exact; // c / det, 2 (-b / det), a / det, (1/255) / max(o, 1e-8): the reference's operations
     uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q; bit 4: slow step
     float2* ctr;    // cx, cy
 };
@@ -298,8 +300,9 @@ struct QuadCullTerms {
 
 // Stage splat g into slot t of a T x T tile: conic from one reciprocal of the
 // clamped determinant in log2 units, pixel rectangle, the reference's conic
-// ((c, -b, a) / det, shaders/cull.glsl:273-275), fragment cutoff
-// log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70) and centre, and the
+// ((c, -b, a) / det, shaders/cull.glsl:273-275), the argument of its
+// fragment cutoff log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70) and
+// centre, and the
 // margin within which the fused test may decide differently from the
 // reference's float test.
 //
@@ -391,7 +394,7 @@ __device__ __forceinline__ StagedTerms stage_splat(const CompositeLaunch& a, con
     s.geo[t] = f32x4_t{A, B, C, E};
     s.par[t] = f32x4_t{F, G, st.margin, __uint_as_float(rect.x)};
     s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
-    s.exact[t] = f32x4_t{k0, 2.0f * k1, k2, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f))};
+    s.exact[t] = f32x4_t{k0, 2.0f * k1, k2, (1.0f / 255.0f) / maxf(r0.z, 1e-8f)};
     s.ctr[t] = make_float2(r0.x, r0.y);
     return st;
 }
@@ -447,12 +450,20 @@ __device__ __forceinline__ void blend_step(const PixelTerms& p, f32x4_t geo, f32
     // power' >= margin: passes; below -margin: fails; between: the reference's test
     const bool lo = power >= -par.z;
     bool pass = lo;
-    if (lo & !(power >= par.z)) {
+    if (__builtin_expect(lo & !(power >= par.z), 0)) {
         const f32x4_t e = *exact;
         const float2 c = *ctr;
         const float dx = p.fx - c.x, dy = p.fy - c.y;
         const float pr = -0.5f * (e.x * dx * dx + e.y * dx * dy + e.z * dy * dy);
-        pass = !(pr > 0.0f) & !(pr < e.w);
+        // pr < cutoff, the cutoff being the deterministic float log of e.w:
+        // v_log_f32 (1 ulp of log2) times ln 2 lies within 4e-6 of it over
+        // the cutoff's range (|cutoff| < 16), so where pr is 1e-5 or more
+        // away from that estimate the two compare alike; closer, the
+        // deterministic log decides
+        const float est = __builtin_amdgcn_logf(e.w) * 0.69314718f;
+        bool below = pr < est;
+        if (__builtin_expect(__builtin_fabsf(pr - est) < 1e-5f, 0)) below = pr < logf_(e.w);
+        pass = !(pr > 0.0f) & !below;
     }
     if (SLOW) pass &= in_rect16(p.pxy, make_uint2(__float_as_uint(par.w), __float_as_uint(col.w)));
     alpha = pass ? alpha : 0.0f;

@@ -68,11 +68,12 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
 // without the cut are identical; it only shortens the tile lists.
 __device__ __forceinline__ bool ellipse_pixel_bounds(float cx, float cy, float sa, float sb, float sc, float opacity,
                                                      int4* e) {
+    // reciprocals to 1 ulp: their error is far inside the 0.3% and 2x slack
     const float det = sa * sc - sb * sb;
-    const float r2 = det / (sa * sc);
+    const float r2 = det * __builtin_amdgcn_rcpf(sa * sc);
     if (!(det >= 1e-6f && r2 >= 0.01f)) return false;
     const float c = maxf(-2.0f * __logf((1.0f / 255.0f) / maxf(opacity, 1e-8f)), 0.0f);
-    const float cg = c * (1.003f + 4.8e-6f / (r2 * r2)) + 1e-5f;
+    const float cg = c * (1.003f + 4.8e-6f * __builtin_amdgcn_rcpf(r2 * r2)) + 1e-5f;
     const float ex = __builtin_sqrtf(cg * sa) + 1e-3f;
     const float ey = __builtin_sqrtf(cg * sc) + 1e-3f;
     // pixel i can pass only if |i + 0.5 - cx| <= ex
