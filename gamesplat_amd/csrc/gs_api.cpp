@@ -1041,9 +1041,13 @@ int prepare_composite(gs_ctx* c, int tile) {
     GS_CHECK(ensure(c, c->image, (size_t)c->total_pixels * 8, "RGBA16F images"));
     GS_CHECK(ensure(c, c->ranges, (size_t)c->total_tiles * 8, "tile ranges"));
     if (!c->dup_cap) {
-        // three pairs per splat of every view to start with, formed in 64
-        // bits and capped at the pair limit; an overflow grows them
-        const size_t want = std::min<size_t>(std::max<size_t>((size_t)3 * c->views * c->n, 65536), 0x7FFFFFFF);
+        // three pairs per splat for the first view and one and a half for
+        // every other one to start with (a 6M-splat 1080p frame takes 1.9
+        // per splat, a batch of 64 mixed views of 3M splats 1.4 per splat
+        // and view), formed in 64 bits and capped at the pair limit; an
+        // overflow grows them
+        const size_t want = std::min<size_t>(
+            std::max<size_t>((size_t)3 * c->n + (size_t)3 * (c->views - 1) * c->n / 2, 65536), 0x7FFFFFFF);
         GS_CHECK(grow_tile_buffers(c, (uint32_t)want));
     }
     (void)tile;
@@ -1370,6 +1374,7 @@ int gs_autotune(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, const G
     const int tiles[3] = {16, 8, 32}, bits[2] = {8, 4}, bpcs[5] = {0, 1, 2, 4, 8};
     GsTuneResult best{};
     best.best_ms = 1e30;
+    uint32_t best_pairs = 0;
     int r = GS_OK, configs = 0;
     for (int ti = 0; ti < 3 && r == GS_OK; ++ti)
         for (int bi = 0; bi < 2 && r == GS_OK; ++bi)
@@ -1398,6 +1403,7 @@ int gs_autotune(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, const G
                     best.tile_size = tiles[ti];
                     best.radix_bits = bits[bi];
                     best.blocks_per_cu = bpcs[pi];
+                    best_pairs = c->last_dup;
                 }
             }
     (void)hipEventDestroy(e0);
@@ -1413,6 +1419,18 @@ int gs_autotune(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, const G
     c->default_tile = best.tile_size;
     c->radix_bits = best.radix_bits;
     c->blocks_per_cu = best.blocks_per_cu;
+    // the tile-pair buffers grew for the configuration with the most pairs
+    // (8-pixel tiles): back to the chosen one's count with a quarter to spare
+    const size_t keep = std::max<size_t>((size_t)best_pairs + best_pairs / 4u, 65536);
+    if (keep < c->dup_cap && !c->dup_pending) {
+        GS_HIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 3; ++k) {
+            release(c, c->dup_keys[k]);
+            release(c, c->dup_vals[k]);
+        }
+        c->dup_cap = 0;
+        GS_CHECK(grow_tile_buffers(c, (uint32_t)keep));
+    }
     *out = best;
     return ok();
 }
