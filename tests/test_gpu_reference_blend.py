@@ -98,6 +98,17 @@ def test_reference_blend_c2_orbit(ctx, orc, gs, frame):
     _check(ref_img, fast, m2, m0, f"c2 frame {frame}")
 
 
+def test_reference_blend_c3_full_frame(ctx, orc, gs):
+    """C3's frame: 6M Gaussians (256 clusters, opacity logit mean 1) at
+    1920x1080, the workload of the benchmark.  Its many small, mostly opaque
+    splats put the most fragments near the 1/255 cutoff, where the fast
+    path's tile-centred power and its exact test meet."""
+    g, cfg = scenes.scene_config("c3")
+    cam = scenes.random_view_camera(cfg["seed"], cfg["width"], cfg["height"])
+    print(f"case: c3 full frame n={len(g)}", flush=True)
+    _check(*_frames(ctx, orc, g, cam), "c3")
+
+
 _C2 = {}
 
 
