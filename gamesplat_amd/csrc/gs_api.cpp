@@ -103,7 +103,9 @@ struct gs_ctx {
     DevBuf vrects[4];  // packed tile rectangles moved by the depth sort
     DevBuf block_range;  // [view][block]: visible key range
     DevBuf pad_rects;  // [view][n]: packed tile rectangles grown by a pixel (StochasticSplats)
-    DevBuf planes, records, keys[4], ids[4], dense_keys, block_counts, block_offsets, block_diag, rects, rect_px,
+    // cull_offsets: the cull's per-block visible offsets, kept for the
+    // compaction (block_offsets is shared by the bin and GPS scans)
+    DevBuf planes, records, keys[4], ids[4], dense_keys, cull_offsets, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
 
@@ -185,6 +187,10 @@ struct gs_ctx {
     std::vector<uint8_t> graph_key;
     uint32_t graph_captures = 0, graph_replays = 0;
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
+    // the last cull's visible list (keys[0], ids[0], vrects[0]) is built:
+    // GPS and the visible-list readbacks build it on demand; the depth sort
+    // otherwise compacts in its first pass
+    bool compacted = false;
     bool stats_collected = false;
     int sort_slot = 0;
     int rect_tile = 0;       // tile size of the packed rectangles the last cull wrote (0: none)
@@ -264,6 +270,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
     GS_CHECK(ensure(c, c->block_range, blocks * 8, "block key ranges"));
     GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
+    GS_CHECK(ensure(c, c->cull_offsets, blocks * 4, "cull block offsets"));
     GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
     GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
     GS_CHECK(ensure(c, c->pad_rects, cap * 4, "padded tile rectangles"));
@@ -459,7 +466,7 @@ void gs_destroy(gs_ctx* c) {
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->keys[3], &c->ids[0],
-                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->block_counts, &c->block_offsets, &c->block_diag,
+                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->cull_offsets, &c->block_counts, &c->block_offsets, &c->block_diag,
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
@@ -914,6 +921,20 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     s.bits = c->radix_bits == 8 ? 7 : c->radix_bits;
     s.passes = (32 + s.bits - 1) / s.bits;
     s.wide_index = c->wide_index;
+    // without a visible list the first pass reads the cull's dense keys and
+    // drops the culled slots (the list it keeps is the one the compaction
+    // would build, in the same order)
+    if (!c->compacted) {
+        const CullLaunch& cl = c->plan.cull;
+        s.dense_keys = cl.dense_keys;
+        s.dense_count = c->views * c->n;
+        if (c->rect_tile && c->rect_pad) {
+            s.dense_rect_packed = cl.pad_rects;
+        } else if (c->rect_tile) {
+            s.dense_rect_px = cl.rect_px;
+            while ((1 << s.dense_rect_shift) < c->rect_tile) ++s.dense_rect_shift;
+        }
+    }
     return s;
 }
 
@@ -992,13 +1013,23 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
     GS_CHECK(timed_begin(c, GS_STAGE_CULL_KERNEL, &tk));
     GS_HIP(launch_cull(p.cull, c->stream));
     GS_CHECK(timed_end(c, &tk));
-    GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->block_offsets),
+    GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->cull_offsets),
                               &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream, 0,
                               nullptr, p.cull.block_range, &c->dc()->cull.key_min));
-    GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->block_offsets),
+    GS_CHECK(timed_end(c, &t));
+    c->compacted = false;
+    return GS_OK;
+}
+
+// The last cull's visible list in index order (keys[0], ids[0], vrects[0]),
+// for the consumers that read it unsorted.
+int ensure_compacted(gs_ctx* c) {
+    if (c->compacted) return GS_OK;
+    const FramePlan& p = c->plan;
+    GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets),
                           ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
                           ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_pad ? p.cull.pad_rects : nullptr));
-    GS_CHECK(timed_end(c, &t));
+    c->compacted = true;
     return GS_OK;
 }
 
@@ -1319,6 +1350,7 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     p.pack_pad = 0;
     c->rect_tile = p.pack_tile;
     c->rect_pad = 0;
+    c->compacted = false;  // this frame's cull builds no visible list
     p.depth = make_depth_sort(c);
     // the depth sort's last needed pass writes the sorted list to buffer 3
     c->sort_slot = 3;
@@ -1479,6 +1511,7 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     c->res_w = W;
     c->res_h = H;
 
+    GS_CHECK(ensure_compacted(c));
     GpsLaunch g{};
     g.records = ptr<float4>(c->records);
     g.vis_ids = ptr<uint32_t>(c->ids[0]);
@@ -1803,6 +1836,7 @@ int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
         case GS_BUFFER_VISIBLE_IDS:
         case GS_BUFFER_VISIBLE_KEYS: {
             if (bytes < (size_t)V * 4) return fail(GS_ERR_INVALID, "visible buffer needs %zu bytes", (size_t)V * 4);
+            GS_CHECK(ensure_compacted(c));
             DevBuf& src = which == GS_BUFFER_VISIBLE_KEYS ? c->keys[0] : c->ids[0];
             GS_HIP(hipMemcpyAsync(host, src.p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
             GS_HIP(hipStreamSynchronize(c->stream));
@@ -1810,6 +1844,7 @@ int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
         }
         case GS_BUFFER_PROJECTED: {
             if (bytes != n * 64) return fail(GS_ERR_INVALID, "projected buffer needs %zu bytes", n * 64);
+            GS_CHECK(ensure_compacted(c));
             std::vector<uint32_t> vis(V);
             std::vector<float> rec(n * kRecordVec4 * 4);
             if (V) GS_HIP(hipMemcpyAsync(vis.data(), c->ids[0].p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));

@@ -220,6 +220,16 @@ struct SortLaunch {
     int passes;
     int bits;                // digit width (8, 7, 6 or 4)
     int wide_index;          // 1: 64-bit element indexing even where 32-bit offsets suffice
+    // optional dense first pass (digit widths 7 and 4): it reads dense_count
+    // slots of dense_keys (0xFFFFFFFF: culled) instead of keys[0] and count,
+    // keeps the others in slot order with the slot as value, and with vals2
+    // their tile rectangles: dense_rect_packed[slot] as it is, else
+    // dense_rect_px[slot] packed at 1 << dense_rect_shift pixels a tile
+    const uint32_t* dense_keys;
+    uint32_t dense_count;
+    const uint2* dense_rect_px;
+    const uint32_t* dense_rect_packed;
+    int dense_rect_shift;
 };
 // Stable LSD radix sort.  Returns the index (1, 2, or 3 with key_range) of
 // the buffer holding the result through *result_slot.

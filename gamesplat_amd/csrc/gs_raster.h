@@ -128,10 +128,9 @@ __device__ __forceinline__ uint2 pixel_rect(float cx, float cy, float sa, float 
 // 1 << shift, x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when empty; the view must
 // span at most 255 tiles a side).
 __device__ __forceinline__ uint32_t pack_tile_rect(uint2 px, int shift) {
-    if (!px.x) return 0u;
     const uint32_t x0 = (px.x & 0xFFFFu) >> shift, x1 = (((px.x >> 16) - 1u) >> shift) + 1u;
     const uint32_t y0 = (px.y & 0xFFFFu) >> shift, y1 = (((px.y >> 16) - 1u) >> shift) + 1u;
-    return x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
+    return px.x ? x0 | (x1 << 8) | (y0 << 16) | (y1 << 24) : 0u;
 }
 
 }  // namespace gs

@@ -16,6 +16,11 @@
 //              array, which keeps the block's LDS small enough for seven
 //              blocks per CU.
 // An optional second 32-bit value array moves with the same permutation.
+// The first pass can also read the cull's dense [view][n] keys directly
+// (DENSE): slots holding the culled key 0xFFFFFFFF take no digit count and no
+// rank, so that pass also does the order-preserving compaction of the visible
+// splats; their values are the slots themselves and their second values the
+// splats' packed tile rectangles.
 // No kernel waits on another block or takes a global ticket (device-scope
 // atomics on one address serialize across XCDs on MI355X).  The element
 // count lives in device memory; blocks past the tiles it needs exit at once,
@@ -23,6 +28,7 @@
 // neighbouring tiles, which end and start in the same cache lines of the
 // output, are then written through the same L2.
 #include "gs_internal.h"
+#include "gs_raster.h"
 
 namespace gs {
 
@@ -51,6 +57,14 @@ struct PassArgs {
     int adaptive;
     int pass, passes;           // this pass, passes launched
     int shift;
+    // dense first pass: keys_in holds dense_count slots (0xFFFFFFFF: culled);
+    // the second value is rect_packed[slot] or rect_px[slot] packed at tile
+    // size 1 << rect_shift (neither: no second value)
+    int dense;
+    uint32_t dense_count;
+    const uint2* rect_px;
+    const uint32_t* rect_packed;
+    int rect_shift;
 };
 
 // Whether pass `pass` runs and what it adds to or subtracts from the keys.
@@ -78,9 +92,10 @@ __device__ __forceinline__ PassPlan pass_plan(const PassArgs& a, uint2 kr) {
     return p;
 }
 
-// The element count and the key range, loaded together (one memory wait).
+// The element count (the slot count of a dense pass) and the key range,
+// loaded together (one memory wait).
 __device__ __forceinline__ uint32_t load_count_range(const PassArgs& a, uint2* kr) {
-    const uint32_t count = *a.count;
+    const uint32_t count = a.dense ? a.dense_count : *a.count;
     *kr = *reinterpret_cast<const uint2*>(a.key_range);
     return count;
 }
@@ -88,7 +103,7 @@ __device__ __forceinline__ uint32_t load_count_range(const PassArgs& a, uint2* k
 // Digit counts of one tile.  Each wave counts into kCopies histograms, lane
 // l into copy l % kCopies, so a digit shared by many lanes (the top digit of
 // clustered keys) serializes kCopies times fewer LDS atomics.
-template <int BITS, bool O32>
+template <int BITS, bool O32, bool DENSE>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -112,12 +127,13 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        keys[i] = g < count ? at<O32>(keys_in, g) : 0u;
+        keys[i] = g < count ? at<O32>(keys_in, g) : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        if (g < count) atomicAdd(&hist[((keys[i] - plan.sub) >> a.shift) & MASK], 1u);
+        if (DENSE ? keys[i] != 0xFFFFFFFFu : g < count)
+            atomicAdd(&hist[((keys[i] - plan.sub) >> a.shift) & MASK], 1u);
     }
     __syncthreads();
     if (tid < (uint32_t)RADIX) {
@@ -163,7 +179,7 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     if (tid == 0) a.col_total[d] = carry;
 }
 
-template <int BITS, bool P2, bool O32>
+template <int BITS, bool P2, bool O32, bool DENSE>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -204,11 +220,18 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int kPairs = (kItems + 1) / 2, kQuads = (kItems + 3) / 4;
     static_assert(kTile <= 65536 && BITS <= 8, "packed ranks and digits");
     uint32_t keys[kItems], ranks[kPairs];
+    uint32_t kept = 0u;  // DENSE: bit i set when item i holds a visible splat
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
         const uint32_t k = at<O32>(keys_in, min(g, last));
-        keys[i] = g < count ? k - plan.sub : 0xFFFFFFFFu;
+        if constexpr (DENSE) {
+            const bool keep = g < count && k != 0xFFFFFFFFu;
+            kept |= (uint32_t)keep << i;
+            keys[i] = keep ? k - plan.sub : 0xFFFFFFFFu;
+        } else {
+            keys[i] = g < count ? k - plan.sub : 0xFFFFFFFFu;
+        }
     }
 
     // 2. stable rank within the wave: peers = lanes with the same digit,
@@ -217,10 +240,16 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     //    its rank.  Every lane reads the wave's digit counter before the
     //    lowest peer (no peers below it) advances it: LDS accesses of one
     //    wave complete in program order, and no other wave uses the counter.
+    //    In a dense pass only kept lanes are peers, and only they count.
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const uint32_t d = (keys[i] >> shift) & MASK;
         uint32_t plo = ~0u, phi = ~0u;
+        if constexpr (DENSE) {
+            const uint64_t kb = __ballot((kept >> i) & 1u);
+            plo = (uint32_t)kb;
+            phi = (uint32_t)(kb >> 32);
+        }
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
             const uint32_t t = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // 0 or ~0: bit b of d
@@ -230,7 +259,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
         }
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
         const uint32_t old = s_whist[wave][d];
-        if (below == 0u) s_whist[wave][d] = old + (uint32_t)(__popc(plo) + __popc(phi));
+        if (below == 0u && (!DENSE || ((kept >> i) & 1u))) s_whist[wave][d] = old + (uint32_t)(__popc(plo) + __popc(phi));
         const uint32_t rank = old + below;
         ranks[i >> 1] = (i & 1) ? ranks[i >> 1] | (rank << 16) : rank;
     }
@@ -240,6 +269,11 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     // statement that the compiler must assume changes the keys)
 #pragma unroll
     for (int i = 0; i < kItems; ++i) asm volatile("" : "+v"(keys[i]));
+    if constexpr (DENSE) asm volatile("" : "+v"(kept));
+    // likewise the lane's first position: the later loads and slots form
+    // their positions from it instead of holding those of the key loads
+    uint32_t lane_slot = wbase + lane;
+    asm volatile("" : "+v"(lane_slot));
 
     // 3. per-digit offsets: within the tile (over waves, then digits) and global
     uint32_t bin_total = 0u;
@@ -255,6 +289,9 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     }
     uint32_t total;
     uint32_t local = block_exclusive_sum<kWaves>(bin_total, s_scan, &total);
+    // a dense tile's kept splats, ranked first: they fill local positions
+    // [0, total)
+    const uint32_t out_count = DENSE ? total : valid;
     // digit bases: every block scans the column totals itself (no separate
     // single-block launch between colscan and downsweep)
     uint32_t all;
@@ -275,12 +312,17 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
         const uint32_t d = (keys[i] >> shift) & MASK;
         const uint32_t rank = ((ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) + s_local[d] + s_whist[wave][d];
         ranks[i >> 1] = (i & 1) ? (ranks[i >> 1] & 0xFFFFu) | (rank << 16) : (ranks[i >> 1] & 0xFFFF0000u) | rank;
-        s_buf[rank] = keys[i];
+        // a culled slot of a dense pass has no rank (and its position may lie
+        // past the tile)
+        if (!DENSE || ((kept >> i) & 1u)) s_buf[rank] = keys[i];
     }
-    // values of positions past the count are staged but never written out
+    // values of positions past the count are staged but never written out;
+    // a dense pass's value is the slot (formed when it is staged)
     uint32_t v[kItems];
+    if constexpr (!DENSE) {
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals_in, min(wbase + (uint32_t)i * 64u + lane, last));
+        for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals_in, min(lane_slot + (uint32_t)i * 64u, last));
+    }
     __syncthreads();
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
@@ -289,7 +331,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
-        if (p < valid) {
+        if (p < out_count) {
             uint32_t k = s_buf[p];
             const uint32_t d = (k >> shift) & MASK;
             digits[j >> 2] |= d << ((j & 3) * 8);
@@ -298,31 +340,68 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
+    for (int i = 0; i < kItems; ++i) {
+        if constexpr (DENSE) {
+            if ((kept >> i) & 1u) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = lane_slot + (uint32_t)i * 64u;
+        } else {
+            s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
+        }
+    }
     if constexpr (P2) {
+        // a dense pass packs each slot's rectangle (all loads issued before
+        // any is used; culled slots load too and are never staged)
+        if (DENSE && a.rect_packed) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals2_in, min(wbase + (uint32_t)i * 64u + lane, last));
+            for (int i = 0; i < kItems; ++i) v[i] = at<O32>(a.rect_packed, min(lane_slot + (uint32_t)i * 64u, last));
+        } else if (DENSE) {
+            // every rectangle is loaded whole before any is packed (the
+            // packing tests the first word, and a branch on it would load
+            // the second word only after the first arrived)
+            uint2 px[kItems];
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) px[i] = a.rect_px[min(lane_slot + (uint32_t)i * 64u, last)];
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) asm volatile("" : "+v"(px[i].x), "+v"(px[i].y));
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) v[i] = pack_tile_rect(px[i], a.rect_shift);
+        } else {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) v[i] = at<O32>(vals2_in, min(lane_slot + (uint32_t)i * 64u, last));
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
-        if (p < valid) at<O32>(vals_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
+        if (p < out_count) at<O32>(vals_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
     }
     if constexpr (P2) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
+        for (int i = 0; i < kItems; ++i)
+            if (!DENSE || ((kept >> i) & 1u)) s_buf[(ranks[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu] = v[i];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t p = tid + (uint32_t)j * kThreads;
-            if (p < valid) at<O32>(vals2_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
+            if (p < out_count) at<O32>(vals2_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
         }
     }
 }
 
-template <int BITS>
+template <int BITS, bool O32, bool DENSE>
+void launch_pass(const PassArgs& pa, bool p2, uint32_t tiles, hipStream_t s) {
+    hipLaunchKernelGGL((upsweep_kernel<BITS, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+    hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
+    if (p2)
+        hipLaunchKernelGGL((downsweep_kernel<BITS, true, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+    else
+        hipLaunchKernelGGL((downsweep_kernel<BITS, false, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+}
+
+// DENSE_OK: the digit widths a dense first pass is built for (the depth
+// sort's)
+template <int BITS, bool DENSE_OK>
 hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     uint32_t tiles = (a.capacity + kTile - 1) / kTile;
     if (tiles < 1) tiles = 1;
@@ -340,9 +419,11 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     pa.col_total = a.scratch;
     // every array holds at most capacity 4-byte elements
     const bool o32 = (uint64_t)a.capacity * 4u <= 0xFFFFFFFFull && !a.wide_index;
+    const bool p2 = a.vals2[0] != nullptr;
     int src = 0, dst = 1;
     for (int p = 0; p < a.passes; ++p) {
-        pa.keys_in = a.keys[src];
+        const bool dense = p == 0 && a.dense_keys;
+        pa.keys_in = dense ? a.dense_keys : a.keys[src];
         pa.vals_in = a.vals[src];
         pa.vals2_in = a.vals2[src];
         pa.keys_out = a.keys[dst];
@@ -350,19 +431,27 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.vals2_out = a.vals2[dst];
         pa.pass = p;
         pa.shift = p * BITS;
-        if (o32)
-            hipLaunchKernelGGL((upsweep_kernel<BITS, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
-        else
-            hipLaunchKernelGGL((upsweep_kernel<BITS, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
-        hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
-        if (a.vals2[0] && o32)
-            hipLaunchKernelGGL((downsweep_kernel<BITS, true, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
-        else if (a.vals2[0])
-            hipLaunchKernelGGL((downsweep_kernel<BITS, true, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
-        else if (o32)
-            hipLaunchKernelGGL((downsweep_kernel<BITS, false, true>), dim3(tiles), dim3(kThreads), 0, s, pa);
-        else
-            hipLaunchKernelGGL((downsweep_kernel<BITS, false, false>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        pa.dense = dense ? 1 : 0;
+        pa.dense_count = dense ? a.dense_count : 0u;
+        pa.rect_px = dense ? a.dense_rect_px : nullptr;
+        pa.rect_packed = dense ? a.dense_rect_packed : nullptr;
+        pa.rect_shift = dense ? a.dense_rect_shift : 0;
+        if constexpr (DENSE_OK) {
+            if (dense && o32)
+                launch_pass<BITS, true, true>(pa, p2, tiles, s);
+            else if (dense)
+                launch_pass<BITS, false, true>(pa, p2, tiles, s);
+            else if (o32)
+                launch_pass<BITS, true, false>(pa, p2, tiles, s);
+            else
+                launch_pass<BITS, false, false>(pa, p2, tiles, s);
+        } else {
+            if (dense) return hipErrorInvalidValue;
+            if (o32)
+                launch_pass<BITS, true, false>(pa, p2, tiles, s);
+            else
+                launch_pass<BITS, false, false>(pa, p2, tiles, s);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         src = dst;
@@ -384,10 +473,18 @@ hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t 
     if (a.passes < 1 || (a.passes - 1) * a.bits >= 32) return hipErrorInvalidValue;
     if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
     if (a.key_range && (!a.keys[3] || !a.vals[3] || (a.vals2[0] && !a.vals2[3]))) return hipErrorInvalidValue;
-    if (a.bits == 8) return run_passes<8>(a, result_slot, s);
-    if (a.bits == 7) return run_passes<7>(a, result_slot, s);
-    if (a.bits == 6) return run_passes<6>(a, result_slot, s);
-    if (a.bits == 4) return run_passes<4>(a, result_slot, s);
+    if (a.dense_keys) {
+        // the dense slots fit the grid; their second value is a packed
+        // rectangle, from pixel rectangles at 8, 16 or 32 pixels a tile
+        if (a.dense_count > a.capacity) return hipErrorInvalidValue;
+        if (a.vals2[0] && !a.dense_rect_packed &&
+            (!a.dense_rect_px || a.dense_rect_shift < 3 || a.dense_rect_shift > 5))
+            return hipErrorInvalidValue;
+    }
+    if (a.bits == 8) return run_passes<8, false>(a, result_slot, s);
+    if (a.bits == 7) return run_passes<7, true>(a, result_slot, s);
+    if (a.bits == 6) return run_passes<6, false>(a, result_slot, s);
+    if (a.bits == 4) return run_passes<4, true>(a, result_slot, s);
     return hipErrorInvalidValue;
 }
 
