@@ -915,10 +915,10 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     s.capacity = c->views * c->n;
     s.scratch = c->dc()->depth_sort.scratch;
     s.table = ptr<uint32_t>(c->table);
-    // 32-bit depth keys: 8-bit digits take 4 passes, 4-bit ones 8; with
-    // 8-bit digits the keys go in 7-bit passes (shorter digit tables, longer
-    // runs of equal digits), at most five, as many as the key range needs
-    s.bits = c->radix_bits == 8 ? 7 : c->radix_bits;
+    // 32-bit depth keys: 4-bit digits take 8 passes; with the 8-bit setting
+    // the keys go in 9-bit passes, at most four, as many as the key range
+    // needs (three for a range under 2^27: depths within a factor of 2^16)
+    s.bits = c->radix_bits == 8 ? 9 : c->radix_bits;
     s.passes = (32 + s.bits - 1) / s.bits;
     s.wide_index = c->wide_index;
     // without a visible list the first pass reads the cull's dense keys and

@@ -107,7 +107,7 @@ template <int BITS, bool O32, bool DENSE>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
-    constexpr int kCopies = 4;
+    constexpr int kCopies = RADIX > 256 ? 2 : 4;
     __shared__ uint32_t s_hist[kWaves * kCopies][RADIX];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint2 kr;
@@ -136,11 +136,11 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
             atomicAdd(&hist[((keys[i] - plan.sub) >> a.shift) & MASK], 1u);
     }
     __syncthreads();
-    if (tid < (uint32_t)RADIX) {
+    for (uint32_t d = tid; d < (uint32_t)RADIX; d += kThreads) {
         uint32_t c = 0u;
 #pragma unroll
-        for (int w = 0; w < kWaves * kCopies; ++w) c += s_hist[w][tid];
-        a.table[(size_t)tid * a.tiles + tile] = c;
+        for (int w = 0; w < kWaves * kCopies; ++w) c += s_hist[w][d];
+        a.table[(size_t)d * a.tiles + tile] = c;
     }
 }
 
@@ -215,10 +215,12 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     //    the key 0xFFFFFFFF: the last digit, ranked after every valid key.
     const uint32_t wbase = base + wave * (64u * kItems);
     const uint32_t last = count - 1u;
-    // ranks (< kTile) two to a register and output digits four to a
-    // register: fewer registers hold more waves in flight
-    constexpr int kPairs = (kItems + 1) / 2, kQuads = (kItems + 3) / 4;
-    static_assert(kTile <= 65536 && BITS <= 8, "packed ranks and digits");
+    // ranks (< kTile) two to a register and output digits four (BITS > 8:
+    // two) to a register: fewer registers hold more waves in flight
+    constexpr int kDigitBits = BITS <= 8 ? 8 : 16, kPerReg = 32 / kDigitBits;
+    constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
+    constexpr int kPairs = (kItems + 1) / 2, kDigitRegs = (kItems + kPerReg - 1) / kPerReg;
+    static_assert(kTile <= 65536 && BITS <= 16, "packed ranks and digits");
     uint32_t keys[kItems], ranks[kPairs];
     uint32_t kept = 0u;  // DENSE: bit i set when item i holds a visible splat
 #pragma unroll
@@ -275,30 +277,48 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     uint32_t lane_slot = wbase + lane;
     asm volatile("" : "+v"(lane_slot));
 
-    // 3. per-digit offsets: within the tile (over waves, then digits) and global
-    uint32_t bin_total = 0u;
-    if (tid < (uint32_t)RADIX) {
-        uint32_t sum = 0u;
+    // 3. per-digit offsets: within the tile (over waves, then digits) and
+    //    global.  Thread t holds digits [t * kDPT, (t + 1) * kDPT).
+    constexpr int kDPT = RADIX > kThreads ? RADIX / kThreads : 1;
+    const bool has_digits = tid * kDPT < (uint32_t)RADIX;
+    uint32_t dsum[kDPT], ctot[kDPT];
+    uint32_t tsum = 0u, csum = 0u;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            uint32_t c = s_whist[w][tid];
-            s_whist[w][tid] = sum;
-            sum += c;
+    for (int j = 0; j < kDPT; ++j) {
+        const uint32_t d = tid * kDPT + (uint32_t)j;
+        uint32_t sum = 0u;
+        ctot[j] = 0u;
+        if (has_digits) {
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                uint32_t c = s_whist[w][d];
+                s_whist[w][d] = sum;
+                sum += c;
+            }
+            // digit bases: every block scans the column totals itself (no
+            // separate single-block launch between colscan and downsweep)
+            ctot[j] = a.col_total[d];
         }
-        bin_total = sum;
+        dsum[j] = sum;
+        tsum += sum;
+        csum += ctot[j];
     }
     uint32_t total;
-    uint32_t local = block_exclusive_sum<kWaves>(bin_total, s_scan, &total);
+    uint32_t local = block_exclusive_sum<kWaves>(tsum, s_scan, &total);
     // a dense tile's kept splats, ranked first: they fill local positions
     // [0, total)
     const uint32_t out_count = DENSE ? total : valid;
-    // digit bases: every block scans the column totals itself (no separate
-    // single-block launch between colscan and downsweep)
     uint32_t all;
-    const uint32_t dbase = block_exclusive_sum<kWaves>(tid < (uint32_t)RADIX ? a.col_total[tid] : 0u, s_scan, &all);
-    if (tid < (uint32_t)RADIX) {
-        s_local[tid] = local;
-        s_dest[tid] = dbase + a.table[(size_t)tid * a.tiles + tile] - local;
+    uint32_t dbase = block_exclusive_sum<kWaves>(csum, s_scan, &all);
+    if (has_digits) {
+#pragma unroll
+        for (int j = 0; j < kDPT; ++j) {
+            const uint32_t d = tid * kDPT + (uint32_t)j;
+            s_local[d] = local;
+            s_dest[d] = dbase + a.table[(size_t)d * a.tiles + tile] - local;
+            local += dsum[j];
+            dbase += ctot[j];
+        }
     }
     __syncthreads();
 
@@ -327,14 +347,14 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     // invalid lanes carry the last digit and rank after every valid key, so
     // the valid keys occupy local positions [0, valid)
     // the output position of local position p is s_dest[digit] + p
-    uint32_t digits[kQuads] = {};
+    uint32_t digits[kDigitRegs] = {};
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
         if (p < out_count) {
             uint32_t k = s_buf[p];
             const uint32_t d = (k >> shift) & MASK;
-            digits[j >> 2] |= d << ((j & 3) * 8);
+            digits[j / kPerReg] |= d << ((j % kPerReg) * kDigitBits);
             at<O32>(keys_out, s_dest[d] + p) = k + plan.add;
         }
     }
@@ -373,7 +393,8 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const uint32_t p = tid + (uint32_t)j * kThreads;
-        if (p < out_count) at<O32>(vals_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
+        if (p < out_count)
+            at<O32>(vals_out, s_dest[(digits[j / kPerReg] >> ((j % kPerReg) * kDigitBits)) & kDigitMask] + p) = s_buf[p];
     }
     if constexpr (P2) {
         __syncthreads();
@@ -384,7 +405,9 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t p = tid + (uint32_t)j * kThreads;
-            if (p < out_count) at<O32>(vals2_out, s_dest[(digits[j >> 2] >> ((j & 3) * 8)) & 0xFFu] + p) = s_buf[p];
+            if (p < out_count)
+                at<O32>(vals2_out, s_dest[(digits[j / kPerReg] >> ((j % kPerReg) * kDigitBits)) & kDigitMask] + p) =
+                    s_buf[p];
         }
     }
 }
@@ -465,7 +488,7 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
 
 uint32_t sort_table_words(uint32_t capacity) {
     uint32_t tiles = (capacity + kTile - 1) / kTile;
-    return (tiles < 1 ? 1 : tiles) * 256u;
+    return (tiles < 1 ? 1 : tiles) * 512u;  // up to 9-bit digits
 }
 
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {
@@ -481,6 +504,7 @@ hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t 
             (!a.dense_rect_px || a.dense_rect_shift < 3 || a.dense_rect_shift > 5))
             return hipErrorInvalidValue;
     }
+    if (a.bits == 9) return run_passes<9, true>(a, result_slot, s);
     if (a.bits == 8) return run_passes<8, false>(a, result_slot, s);
     if (a.bits == 7) return run_passes<7, true>(a, result_slot, s);
     if (a.bits == 6) return run_passes<6, false>(a, result_slot, s);

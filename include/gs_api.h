@@ -250,7 +250,7 @@ int gs_sort(gs_ctx* ctx);
  * device pointers, 4-byte aligned, not overlapping.  Runs on the context's
  * stream and returns without waiting.  The rest of the sort's memory
  * (ping-pong pair, digit table, column totals, count) is a static device
- * array of about 33 MiB per GPU, held until the process ends: the call makes
+ * array of about 34 MiB per GPU, held until the process ends: the call makes
  * no hipMalloc, hipFree or other allocation.  count above
  * GS_SORT_STATIC_MAX_PAIRS is refused with GS_ERR_LIMIT.  Every call on one
  * GPU shares that array: calls that overlap in time on one GPU race. */
