@@ -92,14 +92,18 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// Inclusive wave64 prefix sum.
+// Inclusive wave64 prefix sum, all 64 lanes active, in registers: shifts by
+// 1, 2, 4 and 8 lanes within each row of 16 (lanes shifted in from outside
+// the row read 0), then row 0's total is broadcast into row 1 and row 2's
+// into row 3 (row_bcast:15), and rows 0-1's total into rows 2 and 3
+// (row_bcast:31).  Rows a broadcast does not write keep 0 as its result.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    uint32_t l = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (l >= (uint32_t)d) v += t;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
@@ -218,7 +222,7 @@ struct SortLaunch {
     uint32_t* scratch;       // 512 words
     uint32_t* table;         // sort_table_words(capacity) words
     int passes;
-    int bits;                // digit width (8, 7, 6 or 4)
+    int bits;                // digit width (9, 8, 7, 6 or 4)
     int wide_index;          // 1: 64-bit element indexing even where 32-bit offsets suffice
     // optional dense first pass (digit widths 7 and 4): it reads dense_count
     // slots of dense_keys (0xFFFFFFFF: culled) instead of keys[0] and count,

@@ -145,8 +145,9 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
 }
 
 // One block per digit: exclusive scan of the digit's column over the tiles in
-// use (all loads of a thread's run issued before use), column total to
-// col_total[digit].
+// use, column total to col_total[digit].  A thread's run of 8 counts is two
+// 16-byte accesses (the column stride is a multiple of 4 words, and a group
+// of 4 that starts below the tiles in use lies inside the column).
 template <int BITS>
 __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     constexpr int kRun = 8;
@@ -163,16 +164,25 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
         uint32_t v[kRun];
         uint32_t sum = 0u;
 #pragma unroll
-        for (int k = 0; k < kRun; ++k) {
-            v[k] = t0 + k < used ? col[t0 + k] : 0u;
-            sum += v[k];
+        for (int k = 0; k < kRun; k += 4) {
+            uint4 q = make_uint4(0u, 0u, 0u, 0u);
+            if (t0 + k < used) q = *reinterpret_cast<const uint4*>(col + t0 + k);
+            v[k] = t0 + k < used ? q.x : 0u;
+            v[k + 1] = t0 + k + 1 < used ? q.y : 0u;
+            v[k + 2] = t0 + k + 2 < used ? q.z : 0u;
+            v[k + 3] = t0 + k + 3 < used ? q.w : 0u;
+            sum += v[k] + v[k + 1] + v[k + 2] + v[k + 3];
         }
         uint32_t chunk;
         uint32_t run = carry + block_exclusive_sum<kWaves>(sum, s_scan, &chunk);
 #pragma unroll
-        for (int k = 0; k < kRun; ++k) {
-            if (t0 + k < used) col[t0 + k] = run;
-            run += v[k];
+        for (int k = 0; k < kRun; k += 4) {
+            uint4 o;
+            o.x = run; run += v[k];
+            o.y = run; run += v[k + 1];
+            o.z = run; run += v[k + 2];
+            o.w = run; run += v[k + 3];
+            if (t0 + k < used) *reinterpret_cast<uint4*>(col + t0 + k) = o;
         }
         carry += chunk;
     }
@@ -424,10 +434,17 @@ void launch_pass(const PassArgs& pa, bool p2, uint32_t tiles, hipStream_t s) {
 
 // DENSE_OK: the digit widths a dense first pass is built for (the depth
 // sort's)
+// Tiles of a sort over `capacity` elements, rounded up to a multiple of 4:
+// the digit table's column stride.
+uint32_t table_stride(uint32_t capacity) {
+    uint32_t tiles = (capacity + kTile - 1) / kTile;
+    if (tiles < 1) tiles = 1;
+    return (tiles + 3u) & ~3u;
+}
+
 template <int BITS, bool DENSE_OK>
 hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
-    uint32_t tiles = (a.capacity + kTile - 1) / kTile;
-    if (tiles < 1) tiles = 1;
+    const uint32_t tiles = table_stride(a.capacity);
     PassArgs pa{};
     pa.count = a.count;
     // a readable placeholder when the passes are fixed (not used for them)
@@ -487,8 +504,7 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
 }  // namespace
 
 uint32_t sort_table_words(uint32_t capacity) {
-    uint32_t tiles = (capacity + kTile - 1) / kTile;
-    return (tiles < 1 ? 1 : tiles) * 512u;  // up to 9-bit digits
+    return table_stride(capacity) * 512u;  // up to 9-bit digits
 }
 
 hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t s) {

@@ -29,7 +29,7 @@ namespace {
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 constexpr uint32_t kMaxPairs = GS_SORT_STATIC_MAX_PAIRS;
-constexpr uint32_t kTableWords = ((kMaxPairs + 3839u) / 3840u) * 512u;  // sort_table_words(kMaxPairs)
+constexpr uint32_t kTableWords = ((((kMaxPairs + 3839u) / 3840u) + 3u) & ~3u) * 512u;  // sort_table_words(kMaxPairs)
 constexpr size_t kKeysOff = 0;
 constexpr size_t kIdsOff = align256(kKeysOff + 4ull * kMaxPairs);
 constexpr size_t kTableOff = align256(kIdsOff + 4ull * kMaxPairs);
