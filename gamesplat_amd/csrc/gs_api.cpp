@@ -103,9 +103,10 @@ struct gs_ctx {
     DevBuf vrects[4];  // packed tile rectangles moved by the depth sort
     DevBuf block_range;  // [view][block]: visible key range
     DevBuf pad_rects;  // [view][n]: packed tile rectangles grown by a pixel (StochasticSplats)
-    // cull_offsets: the cull's per-block visible offsets, kept for the
-    // compaction (block_offsets is shared by the bin and GPS scans)
-    DevBuf planes, records, keys[4], ids[4], dense_keys, cull_offsets, block_counts, block_offsets, block_diag, rects, rect_px,
+    // cull_counts, cull_offsets: the cull's per-block visible counts and
+    // their offsets, kept for the compaction (block_counts and block_offsets
+    // are shared by the bin and GPS scans)
+    DevBuf planes, records, keys[4], ids[4], dense_keys, cull_counts, cull_offsets, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
 
@@ -270,6 +271,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     GS_CHECK(ensure(c, c->block_counts, blocks * 4, "block counts"));
     GS_CHECK(ensure(c, c->block_range, blocks * 8, "block key ranges"));
     GS_CHECK(ensure(c, c->block_offsets, blocks * 4, "block offsets"));
+    GS_CHECK(ensure(c, c->cull_counts, blocks * 4, "cull block counts"));
     GS_CHECK(ensure(c, c->cull_offsets, blocks * 4, "cull block offsets"));
     GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
     GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
@@ -466,7 +468,7 @@ void gs_destroy(gs_ctx* c) {
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->keys[3], &c->ids[0],
-                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->cull_offsets, &c->block_counts, &c->block_offsets, &c->block_diag,
+                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->cull_counts, &c->cull_offsets, &c->block_counts, &c->block_offsets, &c->block_diag,
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
@@ -864,7 +866,7 @@ CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullO
     a.collect_stats = opts ? opts->collect_stats : 0;
     a.records = ptr<float4>(c->records);
     a.dense_keys = ptr<uint32_t>(c->dense_keys);
-    a.block_counts = ptr<uint32_t>(c->block_counts);
+    a.block_counts = ptr<uint32_t>(c->cull_counts);
     a.block_range = ptr<uint2>(c->block_range);
     a.block_diag = a.collect_stats ? ptr<uint32_t>(c->block_diag) : nullptr;
     a.rect_px = ptr<uint2>(c->rect_px);
@@ -1013,9 +1015,11 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
     GS_CHECK(timed_begin(c, GS_STAGE_CULL_KERNEL, &tk));
     GS_HIP(launch_cull(p.cull, c->stream));
     GS_CHECK(timed_end(c, &tk));
-    GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->cull_offsets),
-                              &c->dc()->cull.visible_count, p.cull.block_diag, c->dc()->cull.diag, c->stream, 0,
-                              nullptr, p.cull.block_range, &c->dc()->cull.key_min));
+    // the visible count, key range and diagnostics; the block offsets only
+    // when the visible list is compacted
+    GS_HIP(launch_reduce_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, &c->dc()->cull.visible_count,
+                                p.cull.block_diag, c->dc()->cull.diag, c->stream, p.cull.block_range,
+                                &c->dc()->cull.key_min));
     GS_CHECK(timed_end(c, &t));
     c->compacted = false;
     return GS_OK;
@@ -1026,6 +1030,9 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
 int ensure_compacted(gs_ctx* c) {
     if (c->compacted) return GS_OK;
     const FramePlan& p = c->plan;
+    uint32_t* visible = &c->dc()->cull.visible_count;
+    GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->cull_offsets),
+                              visible, nullptr, nullptr, c->stream));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets),
                           ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
                           ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_pad ? p.cull.pad_rects : nullptr));

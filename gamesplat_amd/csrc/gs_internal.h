@@ -162,6 +162,10 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* off
                               uint32_t clamp_cap = 0, uint32_t* clamped = nullptr,
                               const uint2* block_range = nullptr, uint32_t* range_out = nullptr);
 // block_range (nb x (min, max), may be NULL) is reduced into range_out[0..1].
+// The same totals, diagnostics and range without the offsets.
+hipError_t launch_reduce_counts(const uint32_t* counts, uint32_t nb, uint32_t* total, const uint32_t* block_diag,
+                                uint32_t* diag_out, hipStream_t s, const uint2* block_range = nullptr,
+                                uint32_t* range_out = nullptr);
 
 // gs_cull.hip
 struct CullLaunch {

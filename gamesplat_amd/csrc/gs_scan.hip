@@ -191,7 +191,64 @@ __global__ void __launch_bounds__(kScanThreads) scan_counts_kernel(const uint32_
     if (block_range) reduce_range(block_range, nb, range_out, s_lo, s_hi);
 }
 
+// Totals only (no offsets): the sum of nb counts into *total, and the
+// diagnostics and key range as above.  Eight loads in flight per thread.
+__global__ void __launch_bounds__(kScanThreads) reduce_counts_kernel(const uint32_t* __restrict__ counts,
+                                                                     uint32_t nb, uint32_t* total,
+                                                                     const uint32_t* __restrict__ block_diag,
+                                                                     uint32_t* diag_out,
+                                                                     const uint2* __restrict__ block_range,
+                                                                     uint32_t* range_out) {
+    __shared__ uint32_t s_sum[kScanThreads / 64];
+    __shared__ uint32_t s_diag[8];
+    __shared__ uint32_t s_lo[kScanThreads / 64], s_hi[kScanThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    constexpr int kBatch = 8;
+    uint32_t sum = 0u, lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t i0 = tid; i0 < nb; i0 += kScanThreads * kBatch) {
+        uint32_t v[kBatch];
+        uint2 rg[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * kScanThreads;
+            v[j] = i < nb ? counts[i] : 0u;
+        }
+        if (block_range) {
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) rg[j] = block_range[min(i0 + (uint32_t)j * kScanThreads, nb - 1u)];
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) sum += v[j];
+        if (block_range) {
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                lo = min(lo, rg[j].x);
+                hi = max(hi, rg[j].y);
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += (uint32_t)__shfl_xor((int)sum, d, 64);
+    if ((tid & 63u) == 0) s_sum[tid >> 6] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t all = 0u;
+        for (int w = 0; w < kScanThreads / 64; ++w) all += s_sum[w];
+        *total = all;
+    }
+    if (block_diag) reduce_diag(block_diag, nb, diag_out, s_diag);
+    if (block_range) finish_range(lo, hi, range_out, s_lo, s_hi);
+}
+
 }  // namespace
+
+hipError_t launch_reduce_counts(const uint32_t* counts, uint32_t nb, uint32_t* total, const uint32_t* block_diag,
+                                uint32_t* diag_out, hipStream_t s, const uint2* block_range, uint32_t* range_out) {
+    if (block_range && !range_out) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reduce_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, total, block_diag,
+                       diag_out, block_range, range_out);
+    return hipGetLastError();
+}
 
 hipError_t init_kernel_attributes() {
     // once per process; the attribute belongs to the kernel on every device
