@@ -490,9 +490,9 @@ def main():
     ms = elapsed * 1000.0 / args.steps
     st = ctx.stats()
     V, D = st["visible_count"], st["tile_entries"]
-    # the depth sort runs the 7-bit passes the visible keys' range needs
+    # the depth sort runs the 9-bit passes the visible keys' range needs
     sk = ctx.read_buffer(gs.abi.GS_BUFFER_SORTED_KEYS)[:V].astype(np.int64)
-    depth_passes = max(1, -(-int(sk.max() - sk.min()).bit_length() // 7)) if V else 1
+    depth_passes = max(1, -(-int(sk.max() - sk.min()).bit_length() // 9)) if V else 1
 
     # per-frame GPU times from events on the context stream
     ft = sorted(frame_times(ctx, lambda k: ctx.render_alpha(cam, tile=args.tile), max(100, args.steps)))
@@ -595,7 +595,11 @@ def main():
     cull_bytes = 20 * N + 272 * V
     sort_bytes = 68 * V
     comp_bytes = 56 * D + 8 * W * H
-    impl_bytes = {"cull": 20 * N + 280 * V, "sort": 28 * depth_passes * V, "composite": comp_bytes}
+    # the sort's first pass reads the cull's dense keys and pixel rectangles
+    # of all N slots (its upsweep the keys again) and writes the visible
+    # keys, ids and tile rectangles; each later pass reads and writes them
+    impl_bytes = {"cull": 20 * N + 280 * V, "sort": 16 * N + 12 * V + 28 * (depth_passes - 1) * V,
+                  "composite": comp_bytes}
     cands = {
         "cull": (stage_ms["cull_kernel"], cull_bytes),
         "sort": (stage_ms["sort"], sort_bytes),

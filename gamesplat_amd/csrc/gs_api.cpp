@@ -106,6 +106,7 @@ struct gs_ctx {
     // cull_counts, cull_offsets: the cull's per-block visible counts and
     // their offsets, kept for the compaction (block_counts and block_offsets
     // are shared by the bin and GPS scans)
+    DevBuf box_partials;  // the centre-box reduction's per-block boxes
     DevBuf planes, records, keys[4], ids[4], dense_keys, cull_counts, cull_offsets, block_counts, block_offsets, block_diag, rects, rect_px,
         table,
         counters;
@@ -192,6 +193,10 @@ struct gs_ctx {
     // GPS and the visible-list readbacks build it on demand; the depth sort
     // otherwise compacts in its first pass
     bool compacted = false;
+    // box of the scene's Gaussian centres (valid: every centre finite), which
+    // bounds the depth keys' range and so the depth-sort passes launched
+    bool box_valid = false;
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
     bool stats_collected = false;
     int sort_slot = 0;
     int rect_tile = 0;       // tile size of the packed rectangles the last cull wrote (0: none)
@@ -468,7 +473,7 @@ void gs_destroy(gs_ctx* c) {
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&c->planes, &c->records, &c->keys[0], &c->keys[1], &c->keys[2], &c->keys[3], &c->ids[0],
-                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->cull_counts, &c->cull_offsets, &c->block_counts, &c->block_offsets, &c->block_diag,
+                      &c->ids[1], &c->ids[2], &c->ids[3], &c->vrects[3], &c->block_range, &c->dense_keys, &c->cull_counts, &c->cull_offsets, &c->box_partials, &c->block_counts, &c->block_offsets, &c->block_diag,
                       &c->rects, &c->rect_px, &c->table, &c->counters, &c->dup_keys[0], &c->dup_keys[1], &c->dup_keys[2],
                       &c->dup_vals[0], &c->dup_vals[1], &c->dup_vals[2], &c->ranges, &c->image, &c->work,
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
@@ -587,6 +592,68 @@ int reserve_planes(gs_ctx* c, uint32_t count, DevBuf* planes) {
     return GS_OK;
 }
 
+// The box of the uploaded centres (plane P0), reduced on the device and
+// finished on the host.
+int measure_centre_box(gs_ctx* c) {
+    c->box_valid = false;
+    if (!c->n) return GS_OK;
+    GS_CHECK(ensure(c, c->box_partials, (size_t)kCentreBoundBlocks * 8 * 4, "centre box"));
+    GS_HIP(launch_centre_bounds(ptr<float4>(c->planes), c->n, ptr<float>(c->box_partials), c->stream));
+    std::vector<float> part((size_t)kCentreBoundBlocks * 8);
+    GS_HIP(hipMemcpyAsync(part.data(), c->box_partials.p, part.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool finite = true;
+    for (uint32_t b = 0; b < kCentreBoundBlocks; ++b) {
+        const float* q = part.data() + (size_t)b * 8;
+        if (q[6] != 0.0f) finite = false;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], (double)q[k]);
+            hi[k] = std::max(hi[k], (double)q[3 + k]);
+        }
+    }
+    for (int k = 0; k < 3; ++k) finite = finite && std::isfinite(lo[k]) && std::isfinite(hi[k]);
+    if (finite) {
+        for (int k = 0; k < 3; ++k) {
+            c->box_lo[k] = lo[k];
+            c->box_hi[k] = hi[k];
+        }
+        c->box_valid = true;
+    }
+    return GS_OK;
+}
+
+// Significant bits of the depth keys' range over the current views: the
+// keys are 0xFFFFFFFF - bits(depth) for depths in [0.2, d], d the deepest
+// corner of the centre box in any view (with a margin for the cull's float
+// evaluation), so the range is below bits(d) - bits(0.2).  32 when the box
+// is unknown or the effect moves the centres.
+int depth_key_bits(const gs_ctx* c) {
+    if (!c->box_valid || c->plan.cull.effect_active || c->host_cams.empty()) return 32;
+    double deepest = 0.0;
+    for (const GsCameraUniforms& cam : c->host_cams) {
+        const float* V = cam.view;
+        double z = V[14], mag = std::fabs((double)V[14]);
+        for (int k = 0; k < 3; ++k) {
+            const double m = V[2 + 4 * k];
+            z += std::min(m * c->box_lo[k], m * c->box_hi[k]);
+            mag += std::fabs(m) * std::max(std::fabs(c->box_lo[k]), std::fabs(c->box_hi[k]));
+        }
+        // depth = -z; the cull's float sum lies within a few ulps of mag
+        deepest = std::max(deepest, -z + 1e-5 * mag + 1e-4);
+    }
+    if (!(deepest < 1e30)) return 32;
+    if (deepest < 0.2) return 1;
+    const float df = (float)deepest, nf = 0.2f;
+    uint32_t bd, bn;
+    memcpy(&bd, &df, 4);
+    memcpy(&bn, &nf, 4);
+    const uint32_t range = bd + 1u - bn;  // float rounding of deepest may round down by one
+    int bits = 1;
+    while (bits < 32 && (range >> bits)) ++bits;
+    return bits;
+}
+
 // The new planes become the scene: per-view buffers are sized for it, and
 // every frame state and the animation of the previous scene are dropped.
 int install_planes(gs_ctx* c, DevBuf& planes, uint32_t count) {
@@ -596,6 +663,7 @@ int install_planes(gs_ctx* c, DevBuf& planes, uint32_t count) {
     c->n = count;
     c->view_cap = 0;
     GS_CHECK(ensure_views(c, 1));
+    GS_CHECK(measure_centre_box(c));
     c->have_cull = c->have_sort = c->have_composite = c->have_gps = false;
     release(c, c->anim_data);
     release(c, c->lut);
@@ -921,7 +989,10 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     // the keys go in 9-bit passes, at most four, as many as the key range
     // needs (three for a range under 2^27: depths within a factor of 2^16)
     s.bits = c->radix_bits == 8 ? 9 : c->radix_bits;
-    s.passes = (32 + s.bits - 1) / s.bits;
+    // launched: what the key range the scene's box allows can need (the
+    // adaptive passes may still stop earlier)
+    s.passes = (depth_key_bits(c) + s.bits - 1) / s.bits;
+    if (s.passes < 1) s.passes = 1;
     s.wide_index = c->wide_index;
     // without a visible list the first pass reads the cull's dense keys and
     // drops the culled slots (the list it keeps is the one the compaction

@@ -524,6 +524,38 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     }
 }
 
+// Box of the Gaussian centres (plane P0): block b reduces the centres
+// b, b + grid, ... into out[8 b ..]: min x, y, z, max x, y, z, and 1 when a
+// coordinate is not finite (else 0).
+__global__ void __launch_bounds__(256) centre_bounds_kernel(const float4* __restrict__ p0, uint32_t n, float* out) {
+    __shared__ float s_v[7][4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const float inf = __builtin_inff();
+    float v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.0f};
+    for (uint32_t i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
+        const float4 q = p0[i];
+        if (!(__builtin_isfinite(q.x) && __builtin_isfinite(q.y) && __builtin_isfinite(q.z))) v[6] = 1.0f;
+        v[0] = fminf(v[0], q.x); v[1] = fminf(v[1], q.y); v[2] = fminf(v[2], q.z);
+        v[3] = fmaxf(v[3], q.x); v[4] = fmaxf(v[4], q.y); v[5] = fmaxf(v[5], q.z);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], __shfl_xor(v[k], d, 64));
+#pragma unroll
+        for (int k = 3; k < 7; ++k) v[k] = fmaxf(v[k], __shfl_xor(v[k], d, 64));
+    }
+    if (lane == 0u)
+        for (int k = 0; k < 7; ++k) s_v[k][wave] = v[k];
+    __syncthreads();
+    if (tid < 8u) {
+        float r = 0.0f;
+        if (tid < 3u) r = fminf(fminf(s_v[tid][0], s_v[tid][1]), fminf(s_v[tid][2], s_v[tid][3]));
+        else if (tid < 7u) r = fmaxf(fmaxf(s_v[tid][0], s_v[tid][1]), fmaxf(s_v[tid][2], s_v[tid][3]));
+        out[blockIdx.x * 8u + tid] = r;
+    }
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -570,6 +602,12 @@ hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views
     else
         hipLaunchKernelGGL(compact_kernel<false>, dim3((n + 255u) / 256u, views), dim3(256), 0, s, dense_keys, n,
                            offsets, vis_keys, vis_ids, rect_px, shift, vis_rects, packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_centre_bounds(const float4* p0, uint32_t n, float* partials, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(centre_bounds_kernel, dim3(kCentreBoundBlocks), dim3(256), 0, s, p0, n, partials);
     return hipGetLastError();
 }
 

@@ -209,6 +209,10 @@ hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views
                           uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
                           uint32_t* vis_rects, hipStream_t s, const uint32_t* packed = nullptr);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+// Partial boxes of the n Gaussian centres of plane P0: kCentreBoundBlocks
+// groups of 8 floats (min x, y, z, max x, y, z, non-finite flag, 0).
+constexpr uint32_t kCentreBoundBlocks = 256;
+hipError_t launch_centre_bounds(const float4* p0, uint32_t n, float* partials, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 
 // gs_sort.hip

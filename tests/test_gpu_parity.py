@@ -82,12 +82,13 @@ def test_sort_ties_and_sizes(ctx, orc, gs):
         np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si, err_msg=f"n={n}")
 
 
-@pytest.mark.parametrize("spread,passes", [(0.0, 1), (1e-3, 2), (0.3, 3), (None, 4), (1e10, 5)])
+@pytest.mark.parametrize("spread,passes", [(0.0, 1), (2e-4, 1), (1e-3, 2), (0.3, 3), (None, 3), (1e10, 4)])
 def test_sort_key_range_passes(ctx, orc, gs, spread, passes):
-    # The depth sort runs only the 7-bit passes that the visible keys' range
-    # (max - min) needs.  Splats on the view axis at depths 15 + U[0, spread)
+    # The depth sort runs only the 9-bit passes that the visible keys' range
+    # (max - min) needs, and launches only those the box of the scene's
+    # centres allows.  Splats on the view axis at depths 15 + U[0, spread)
     # give a range of each pass count; 1e10 puts one splat far out (range
-    # above 2^28, all five passes).
+    # above 2^27: the box then allows, and the keys need, all four passes).
     print(f"case: sort key range spread={spread}", flush=True)
     cam = scenes.random_view_camera(21, 640, 480)
     d = scenes._direction(21).astype(np.float64)
@@ -102,7 +103,7 @@ def test_sort_key_range_passes(ctx, orc, gs, spread, passes):
     vis = ok[oi != 0xFFFFFFFF].astype(np.int64)
     assert vis.size > 1
     span = int(vis.max() - vis.min())
-    need = max(1, -(-span.bit_length() // 7))
+    need = max(1, -(-span.bit_length() // 9))
     assert need == passes, f"key range {span:#x} needs {need} passes"
     ctx.sort()
     sk, si = orc.radix_sort(ok, oi)
