@@ -68,6 +68,8 @@ def _parse():
                     help="only time the learned denoiser on the workload's 1 spp stochastic frame and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (calibrates HBM byte counters) and exit")
+    ap.add_argument("--hbm-ceilings", type=int, default=0, metavar="BYTES",
+                    help="only run the library's HBM rate probes over buffers of BYTES, print them and exit")
     return ap.parse_args()
 
 
@@ -451,6 +453,14 @@ def multiview(steps: int, warmup: int) -> dict:
 
 def main():
     args = _parse()
+    if args.hbm_ceilings:
+        import gamesplat_amd as gsm
+
+        with gsm.Context(0) as ctx:
+            rates = ctx.hbm_probe(args.hbm_ceilings)
+        print(json.dumps({"hbm_probe_bytes": args.hbm_ceilings, "GB/s": rates,
+                          "frac_of_8TBs": {k: round(v / 8000.0, 3) for k, v in rates.items()}}), flush=True)
+        return 0
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return 0

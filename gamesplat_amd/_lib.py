@@ -142,6 +142,11 @@ class GsCullOptions(C.Structure):
     _fields_ = [("sh_degree", C.c_int32), ("sh_lod_distance", C.c_float), ("collect_stats", C.c_int32)]
 
 
+class GsHbmRates(C.Structure):
+    _fields_ = [("read_gbs", C.c_double), ("planes_read_gbs", C.c_double), ("write_gbs", C.c_double),
+                ("copy_gbs", C.c_double)]
+
+
 class GsStats(C.Structure):
     _fields_ = [
         ("gaussian_count", C.c_uint32),
@@ -193,6 +198,7 @@ EXPORTS = {
     "gs_destroy": (None, [C.c_void_p]),
     "gs_last_error": (C.c_char_p, []),
     "gs_device_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "gs_hbm_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(GsHbmRates)]),
     "gs_set_memory_budget": (C.c_int, [C.c_void_p, C.c_size_t]),
     "gs_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gs_upload_gaussians": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
@@ -369,6 +375,13 @@ class Context:
         buf = C.create_string_buffer(512)
         check(self._lib.gs_device_info(self._h, buf, 512))
         return buf.value.decode()
+
+    def hbm_probe(self, nbytes: int = 1 << 30) -> dict:
+        """Read, 15-plane read, write and copy rates (GB/s) of probe kernels
+        over buffers of nbytes."""
+        r = GsHbmRates()
+        check(self._lib.gs_hbm_probe(self._h, nbytes, C.byref(r)))
+        return {k: round(getattr(r, k), 1) for k, _ in GsHbmRates._fields_}
 
     def set_memory_budget(self, nbytes: int):
         check(self._lib.gs_set_memory_budget(self._h, nbytes))

@@ -497,6 +497,48 @@ void gs_destroy(gs_ctx* c) {
     delete c;
 }
 
+int gs_hbm_probe(gs_ctx* c, size_t bytes, GsHbmRates* out) {
+    if (!c || !out) return fail(GS_ERR_INVALID, "bad arguments");
+    if (bytes < ((size_t)64 << 20)) return fail(GS_ERR_INVALID, "probe buffers need at least 64 MiB");
+    GS_CHECK(set_device(c));
+    GS_HIP(hipStreamSynchronize(c->stream));  // nothing else runs while the probes are timed
+    const size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr, *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best[4] = {1e30, 1e30, 1e30, 1e30};
+    hipError_t e = hipMalloc(&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&sink, 16);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * 16, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b, 0, n * 16, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
+        for (int w = 0; w < 4 && e == hipSuccess; ++w) {
+            e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess) e = launch_probe(w, static_cast<const float4*>(a), static_cast<float4*>(b), n,
+                                                  static_cast<float*>(sink), c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && ms > 0.0f) best[w] = std::min(best[w], (double)ms);
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (sink) (void)hipFree(sink);
+    if (e != hipSuccess) return fail(GS_ERR_HIP, "HBM probe failed: %s", hipGetErrorString(e));
+    const double planes_bytes = (double)(n / 15) * 15 * 16;
+    out->read_gbs = (double)n * 16 / best[0] / 1e6;
+    out->planes_read_gbs = planes_bytes / best[1] / 1e6;
+    out->write_gbs = (double)n * 16 / best[2] / 1e6;
+    out->copy_gbs = 2.0 * n * 16 / best[3] / 1e6;
+    return ok();
+}
+
 int gs_device_info(gs_ctx* c, char* out, size_t out_bytes) {
     if (!c || !out || !out_bytes) return fail(GS_ERR_INVALID, "bad arguments");
     snprintf(out, out_bytes, "%s (%s): %d CUs, %.1f GiB HBM, %zu KiB LDS/block, clock %d MHz, mem clock %d MHz, bus %d bit",

@@ -213,6 +213,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // groups of 8 floats (min x, y, z, max x, y, z, non-finite flag, 0).
 constexpr uint32_t kCentreBoundBlocks = 256;
 hipError_t launch_centre_bounds(const float4* p0, uint32_t n, float* partials, hipStream_t s);
+
+// gs_probe.hip: HBM rate probe `which` (0 read, 1 fifteen planes, 2 write,
+// 3 copy) over n float4 of a (read) and b (written); sink takes one float.
+hipError_t launch_probe(int which, const float4* a, float4* b, size_t n, float* sink, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, size_t count, hipStream_t s);
 
 // gs_sort.hip

@@ -151,6 +151,21 @@ const char* gs_last_error(void);
 /* Human-readable device description (name, CUs, HBM, LDS). */
 int gs_device_info(gs_ctx* ctx, char* out, size_t out_bytes);
 
+/* HBM rates this device reaches with the library's access patterns, in
+ * GB/s (1e9 bytes), each the best of five event-timed launches over buffers
+ * of `bytes` (at least 64 MiB, allocated for the call and freed): a 16-byte
+ * load per thread, the same over 15 planes read together (the cull's
+ * prefetch), a 16-byte store per thread, and a load and store per thread
+ * (bytes read plus bytes written).  No counterpart in the reference: the
+ * ceiling against which the HBM-bound kernels here are measured. */
+typedef struct GsHbmRates {
+    double read_gbs;
+    double planes_read_gbs;
+    double write_gbs;
+    double copy_gbs;
+} GsHbmRates;
+int gs_hbm_probe(gs_ctx* ctx, size_t bytes, GsHbmRates* out);
+
 /* Cap on device memory this context may hold (0 = no cap).  Mirrors
  * gpu_budget_bytes / renderer_gpu_buffer_limits (src/renderer.cpp:709-734). */
 int gs_set_memory_budget(gs_ctx* ctx, size_t bytes);

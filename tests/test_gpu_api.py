@@ -284,3 +284,14 @@ def test_upload_paths_give_the_same_scene(gs, tmp_path):
         np.testing.assert_array_equal(projected(sog_cam), sog_ref)
     for k in ("planes", "pinned", "ply"):
         np.testing.assert_array_equal(outs[k], outs["gaussians"], err_msg=k)
+
+
+def test_hbm_probe_rates(gs):
+    print("case: hbm probe 64 MiB", flush=True)
+    with gs.Context(0) as c:
+        r = c.hbm_probe(64 << 20)
+        assert set(r) == {"read_gbs", "planes_read_gbs", "write_gbs", "copy_gbs"}
+        # positive and below the 8 TB/s HBM3E peak
+        assert all(0.0 < v < 8000.0 for v in r.values()), r
+        with pytest.raises(abi.GsError):
+            c.hbm_probe(1 << 20)
