@@ -353,3 +353,38 @@ def test_wide_index_variants(ctx, orc, c1_scene):
     np.testing.assert_array_equal(out[1][1], out[0][1])
     np.testing.assert_array_equal(out[1][2], out[0][2])
     assert out[1][1].any()
+
+
+def test_visible_list_after_frame(ctx, orc, c1_scene):
+    """A frame sorts from the cull's dense keys and reuses the shared block
+    offsets for its bin; the visible list built afterwards on demand (for a
+    readback or GPS) still comes from the cull's own counts."""
+    g, cfg, cam = c1_scene
+    print("case: visible list after a frame c1", flush=True)
+    ctx.upload(g)
+    ctx.render_alpha(cam)
+    ok, oi, op, ov, od = orc.cull(g, cam)
+    exp_ids = np.nonzero(oi != 0xFFFFFFFF)[0].astype(np.uint32)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_VISIBLE_IDS), exp_ids)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_VISIBLE_KEYS), ok[exp_ids])
+    sk, si = orc.radix_sort(ok, oi)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si)
+
+
+def test_sort_far_splat_under_effect(ctx, orc, gs):
+    """With the shockwave effect active the centres move, so the scene's box
+    no longer bounds the depths and all depth passes are launched; a splat
+    far out (key range above 2^27) still sorts exactly."""
+    print("case: sort far splat under the effect", flush=True)
+    cam = scenes.random_view_camera(21, 640, 480)
+    d = scenes._direction(21).astype(np.float64)
+    g = gs.generate_scene(9, 20000, 4, 0.0)
+    g[0, 0:3] = d * (15.0 - 1e10)
+    fx = abi.GsSplatEffectParams((0.5, -0.2, 0.3, 9.0), (0.6, 1.5, 0.12, 1.0), (1.0, 0.55, 0.18, 0.4))
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam, effect=fx)
+    vis = ok[oi != 0xFFFFFFFF].astype(np.int64)
+    assert int(vis.max() - vis.min()).bit_length() > 27
+    ctx.sort()
+    sk, si = orc.radix_sort(ok, oi)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_KEYS), sk)
+    np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si)
