@@ -213,10 +213,10 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint
 // covered pixels [i0, i1) x [j0, j1) ride in par.w and col.w as the packed
 // corners (i0 | j0 << 16, (i1 - 1) | (j1 - 1) << 16), so one step reads
 // three 16-byte records at one address.  The fourth record and the centre
-// hold the reference's own conic, the argument of its cutoff's log and the
-// centre, read only for the fragments whose fused test lies within par.z of
-// the cutoff (under one wave step in a hundred): the log itself is taken
-// there, not at staging.
+// hold the record's covariance (a, b, c), opacity and centre, read only for
+// the fragments whose fused test lies within par.z of the cutoff (under one
+// wave step in a hundred): the reference's conic divisions and cutoff log
+// are taken there, not at staging.
 // (A native 4-float vector, so each record is one 128-bit LDS access.)
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
@@ -224,7 +224,7 @@ struct StagedSplats {
     f32x4_t* geo;   // A, B, C, E
     f32x4_t* par;   // F, G, decision margin (log2 units), rect (low corner bits)
     f32x4_t* col;   // r, g, b, rect (high corner bits)
-    f32x4_t* exact; // c / det, 2 (-b / det), a / det, (1/255) / max(o, 1e-8): the reference's operations
+    f32x4_t* exact; // a, b, c, opacity of the record: the exact test's inputs
     uint32_t* quads;  // 16x16 quadrant walk: bit q set when the splat can touch quadrant q; bit 4: slow step
     float2* ctr;    // cx, cy
 };
@@ -281,28 +281,20 @@ __device__ __forceinline__ TileRef tile_ref(const CompositeLaunch& a, uint32_t i
     return t;
 }
 
-// What the quadrant test needs of a staged splat: centre, the form's
-// coefficients relative to the centre (log2 units), log2 cutoff, decision
-// margin and covered pixels.
+// What the quadrant test needs of a staged splat: centre, covered pixels and
+// the half widths of its alpha ellipse across the diagonals (diag: they
+// apply).
 struct StagedTerms {
-    float cx, cy, A, B, C, cutoff, margin;
+    float cx, cy, wp, wm;
     int4 rect;
-    bool slow;
-};
-
-// Per-splat constants of the quadrant test (ellipse_meets_box): whether it
-// applies (a concave form with 1 - rho^2 >= 0.01), the argmax slopes of the
-// form along y and x, and the band below the cutoff that may still pass.
-struct QuadCullTerms {
-    bool applies;
-    float sy, sx, band;
+    bool slow, diag;
 };
 
 // Stage splat g into slot t of a T x T tile: conic from one reciprocal of the
-// clamped determinant in log2 units, pixel rectangle, the reference's conic
-// ((c, -b, a) / det, shaders/cull.glsl:273-275), the argument of its
-// fragment cutoff log((1/255) / max(o, 1e-8)) (shaders/splat.glsl:70) and
-// centre, and the
+// clamped determinant in log2 units, pixel rectangle, the covariance,
+// opacity and centre the exact test needs (it forms the reference's conic
+// (c, -b, a) / det, shaders/cull.glsl:273-275, and its fragment cutoff
+// log((1/255) / max(o, 1e-8)), shaders/splat.glsl:70, itself), and the
 // margin within which the fused test may decide differently from the
 // reference's float test.
 //
@@ -348,20 +340,29 @@ __device__ __forceinline__ StagedTerms stage_splat(const CompositeLaunch& a, con
     const float4 r0 = rec.r0, r1 = rec.r1, r2 = rec.r2;
     float det = r1.x * r1.z - r1.y * r1.y;
     if (det < 1e-6f) det = 1e-6f;
-    // the reference's conic (IEEE divisions), which the fused form scales
-    const float k0 = r1.z / det, k1 = -r1.y / det, k2 = r1.x / det;
+    // the conic (c, -b, a) / det to about an ulp (the margin below covers
+    // it); the reference's own divisions are taken in the exact test only
+    const float rdet = __builtin_amdgcn_rcpf(det);
+    const float k0 = r1.z * rdet, k1 = -r1.y * rdet, k2 = r1.x * rdet;
     const float log2o = __log2f(maxf(r0.z, 1e-8f));
     StagedTerms st;
     st.cx = r0.x;
     st.cy = r0.y;
-    st.A = -0.5f * kLog2e * k0;
-    st.B = -kLog2e * k1;
-    st.C = -0.5f * kLog2e * k2;
+    const float A = -0.5f * kLog2e * k0, B = -kLog2e * k1, C = -0.5f * kLog2e * k2;
     const float Dc = log2o - kLog2AlphaMin;
-    st.cutoff = -Dc;  // log2((1/255) / o)
+    const float cutoff = -Dc;  // log2((1/255) / o)
     const int4 quad = quad_pixel_bounds(r0.x, r0.y, r1.w, r2.w, tr.width, tr.height);
-    int4 ell;
-    const bool bounded = ellipse_pixel_bounds(r0.x, r0.y, r1.x, r1.y, r1.z, r0.z, &ell);
+    float cgb;
+    const bool bounded = ellipse_cutoff_bound(r1.x, r1.y, r1.z, r0.z, &cgb);
+    int4 ell = quad;
+    st.diag = bounded;
+    st.wp = st.wm = 0.0f;
+    if (bounded) {
+        ell = ellipse_box(r0.x, r0.y, r1.x, r1.z, cgb);
+        const float2 w = ellipse_diagonal_halfwidths(r1.x, r1.y, r1.z, cgb);
+        st.wp = w.x;
+        st.wm = w.y;
+    }
     st.rect = quad;
     if (bounded && a.tight_rects != 0) {
         st.rect.x = max(quad.x, ell.x);
@@ -373,7 +374,6 @@ __device__ __forceinline__ StagedTerms stage_splat(const CompositeLaunch& a, con
     // the expanded polynomial about the tile centre; S bounds the magnitudes
     // of every product and sum formed in its coefficients and its evaluation
     const float u = r0.x - (float)(tr.tx * T + T / 2), v = r0.y - (float)(tr.ty * T + T / 2);
-    const float A = st.A, B = st.B, C = st.C;
     const float Au = A * u, Bv = B * v, Bu = B * u, Cv = C * v;
     const float E = -2.0f * Au - Bv, F = -2.0f * Cv - Bu;
     const float G = (Au + Bv) * u + Cv * v + Dc;
@@ -384,17 +384,17 @@ __device__ __forceinline__ StagedTerms stage_splat(const CompositeLaunch& a, con
     const float ht = 0.5f * (k0 + k2), hd = 0.5f * (k0 - k2);
     const float sp = __builtin_sqrtf(hd * hd + k1 * k1);
     const float kappa = (ht + sp) * __builtin_amdgcn_rcpf(ht - sp);
-    st.margin = ht - sp > 0.0f
-                    ? 8e-6f * kappa * __builtin_fabsf(st.cutoff) + 3e-5f + 16.0f * 5.9604645e-8f * S
-                    : __builtin_inff();
+    const float margin = ht - sp > 0.0f
+                             ? 8e-6f * kappa * __builtin_fabsf(cutoff) + 3e-5f + 16.0f * 5.9604645e-8f * S
+                             : __builtin_inff();
     const bool rect_free = bounded && ell.x >= quad.x && ell.y <= quad.y && ell.z >= quad.z && ell.w <= quad.w;
-    const bool cap_free = Dc + st.margin + 1e-4f < kLog2AlphaCap255;
+    const bool cap_free = Dc + margin + 1e-4f < kLog2AlphaCap255;
     st.slow = !(rect_free && cap_free);
     const uint2 rect = pack_rect16(st.rect);
     s.geo[t] = f32x4_t{A, B, C, E};
-    s.par[t] = f32x4_t{F, G, st.margin, __uint_as_float(rect.x)};
+    s.par[t] = f32x4_t{F, G, margin, __uint_as_float(rect.x)};
     s.col[t] = f32x4_t{r2.x, r2.y, r2.z, __uint_as_float(rect.y)};
-    s.exact[t] = f32x4_t{k0, 2.0f * k1, k2, (1.0f / 255.0f) / maxf(r0.z, 1e-8f)};
+    s.exact[t] = f32x4_t{r1.x, r1.y, r1.z, r0.z};
     s.ctr[t] = make_float2(r0.x, r0.y);
     return st;
 }
@@ -451,18 +451,24 @@ __device__ __forceinline__ void blend_step(const PixelTerms& p, f32x4_t geo, f32
     const bool lo = power >= -par.z;
     bool pass = lo;
     if (__builtin_expect(lo & !(power >= par.z), 0)) {
+        // the reference's conic (shaders/cull.glsl:273-275) from the staged
+        // covariance (a, b, c) and opacity, IEEE divisions
         const f32x4_t e = *exact;
+        float det = e.x * e.z - e.y * e.y;
+        if (det < 1e-6f) det = 1e-6f;
+        const float k0 = e.z / det, k1 = -e.y / det, k2 = e.x / det, k1x2 = 2.0f * k1;
+        const float arg = (1.0f / 255.0f) / maxf(e.w, 1e-8f);
         const float2 c = *ctr;
         const float dx = p.fx - c.x, dy = p.fy - c.y;
-        const float pr = -0.5f * (e.x * dx * dx + e.y * dx * dy + e.z * dy * dy);
-        // pr < cutoff, the cutoff being the deterministic float log of e.w:
+        const float pr = -0.5f * (k0 * dx * dx + k1x2 * dx * dy + k2 * dy * dy);
+        // pr < cutoff, the cutoff being the deterministic float log of arg:
         // v_log_f32 (1 ulp of log2) times ln 2 lies within 4e-6 of it over
         // the cutoff's range (|cutoff| < 16), so where pr is 1e-5 or more
         // away from that estimate the two compare alike; closer, the
         // deterministic log decides
-        const float est = __builtin_amdgcn_logf(e.w) * 0.69314718f;
+        const float est = __builtin_amdgcn_logf(arg) * 0.69314718f;
         bool below = pr < est;
-        if (__builtin_expect(__builtin_fabsf(pr - est) < 1e-5f, 0)) below = pr < logf_(e.w);
+        if (__builtin_expect(__builtin_fabsf(pr - est) < 1e-5f, 0)) below = pr < logf_(arg);
         pass = !(pr > 0.0f) & !below;
     }
     if (SLOW) pass &= in_rect16(p.pxy, make_uint2(__float_as_uint(par.w), __float_as_uint(col.w)));
@@ -541,49 +547,15 @@ __device__ __forceinline__ void composite_tile(const CompositeLaunch& a, uint32_
         if (px[k] < t.width && py[k] < t.height) store_pixel(t, px[k], py[k], cr[k], cg[k], cb[k], tr[k]);
 }
 
-// Whether any pixel centre of the 8x8 quadrant at (qx0, qy0) that the
-// rectangle r covers can pass the fragment test power >= cutoff.  power is a
-// concave quadratic (log2 units: A dx^2 + B dx dy + C dy^2, A, C < 0), so its
-// maximum over the box of those centres is 0 when the splat centre lies in
-// the box and otherwise sits on an edge, at the clamped vertex of the 1D
-// parabola.  The band covers the float evaluation of power here and in the
-// blend step (about 10 eps |power| / (1 - |rho|)^2 each, with 1 - rho^2 =
-// r2 = 1 - B^2 / (4 A C)) and the blend step's decision margin; splats with
-// r2 < 0.01 or a clamped, indefinite conic are always walked.  The slopes
-// and the band are per-splat constants (quad_cull_terms); a vertex placed
-// a few ulps off by the approximate reciprocals lowers the edge maximum by
-// far less than the band.
-__device__ __forceinline__ QuadCullTerms quad_cull_terms(const StagedTerms& s) {
-    QuadCullTerms q;
-    q.applies = false;
-    q.sy = q.sx = q.band = 0.0f;
-    if (!(s.A < 0.0f && s.C < 0.0f)) return q;
-    const float r2 = 1.0f - (s.B * s.B) * __builtin_amdgcn_rcpf(4.0f * s.A * s.C);
-    if (!(r2 >= 0.01f)) return q;
-    q.applies = true;
-    q.sy = -s.B * __builtin_amdgcn_rcpf(2.0f * s.C);  // argmax of the parabola along y
-    q.sx = -s.B * __builtin_amdgcn_rcpf(2.0f * s.A);  // and along x
-    q.band = __builtin_fabsf(s.cutoff) * (0.003f + 1.001e-5f * __builtin_amdgcn_rcpf(r2 * r2)) + 1e-5f + s.margin;
-    return q;
-}
-
-__device__ __forceinline__ bool ellipse_meets_box(const StagedTerms& s, const QuadCullTerms& q, int qx0, int qy0) {
-    if (!q.applies) return true;
-    const float A = s.A, B = s.B, C = s.C;
-    const int4 r = s.rect;
-    const float xl = (float)max(qx0, r.x) + 0.5f - s.cx, xh = (float)min(qx0 + 8, r.y) - 0.5f - s.cx;
-    const float yl = (float)max(qy0, r.z) + 0.5f - s.cy, yh = (float)min(qy0 + 8, r.w) - 0.5f - s.cy;
-    if (xl <= 0.0f && xh >= 0.0f && yl <= 0.0f && yh >= 0.0f) return true;
-    auto along_y = [&](float u) {  // max over dy in [yl, yh] at dx = u
-        const float t = clampf(q.sy * u, yl, yh);
-        return fmaf(fmaf(C, t, B * u), t, (A * u) * u);
-    };
-    auto along_x = [&](float u) {  // max over dx in [xl, xh] at dy = u
-        const float t = clampf(q.sx * u, xl, xh);
-        return fmaf(fmaf(A, t, B * u), t, (C * u) * u);
-    };
-    const float best = maxf(maxf(along_y(xl), along_y(xh)), maxf(along_x(yl), along_x(yh)));
-    return best >= s.cutoff - q.band;
+// Whether the alpha ellipse of a staged splat can reach a pixel centre of
+// the box [xa, xb] x [ya, yb] (pixel-centre offsets from the splat centre):
+// the box must meet both diagonal slabs |dx + dy| <= wp and |dx - dy| <= wm
+// that hold every pixel the fragment test can keep (with the rectangle,
+// already cut to the ellipse's axis-aligned box, an octagon around the
+// ellipse).  The sums and differences of the box corners range over [xa +
+// ya, xb + yb] and [xa - yb, xb - ya].
+__device__ __forceinline__ bool ellipse_meets_box(const StagedTerms& s, float xa, float xb, float ya, float yb) {
+    return (xa + ya <= s.wp) & (xb + yb >= -s.wp) & (xa - yb <= s.wm) & (xb - ya >= -s.wm);
 }
 
 // 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
@@ -618,12 +590,19 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             // the quadrants the splat can touch, tested once here for all
             // four waves (the tests share the splat's terms)
             uint32_t quads = s.slow ? 16u : 0u;
-            const QuadCullTerms qc = quad_cull_terms(s);
+            // pixel-centre offsets of the rectangle's part in the left and
+            // right (top and bottom) halves of the tile
+            const int xm = t.tx * 16 + 8, ym = t.ty * 16 + 8;
+            const float xa[2] = {(float)max(xm - 8, s.rect.x) + 0.5f - s.cx, (float)max(xm, s.rect.x) + 0.5f - s.cx};
+            const float xb[2] = {(float)min(xm, s.rect.y) - 0.5f - s.cx, (float)min(xm + 8, s.rect.y) - 0.5f - s.cx};
+            const float ya[2] = {(float)max(ym - 8, s.rect.z) + 0.5f - s.cy, (float)max(ym, s.rect.z) + 0.5f - s.cy};
+            const float yb[2] = {(float)min(ym, s.rect.w) - 0.5f - s.cy, (float)min(ym + 8, s.rect.w) - 0.5f - s.cy};
+            const bool ellipse_test = a.quadrant_cull == 1 && s.diag;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int x0 = t.tx * 16 + (q & 1) * 8, y0 = t.ty * 16 + (q >> 1) * 8;
+                const int x0 = xm - 8 + (q & 1) * 8, y0 = ym - 8 + (q >> 1) * 8;
                 bool hit = s.rect.x < x0 + 8 && s.rect.y > x0 && s.rect.z < y0 + 8 && s.rect.w > y0;
-                if (hit && a.quadrant_cull == 1) hit = ellipse_meets_box(s, qc, x0, y0);
+                if (ellipse_test) hit = hit && ellipse_meets_box(s, xa[q & 1], xb[q & 1], ya[q >> 1], yb[q >> 1]);
                 quads |= hit ? 1u << q : 0u;
             }
             st.quads[tid] = quads;

@@ -66,22 +66,50 @@ __device__ __forceinline__ int4 quad_fixed(float cx, float cy, float rx, float r
 // 1e-5 on c, and 1e-3 px) and is only used for 1 - rho^2 >= 0.01.  Every
 // pixel that can pass the fragment test stays inside, so composites with and
 // without the cut are identical; it only shortens the tile lists.
-__device__ __forceinline__ bool ellipse_pixel_bounds(float cx, float cy, float sa, float sb, float sc, float opacity,
-                                                     int4* e) {
+// The grown bound cg on q(d) of every pixel that can pass the fragment test
+// (see ellipse_pixel_bounds); false when the bound does not apply.
+__device__ __forceinline__ bool ellipse_cutoff_bound(float sa, float sb, float sc, float opacity, float* cg) {
     // reciprocals to 1 ulp: their error is far inside the 0.3% and 2x slack
     const float det = sa * sc - sb * sb;
     const float r2 = det * __builtin_amdgcn_rcpf(sa * sc);
     if (!(det >= 1e-6f && r2 >= 0.01f)) return false;
     const float c = maxf(-2.0f * __logf((1.0f / 255.0f) / maxf(opacity, 1e-8f)), 0.0f);
-    const float cg = c * (1.003f + 4.8e-6f * __builtin_amdgcn_rcpf(r2 * r2)) + 1e-5f;
+    *cg = c * (1.003f + 4.8e-6f * __builtin_amdgcn_rcpf(r2 * r2)) + 1e-5f;
+    return true;
+}
+
+// Pixel bounds of the ellipse {q <= cg} around (cx, cy).
+__device__ __forceinline__ int4 ellipse_box(float cx, float cy, float sa, float sc, float cg) {
     const float ex = __builtin_sqrtf(cg * sa) + 1e-3f;
     const float ey = __builtin_sqrtf(cg * sc) + 1e-3f;
     // pixel i can pass only if |i + 0.5 - cx| <= ex
-    e->x = (int)__builtin_ceilf(clampf(cx - ex - 0.5f, -2.0f, 65536.0f));
-    e->y = (int)__builtin_floorf(clampf(cx + ex - 0.5f, -2.0f, 65536.0f)) + 1;
-    e->z = (int)__builtin_ceilf(clampf(cy - ey - 0.5f, -2.0f, 65536.0f));
-    e->w = (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1;
+    int4 e;
+    e.x = (int)__builtin_ceilf(clampf(cx - ex - 0.5f, -2.0f, 65536.0f));
+    e.y = (int)__builtin_floorf(clampf(cx + ex - 0.5f, -2.0f, 65536.0f)) + 1;
+    e.z = (int)__builtin_ceilf(clampf(cy - ey - 0.5f, -2.0f, 65536.0f));
+    e.w = (int)__builtin_floorf(clampf(cy + ey - 0.5f, -2.0f, 65536.0f)) + 1;
+    return e;
+}
+
+__device__ __forceinline__ bool ellipse_pixel_bounds(float cx, float cy, float sa, float sb, float sc, float opacity,
+                                                     int4* e) {
+    float cg;
+    if (!ellipse_cutoff_bound(sa, sb, sc, opacity, &cg)) return false;
+    *e = ellipse_box(cx, cy, sa, sc, cg);
     return true;
+}
+
+// Half widths of the same ellipse across the diagonals: every d with
+// d^T S^-1 d <= cg has |dx + dy| <= wp and |dx - dy| <= wm (Cauchy-Schwarz:
+// |n.d| <= sqrt(cg n^T S n) for n = (1, +-1)).  n^T S n = sa + sc +- 2 sb is
+// formed in float to within 2 eps (sa + sc + 2 |sb|), which is added; the
+// square root is grown by 1e-6 relative, and 4e-3 px covers the rounding of
+// the pixel-centre offsets the caller compares with.
+__device__ __forceinline__ float2 ellipse_diagonal_halfwidths(float sa, float sb, float sc, float cg) {
+    const float s = sa + sc, t = 2.0f * sb, slack = 4.8e-7f * (s + __builtin_fabsf(t));
+    const float wp = __builtin_sqrtf(fmaxf(cg * ((s + t) + slack), 0.0f)) * 1.000001f + 4e-3f;
+    const float wm = __builtin_sqrtf(fmaxf(cg * ((s - t) + slack), 0.0f)) * 1.000001f + 4e-3f;
+    return make_float2(wp, wm);
 }
 
 // Pixels [i0, i1) x [j0, j1) (not cut to the image) whose centre the quad
