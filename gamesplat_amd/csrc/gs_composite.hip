@@ -558,20 +558,30 @@ __device__ __forceinline__ bool ellipse_meets_box(const StagedTerms& s, float xa
     return (xa + ya <= s.wp) & (xb + yb >= -s.wp) & (xa - yb <= s.wm) & (xb - ya >= -s.wm);
 }
 
-// 16x16 tile, four wave64s, each wave owning one 8x8 quadrant.  Every splat
-// of an LDS batch is tested against the quadrant's pixel range with one
-// ballot per 64 splats; the wave then walks only the overlapping splats
-// (scalar bit iteration) with one LDS wait per splat, and stops as soon as
-// all 64 of its pixels are saturated.  Whether a splat takes the slow blend
-// step is a second ballot, so the choice is a scalar branch.
+// T x T tile (16 or 32), (T / 8)^2 wave64s, each wave owning one 8x8
+// quadrant.  Every splat of an LDS batch is tested against the quadrant's
+// pixel range with one ballot per 64 splats; the wave then walks only the
+// overlapping splats (scalar bit iteration) with one LDS wait per splat, and
+// stops as soon as all 64 of its pixels are saturated.  Whether a splat takes
+// the slow blend step is a second ballot, so the choice is a scalar branch.
+template <int T>
+struct QuadShape {
+    static constexpr int kSide = T / 8;             // quadrants a side
+    static constexpr int kQuads = kSide * kSide;    // waves
+    static constexpr int kThreads = 64 * kQuads;    // also the batch
+    static constexpr uint32_t kSlowBit = 1u << kQuads;
+};
+
+template <int T>
 __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, uint32_t item, StagedSplats st) {
-    constexpr int BATCH = 256;
+    using Q = QuadShape<T>;
+    constexpr int BATCH = Q::kThreads, SIDE = Q::kSide;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const TileRef t = tile_ref(a, item);
 
-    const int qx0 = t.tx * 16 + (int)(wave & 1u) * 8, qy0 = t.ty * 16 + (int)(wave >> 1) * 8;
+    const int qx0 = t.tx * T + (int)(wave % SIDE) * 8, qy0 = t.ty * T + (int)(wave / SIDE) * 8;
     const int px = qx0 + (int)(lane & 7u), py = qy0 + (int)(lane >> 3);
-    PixelTerms pt = pixel_terms<16>(px, py, t.tx, t.ty);
+    PixelTerms pt = pixel_terms<T>(px, py, t.tx, t.ty);
     // kept in registers through the walk rather than recomputed in it
     asm volatile("" : "+v"(pt.XX), "+v"(pt.XY), "+v"(pt.YY), "+v"(pt.X), "+v"(pt.Y));
     const bool valid = px < t.width && py < t.height;
@@ -586,23 +596,29 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
         if (!__syncthreads_or(tr >= kUMin)) break;
         uint32_t e = base + tid;
         if (e < end) {
-            const StagedTerms s = stage_splat<16>(a, t, load_record(a, g_next), tid, st);
+            const StagedTerms s = stage_splat<T>(a, t, load_record(a, g_next), tid, st);
             // the quadrants the splat can touch, tested once here for all
-            // four waves (the tests share the splat's terms)
-            uint32_t quads = s.slow ? 16u : 0u;
-            // pixel-centre offsets of the rectangle's part in the left and
-            // right (top and bottom) halves of the tile
-            const int xm = t.tx * 16 + 8, ym = t.ty * 16 + 8;
-            const float xa[2] = {(float)max(xm - 8, s.rect.x) + 0.5f - s.cx, (float)max(xm, s.rect.x) + 0.5f - s.cx};
-            const float xb[2] = {(float)min(xm, s.rect.y) - 0.5f - s.cx, (float)min(xm + 8, s.rect.y) - 0.5f - s.cx};
-            const float ya[2] = {(float)max(ym - 8, s.rect.z) + 0.5f - s.cy, (float)max(ym, s.rect.z) + 0.5f - s.cy};
-            const float yb[2] = {(float)min(ym, s.rect.w) - 0.5f - s.cy, (float)min(ym + 8, s.rect.w) - 0.5f - s.cy};
+            // waves (the tests share the splat's terms): the rectangle's
+            // pixel-centre offsets in each column and row of quadrants
+            uint32_t quads = s.slow ? Q::kSlowBit : 0u;
+            float xa[SIDE], xb[SIDE], ya[SIDE], yb[SIDE];
+            bool ox[SIDE], oy[SIDE];
+#pragma unroll
+            for (int i = 0; i < SIDE; ++i) {
+                const int x0 = t.tx * T + 8 * i, y0 = t.ty * T + 8 * i;
+                ox[i] = s.rect.x < x0 + 8 && s.rect.y > x0;
+                oy[i] = s.rect.z < y0 + 8 && s.rect.w > y0;
+                xa[i] = (float)max(x0, s.rect.x) + 0.5f - s.cx;
+                xb[i] = (float)min(x0 + 8, s.rect.y) - 0.5f - s.cx;
+                ya[i] = (float)max(y0, s.rect.z) + 0.5f - s.cy;
+                yb[i] = (float)min(y0 + 8, s.rect.w) - 0.5f - s.cy;
+            }
             const bool ellipse_test = a.quadrant_cull == 1 && s.diag;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int x0 = xm - 8 + (q & 1) * 8, y0 = ym - 8 + (q >> 1) * 8;
-                bool hit = s.rect.x < x0 + 8 && s.rect.y > x0 && s.rect.z < y0 + 8 && s.rect.w > y0;
-                if (ellipse_test) hit = hit && ellipse_meets_box(s, xa[q & 1], xb[q & 1], ya[q >> 1], yb[q >> 1]);
+            for (int q = 0; q < Q::kQuads; ++q) {
+                const int i = q % SIDE, j = q / SIDE;
+                bool hit = ox[i] && oy[j];
+                if (ellipse_test) hit = hit && ellipse_meets_box(s, xa[i], xb[i], ya[j], yb[j]);
                 quads |= hit ? 1u << q : 0u;
             }
             st.quads[tid] = quads;
@@ -617,7 +633,7 @@ __device__ __forceinline__ void composite_tile_quad(const CompositeLaunch& a, ui
             const uint32_t j = chunk + lane;
             const uint32_t q = j < cnt ? st.quads[j] : 0u;
             uint64_t mask = __ballot((q >> wave) & 1u);
-            const uint64_t slow = __ballot((q >> 4) & 1u);
+            const uint64_t slow = __ballot((q & Q::kSlowBit) != 0u);
             while (mask) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(mask);
                 const uint32_t k = chunk + b;
@@ -880,7 +896,7 @@ __device__ __forceinline__ uint32_t next_tile(const CompositeLaunch& a, uint32_t
 
 template <int T, bool QUAD>
 struct CompositeKernelShape {
-    static constexpr int kThreads = QUAD ? 256 : TileShape<T>::kThreads;
+    static constexpr int kThreads = QUAD ? QuadShape<T>::kThreads : TileShape<T>::kThreads;
 };
 
 // The reference-arithmetic compositor over 16x16 tiles, launched like
@@ -925,7 +941,7 @@ __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads), (QU
     const StagedSplats st{s_geo, s_par, s_col, s_exact, s_quads, s_ctr};
     if (a.persistent_blocks == 0u) {
         const uint32_t item = xcd_remap(blockIdx.x, a.total_tiles);
-        if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
+        if constexpr (QUAD) composite_tile_quad<T>(a, item, st); else composite_tile<T>(a, item, st);
         return;
     }
     const uint32_t xcd = blockIdx.x & 7u;
@@ -935,7 +951,7 @@ __global__ void __launch_bounds__((CompositeKernelShape<T, QUAD>::kThreads), (QU
         const uint32_t item = s_item;
         __syncthreads();
         if (item >= a.total_tiles) break;
-        if (QUAD) composite_tile_quad(a, item, st); else composite_tile<T>(a, item, st);
+        if constexpr (QUAD) composite_tile_quad<T>(a, item, st); else composite_tile<T>(a, item, st);
     }
 }
 
@@ -989,12 +1005,15 @@ hipError_t launch_composite(const CompositeLaunch& a, hipStream_t s) {
             break;
         case 16:
             if (a.quadrant_cull)
-                hipLaunchKernelGGL((composite_kernel<16, true>), dim3(grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((composite_kernel<16, true>), dim3(grid), dim3(QuadShape<16>::kThreads), 0, s, a);
             else
                 hipLaunchKernelGGL((composite_kernel<16, false>), dim3(grid), dim3(TileShape<16>::kThreads), 0, s, a);
             break;
         case 32:
-            hipLaunchKernelGGL((composite_kernel<32, false>), dim3(grid), dim3(TileShape<32>::kThreads), 0, s, a);
+            if (a.quadrant_cull)
+                hipLaunchKernelGGL((composite_kernel<32, true>), dim3(grid), dim3(QuadShape<32>::kThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((composite_kernel<32, false>), dim3(grid), dim3(TileShape<32>::kThreads), 0, s, a);
             break;
         default:
             return hipErrorInvalidValue;

@@ -191,23 +191,24 @@ def _needle_scene(gs, seed=21, n=20000):
     return g
 
 
-@pytest.mark.parametrize("scene", ["c1", "needles"])
-def test_quadrant_ellipse_cull_identical_image(ctx, gs, c1_scene, scene):
+@pytest.mark.parametrize("scene,tile", [("c1", 16), ("needles", 16), ("c1", 32), ("needles", 32)])
+def test_quadrant_ellipse_cull_identical_image(ctx, gs, c1_scene, scene, tile):
     """Skipping (splat, quadrant) pairs whose alpha >= 1/255 ellipse misses
-    the quadrant must not change a single pixel."""
+    the quadrant must not change a single pixel (16x16 tiles: four 8x8
+    quadrants, 32x32 tiles: sixteen)."""
     g, cfg, cam = c1_scene
     if scene == "needles":
         g = _needle_scene(gs)
+    print(f"case: quadrant ellipse cull scene={scene} tile={tile}", flush=True)
     ctx.upload(g)
     out = {}
     try:
         for quad in (2, 1):
             ctx.set_option(abi.GS_OPT_QUADRANT_CULL, quad)
-            ctx.render_alpha(cam, tile=16)
+            ctx.render_alpha(cam, tile=tile)
             out[quad] = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy()
     finally:
         ctx.set_option(abi.GS_OPT_QUADRANT_CULL, 1)
-    print(f"case: quadrant ellipse cull scene={scene}", flush=True)
     np.testing.assert_array_equal(out[1], out[2])
     assert out[1].any()
 
