@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "gs_internal.h"
 
@@ -42,14 +43,20 @@ __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 u) { return __builtin_bit_ca
 
 // Block tile: 32 pixels wide, 4 * RG / 2 rows (RG 16-pixel row groups per
 // wave).  The input tile with a one-pixel halo is staged in LDS once (channel
-// pitch cin + 8 spreads the 16-byte reads of consecutive pixels over the
-// banks), followed by a table that maps each group of 8 k to its LDS offset
-// (tap row * TW + tap column) * pitch + channel.  The MFMA's A operand is the
-// weights (M = output channel, through the L1/L2 caches) and its B operand
-// the input pixels (N = pixel, from LDS), so a lane's accumulator holds four
-// consecutive output channels of one pixel: one packed 8-byte store each.
-// k beyond 9 * cin (the zero padding of K) reads channel 0 of the tile: the
-// padded weights are zero, so those products add nothing.
+// pitch + 8 spreads the 16-byte reads of consecutive pixels over the banks).
+// Channels taken from the nearest 2x upsample of a half-size tensor are
+// staged at half size (the (TR / 2 + 1) x (TW / 2 + 1) half-size pixels the
+// tile's halo covers, a quarter of the bytes), so a wide decoder input holds
+// more blocks per CU.  A table maps each group of 8 k to its LDS offset from
+// the lane's pixel: (tap row * TW + tap column) * pitch + channel at full
+// size; at half size the offset depends on the parities of the lane's row and
+// column, so each group has four entries (bit 31 marks the half-size part).
+// The MFMA's A operand is the weights (M = output channel, through the L1/L2
+// caches) and its B operand the input pixels (N = pixel, from LDS), so a
+// lane's accumulator holds four consecutive output channels of one pixel: one
+// packed 8-byte store each.  k beyond 9 * cin (the zero padding of K) reads
+// the lane's own pixel: the padded weights are zero, so those products add
+// nothing.
 constexpr int kConvTileW = 32;
 
 __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
@@ -99,55 +106,97 @@ __device__ __forceinline__ void stage_tile(uint16_t* s_dst, int cp, uint16_t* s_
     }
 }
 
-template <int NT, int RG>
-__global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
+// Half-size rows and columns of a block's upsampled part: the full-size
+// halo rows y0 - 1 .. y0 + BR (y0 even) lie on half-size rows y0 / 2 - 1 ..
+// y0 / 2 + BR / 2.
+template <int BR>
+struct HalfTile {
+    static constexpr int kRows = BR / 2 + 2, kCols = kConvTileW / 2 + 2;
+};
+
+// LDS element offset, from the lane's pixel, of k-group kgp (8 channels of one
+// tap) for a lane whose output row and column have parities (ya, xb); bit 31
+// set: the group lies in the half-size part.  k past 9 * cin: offset 0.
+__device__ __forceinline__ uint32_t conv_tab_entry(int kgp, int ya, int xb, int cin, int split, int cpf, int cph,
+                                                   int tw, int hw) {
+    const int k = kgp * 8;
+    if (k >= 9 * cin) return 0u;
+    const int tap = k / cin, ci = k - tap * cin, dy = tap / 3, dx = tap % 3;
+    if (ci >= split) return (uint32_t)((dy * tw + dx) * cpf + (ci - split));
+    // half-size row of input row (lane row + dy - 1): (row >> 1) + ((ya + dy - 1) >> 1) + 1
+    // relative to the half tile's first row, and likewise for the column
+    const int oy = ((ya + dy + 1) >> 1), ox = ((xb + dx + 1) >> 1);
+    return 0x80000000u | (uint32_t)((oy * hw + ox) * cph + ci);
+}
+
+// HALF: channels [0, split) come from the half-size tensor x2 (kept at half
+// size in LDS); otherwise every channel comes from x and the table holds one
+// entry per k-group.
+// The 32-channel kernel without a half-size part (whose input tiles allow
+// five blocks per CU) is held to 96 VGPRs, five waves per SIMD.
+template <int NT, int RG, bool HALF>
+__global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kernel(ConvLaunch a) {
     extern __shared__ __align__(16) uint16_t s_in[];
     constexpr int RPW = RG / 2;            // rows per wave
     constexpr int BR = 4 * RPW;            // rows per block
     constexpr int TW = kConvTileW + 2, TR = BR + 2;
+    constexpr int HR = HalfTile<BR>::kRows, HW = HalfTile<BR>::kCols;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int y0 = (int)blockIdx.y * BR;
     const int x0 = (int)blockIdx.x * kConvTileW;
     const int n = (int)blockIdx.z;
-    const int cin = a.cin, cp = cin + 8, groups = cin / 8;
-    uint16_t* s_dummy = s_in + TR * TW * cp;  // 16 bytes
-    int* s_tab = reinterpret_cast<int*>(s_dummy + 8);
+    const int cin = a.cin;
+    const int split = HALF ? a.split : 0;  // channels [0, split) at half size
+    const int cpf = cin - split + 8, cph = split + 8;
+    const int hbase = TR * TW * cpf;       // element offset of the half-size part
+    const int words_in = hbase + (split ? HR * HW * cph : 0);
+    uint16_t* s_dummy = s_in + words_in;   // 16 bytes
+    uint32_t* s_tab = reinterpret_cast<uint32_t*>(s_dummy + 8);
 
-    // k-group table (kpad / 8 <= 256 entries)
-    if (tid < a.kpad / 8) {
-        const int k = tid * 8;
-        int off = 0;
-        if (k < 9 * cin) {
-            const int tap = k / cin, ci = k - tap * cin;
-            off = ((tap / 3) * TW + tap % 3) * cp + ci;
-        }
-        s_tab[tid] = off;
-    }
+    // k-group table: kpad / 8 <= 256 groups; with HALF four entries each,
+    // ordered [group][column parity][row parity]
+    constexpr int kEntries = HALF ? 4 : 1;
+    for (int i = tid; i < a.kpad / 8 * kEntries; i += 256)
+        s_tab[i] = HALF ? conv_tab_entry(i >> 2, i & 1, (i >> 1) & 1, cin, split, cpf, cph, TW, HW)
+                        : conv_tab_entry(i, 0, 0, cin, 0, cpf, cph, TW, HW);
 
     // stage rows y0-1 .. y0+BR, columns x0-1 .. x0+32 (zero outside the
-    // image).  Thread tid owns channel group tid % groups of every ppass-th
-    // tile pixel from tid / groups on, so its source (x, or with x2 the
-    // half-size tensor for channels below split), stride and upsample shift
-    // are fixed.  Offsets are 32-bit within one image (the host checks
-    // h * w * stride < 2^31).  Blocks clear of the image border skip the
+    // image) of the full-size channels, and the half-size pixels under them
+    // of the upsampled ones.  Thread tid owns channel group tid % groups of
+    // every ppass-th tile pixel from tid / groups on.  Offsets are 32-bit
+    // within one image (the host checks h * w * stride < 2^31).  Blocks
+    // clear of the image border (then the half-size part is too) skip the
     // bounds checks; the others zero the tile first.
     {
-        const int g = tid % groups, p0 = tid / groups, ppass = 256 / groups;
-        const bool up = a.x2 && g * 8 < a.split;
-        const int sh = up ? 1 : 0;
-        const int sw = up ? a.w / 2 : a.w, ss = up ? a.x2s : a.xs;
-        const uint16_t* src = up ? a.x2 + (size_t)n * (a.h / 2) * (a.w / 2) * a.x2s + a.x2o + g * 8
-                                 : a.x + (size_t)n * a.h * a.w * a.xs + a.xo + g * 8;
         const bool interior = y0 >= 1 && x0 >= 1 && y0 + BR < a.h && x0 + kConvTileW < a.w;
-        if (interior) {
-            if (p0 < ppass)
-                stage_tile<TR, TW, false>(s_in + g * 8, cp, s_dummy, src, sh, sw, ss, y0, x0, a.h, a.w, p0, ppass);
-        } else {
-            const int words = TR * TW * cp / 8;
-            for (int i = tid; i < words; i += 256) reinterpret_cast<uint4*>(s_in)[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (!interior) {
+            for (int i = tid; i < words_in / 8; i += 256) reinterpret_cast<uint4*>(s_in)[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
-            if (p0 < ppass)
-                stage_tile<TR, TW, true>(s_in + g * 8, cp, s_dummy, src, sh, sw, ss, y0, x0, a.h, a.w, p0, ppass);
+        }
+        const int gf = (cin - split) / 8;
+        if (gf > 0) {
+            const int g = tid % gf, p0 = tid / gf, ppass = 256 / gf;
+            const uint16_t* src = a.x + (size_t)n * a.h * a.w * a.xs + a.xo + split + g * 8;
+            if (p0 < ppass) {
+                if (interior)
+                    stage_tile<TR, TW, false>(s_in + g * 8, cpf, s_dummy, src, 0, a.w, a.xs, y0, x0, a.h, a.w, p0, ppass);
+                else
+                    stage_tile<TR, TW, true>(s_in + g * 8, cpf, s_dummy, src, 0, a.w, a.xs, y0, x0, a.h, a.w, p0, ppass);
+            }
+        }
+        if (HALF) {
+            const int gh = split / 8;
+            const int g = tid % gh, p0 = tid / gh, ppass = 256 / gh;
+            const int h2 = a.h / 2, w2 = a.w / 2;
+            const uint16_t* src = a.x2 + (size_t)n * h2 * w2 * a.x2s + a.x2o + g * 8;
+            if (p0 < ppass) {
+                if (interior)
+                    stage_tile<HR, HW, false>(s_in + hbase + g * 8, cph, s_dummy, src, 0, w2, a.x2s, y0 / 2, x0 / 2, h2,
+                                              w2, p0, ppass);
+                else
+                    stage_tile<HR, HW, true>(s_in + hbase + g * 8, cph, s_dummy, src, 0, w2, a.x2s, y0 / 2, x0 / 2, h2,
+                                             w2, p0, ppass);
+            }
         }
     }
     __syncthreads();
@@ -159,10 +208,16 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[r][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
-    // LDS element offset of this lane's pixel in each row group (tap 0)
-    int pix[RG];
+    // LDS element offset of this lane's pixel in each row group (tap 0), at
+    // full and at half size
+    int pix[RG], pixh[HALF ? RG : 1];
 #pragma unroll
-    for (int r = 0; r < RG; ++r) pix[r] = ((wave * RPW + r / 2) * TW + (r & 1) * 16 + row) * cp;
+    for (int r = 0; r < RG; ++r) {
+        const int ty = wave * RPW + r / 2, tx = (r & 1) * 16 + row;
+        pix[r] = (ty * TW + tx) * cpf;
+        if constexpr (HALF) pixh[r] = hbase + ((ty >> 1) * HW + (tx >> 1)) * cph;
+    }
+    const int xb = row & 1;
     const uint16_t* wl = a.wt + row * a.kpad + kg * 8;
     const int wtile = 16 * a.kpad;
     const int chunks = a.kpad / 32;
@@ -170,10 +225,24 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) w[t] = *reinterpret_cast<const uint4*>(wl + t * wtile + c * 32);
     };
-    auto chunk = [&](const uint4 (&w)[NT], int off) {
+    // with HALF the k-group's two entries for this lane's column parity (row
+    // parity 0, 1)
+    typedef std::conditional_t<HALF, uint2, uint32_t> TabEntry;
+    auto tab = [&](int c) -> TabEntry {
+        if constexpr (HALF) return *reinterpret_cast<const uint2*>(s_tab + ((c * 4 + kg) * 4 + xb * 2));
+        else return s_tab[c * 4 + kg];
+    };
+    auto chunk = [&](const uint4 (&w)[NT], TabEntry off) {
 #pragma unroll
         for (int r = 0; r < RG; ++r) {
-            const uint4 pv = *reinterpret_cast<const uint4*>(s_in + pix[r] + off);
+            int adr;
+            if constexpr (HALF) {
+                const uint32_t e = ((wave * RPW + r / 2) & 1) ? off.y : off.x;
+                adr = (int)(e & 0x7FFFFFFFu) + ((e >> 31) ? pixh[r] : pix[r]);
+            } else {
+                adr = (int)off + pix[r];
+            }
+            const uint4 pv = *reinterpret_cast<const uint4*>(s_in + adr);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(w[t]), as_bf16x8(pv), acc[r][t], 0,
@@ -190,13 +259,13 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
     // to cover an L2 load with the two-chunk lead of a ping-pong pair
     constexpr int PD = NT <= 2 ? 4 : (NT <= 4 ? 3 : 2);
     uint4 w[PD][NT];
-    int off[PD];
+    TabEntry off[PD];
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
-        off[j] = 0;
+        off[j] = TabEntry{};
         if (j < chunks) {
             load_w(w[j], j);
-            off[j] = s_tab[j * 4 + kg];
+            off[j] = tab(j);
         }
     }
     for (int c = 0; c < chunks; c += PD) {
@@ -206,7 +275,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvLaunch a) {
                 chunk(w[j], off[j]);
                 if (c + j + PD < chunks) {
                     load_w(w[j], c + j + PD);
-                    off[j] = s_tab[(c + j + PD) * 4 + kg];
+                    off[j] = tab(c + j + PD);
                 }
             }
         }
@@ -656,8 +725,8 @@ inline uint32_t blocks_for(long n, int t = 256) { return (uint32_t)((n + t - 1) 
 
 }  // namespace
 
-template <int NT, int RG>
-hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
+template <int NT, int RG, bool HALF>
+hipError_t conv3x3_launch_as(const ConvLaunch& a, hipStream_t s) {
     constexpr int BR = 4 * (RG / 2);
     if (a.pool && (RG != 4 || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
     if (a.x2 && (a.split % 8 || a.split > a.cin || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
@@ -665,18 +734,27 @@ hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
     // element offsets inside one image are 32-bit
     if ((size_t)a.h * a.w * std::max(std::max(a.xs, a.ys), a.x2 ? a.x2s : 0) >= (1u << 31)) return hipErrorInvalidValue;
     if (a.pool && (size_t)a.h * a.w * a.ps >= (1u << 31)) return hipErrorInvalidValue;
-    const size_t lds = (size_t)(BR + 2) * (kConvTileW + 2) * (a.cin + 8) * 2 + 16 + (size_t)a.kpad / 8 * 4;
+    const int split = HALF ? a.split : 0;
+    if (HALF && (split == a.cin || split == 0)) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)(BR + 2) * (kConvTileW + 2) * (a.cin - split + 8) +
+                        (HALF ? (size_t)HalfTile<BR>::kRows * HalfTile<BR>::kCols * (split + 8) : 0)) * 2 +
+                       16 + (size_t)a.kpad / 8 * 4 * (HALF ? 4 : 1);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv3x3_kernel<NT, RG>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv3x3_kernel<NT, RG, HALF>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     dim3 grid((a.w + kConvTileW - 1) / kConvTileW, (a.h + BR - 1) / BR, a.n);
-    hipLaunchKernelGGL((conv3x3_kernel<NT, RG>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((conv3x3_kernel<NT, RG, HALF>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
+}
+
+template <int NT, int RG>
+hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
+    return a.x2 ? conv3x3_launch_as<NT, RG, true>(a, s) : conv3x3_launch_as<NT, RG, false>(a, s);
 }
 
 hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s) {
