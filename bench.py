@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -29,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Kernel timed by each roofline candidate (the cull and composite brackets
 # hold exactly one launch of it; the sort bracket holds its 15 launches, of
 # which those past the passes the key range needs exit at once).
-STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<7,true,true>",
+STAGE_KERNEL = {"cull": "cull_kernel<0>", "sort": "downsweep_kernel<9,true,true,false>",
                 "composite": "composite_kernel<16,true>"}
 PMC_PROFILE = ROOT / "profiles" / "pmc_sorted_alpha_6M_1080p.md"
 
@@ -44,6 +45,21 @@ def pmc_traffic(kernel: str):
         if len(cells) >= 3 and cells[0] == kernel:
             try:
                 return int(round((float(cells[1]) + float(cells[2])) * 1e6))
+            except ValueError:
+                return None
+    return None
+
+
+def pmc_valu(kernel: str):
+    """VALU wave instructions per dispatch of `kernel` from the committed
+    counter profile of the default workload (SQ_INSTS_VALU), or None."""
+    if not PMC_PROFILE.exists():
+        return None
+    for line in PMC_PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == kernel:
+            try:
+                return float(cells[3])
             except ValueError:
                 return None
     return None
@@ -501,7 +517,12 @@ def main():
     print(f"# upload: {json.dumps(upload)}", file=sys.stderr, flush=True)
     if args.graph:
         ctx.set_option(gs.abi.GS_OPT_GRAPH, 1)
-    print(f"# {ctx.device_info()}", file=sys.stderr, flush=True)
+    dinfo = ctx.device_info()
+    print(f"# {dinfo}", file=sys.stderr, flush=True)
+    m = re.search(r"(\d+) CUs", dinfo)
+    info_cus = int(m.group(1)) if m else 256
+    m = re.search(r"clock (\d+) MHz", dinfo)
+    info_clock_mhz = int(m.group(1)) if m else 2400
 
     # sorted-alpha frames
     for _ in range(args.warmup):
@@ -630,7 +651,12 @@ def main():
         "sort": (stage_ms["sort"], sort_bytes),
         "composite": (stage_ms["composite"], comp_bytes),
     }
-    dom = max(cands, key=lambda k: cands[k][0])
+    # the roofline kernel is the longest-running of the HBM-bound ones (cull
+    # kernel, depth sort); the composite kernel, which takes about as long
+    # as the cull kernel, is bound by VALU issue, and its figure is that
+    # fraction (VALU wave instructions x 4 cycles over the SIMD cycles of
+    # its time, from the committed counter profile) under "others"
+    dom = max(("cull", "sort"), key=lambda k: cands[k][0])
     dms, dbytes = cands[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
     # cull + sort against the 60%-of-peak target: the contract bytes of both
@@ -658,6 +684,13 @@ def main():
                        "implementation_bytes": impl_bytes[k]}
                    for k, v in cands.items()},
     }
+    valu = pmc_valu(STAGE_KERNEL["composite"]) if args.workload == "c3" and args.tile in (0, 16) else None
+    cms = stage_ms["composite"]
+    if valu and cms > 0:
+        simd_cycles = info_cus * 4 * info_clock_mhz * 1e6 * cms * 1e-3
+        roofline["others"]["composite"].update({
+            "bound": "valu", "valu_wave_instructions": valu,
+            "valu_issue_frac": round(valu * 4.0 / simd_cycles, 4) if simd_cycles else None})
 
     mv = None
     if not args.no_multiview:

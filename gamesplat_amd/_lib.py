@@ -48,6 +48,7 @@ GS_OPT_BLOCKS_PER_CU = 5
 GS_OPT_GRAPH = 6
 GS_OPT_REFERENCE_BLEND = 7
 GS_OPT_WIDE_INDEX = 8
+GS_OPT_TILE_KEY16 = 9
 GS_SORT_STATIC_MAX_PAIRS = 1 << 22
 MAX_VIEWS = 64
 

@@ -91,7 +91,14 @@ __global__ void __launch_bounds__(256) bin_count_kernel(BinLaunch a) {
 // splats' offsets.  Pairs past the capacity are not written.
 constexpr uint32_t kOwnPairs = 16u;
 
-template <bool O32>
+// A tile key of the pairs: 32-bit, or 16-bit with K16.
+template <bool O32, bool K16>
+__device__ __forceinline__ void store_tile_key(uint32_t* keys, uint32_t i, uint32_t k) {
+    if constexpr (K16) at<O32>(reinterpret_cast<uint16_t*>(keys), i) = (uint16_t)k;
+    else at<O32>(keys, i) = k;
+}
+
+template <bool O32, bool K16>
 __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
     __shared__ uint32_t s_scan[4];
     __shared__ uint32_t s_lstart[256];   // large splats: first pair (block-relative) ...
@@ -132,7 +139,7 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         const uint32_t x0 = r.x & 0xFFFFu, x1 = r.x >> 16;
         uint32_t xx = x0, row = tbase + (r.y & 0xFFFFu) * tx;
         for (uint32_t i = start; i < start + cnt && i < emit; ++i) {
-            at<O32>(a.dup_keys, base + i) = row + xx;
+            store_tile_key<O32, K16>(a.dup_keys, base + i, row + xx);
             at<O32>(a.dup_vals, base + i) = val;
             if (++xx == x1) {
                 xx = x0;
@@ -163,29 +170,38 @@ __global__ void __launch_bounds__(256) bin_emit_kernel(BinLaunch a) {
         if (i >= emit) continue;
         const uint2 rr = s_rect[o];
         const uint32_t x0 = rr.x & 0xFFFFu, w = (rr.x >> 16) - x0, y0 = rr.y & 0xFFFFu;
-        at<O32>(a.dup_keys, base + i) = s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w;
+        store_tile_key<O32, K16>(a.dup_keys, base + i, s_tbase[o] + (y0 + k / w) * s_tx[o] + x0 + k % w);
         at<O32>(a.dup_vals, base + i) = s_id[o];
     }
 }
 
 // [begin, end) of every tile's entries in the tile-sorted keys: each run
 // start writes its own begin and the previous run's end.  A thread takes four
-// consecutive keys (one 16-byte load when they are all in range); the key
-// before them is the last of the neighbouring lane's four.
-__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, const uint32_t* count_ptr, uint2* ranges) {
+// consecutive keys (one 16-byte load, 8-byte with 16-bit keys, when they are
+// all in range); the key before them is the last of the neighbouring lane's
+// four.
+template <bool K16>
+__global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys32, const uint32_t* count_ptr, uint2* ranges) {
+    const uint16_t* keys16 = reinterpret_cast<const uint16_t*>(keys32);
+    auto key = [&](uint32_t i) -> uint32_t { return K16 ? (uint32_t)keys16[i] : keys32[i]; };
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *count_ptr;
     const uint32_t i0 = t * 4u;
     uint32_t k[4];
     if (i0 + 3u < count) {
-        const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
-        k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+        if constexpr (K16) {
+            const uint2 v = *reinterpret_cast<const uint2*>(keys16 + i0);
+            k[0] = v.x & 0xFFFFu; k[1] = v.x >> 16; k[2] = v.y & 0xFFFFu; k[3] = v.y >> 16;
+        } else {
+            const uint4 v = *reinterpret_cast<const uint4*>(keys32 + i0);
+            k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+        }
     } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < count ? keys[i0 + j] : 0xFFFFFFFFu;
+        for (int j = 0; j < 4; ++j) k[j] = i0 + (uint32_t)j < count ? key(i0 + j) : 0xFFFFFFFFu;
     }
     uint32_t prev = __shfl_up(k[3], 1, 64);
-    if (__lane_id() == 0) prev = (i0 > 0 && i0 - 1u < count) ? keys[i0 - 1u] : 0xFFFFFFFFu;
+    if (__lane_id() == 0) prev = (i0 > 0 && i0 - 1u < count) ? key(i0 - 1u) : 0xFFFFFFFFu;
     if (i0 >= count) return;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -968,19 +984,28 @@ hipError_t launch_bin(const BinLaunch& a, hipStream_t s) {
                                       nullptr, s, a.dup_capacity, &a.counters->bin.dup_count);
     if (e != hipSuccess) return e;
     // pair positions stay below the capacity
-    if ((uint64_t)a.dup_capacity * 4u <= 0xFFFFFFFFull && !a.wide_index)
-        hipLaunchKernelGGL(bin_emit_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
+    const bool o32 = (uint64_t)a.dup_capacity * 4u <= 0xFFFFFFFFull && !a.wide_index;
+    if (o32 && a.key16)
+        hipLaunchKernelGGL((bin_emit_kernel<true, true>), dim3(blocks), dim3(256), 0, s, a);
+    else if (o32)
+        hipLaunchKernelGGL((bin_emit_kernel<true, false>), dim3(blocks), dim3(256), 0, s, a);
+    else if (a.key16)
+        hipLaunchKernelGGL((bin_emit_kernel<false, true>), dim3(blocks), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(bin_emit_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((bin_emit_kernel<false, false>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint32_t capacity, uint2* ranges,
+hipError_t launch_tile_ranges(const uint32_t* keys, int key16, const uint32_t* count, uint32_t capacity, uint2* ranges,
                               hipStream_t s) {
     if (!capacity) return hipSuccess;
     if (reinterpret_cast<uintptr_t>(keys) % 16) return hipErrorInvalidValue;
     const uint32_t threads = (capacity + 3u) / 4u;
-    hipLaunchKernelGGL(tile_ranges_kernel, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+    if (key16)
+        hipLaunchKernelGGL(tile_ranges_kernel<true>, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, count, ranges);
+    else
+        hipLaunchKernelGGL(tile_ranges_kernel<false>, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, count,
+                           ranges);
     return hipGetLastError();
 }
 

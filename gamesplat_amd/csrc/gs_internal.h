@@ -236,6 +236,7 @@ struct SortLaunch {
     int passes;
     int bits;                // digit width (9, 8, 7, 6 or 4)
     int wide_index;          // 1: 64-bit element indexing even where 32-bit offsets suffice
+    int key16;               // 1: keys are 16-bit (below 2^16; no dense pass, no key range)
     // optional dense first pass (digit widths 7 and 4): it reads dense_count
     // slots of dense_keys (0xFFFFFFFF: culled) instead of keys[0] and count,
     // keeps the others in slot order with the slot as value, and with vals2
@@ -284,11 +285,12 @@ struct BinLaunch {
     uint32_t* dup_vals;
     uint32_t dup_capacity;   // pairs that fit in dup_keys / dup_vals
     int wide_index;          // 1: 64-bit pair indexing even where 32-bit offsets suffice
+    int key16;               // 1: dup_keys holds 16-bit tile ids (every tile id below 2^16)
 };
 // Counts and emits (tile id, splat id) pairs; leaves the requested total in
 // counters->bin.dup_total and min(total, capacity) in counters->bin.dup_count.
 hipError_t launch_bin(const BinLaunch& a, hipStream_t s);
-hipError_t launch_tile_ranges(const uint32_t* keys, const uint32_t* count, uint32_t capacity, uint2* ranges,
+hipError_t launch_tile_ranges(const uint32_t* keys, int key16, const uint32_t* count, uint32_t capacity, uint2* ranges,
                               hipStream_t s);
 
 struct CompositeLaunch {

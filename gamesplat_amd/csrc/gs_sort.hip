@@ -16,6 +16,8 @@
 //              array, which keeps the block's LDS small enough for seven
 //              blocks per CU.
 // An optional second 32-bit value array moves with the same permutation.
+// Keys can also be 16-bit (K16: the tile ids of frames with at most 65536
+// tiles), held as 32-bit values in registers and LDS.
 // The first pass can also read the cull's dense [view][n] keys directly
 // (DENSE): slots holding the culled key 0xFFFFFFFF take no digit count and no
 // rank, so that pass also does the order-preserving compaction of the visible
@@ -76,6 +78,19 @@ struct PassPlan {
     uint32_t sub, add;
 };
 
+// Key i of an array of 32-bit or (K16) 16-bit keys.
+template <bool O32, bool K16>
+__device__ __forceinline__ uint32_t load_key(const uint32_t* keys, uint32_t i) {
+    if constexpr (K16) return at<O32>(reinterpret_cast<const uint16_t*>(keys), i);
+    else return at<O32>(keys, i);
+}
+
+template <bool O32, bool K16>
+__device__ __forceinline__ void store_key(uint32_t* keys, uint32_t i, uint32_t k) {
+    if constexpr (K16) at<O32>(reinterpret_cast<uint16_t*>(keys), i) = (uint16_t)k;
+    else at<O32>(keys, i) = k;
+}
+
 template <int BITS>
 __device__ __forceinline__ PassPlan pass_plan(const PassArgs& a, uint2 kr) {
     int needed = a.passes;
@@ -103,7 +118,7 @@ __device__ __forceinline__ uint32_t load_count_range(const PassArgs& a, uint2* k
 // Digit counts of one tile.  Each wave counts into kCopies histograms, lane
 // l into copy l % kCopies, so a digit shared by many lanes (the top digit of
 // clustered keys) serializes kCopies times fewer LDS atomics.
-template <int BITS, bool O32, bool DENSE>
+template <int BITS, bool O32, bool DENSE, bool K16>
 __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -127,7 +142,7 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        keys[i] = g < count ? at<O32>(keys_in, g) : 0xFFFFFFFFu;
+        keys[i] = g < count ? load_key<O32, K16>(keys_in, g) : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
@@ -189,7 +204,7 @@ __global__ void __launch_bounds__(kThreads) colscan_kernel(PassArgs a) {
     if (tid == 0) a.col_total[d] = carry;
 }
 
-template <int BITS, bool P2, bool O32, bool DENSE>
+template <int BITS, bool P2, bool O32, bool DENSE, bool K16>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     constexpr int RADIX = 1 << BITS;
     constexpr uint32_t MASK = RADIX - 1;
@@ -236,7 +251,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         uint32_t g = wbase + (uint32_t)i * 64u + lane;
-        const uint32_t k = at<O32>(keys_in, min(g, last));
+        const uint32_t k = load_key<O32, K16>(keys_in, min(g, last));
         if constexpr (DENSE) {
             const bool keep = g < count && k != 0xFFFFFFFFu;
             kept |= (uint32_t)keep << i;
@@ -365,7 +380,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
             uint32_t k = s_buf[p];
             const uint32_t d = (k >> shift) & MASK;
             digits[j / kPerReg] |= d << ((j % kPerReg) * kDigitBits);
-            at<O32>(keys_out, s_dest[d] + p) = k + plan.add;
+            store_key<O32, K16>(keys_out, s_dest[d] + p, k + plan.add);
         }
     }
     __syncthreads();
@@ -422,14 +437,14 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     }
 }
 
-template <int BITS, bool O32, bool DENSE>
+template <int BITS, bool O32, bool DENSE, bool K16 = false>
 void launch_pass(const PassArgs& pa, bool p2, uint32_t tiles, hipStream_t s) {
-    hipLaunchKernelGGL((upsweep_kernel<BITS, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+    hipLaunchKernelGGL((upsweep_kernel<BITS, O32, DENSE, K16>), dim3(tiles), dim3(kThreads), 0, s, pa);
     hipLaunchKernelGGL(colscan_kernel<BITS>, dim3(1 << BITS), dim3(kThreads), 0, s, pa);
     if (p2)
-        hipLaunchKernelGGL((downsweep_kernel<BITS, true, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL((downsweep_kernel<BITS, true, O32, DENSE, K16>), dim3(tiles), dim3(kThreads), 0, s, pa);
     else
-        hipLaunchKernelGGL((downsweep_kernel<BITS, false, O32, DENSE>), dim3(tiles), dim3(kThreads), 0, s, pa);
+        hipLaunchKernelGGL((downsweep_kernel<BITS, false, O32, DENSE, K16>), dim3(tiles), dim3(kThreads), 0, s, pa);
 }
 
 // DENSE_OK: the digit widths a dense first pass is built for (the depth
@@ -476,7 +491,11 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
         pa.rect_px = dense ? a.dense_rect_px : nullptr;
         pa.rect_packed = dense ? a.dense_rect_packed : nullptr;
         pa.rect_shift = dense ? a.dense_rect_shift : 0;
-        if constexpr (DENSE_OK) {
+        if (a.key16 && o32)
+            launch_pass<BITS, true, false, true>(pa, p2, tiles, s);
+        else if (a.key16)
+            launch_pass<BITS, false, false, true>(pa, p2, tiles, s);
+        else if constexpr (DENSE_OK) {
             if (dense && o32)
                 launch_pass<BITS, true, true>(pa, p2, tiles, s);
             else if (dense)
@@ -512,6 +531,8 @@ hipError_t launch_radix_sort(const SortLaunch& a, int* result_slot, hipStream_t 
     if (a.passes < 1 || (a.passes - 1) * a.bits >= 32) return hipErrorInvalidValue;
     if (a.vals2[0] && (!a.vals2[1] || !a.vals2[2])) return hipErrorInvalidValue;
     if (a.key_range && (!a.keys[3] || !a.vals[3] || (a.vals2[0] && !a.vals2[3]))) return hipErrorInvalidValue;
+    // 16-bit keys: no dense pass, no key range, and every key below 2^16
+    if (a.key16 && (a.dense_keys || a.key_range || a.passes * a.bits > 32)) return hipErrorInvalidValue;
     if (a.dense_keys) {
         // the dense slots fit the grid; their second value is a packed
         // rectangle, from pixel rectangles at 8, 16 or 32 pixels a tile

@@ -503,7 +503,11 @@ enum {
      * element indices even where 32-bit byte offsets suffice (the variants
      * arrays of 2^30 or more elements take); results are identical.  For
      * testing those variants at small sizes.  0 (default): off. */
-    GS_OPT_WIDE_INDEX = 8
+    GS_OPT_WIDE_INDEX = 8,
+    /* 1 (default): frames with at most 65536 tiles (over all views) bin
+     * their (tile, splat) pairs with 16-bit tile keys; 0: 32-bit keys.
+     * Results are identical. */
+    GS_OPT_TILE_KEY16 = 9
 };
 int gs_set_option(gs_ctx* ctx, int option, int value);
 

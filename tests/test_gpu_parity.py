@@ -356,6 +356,28 @@ def test_wide_index_variants(ctx, orc, c1_scene):
     assert out[1][1].any()
 
 
+@pytest.mark.parametrize("tile,size", [(16, (1280, 720)), (8, (1280, 720)), (32, (1280, 720)), (8, (2048, 2048))])
+def test_tile_key_widths(ctx, gs, c1_scene, tile, size):
+    """16-bit tile keys (frames of at most 65536 tiles; 2048x2048 at 8-pixel
+    tiles has exactly 65536, ids up to 0xFFFF) give the same image as
+    32-bit ones."""
+    g, cfg, _ = c1_scene
+    W, H = size
+    print(f"case: tile key widths tile={tile} size={W}x{H}", flush=True)
+    cam = scenes.random_view_camera(cfg["seed"], W, H)
+    ctx.upload(g)
+    out = {}
+    try:
+        for k16 in (1, 0):
+            ctx.set_option(abi.GS_OPT_TILE_KEY16, k16)
+            ctx.render_alpha(cam, tile=tile)
+            out[k16] = ctx.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy()
+    finally:
+        ctx.set_option(abi.GS_OPT_TILE_KEY16, 1)
+    np.testing.assert_array_equal(out[1], out[0])
+    assert out[1].any()
+
+
 def test_visible_list_after_frame(ctx, orc, c1_scene):
     """A frame sorts from the cull's dense keys and reuses the shared block
     offsets for its bin; the visible list built afterwards on demand (for a
