@@ -102,7 +102,7 @@ struct gs_ctx {
     uint32_t n = 0;
     DevBuf vrects[4];  // packed tile rectangles moved by the depth sort
     DevBuf block_range;  // [view][block]: visible key range
-    DevBuf pad_rects;  // [view][n]: packed tile rectangles grown by a pixel (StochasticSplats)
+    DevBuf packed_rects;  // [view][n]: the cull's packed tile rectangles (grown by a pixel for StochasticSplats)
     // cull_counts, cull_offsets: the cull's per-block visible counts and
     // their offsets, kept for the compaction (block_counts and block_offsets
     // are shared by the bin and GPS scans)
@@ -165,6 +165,7 @@ struct gs_ctx {
     int reference_blend = 0;  // the composite's blend arithmetic (GS_OPT_REFERENCE_BLEND)
     int wide_index = 0;       // GS_OPT_WIDE_INDEX
     int tile_key16 = 1;       // GS_OPT_TILE_KEY16
+    bool rect_px_valid = false;  // the last cull wrote pixel rectangles (else packed tile rectangles only)
     int tight_rects = 1;
     int cull_tight = 1;      // tight_rects as the last cull used it
 
@@ -281,7 +282,7 @@ int ensure_views(gs_ctx* c, uint32_t views) {
     GS_CHECK(ensure(c, c->cull_offsets, blocks * 4, "cull block offsets"));
     GS_CHECK(ensure(c, c->rects, cap * 8, "bin rectangles"));
     GS_CHECK(ensure(c, c->rect_px, cap * 8, "pixel rectangles"));
-    GS_CHECK(ensure(c, c->pad_rects, cap * 4, "padded tile rectangles"));
+    GS_CHECK(ensure(c, c->packed_rects, cap * 4, "packed tile rectangles"));
     GS_CHECK(ensure_table(c, (uint32_t)std::max<size_t>(cap, c->dup_cap)));
     c->view_cap = views;
     return GS_OK;
@@ -480,7 +481,7 @@ void gs_destroy(gs_ctx* c) {
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
                       &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1],
                       &c->anim_data, &c->lut, &c->display, &c->vrects[0], &c->vrects[1], &c->vrects[2],
-                      &c->pad_rects};
+                      &c->packed_rects};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -613,7 +614,7 @@ int reserve_planes(gs_ctx* c, uint32_t count, DevBuf* planes) {
     size_t held_other = c->held - c->planes.bytes - c->records.bytes - c->keys[0].bytes - c->keys[1].bytes -
                         c->keys[2].bytes - c->keys[3].bytes - c->ids[0].bytes - c->ids[1].bytes -
                         c->ids[2].bytes - c->ids[3].bytes - c->vrects[0].bytes - c->vrects[1].bytes -
-                        c->vrects[2].bytes - c->vrects[3].bytes - c->pad_rects.bytes - c->block_range.bytes -
+                        c->vrects[2].bytes - c->vrects[3].bytes - c->packed_rects.bytes - c->block_range.bytes -
                         c->dense_keys.bytes - c->rects.bytes - c->rect_px.bytes - c->prepared.bytes - c->item_count.bytes -
                         c->item_base.bytes - c->work.bytes;
     if (c->budget && held_other + required > c->budget)
@@ -1002,6 +1003,25 @@ CullLaunch make_cull(gs_ctx* c, const GsSplatEffectParams* effect, const GsCullO
     return a;
 }
 
+// The cull's rectangle outputs for a plan: with a packing tile size the tile
+// rectangles packed at that size (grown by a pixel with pack_pad), which the
+// depth sort and the compaction carry, and no pixel rectangles (a bin at
+// another tile size then recomputes them from the records); else the pixel
+// rectangles, which the bin gathers.
+void set_cull_rects(gs_ctx* c, FramePlan* p) {
+    if (p->pack_tile) {
+        p->cull.rect_px = nullptr;
+        p->cull.packed_rects = ptr<uint32_t>(c->packed_rects);
+        p->cull.packed_shift = 0;
+        while ((1 << p->cull.packed_shift) < p->pack_tile) ++p->cull.packed_shift;
+        p->cull.packed_pad = p->pack_pad;
+    } else {
+        p->cull.rect_px = ptr<uint2>(c->rect_px);
+        p->cull.packed_rects = nullptr;
+    }
+    c->rect_px_valid = p->cull.rect_px != nullptr;
+}
+
 // Tile size the compact packs each visible splat's tile rectangle for (the
 // depth sort carries it to the binning, which then reads it in order instead
 // of gathering the pixel rectangle): `tile` when every view spans at most 255
@@ -1047,12 +1067,8 @@ SortLaunch make_depth_sort(gs_ctx* c) {
         const CullLaunch& cl = c->plan.cull;
         s.dense_keys = cl.dense_keys;
         s.dense_count = c->views * c->n;
-        if (c->rect_tile && c->rect_pad) {
-            s.dense_rect_packed = cl.pad_rects;
-        } else if (c->rect_tile) {
-            s.dense_rect_px = cl.rect_px;
-            while ((1 << s.dense_rect_shift) < c->rect_tile) ++s.dense_rect_shift;
-        }
+        // the cull packed the tile rectangles itself whenever they are carried
+        if (c->rect_tile) s.dense_rect_packed = cl.packed_rects;
     }
     return s;
 }
@@ -1082,6 +1098,14 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     b.counters = c->dc();
     b.rect_px = ptr<uint2>(c->rect_px);
     b.sorted_rects = c->rect_tile == tile && !c->rect_pad ? ptr<uint32_t>(c->vrects[c->sort_slot]) : nullptr;
+    if (!b.sorted_rects && !c->rect_px_valid) {
+        // the cull packed its rectangles for another tile size and wrote no
+        // pixel rectangles: they come from the records (unmoved, unpadded)
+        b.jittered = 1;
+        b.records = ptr<float4>(c->records);
+        b.tight = c->cull_tight;
+        b.clip_z_01 = c->cam.clip_z_01;
+    }
     b.rects = ptr<uint2>(c->rects);
     b.block_counts = ptr<uint32_t>(c->block_counts);
     b.block_offsets = ptr<uint32_t>(c->block_offsets);
@@ -1156,7 +1180,7 @@ int ensure_compacted(gs_ctx* c) {
                               visible, nullptr, nullptr, c->stream));
     GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets),
                           ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
-                          ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_pad ? p.cull.pad_rects : nullptr));
+                          ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_tile ? p.cull.packed_rects : nullptr));
     c->compacted = true;
     return GS_OK;
 }
@@ -1245,6 +1269,7 @@ int enqueue_stochastic(gs_ctx* c, bool* binned) {
         const bool padded = c->rect_pad && c->rect_tile == 16 && mode <= 1;
         if (padded) {
             c->plan.bin.sorted_rects = ptr<uint32_t>(c->vrects[c->sort_slot]);
+            c->plan.bin.jittered = 0;
         } else if (mode) {
             // the tile rectangles come from the records: grown by a pixel,
             // which holds every jitter up to half a pixel, or moved exactly
@@ -1383,10 +1408,7 @@ int gs_cull(gs_ctx* c, const GsCameraUniforms* cam, const GsSplatEffectParams* e
     const int pad_tile = c->composite_kind == 1 ? rect_pack_tile(cam, 1, 16) : 0;
     c->plan.pack_pad = pad_tile ? 1 : 0;
     c->plan.pack_tile = pad_tile ? pad_tile : rect_pack_tile(cam, 1, c->default_tile);
-    if (c->plan.pack_pad) {
-        c->plan.cull.pad_rects = ptr<uint32_t>(c->pad_rects);
-        c->plan.cull.pad_tile_shift = 4;
-    }
+    set_cull_rects(c, &c->plan);
     c->rect_tile = c->plan.pack_tile;
     c->rect_pad = c->plan.pack_pad;
     GS_CHECK(stage_cull(c, c->plan));
@@ -1476,6 +1498,7 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     p.cull_blocks = blocks;
     p.pack_tile = rect_pack_tile(cams, count, tile);
     p.pack_pad = 0;
+    set_cull_rects(c, &p);
     c->rect_tile = p.pack_tile;
     c->rect_pad = 0;
     c->compacted = false;  // this frame's cull builds no visible list

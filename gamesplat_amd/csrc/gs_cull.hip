@@ -441,14 +441,17 @@ __global__ void __launch_bounds__(256) cull_kernel(CullLaunch a) {
             // the rasteriser drops a quad outside the clip volume's depth
             // range (far plane): it covers no pixel, so it gets no tile
             const bool drawn = !depth_clipped(ndc_z, cam.clip_z_01);
-            a.rect_px[slot] = drawn ? pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
-                                                 (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0)
-                                    : make_uint2(0u, 0u);
-            if (a.pad_rects)
-                a.pad_rects[slot] = drawn ? pack_tile_rect(
+            if (a.rect_px)
+                a.rect_px[slot] = drawn ? pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y,
+                                                     opacity, (int)cam.viewport[0], (int)cam.viewport[1],
+                                                     a.tight_rects != 0)
+                                        : make_uint2(0u, 0u);
+            if (a.packed_rects)
+                a.packed_rects[slot] = drawn ? pack_tile_rect(
                     pixel_rect(cx, rcy, ca, cb * cam.clip_y_sign, cc, radius_x, radius_y, opacity,
-                               (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0, 0.0f, 0.0f, 1),
-                    a.pad_tile_shift) : 0u;
+                               (int)cam.viewport[0], (int)cam.viewport[1], a.tight_rects != 0, 0.0f, 0.0f,
+                               a.packed_pad),
+                    a.packed_shift) : 0u;
         }
         // dense key (culled = 0xFFFFFFFF, as cull_reset_cs leaves it) and the
         // block's visible count; compaction runs after the counts are scanned
@@ -589,7 +592,7 @@ hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views
     if (n == 0 || views == 0) return hipSuccess;
     int shift = 0;
     if (pack_tile) {
-        if (!rect_px || !vis_rects || (pack_tile != 8 && pack_tile != 16 && pack_tile != 32))
+        if ((!rect_px && !packed) || !vis_rects || (pack_tile != 8 && pack_tile != 16 && pack_tile != 32))
             return hipErrorInvalidValue;
         while ((1 << shift) < pack_tile) ++shift;
     } else {

@@ -183,12 +183,14 @@ struct CullLaunch {
     uint32_t* block_counts;  // [view][block]: visible Gaussians per 256-block
     uint2* block_range;      // [view][block]: min and max of the visible depth keys (or NULL)
     uint32_t* block_diag;    // 8 diagnostics per block (collect_stats, one view)
-    uint2* rect_px;          // [view][n]: covered pixel rectangle of each visible splat
+    // optional [view][n]: covered pixel rectangle of each visible splat
+    uint2* rect_px;
     int tight_rects;         // cut the rectangle to the opacity-aware ellipse box
-    // optional [view][n]: the rectangle grown by a pixel (every StochasticSplats
-    // jitter up to half a pixel), packed as tiles of 1 << pad_tile_shift
-    uint32_t* pad_rects;
-    int pad_tile_shift;
+    // optional [view][n]: the same rectangle grown by packed_pad pixels (1:
+    // every StochasticSplats jitter up to half a pixel) as its tile
+    // rectangle at tile size 1 << packed_shift (pack_tile_rect)
+    uint32_t* packed_rects;
+    int packed_shift, packed_pad;
     // GSAnim appearance: 0 none, 1 int8 deltas (48 per row and state, state
     // 0 excluded), 2 half coefficients (48 per row and layer)
     int anim;
@@ -204,7 +206,8 @@ hipError_t launch_cull(const CullLaunch& a, hipStream_t s);
 // rectangle of its pixel rectangle rect_px at that tile size as
 // x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when empty; every view must span at
 // most 255 tiles a side), which the depth sort carries to the binning; with
-// packed (the cull's pad_rects) that packed rectangle is moved instead.
+// packed (the cull's packed_rects) that packed rectangle is moved instead
+// and rect_px is not read.
 hipError_t launch_compact(const uint32_t* dense_keys, uint32_t n, uint32_t views, const uint32_t* offsets,
                           uint32_t* vis_keys, uint32_t* vis_ids, const uint2* rect_px, int pack_tile,
                           uint32_t* vis_rects, hipStream_t s, const uint32_t* packed = nullptr);
