@@ -636,15 +636,17 @@ def main():
     #   composite  56 D (48-byte record re-read per tile entry, 4-byte list
     #              entry written and read) + 8 P (RGBA16F image)
     # Next to each: the bytes this implementation moves by design (the cull
-    # kernel reads 16 N + 224 V and writes 4 N + 56 V; the depth sort moves
-    # key, id and packed tile rectangle, 4 + 24 bytes per splat and pass).
+    # kernel reads 16 N + 224 V and writes 4 N + 52 V: dense key, 48-byte
+    # record and packed tile rectangle; the depth sort moves key, id and
+    # packed tile rectangle, 4 + 24 bytes per splat and pass).
     cull_bytes = 20 * N + 272 * V
     sort_bytes = 68 * V
     comp_bytes = 56 * D + 8 * W * H
-    # the sort's first pass reads the cull's dense keys and pixel rectangles
-    # of all N slots (its upsweep the keys again) and writes the visible
-    # keys, ids and tile rectangles; each later pass reads and writes them
-    impl_bytes = {"cull": 20 * N + 280 * V, "sort": 16 * N + 12 * V + 28 * (depth_passes - 1) * V,
+    # the sort's first pass reads the cull's dense keys and packed
+    # rectangles of all N slots (its upsweep the keys again) and writes the
+    # visible keys, ids and tile rectangles; each later pass reads and
+    # writes them
+    impl_bytes = {"cull": 20 * N + 276 * V, "sort": 12 * N + 12 * V + 28 * (depth_passes - 1) * V,
                   "composite": comp_bytes}
     cands = {
         "cull": (stage_ms["cull_kernel"], cull_bytes),
