@@ -1078,13 +1078,17 @@ void make_composite(gs_ctx* c, int tile, FramePlan* p) {
     const uint32_t ntiles = c->total_tiles;
     int bits_needed = 1;
     while (bits_needed < 31 && ((ntiles - 1u) >> bits_needed)) ++bits_needed;
-    const int passes = (bits_needed + c->radix_bits - 1) / c->radix_bits;
-    // the pass count of the configured width, with the digits balanced over
-    // the passes (13 tile bits: two 7-bit passes, not 8 + 5): fewer digits
-    // per pass mean fewer ballots and longer runs of equal digits to write
+    // digits of up to 9 bits with the 8-bit setting (as the depth sort), 4
+    // with the 4-bit one
+    const int max_bits = c->radix_bits == 8 ? 9 : c->radix_bits;
+    const int passes = (bits_needed + max_bits - 1) / max_bits;
+    // the pass count of that width, with the digits balanced over the passes
+    // (13 tile bits: two 7-bit passes, not 9 + 4; 18 bits of 64 views: two
+    // 9-bit passes): fewer digits per pass mean fewer ballots and longer runs
+    // of equal digits to write
     int tile_bits = (bits_needed + passes - 1) / passes;
-    tile_bits = tile_bits <= 4 ? 4 : (tile_bits <= 6 ? 6 : (tile_bits <= 7 ? 7 : 8));
-    if (tile_bits > c->radix_bits) tile_bits = c->radix_bits;
+    tile_bits = tile_bits <= 4 ? 4 : (tile_bits <= 6 ? 6 : (tile_bits <= 7 ? 7 : (tile_bits <= 8 ? 8 : 9)));
+    if (tile_bits > max_bits) tile_bits = max_bits;
 
     BinLaunch& b = p->bin;
     b = BinLaunch{};

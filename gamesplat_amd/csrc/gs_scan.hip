@@ -5,8 +5,9 @@
 // The kernels that produce counts never wait on each other and never take a
 // global ticket: on MI355X device-scope atomics on one address serialize
 // across the XCDs, which made a per-block ticket or total counter cost
-// ~25 ns per block.  One 1024-thread block scans the counts instead.  The
-// total goes to *total.
+// ~25 ns per block.  One 1024-thread block scans up to 32768 counts staged
+// in LDS; more counts take a reduce-then-scan over blocks of 8192 (below).
+// The total goes to *total.
 #include <mutex>
 
 #include "gs_internal.h"
@@ -240,6 +241,98 @@ __global__ void __launch_bounds__(kScanThreads) reduce_counts_kernel(const uint3
     if (block_range) finish_range(lo, hi, range_out, s_lo, s_hi);
 }
 
+// Counts past the LDS staging limit, without diagnostics or key ranges, are
+// scanned by many blocks: each block of kChunkThreads threads owns kChunk
+// consecutive counts (kChunkItems per thread, two 16-byte loads).
+//   1. chunk_sum_kernel    each block's sum into offsets[block * kChunk]
+//                          (the first slot of its own range);
+//   2. chunk_scan_kernel   one block scans those sums in place (exclusive)
+//                          and writes the total;
+//   3. chunk_apply_kernel  each block reads its base from that slot, then
+//                          writes the exclusive offsets of its counts.
+// A slot is written in step 1 and read and overwritten in step 3 only by its
+// own block, so no block waits on another.
+constexpr int kChunkThreads = 1024;
+constexpr int kChunkItems = 8;
+constexpr uint32_t kChunk = kChunkThreads * kChunkItems;
+
+// This thread's kChunkItems counts of block b's chunk (zero past nb).
+__device__ __forceinline__ void load_chunk(const uint32_t* __restrict__ counts, uint32_t nb, uint32_t i0,
+                                           uint32_t (&v)[kChunkItems]) {
+    if (i0 + kChunkItems <= nb) {
+        const uint4 a = *reinterpret_cast<const uint4*>(counts + i0);
+        const uint4 b = *reinterpret_cast<const uint4*>(counts + i0 + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kChunkItems; ++k) v[k] = i0 + k < nb ? counts[i0 + k] : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kChunkThreads) chunk_sum_kernel(const uint32_t* __restrict__ counts, uint32_t nb,
+                                                                  uint32_t* offsets) {
+    __shared__ uint32_t s_scan[kChunkThreads / 64];
+    const uint32_t i0 = blockIdx.x * kChunk + threadIdx.x * kChunkItems;
+    uint32_t v[kChunkItems];
+    load_chunk(counts, nb, i0, v);
+    uint32_t sum = 0u;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) sum += v[k];
+    uint32_t all;
+    (void)block_exclusive_sum<kChunkThreads / 64>(sum, s_scan, &all);
+    if (threadIdx.x == 0) offsets[blockIdx.x * kChunk] = all;
+}
+
+__global__ void __launch_bounds__(kScanThreads) chunk_scan_kernel(uint32_t* offsets, uint32_t chunks, uint32_t* total,
+                                                                  uint32_t clamp_cap, uint32_t* clamped) {
+    __shared__ uint32_t s_scan[kScanThreads / 64];
+    uint32_t carry = 0u;
+    for (uint32_t first = 0; first < chunks; first += kScanThreads) {
+        const uint32_t c = first + threadIdx.x;
+        const uint32_t v = c < chunks ? offsets[(size_t)c * kChunk] : 0u;
+        uint32_t all;
+        const uint32_t before = block_exclusive_sum<kScanThreads / 64>(v, s_scan, &all);
+        if (c < chunks) offsets[(size_t)c * kChunk] = carry + before;
+        carry += all;
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (clamped) *clamped = min(carry, clamp_cap);
+    }
+}
+
+__global__ void __launch_bounds__(kChunkThreads) chunk_apply_kernel(const uint32_t* __restrict__ counts, uint32_t nb,
+                                                                    uint32_t* offsets) {
+    __shared__ uint32_t s_scan[kChunkThreads / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t i0 = blockIdx.x * kChunk + threadIdx.x * kChunkItems;
+    if (threadIdx.x == 0) s_base = offsets[blockIdx.x * kChunk];
+    uint32_t v[kChunkItems];
+    load_chunk(counts, nb, i0, v);
+    uint32_t sum = 0u;
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) sum += v[k];
+    uint32_t all;
+    // the barriers inside the block scan order the base's read before any
+    // write of the block's range
+    uint32_t run = block_exclusive_sum<kChunkThreads / 64>(sum, s_scan, &all) + s_base;
+    uint32_t o[kChunkItems];
+#pragma unroll
+    for (int k = 0; k < kChunkItems; ++k) {
+        o[k] = run;
+        run += v[k];
+    }
+    if (i0 + kChunkItems <= nb) {
+        *reinterpret_cast<uint4*>(offsets + i0) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(offsets + i0 + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kChunkItems; ++k)
+            if (i0 + k < nb) offsets[i0 + k] = o[k];
+    }
+}
+
 }  // namespace
 
 hipError_t launch_reduce_counts(const uint32_t* counts, uint32_t nb, uint32_t* total, const uint32_t* block_diag,
@@ -270,6 +363,13 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t nb, uint32_t* off
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(scan_counts_lds_kernel, dim3(1), dim3(kScanThreads), (size_t)(nb ? nb : 1) * 4, s, counts,
                            nb, offsets, total, block_diag, diag_out, clamp_cap, clamped, block_range, range_out);
+    } else if (!block_diag && !block_range && reinterpret_cast<uintptr_t>(counts) % 16 == 0 &&
+               reinterpret_cast<uintptr_t>(offsets) % 16 == 0) {
+        const uint32_t chunks = (uint32_t)(((uint64_t)nb + kChunk - 1) / kChunk);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3(chunks), dim3(kChunkThreads), 0, s, counts, nb, offsets);
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, offsets, chunks, total, clamp_cap,
+                           clamped);
+        hipLaunchKernelGGL(chunk_apply_kernel, dim3(chunks), dim3(kChunkThreads), 0, s, counts, nb, offsets);
     } else {
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanThreads), 0, s, counts, nb, offsets, total,
                            block_diag, diag_out, clamp_cap, clamped, block_range, range_out);
