@@ -29,6 +29,7 @@ namespace {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4v_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -232,7 +233,10 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
         if constexpr (HALF) return *reinterpret_cast<const uint2*>(s_tab + ((c * 4 + kg) * 4 + xb * 2));
         else return s_tab[c * 4 + kg];
     };
+    // a chunk's RG pixel fragments are all read before its first MFMA, so
+    // one LDS latency (not RG) precedes the chunk's MFMAs
     auto chunk = [&](const uint4 (&w)[NT], TabEntry off) {
+        uint4 pv[RG];
 #pragma unroll
         for (int r = 0; r < RG; ++r) {
             int adr;
@@ -242,12 +246,22 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
             } else {
                 adr = (int)off + pix[r];
             }
-            const uint4 pv = *reinterpret_cast<const uint4*>(s_in + adr);
+            pv[r] = *reinterpret_cast<const uint4*>(s_in + adr);
+        }
+        // (the register pressure would otherwise lead the scheduler to read
+        // each fragment into one register tuple just before its MFMAs)
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+            u32x4v_t t = __builtin_bit_cast(u32x4v_t, pv[r]);
+            asm volatile("" : "+v"(t));
+            pv[r] = __builtin_bit_cast(uint4, t);
+        }
+#pragma unroll
+        for (int r = 0; r < RG; ++r)
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(w[t]), as_bf16x8(pv), acc[r][t], 0,
+                acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(w[t]), as_bf16x8(pv[r]), acc[r][t], 0,
                                                                      0, 0);
-        }
     };
     // the bias is loaded before the main loop, which hides its latency
     float bias[NT][4];
@@ -257,7 +271,7 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
         for (int i = 0; i < 4; ++i) bias[t][i] = a.bias ? a.bias[t * 16 + 4 * kg + i] : 0.0f;
     // weights PD chunks ahead: a chunk's MFMAs take ~16 NT RG cycles, too few
     // to cover an L2 load with the two-chunk lead of a ping-pong pair
-    constexpr int PD = NT <= 2 ? 4 : (NT <= 4 ? 3 : 2);
+    constexpr int PD = NT <= 2 ? 3 : (NT <= 4 ? 3 : 2);
     uint4 w[PD][NT];
     TabEntry off[PD];
 #pragma unroll
