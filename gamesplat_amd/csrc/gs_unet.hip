@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
         for (int i = 0; i < 4; ++i) bias[t][i] = a.bias ? a.bias[t * 16 + 4 * kg + i] : 0.0f;
     // weights PD chunks ahead: a chunk's MFMAs take ~16 NT RG cycles, too few
     // to cover an L2 load with the two-chunk lead of a ping-pong pair
-    constexpr int PD = NT <= 2 ? 3 : (NT <= 4 ? 3 : 2);
+    constexpr int PD = NT <= 4 ? 3 : 2;
     uint4 w[PD][NT];
     TabEntry off[PD];
 #pragma unroll
