@@ -1060,6 +1060,9 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     s.passes = (depth_key_bits(c) + s.bits - 1) / s.bits;
     if (s.passes < 1) s.passes = 1;
     s.wide_index = c->wide_index;
+    // nothing on the device reads the sorted keys: the readback gathers them
+    // from the cull's dense keys by the sorted ids
+    s.skip_last_keys = 1;
     // without a visible list the first pass reads the cull's dense keys and
     // drops the culled slots (the list it keeps is the one the compaction
     // would build, in the same order)
@@ -1981,10 +1984,22 @@ int gs_read_buffer(gs_ctx* c, int which, void* host, size_t bytes) {
         case GS_BUFFER_SORTED_IDS: {
             if (!c->have_sort) return fail(GS_ERR_STATE, "no sort yet");
             if (bytes != n * 4) return fail(GS_ERR_INVALID, "sorted buffer needs %zu bytes", n * 4);
-            DevBuf& src = which == GS_BUFFER_SORTED_KEYS ? c->keys[c->sort_slot] : c->ids[c->sort_slot];
-            GS_HIP(hipMemcpyAsync(host, src.p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
-            GS_HIP(hipStreamSynchronize(c->stream));
             uint32_t* h = static_cast<uint32_t*>(host);
+            GS_HIP(hipMemcpyAsync(host, c->ids[c->sort_slot].p, (size_t)V * 4, hipMemcpyDeviceToHost, c->stream));
+            if (which == GS_BUFFER_SORTED_KEYS) {
+                // the depth sort writes no keys: each sorted id (a dense
+                // [view][n] slot) indexes the cull's dense keys
+                std::vector<uint32_t> dense((size_t)c->views * c->n);
+                GS_HIP(hipMemcpyAsync(dense.data(), c->dense_keys.p, dense.size() * 4, hipMemcpyDeviceToHost,
+                                      c->stream));
+                GS_HIP(hipStreamSynchronize(c->stream));
+                for (size_t i = 0; i < V; ++i) {
+                    if (h[i] >= dense.size()) return fail(GS_ERR_INVALID, "sorted id %u out of range", h[i]);
+                    h[i] = dense[h[i]];
+                }
+            } else {
+                GS_HIP(hipStreamSynchronize(c->stream));
+            }
             for (size_t i = V; i < n; ++i) h[i] = 0xFFFFFFFFu;
             return ok();
         }

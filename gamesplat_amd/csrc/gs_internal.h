@@ -240,6 +240,7 @@ struct SortLaunch {
     int bits;                // digit width (9, 8, 7, 6 or 4)
     int wide_index;          // 1: 64-bit element indexing even where 32-bit offsets suffice
     int key16;               // 1: keys are 16-bit (below 2^16; no dense pass, no key range)
+    int skip_last_keys;      // 1 (with key_range): the last pass writes values only, not keys
     // optional dense first pass (digit widths 7 and 4): it reads dense_count
     // slots of dense_keys (0xFFFFFFFF: culled) instead of keys[0] and count,
     // keeps the others in slot order with the slot as value, and with vals2

@@ -57,6 +57,7 @@ struct PassArgs {
     uint32_t tiles;             // table stride (grid size)
     uint32_t* col_total;        // [RADIX]
     int adaptive;
+    int skip_last_keys;         // the last pass the range needs writes no keys
     int pass, passes;           // this pass, passes launched
     int shift;
     // dense first pass: keys_in holds dense_count slots (0xFFFFFFFF: culled);
@@ -333,6 +334,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
     // a dense tile's kept splats, ranked first: they fill local positions
     // [0, total)
     const uint32_t out_count = DENSE ? total : valid;
+    const bool write_keys = !(plan.last && a.skip_last_keys);
     uint32_t all;
     uint32_t dbase = block_exclusive_sum<kWaves>(csum, s_scan, &all);
     if (has_digits) {
@@ -380,7 +382,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(PassArgs a) {
             uint32_t k = s_buf[p];
             const uint32_t d = (k >> shift) & MASK;
             digits[j / kPerReg] |= d << ((j % kPerReg) * kDigitBits);
-            store_key<O32, K16>(keys_out, s_dest[d] + p, k + plan.add);
+            if (write_keys) store_key<O32, K16>(keys_out, s_dest[d] + p, k + plan.add);
         }
     }
     __syncthreads();
@@ -465,6 +467,7 @@ hipError_t run_passes(const SortLaunch& a, int* result_slot, hipStream_t s) {
     // a readable placeholder when the passes are fixed (not used for them)
     pa.key_range = a.key_range ? a.key_range : a.scratch;
     pa.adaptive = a.key_range ? 1 : 0;
+    pa.skip_last_keys = a.key_range && a.skip_last_keys ? 1 : 0;
     pa.keys_last = a.keys[3];
     pa.vals_last = a.vals[3];
     pa.vals2_last = a.vals2[3];
