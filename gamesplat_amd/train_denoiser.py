@@ -84,6 +84,8 @@ def main(argv=None):
     ap.add_argument("--steps-per-scene", type=int, default=60)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--lr-final", type=float, default=None,
+                    help="learning rate of the last scene (linear from --lr); default: --lr throughout")
     ap.add_argument("--seed", type=int, default=0x5EED7000)
     ap.add_argument("--init", type=str, default="", help="start from these parameters")
     ap.add_argument("--out", type=str, default=str(DEFAULT_OUT))
@@ -113,9 +115,12 @@ def main(argv=None):
                 ctx.set_gps_budget(DROP_BUDGET if rng.uniform() < args.drop_fraction else FULL_BUDGET)
                 render_pair(ctx, random_camera(rng, size), spp, (s * 64 + v) * 16)
                 den.capture(args.crops, args.seed * 31 + s * 64 + v)
-            losses = den.train(args.steps_per_scene, args.batch, args.lr, args.seed + 7 * s)
+            lr = args.lr
+            if args.lr_final is not None and args.scenes > 1:
+                lr = args.lr + (args.lr_final - args.lr) * s / (args.scenes - 1)
+            losses = den.train(args.steps_per_scene, args.batch, lr, args.seed + 7 * s)
             steps += args.steps_per_scene
-            rec = {"scene": s, "gaussians": count, "clusters": clusters, "opacity_mean": opacity_mean,
+            rec = {"scene": s, "lr": lr, "gaussians": count, "clusters": clusters, "opacity_mean": opacity_mean,
                    "steps": steps, "loss": float(losses.mean()), "pool": den.pool_size(),
                    "elapsed_s": round(time.time() - t0, 1)}
             log.append(rec)
@@ -137,6 +142,7 @@ def main(argv=None):
     out.parent.mkdir(parents=True, exist_ok=True)
     params.astype(np.float32).tofile(out)
     summary = {"param_count": int(params.size), "steps": steps, "batch": args.batch, "lr": args.lr,
+               "lr_final": args.lr_final,
                "gaussians": [COUNTS[0], COUNTS[-1]], "drop_fraction": args.drop_fraction,
                "scenes": len(log), "final_loss": log[-1]["loss"] if log else None, "held_out": evals,
                "seconds": round(time.time() - t0, 1)}
