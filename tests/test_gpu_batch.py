@@ -178,3 +178,24 @@ def test_batch_errors(gs, scene):
             c.gps_sample(1)  # GPS needs a single-view cull
         with pytest.raises(gs.GsError):
             c.read_buffer(abi.GS_BUFFER_SORTED_IDS)
+
+
+def test_many_count_blocks(gs):
+    """A batch whose bin has more than 32768 count blocks (3 views of a
+    3,000,001-Gaussian scene: 35,157 blocks, not a multiple of the 8 counts a
+    thread scans) takes the scan over blocks of 8192 counts, tail included;
+    each view equals its single-view frame."""
+    print("case: many count blocks (3 views x 3000001 Gaussians)", flush=True)
+    g = gs.generate_scene(0x5EED0B02, 3_000_001, 64, 0.0)
+    cams = _cams(11)[:3]
+    with gs.Context(0) as c:
+        c.upload(g)
+        ref = []
+        for cam in cams:
+            c.render_alpha(cam, tile=16)
+            ref.append(c.read_image(abi.GS_IMAGE_ALPHA_RGBA16F).view(np.uint16).copy())
+        c.render_views(cams, tile=16)
+        for k in range(3):
+            np.testing.assert_array_equal(c.read_view_image(k).view(np.uint16), ref[k])
+        assert any(r.any() for r in ref)
+
