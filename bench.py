@@ -616,8 +616,9 @@ def main():
             "dropped_work_items": gst["gps_dropped_work_items"],
             "budget": gst["gps_max_work_items"],
             "stage_ms": {k: round(v[0] / v[1], 4) for k, v in sstages.items() if v[1]},
-            # one 64-bit atomicMin per work item kept (ss = 1)
-            "gps_splat_atomics_per_s": round(min(gst["gps_requested_work_items"], gst["gps_max_work_items"]) /
+            # work items kept (ss = 1: one point each, one cell load, an
+            # atomicMin only where the point beats the cell's nearest so far)
+            "gps_splat_items_per_s": round(min(gst["gps_requested_work_items"], gst["gps_max_work_items"]) /
                                              (sstages["gps_splat"][0] / sstages["gps_splat"][1] * 1e-3), 1)
             if sstages["gps_splat"][1] else None,
             "denoised": None if args.no_denoiser else den,

@@ -143,9 +143,18 @@ __global__ void __launch_bounds__(256) gps_splat_kernel(GpsLaunch a) {
             float ptx = (mx + l00 * lx) * ssm;
             float pty = (my + (l10 * lx + l11 * ly)) * ssm;
             int px = (int)__builtin_floorf(ptx + 0.5f), py = (int)__builtin_floorf(pty + 0.5f);
-            if (px >= 0 && py >= 0 && px < a.gps_width && py < a.gps_height)
-                atomicMin(reinterpret_cast<unsigned long long*>(a.cells + (size_t)py * a.gps_width + px),
-                          (unsigned long long)value);
+            if (px >= 0 && py >= 0 && px < a.gps_width && py < a.gps_height) {
+                // A cell only decreases during this kernel: every value in its
+                // modification order is at most the one before, and the later
+                // atomicMin of this thread reads no earlier value than this
+                // load.  A cell already at or below `value` makes the atomic a
+                // no-op, so it is skipped (most are: a pixel receives several
+                // points, and each new one beats the nearest so far ever less
+                // often).
+                auto* cp = reinterpret_cast<unsigned long long*>(a.cells + (size_t)py * a.gps_width + px);
+                if (__hip_atomic_load(cp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > value)
+                    atomicMin(cp, (unsigned long long)value);
+            }
         }
     }
 }
