@@ -152,6 +152,7 @@ struct gs_ctx {
 
     uint32_t gps_budget = 16u * 1024u * 1024u;  // renderer_init default (src/renderer.cpp:595)
     DevBuf work, prepared, item_count, item_base, cells, sample, depth, accum;
+    DevBuf gps_hint;  // per GPS count block: prepare in the count pass (see gps_count_kernel)
     int gps_w = 0, gps_h = 0, gps_ss = 1, res_w = 0, res_h = 0, acc_w = 0, acc_h = 0;
     // StochasticSplats + TAA: frame average and the double-buffered history
     DevBuf frame_avg, taa_color[2], taa_xyz[2];
@@ -481,7 +482,7 @@ void gs_destroy(gs_ctx* c) {
                       &c->prepared, &c->item_count, &c->item_base, &c->cells, &c->sample, &c->depth, &c->accum,
                       &c->frame_avg, &c->taa_color[0], &c->taa_color[1], &c->taa_xyz[0], &c->taa_xyz[1],
                       &c->anim_data, &c->lut, &c->display, &c->vrects[0], &c->vrects[1], &c->vrects[2],
-                      &c->packed_rects};
+                      &c->packed_rects, &c->gps_hint};
     for (DevBuf* b : bufs) release(c, *b);
     release(c, c->frame_dev);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -1657,6 +1658,11 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     GS_CHECK(ensure(c, c->prepared, (size_t)c->n * 48, "GPS prepared splats"));
     GS_CHECK(ensure(c, c->item_count, (size_t)c->n * 4, "GPS item counts"));
     GS_CHECK(ensure(c, c->item_base, (size_t)c->n * 4, "GPS item offsets"));
+    const uint32_t count_blocks = (c->n + 255u) / 256u;
+    const void* old_hint = c->gps_hint.p;
+    GS_CHECK(ensure(c, c->gps_hint, (size_t)count_blocks * 4, "GPS prepare hints"));
+    if (c->gps_hint.p != old_hint)  // new blocks start by preparing in the count pass
+        GS_HIP(launch_fill_u32(ptr<uint32_t>(c->gps_hint), 1u, c->gps_hint.bytes / 4, c->stream));
     GS_CHECK(ensure(c, c->sample, (size_t)W * H * 16, "GPS sample"));
     GS_CHECK(ensure(c, c->depth, (size_t)W * H * 4, "GPS depth"));
     if (c->cells_dirty) {
@@ -1685,6 +1691,7 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     g.item_base = ptr<uint32_t>(c->item_base);
     g.block_counts = ptr<uint32_t>(c->block_counts);
     g.block_offsets = ptr<uint32_t>(c->block_offsets);
+    g.prepare_in_count = ptr<uint32_t>(c->gps_hint);
     g.counters = c->dc();
     g.cells = ptr<uint64_t>(c->cells);
     g.gps_width = gw;

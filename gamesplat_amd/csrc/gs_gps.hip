@@ -30,16 +30,48 @@ namespace {
 
 constexpr uint64_t kEmptyCell = 0xFFFFFFFF00000000ull;
 
-// Pass 1: work-item count (gps.glsl:127-142) and prepared record
-// (gps.glsl:152-169) per visible splat, item count per block.  The prepared
-// record is written for every splat with items; the reference writes it only
-// for reservations that start inside the budget, and no other record is read.
+// Prepared record of visible position v (gps.glsl:152-169): mean, Cholesky
+// factor of the twice-dilated covariance, opacity and its Li2, depth key,
+// colour (or the splat id for linear colour) and the splat id.
+__device__ void write_prepared(const GpsLaunch& a, uint32_t v, uint32_t splat_id, float4 r0, float4 r1) {
+    const float4 r2 = a.records[(size_t)splat_id * kRecordVec4 + 2];
+    const float cov_a = r1.x + 0.3f;
+    const float cov_b = r1.y;
+    const float cov_c = r1.z + 0.3f;
+    float l00 = __builtin_sqrtf(maxf(cov_a, 1.0e-5f));
+    float l10 = cov_b / l00;
+    float l11 = __builtin_sqrtf(maxf(cov_c - l10 * l10, 1.0e-5f));
+    float alpha = clampf(r0.z, 1.0e-6f, 1.0f);
+    float dilog_alpha = dilog(alpha);
+    float ndc_depth = r0.w;
+    float z = mixf(ndc_depth * 0.5f + 0.5f, ndc_depth, a.clip_z_01);
+    uint32_t depth_key = f2u_sat(clampf(z, 0.0f, 1.0f) * 4294967294.0f);
+    uint32_t rr = (uint32_t)__builtin_rintf(clampf(r2.x, 0.0f, 1.0f) * 255.0f);
+    uint32_t gg = (uint32_t)__builtin_rintf(clampf(r2.y, 0.0f, 1.0f) * 255.0f);
+    uint32_t bb = (uint32_t)__builtin_rintf(clampf(r2.z, 0.0f, 1.0f) * 255.0f);
+    uint32_t packed = rr | (gg << 8) | (bb << 16) | 0xFF000000u;
+    if (a.linear_color) packed = splat_id;
+    uint4* pr = a.prepared + (size_t)v * 3;
+    pr[0] = make_uint4(fbits(r0.x), fbits(r0.y), fbits(l00), fbits(l10));
+    pr[1] = make_uint4(fbits(l11), fbits(alpha), fbits(dilog_alpha), 0u);
+    pr[2] = make_uint4(depth_key, packed, splat_id, 0u);
+}
+
+// Pass 1: work-item count (gps.glsl:127-142) per visible splat, item count
+// per block.  The prepared record is needed only by splats whose
+// reservation starts inside the budget (the reference writes it for those
+// alone, gps.glsl:145-170), which only the scan decides: a block writes it
+// here, for every splat with items, when its hint says that the block kept
+// work in the frame before, and leaves it to pass 2 (kept splats only)
+// otherwise.  Either way the same records reach the splat pass; the hint
+// only saves writing records of blocks that lie past the budget.
 __global__ void __launch_bounds__(256) gps_count_kernel(GpsLaunch a) {
     __shared__ uint32_t s_scan[4];
     const uint32_t tid = threadIdx.x;
     const uint32_t V = *a.vis_count;
     const uint32_t v = blockIdx.x * 256u + tid;
     const float ss = maxf(a.ss, 1.0f);
+    const bool prepare_here = a.prepare_in_count[blockIdx.x] != 0u;
     uint32_t count = 0u;
     if (v < V) {
         const uint32_t splat_id = a.vis_ids[v];
@@ -58,26 +90,7 @@ __global__ void __launch_bounds__(256) gps_count_kernel(GpsLaunch a) {
         count = f2u_sat(fl);
         if (hash01(splat_id + 0x9e3779b9u) < fractf(scaled) && count < 0xFFFFFFFFu) count += 1u;
         a.item_count[v] = count;
-        if (count) {
-            const float4 r2 = a.records[(size_t)splat_id * kRecordVec4 + 2];
-            float l00 = __builtin_sqrtf(maxf(cov_a, 1.0e-5f));
-            float l10 = cov_b / l00;
-            float l11 = __builtin_sqrtf(maxf(cov_c - l10 * l10, 1.0e-5f));
-            float alpha = clampf(r0.z, 1.0e-6f, 1.0f);
-            float dilog_alpha = dilog(alpha);
-            float ndc_depth = r0.w;
-            float z = mixf(ndc_depth * 0.5f + 0.5f, ndc_depth, a.clip_z_01);
-            uint32_t depth_key = f2u_sat(clampf(z, 0.0f, 1.0f) * 4294967294.0f);
-            uint32_t rr = (uint32_t)__builtin_rintf(clampf(r2.x, 0.0f, 1.0f) * 255.0f);
-            uint32_t gg = (uint32_t)__builtin_rintf(clampf(r2.y, 0.0f, 1.0f) * 255.0f);
-            uint32_t bb = (uint32_t)__builtin_rintf(clampf(r2.z, 0.0f, 1.0f) * 255.0f);
-            uint32_t packed = rr | (gg << 8) | (bb << 16) | 0xFF000000u;
-            if (a.linear_color) packed = splat_id;
-            uint4* pr = a.prepared + (size_t)v * 3;
-            pr[0] = make_uint4(fbits(r0.x), fbits(r0.y), fbits(l00), fbits(l10));
-            pr[1] = make_uint4(fbits(l11), fbits(alpha), fbits(dilog_alpha), 0u);
-            pr[2] = make_uint4(depth_key, packed, splat_id, 0u);
-        }
+        if (count && prepare_here) write_prepared(a, v, splat_id, r0, r1);
     }
     uint32_t total;
     (void)block_exclusive_sum<4>(count, s_scan, &total);
@@ -104,7 +117,18 @@ __global__ void __launch_bounds__(256) gps_reserve_kernel(GpsLaunch a) {
     const uint32_t room = block_base < capacity ? capacity - block_base : 0u;
     const uint32_t emit = min(total, room);
     for (uint32_t i = tid; i < emit; i += 256u) a.work[block_base + i] = blockIdx.x * 256u + owner_of(s_start, 256u, i);
-    if (count && block_base + local < capacity) a.item_base[v] = block_base + local;
+    const bool prepared = a.prepare_in_count[blockIdx.x] != 0u;
+    if (count && block_base + local < capacity) {
+        a.item_base[v] = block_base + local;
+        if (!prepared) {
+            const uint32_t splat_id = a.vis_ids[v];
+            write_prepared(a, v, splat_id, a.records[(size_t)splat_id * kRecordVec4 + 0],
+                           a.records[(size_t)splat_id * kRecordVec4 + 1]);
+        }
+    }
+    // hint for the next frame's pass 1: this block kept work
+    __syncthreads();
+    if (tid == 0) a.prepare_in_count[blockIdx.x] = block_base < capacity ? 1u : 0u;
 }
 
 __global__ void __launch_bounds__(256) gps_splat_kernel(GpsLaunch a) {

@@ -346,6 +346,9 @@ struct GpsLaunch {
     uint32_t* item_base;         // per visible position
     uint32_t* block_counts;
     uint32_t* block_offsets;
+    uint32_t* prepare_in_count;  // per count block: 1 = pass 1 writes the prepared records (the
+                                 // block kept work last frame), 0 = pass 2 writes them for kept
+                                 // splats; pass 2 updates it
     DevCounters* counters;
     // splat
     uint64_t* cells;

@@ -310,6 +310,32 @@ def test_gps_budget_overflow(ctx, orc, c1_scene):
         ctx.set_gps_budget(16 * 1024 * 1024)
 
 
+def test_gps_budget_sequence(ctx, orc, c1_scene):
+    """Budgets that rise and fall between frames: each count block writes its
+    prepared records in the count pass or, for kept splats only, in the
+    reserve pass, as the frame before suggests (wrongly after each change of
+    budget); the cells equal the oracle's every time."""
+    g, cfg, cam = c1_scene
+    W, H = cfg["width"], cfg["height"]
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    _, _, requested, _ = orc.gps_expand(op, oi, 1 << 30, 1.0, 0.0)
+    budgets = [requested // 4, requested // 4, requested // 2, requested + 5, requested // 3, 1 << 22, requested // 8]
+    ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 1)
+    try:
+        for k, budget in enumerate(budgets):
+            print(f"case: gps budget sequence step {k} budget {budget} of {requested}", flush=True)
+            ctx.set_gps_budget(budget)
+            ctx.gps_sample(frame_seed=k + 1, supersample=1)
+            prepared, work, req, overflow = orc.gps_expand(op, oi, budget, 1.0, 0.0)
+            cells = ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * W,))
+            ref = orc.gps_splat(prepared, work, min(req, budget), W, H, k + 1, 1.0)
+            assert ctx.stats()["gps_work_budget_exhausted"] == overflow
+            np.testing.assert_array_equal(cells, ref, err_msg=f"step {k}")
+    finally:
+        ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 0)
+        ctx.set_gps_budget(16 * 1024 * 1024)
+
+
 def test_gps_accumulate(ctx, orc, c1_scene):
     g, cfg, cam = c1_scene
     W, H = cfg["width"], cfg["height"]
