@@ -159,10 +159,10 @@ __global__ void __launch_bounds__(256) gps_splat_kernel(GpsLaunch a) {
             float r1 = (float)y * 2.32830643654e-10f;
             float arg = inv_dilog((1.0f - r0) * dla) / alpha;
             arg = clampf(arg, 1.0e-7f, 1.0f);
-            float radius = __builtin_sqrtf(-2.0f * logf_(arg));
+            float radius = __builtin_sqrtf(-2.0f * gps_logf(arg));
             float az = 6.28318530718f * r1;
             float sn, cs;
-            sincos_f(az, &sn, &cs);
+            gps_sincosf(az, &sn, &cs);
             float lx = cs * radius, ly = sn * radius;
             float ptx = (mx + l00 * lx) * ssm;
             float pty = (my + (l10 * lx + l11 * ly)) * ssm;

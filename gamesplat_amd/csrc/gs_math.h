@@ -4,9 +4,11 @@
 // evaluates them with one fixed double-precision algorithm built only from
 // IEEE double operations (no FMA: the library is compiled with
 // -ffp-contract=off), then rounds to float.  The same algorithm gives the
-// same bits on the host and on gfx950, so the projected splats, the
-// compositor's discard decisions and every GPS point position are
-// reproducible bit for bit on the CPU.
+// same bits on the host and on gfx950, so the projected splats and the
+// compositor's discard decisions are reproducible bit for bit on the CPU.
+// The GPS point sampler, which evaluates a log and a sin/cos per point, uses
+// the binary32 fused-multiply-add forms further down instead (equally
+// reproducible).
 //
 //   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
 //        s = (m-1)/(m+1), series to s^21.
@@ -161,6 +163,52 @@ GS_HD float logf_(float x) { return (float)log_d((double)x); }
 GS_HD float expf_(float x) { return (float)exp_d((double)x); }
 GS_HD float sinf_(float x) { return (float)sin_d((double)x); }
 GS_HD float cosf_(float x) { return (float)cos_d((double)x); }
+
+// GPS point radius and angle: log, sin and cos in binary32 arithmetic with
+// explicit fused multiply-adds (one rounding each, the same on the host and
+// on gfx950), a third of the instructions of the double-precision forms.
+// Coefficients: minimax fits by Lawson's iteratively reweighted least
+// squares on Chebyshev nodes, each coefficient rounded to float in turn and
+// the rest refitted (oracle/fit_gps_math.py).
+//   gps_logf: positive normal x; x = 2^k m, m in [sqrt(1/2), sqrt(2)),
+//     log m = f + f^2 Q(f), f = m - 1, Q of degree 8 with Q(0) = -1/2.
+//     Within 1 ulp of the correctly rounded log for every normal float up
+//     to 1e30 (checked exhaustively).
+//   gps_sincosf: |x| < 1e4; reduction by pi/2 with a three-part constant,
+//     sin r = r + r^3 S(r^2), cos r = 1 - r^2/2 + r^4 C(r^2), S and C of
+//     degree 2 on |r| <= 0.801.  For every float in [0, 2 pi): within 1 ulp
+//     (sin), 2 ulp (cos) and 9.3e-8 absolute.
+constexpr float kGpsLogQ[9] = {-0.5f, 0.3333333134651184f, -0.2500081956386566f, 0.2000124156475067f, -0.166234090924263f, 0.14201579988002777f, -0.13159610331058502f, 0.12762217223644257f, -0.07636439055204391f};
+constexpr float kGpsSinS[3] = {-0.16666653752326965f, 0.008332093246281147f, -0.00019505145610310137f};
+constexpr float kGpsCosC[3] = {0.04166664183139801f, -0.0013887130189687014f, 2.4412080165348016e-05f};
+
+GS_HD float gps_logf(float x) {
+    uint32_t ix = fbits(x) + (0x3f800000u - 0x3f3504f3u);
+    const int k = (int)(ix >> 23) - 127;
+    ix = (ix & 0x007fffffu) + 0x3f3504f3u;
+    const float f = bitsf(ix) - 1.0f;
+    float q = kGpsLogQ[8];
+    for (int i = 7; i >= 0; --i) q = __builtin_fmaf(q, f, kGpsLogQ[i]);
+    const float r = __builtin_fmaf(f * f, q, f);
+    const float fk = (float)k;
+    return __builtin_fmaf(fk, 0.693147182f, __builtin_fmaf(fk, -1.90465421e-09f, r));
+}
+
+GS_HD void gps_sincosf(float x, float* sn, float* cs) {
+    const float q = __builtin_rintf(x * 0.636619747f);
+    const int iq = (int)q;
+    float r = __builtin_fmaf(-q, 1.57079637f, x);
+    r = __builtin_fmaf(-q, -4.37113883e-08f, r);
+    r = __builtin_fmaf(-q, -1.71512451e-15f, r);
+    const float z = r * r;
+    const float ps = __builtin_fmaf(__builtin_fmaf(kGpsSinS[2], z, kGpsSinS[1]), z, kGpsSinS[0]);
+    const float pc = __builtin_fmaf(__builtin_fmaf(kGpsCosC[2], z, kGpsCosC[1]), z, kGpsCosC[0]);
+    const float s = __builtin_fmaf(r * z, ps, r);
+    const float c = __builtin_fmaf(z * z, pc, __builtin_fmaf(-0.5f, z, 1.0f));
+    const int k = iq & 3;
+    *sn = k == 0 ? s : k == 1 ? c : k == 2 ? -s : -c;
+    *cs = k == 0 ? c : k == 1 ? -s : k == 2 ? -c : s;
+}
 
 GS_HD float minf(float a, float b) { return a < b ? a : b; }
 GS_HD float maxf(float a, float b) { return a > b ? a : b; }

@@ -28,6 +28,9 @@ _SIGS = {
     "or_cosf": (C.c_float, [C.c_float]),
     "or_dilog": (C.c_float, [C.c_float]),
     "or_inv_dilog": (C.c_float, [C.c_float]),
+    "or_gps_logf": (C.c_float, [C.c_float]),
+    "or_gps_logf_n": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "or_gps_sincosf_n": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "or_hash01_bits": (C.c_uint32, [C.c_uint32]),
     "or_hash01": (C.c_float, [C.c_uint32]),
     "or_pcg2d": (None, [C.POINTER(C.c_uint32)]),
@@ -250,6 +253,22 @@ def accumulate(history, sample, n):
     out = np.empty_like(h)
     lib.or_accumulate(_p(h), _p(s), float(n), _p(out), h.size)
     return out
+
+
+def gps_logf(x: np.ndarray) -> np.ndarray:
+    """The GPS sampler's binary32 log of each element (positive normal floats)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    load().or_gps_logf_n(_p(x), _p(y), x.size)
+    return y
+
+
+def gps_sincosf(x: np.ndarray):
+    """The GPS sampler's binary32 (sin, cos) of each element."""
+    x = np.ascontiguousarray(x, np.float32)
+    s, c = np.empty_like(x), np.empty_like(x)
+    load().or_gps_sincosf_n(_p(x), _p(s), _p(c), x.size)
+    return s, c
 
 
 def scalar(name, *args):

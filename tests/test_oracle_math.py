@@ -124,3 +124,49 @@ def test_f16_conversion_matches_numpy(orc):
     ref = allh.astype(np.uint16).view(np.float16).astype(np.float32)
     fin = np.isfinite(ref)
     np.testing.assert_array_equal(back[fin].view(np.uint32), ref[fin].view(np.uint32))
+
+
+def _ulps(a, b):
+    ia = a.view(np.int32).astype(np.int64)
+    ib = b.view(np.int32).astype(np.int64)
+    ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
+    ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
+    return np.abs(ia - ib)
+
+
+def test_gps_logf_against_libm(orc):
+    """The GPS sampler's binary32 log: every 61st float from the smallest
+    normal to 1e30 and every 7th of its own domain [1e-7, 1] within 1 ulp of
+    libm's log rounded to float, nearly all of them exact."""
+    lo, hi = np.float32(1.17549435e-38).view(np.uint32), np.float32(1e30).view(np.uint32)
+    bits = np.concatenate([np.arange(lo, hi, 61, dtype=np.uint32),
+                           np.arange(np.float32(1e-7).view(np.uint32), np.float32(1.0).view(np.uint32) + 1, 7,
+                                     dtype=np.uint32)])
+    x = bits.view(np.float32)
+    got = orc.gps_logf(x)
+    ref = np.log(x.astype(np.float64)).astype(np.float32)
+    u = _ulps(got, ref)
+    assert u.max() <= 1 and (u == 0).mean() > 0.98
+    k = np.arange(-126, 100)
+    np.testing.assert_array_equal(orc.gps_logf(np.ldexp(np.float32(1), k).astype(np.float32)),
+                                  (k * np.log(2.0)).astype(np.float32))
+    assert orc.scalar("or_gps_logf", 1.0) == 0.0
+
+
+def test_gps_sincosf_against_libm(orc):
+    """The GPS sampler's binary32 sin/cos over every 37th float of [0, 2 pi)
+    and the angles the sampler forms (2 pi times a 32-bit fraction): within
+    1 ulp (sin) and 2 ulp (cos) of libm rounded to float and 1e-7 absolute,
+    and the quadrant values exact."""
+    two_pi = np.float32(6.28318530718)
+    bits = np.arange(0, two_pi.view(np.uint32), 37, dtype=np.uint32)
+    rng = np.random.default_rng(5)
+    r1 = rng.integers(0, 2 ** 32, 2_000_000, dtype=np.uint64).astype(np.float32) * np.float32(2.32830643654e-10)
+    x = np.concatenate([bits.view(np.float32), (two_pi * r1).astype(np.float32)])
+    s, c = orc.gps_sincosf(x)
+    xd = x.astype(np.float64)
+    assert _ulps(s, np.sin(xd).astype(np.float32)).max() <= 1
+    assert _ulps(c, np.cos(xd).astype(np.float32)).max() <= 2
+    assert np.abs(s - np.sin(xd)).max() < 1e-7 and np.abs(c - np.cos(xd)).max() < 1e-7
+    s0, c0 = orc.gps_sincosf(np.zeros(1, np.float32))
+    assert s0[0] == 0.0 and c0[0] == 1.0
