@@ -210,6 +210,10 @@ struct ConvExtra {
     int ps = 0;
     const uint16_t* x2 = nullptr;
     int x2s = 0, split = 0;
+    const float* head = nullptr;  // fused 1x1 head (see ConvLaunch)
+    const uint16_t* head_inp = nullptr;
+    uint16_t* head_out = nullptr;
+    int head_w = 0, head_h = 0;
 };
 
 int conv(gs_denoiser* d, int li, const uint16_t* x, int xs, int xo, uint16_t* y, int ys, int yo, int n, int h,
@@ -221,6 +225,11 @@ int conv(gs_denoiser* d, int li, const uint16_t* x, int xs, int xo, uint16_t* y,
     a.x2 = ex.x2;
     a.x2s = ex.x2s;
     a.split = ex.split;
+    a.head = ex.head;
+    a.head_inp = ex.head_inp;
+    a.head_out = ex.head_out;
+    a.head_w = ex.head_w;
+    a.head_h = ex.head_h;
     a.x = x; a.xs = xs; a.xo = xo;
     a.y = y; a.ys = ys; a.yo = yo;
     a.wt = L.wf;
@@ -264,8 +273,11 @@ int conv_bwd(gs_denoiser* d, int li, const uint16_t* x, int xs, int xo, TensorVi
 TensorView tv(uint16_t* p, int cs, int co) { return TensorView{p, cs, co}; }
 
 // Forward pass over d->inp (an x ah x aw); the head writes fp32 out, the
-// RGBA16F image, and with a target the loss, head gradients and du0.
-int forward(gs_denoiser* d, bool with_target) {
+// RGBA16F image, and with a target the loss, head gradients and du0.  With
+// `den` (inference on one frame) the last convolution runs the head itself
+// and writes its den_w x den_h corner to den as RGBA16F; u0, out and the
+// padded image are then not written.
+int forward(gs_denoiser* d, bool with_target, uint16_t* den = nullptr, int den_w = 0, int den_h = 0) {
     const int n = d->an, h0 = d->ah, w0 = d->aw;
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8;
     hipStream_t s = d->s;
@@ -304,6 +316,15 @@ int forward(gs_denoiser* d, bool with_target) {
     DN_CHECK(conv(d, D1B, d->t1d, 48, 0, d->u1, 48, 0, n, h1, w1));
     if (with_target) DN_HIP(launch_up2(tv(d->u1, 48, 0), tv(d->cat0, 80, 0), n, h1, w1, 48, s));
     DN_CHECK(conv(d, D0A, d->cat0, 80, 0, d->t0d, 32, 0, n, h0, w0, upsampled(d->u1, 48)));
+    if (den && !with_target) {
+        ConvExtra e;
+        e.head = d->params + d->head_off;
+        e.head_inp = d->inp;
+        e.head_out = den;
+        e.head_w = den_w;
+        e.head_h = den_h;
+        return conv(d, D0B, d->t0d, 32, 0, d->u0, 32, 0, n, h0, w0, e);
+    }
     DN_CHECK(conv(d, D0B, d->t0d, 32, 0, d->u0, 32, 0, n, h0, w0));
     HeadLaunch hl{};
     hl.u = d->u0;
@@ -620,8 +641,7 @@ int gs_denoise(gs_denoiser* d) {
     p.wp = wp;
     p.inp = d->inp;
     DN_HIP(launch_pack_input(p, d->s));
-    DN_CHECK(forward(d, false));
-    DN_HIP(launch_crop_image(d->image, wp, d->den, f.gps_w, f.gps_h, d->s));
+    DN_CHECK(forward(d, false, d->den, f.gps_w, f.gps_h));
     d->den_w = f.gps_w;
     d->den_h = f.gps_h;
     d->den_flip = f.clip_y_sign;

@@ -413,6 +413,14 @@ struct ConvLaunch {
     // optional: 2x2 average pool of the (bf16) output into pool (n, h/2, w/2)
     uint16_t* pool;
     int ps, po;
+    // optional (n = 1, 32 output channels, no x2 / pool): the 1x1 head fused
+    // after the ReLU, out = b + W u + head_inp[0..3] with u the bf16 output,
+    // W [4][32] then b [4] in head; written as RGBA16F at the pixels
+    // (x, y) < (head_w, head_h), row stride head_w; y is then not written
+    const float* head;
+    const uint16_t* head_inp;  // 8 channels per pixel, h x w
+    uint16_t* head_out;
+    int head_w, head_h;
 };
 hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s);  // nt = cout / 16
 struct WGradLaunch {
@@ -459,7 +467,6 @@ struct PackLaunch {
     float* target;           // [hp][wp][4] or NULL
 };
 hipError_t launch_pack_input(const PackLaunch& a, hipStream_t s);
-hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int width, int height, hipStream_t s);
 // Squared-error sum and max abs difference of two half arrays, one float2
 // (sum, max) per block.
 hipError_t launch_image_error(const uint16_t* a, const uint16_t* b, uint32_t count, float2* partial, uint32_t blocks,

@@ -135,7 +135,7 @@ __device__ __forceinline__ uint32_t conv_tab_entry(int kgp, int ya, int xb, int 
 // entry per k-group.
 // The 32-channel kernel without a half-size part (whose input tiles allow
 // five blocks per CU) is held to 96 VGPRs, five waves per SIMD.
-template <int NT, int RG, bool HALF>
+template <int NT, int RG, bool HALF, bool HEAD>
 __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kernel(ConvLaunch a) {
     extern __shared__ __align__(16) uint16_t s_in[];
     constexpr int RPW = RG / 2;            // rows per wave
@@ -297,6 +297,18 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
     // acc[r][t][i] = output channel t * 16 + 4 kg + i of pixel `row` of row group r
     uint16_t* yb = a.y + (size_t)n * a.h * a.w * a.ys + a.yo + 4 * kg;
     uint32_t q[RG][NT][2];
+    // fused 1x1 head: the four lanes of a pixel (k-groups kg) each sum the
+    // terms of their eight channels, two butterfly steps add the partial
+    // sums, and the k-group-0 lane writes the pixel
+    float4 hw[HEAD ? 4 : 1][NT];
+    float4 hb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (HEAD) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) hw[j][t] = *reinterpret_cast<const float4*>(a.head + j * 32 + t * 16 + 4 * kg);
+        hb = *reinterpret_cast<const float4*>(a.head + 128);
+    }
 #pragma unroll
     for (int r = 0; r < RG; ++r) {
         const int y = y0 + wave * RPW + r / 2, xx = x0 + (r & 1) * 16 + row;
@@ -311,18 +323,47 @@ __global__ void __launch_bounds__(256, (NT == 2 && !HALF ? 5 : 1)) conv3x3_kerne
             q[r][t][0] = pack_bf16x2(v[0], v[1]);
             q[r][t][1] = pack_bf16x2(v[2], v[3]);
         }
-        if (y < a.h && xx < a.w) {
+        if (!HEAD && y < a.h && xx < a.w) {
             uint16_t* yp = yb + (y * a.w + xx) * a.ys;
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 *reinterpret_cast<uint2*>(yp + t * 16) = make_uint2(q[r][t][0], q[r][t][1]);
+        }
+        if constexpr (HEAD) {
+            float h[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const float u[4] = {__uint_as_float(q[r][t][0] << 16), __uint_as_float(q[r][t][0] & 0xFFFF0000u),
+                                    __uint_as_float(q[r][t][1] << 16), __uint_as_float(q[r][t][1] & 0xFFFF0000u)};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    h[j] = __builtin_fmaf(hw[j][t].x, u[0], h[j]);
+                    h[j] = __builtin_fmaf(hw[j][t].y, u[1], h[j]);
+                    h[j] = __builtin_fmaf(hw[j][t].z, u[2], h[j]);
+                    h[j] = __builtin_fmaf(hw[j][t].w, u[3], h[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                h[j] += __shfl_xor(h[j], 16, 64);
+                h[j] += __shfl_xor(h[j], 32, 64);
+            }
+            if (kg == 0 && y < a.head_h && xx < a.head_w) {
+                const uint2 iv = *reinterpret_cast<const uint2*>(a.head_inp + ((size_t)y * a.w + xx) * 8);
+                ushort4 o;
+                o.x = __half_as_ushort(__float2half_rn(h[0] + hb.x + __uint_as_float(iv.x << 16)));
+                o.y = __half_as_ushort(__float2half_rn(h[1] + hb.y + __uint_as_float(iv.x & 0xFFFF0000u)));
+                o.z = __half_as_ushort(__float2half_rn(h[2] + hb.z + __uint_as_float(iv.y << 16)));
+                o.w = __half_as_ushort(__float2half_rn(h[3] + hb.w + __uint_as_float(iv.y & 0xFFFF0000u)));
+                *reinterpret_cast<ushort4*>(a.head_out + ((size_t)y * a.head_w + xx) * 4) = o;
+            }
         }
     }
     // 2x2 average pool: with two rows per wave, row groups r and r + 2 are
     // rows y and y + 1 of the same pixels, and the pixel x + 1 is the
     // neighbouring lane; summed in the order of pool2_kernel, so the result
     // is the same bits
-    if constexpr (RG == 4) {
+    if constexpr (RG == 4 && !HEAD) {
       if (a.pool) {
         const int yp = (y0 + wave * RPW) / 2;
         uint16_t* pb = a.pool + (size_t)n * (a.h / 2) * (a.w / 2) * a.ps + a.po + 4 * kg;
@@ -716,14 +757,6 @@ __global__ void pack_input_kernel(PackLaunch a) {
     }
 }
 
-// RGBA16F (hp x wp, row pitch wp) -> the width x height image.
-__global__ void crop_image_kernel(const uint16_t* src, int wp, uint16_t* dst, int width, int height) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)width * height) return;
-    const int x = (int)(i % width), y = (int)(i / width);
-    *reinterpret_cast<ushort4*>(dst + i * 4) = *reinterpret_cast<const ushort4*>(src + ((size_t)y * wp + x) * 4);
-}
-
 // Gather batch crops from the pool: inp[b] = pool_in[idx[b]], tgt likewise.
 __global__ void gather_kernel(const uint4* pool_in, const float4* pool_tgt, const uint32_t* idx, uint32_t per,
                               uint32_t batch, uint4* inp, float4* tgt) {
@@ -739,9 +772,12 @@ inline uint32_t blocks_for(long n, int t = 256) { return (uint32_t)((n + t - 1) 
 
 }  // namespace
 
-template <int NT, int RG, bool HALF>
+template <int NT, int RG, bool HALF, bool HEAD = false>
 hipError_t conv3x3_launch_as(const ConvLaunch& a, hipStream_t s) {
     constexpr int BR = 4 * (RG / 2);
+    if (HEAD && (a.n != 1 || a.cout != 32 || a.pool || a.x2 || !a.head_inp || !a.head_out || a.head_w > a.w ||
+                 a.head_h > a.h || (reinterpret_cast<uintptr_t>(a.head) & 15)))
+        return hipErrorInvalidValue;
     if (a.pool && (RG != 4 || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
     if (a.x2 && (a.split % 8 || a.split > a.cin || a.h % 2 || a.w % 2)) return hipErrorInvalidValue;
     if (a.kpad % 32 || a.kpad / 8 > 256 || a.kpad < 9 * a.cin) return hipErrorInvalidValue;
@@ -756,13 +792,13 @@ hipError_t conv3x3_launch_as(const ConvLaunch& a, hipStream_t s) {
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv3x3_kernel<NT, RG, HALF>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conv3x3_kernel<NT, RG, HALF, HEAD>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     dim3 grid((a.w + kConvTileW - 1) / kConvTileW, (a.h + BR - 1) / BR, a.n);
-    hipLaunchKernelGGL((conv3x3_kernel<NT, RG, HALF>), grid, dim3(256), lds, s, a);
+    hipLaunchKernelGGL((conv3x3_kernel<NT, RG, HALF, HEAD>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
@@ -773,6 +809,7 @@ hipError_t conv3x3_launch(const ConvLaunch& a, hipStream_t s) {
 
 hipError_t launch_conv3x3(const ConvLaunch& a, int nt, hipStream_t s) {
     if (a.cin % 8 || a.cin > 160) return hipErrorInvalidValue;
+    if (a.head) return nt == 2 ? conv3x3_launch_as<2, 4, false, true>(a, s) : hipErrorInvalidValue;
     switch (nt) {
         case 2: return conv3x3_launch<2, 4>(a, s);
         case 3: return conv3x3_launch<3, 4>(a, s);
@@ -858,11 +895,6 @@ hipError_t launch_pack_input(const PackLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_crop_image(const uint16_t* src, int wp, uint16_t* dst, int width, int height, hipStream_t s) {
-    hipLaunchKernelGGL(crop_image_kernel, dim3(blocks_for((long)width * height)), dim3(256), 0, s, src, wp, dst, width,
-                       height);
-    return hipGetLastError();
-}
 
 // Squared-error sum and max abs difference of two half-precision arrays of
 // `count` values (grid-stride, one (sum, max) per block; the caller adds the

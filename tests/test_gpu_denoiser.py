@@ -125,10 +125,13 @@ def _frames(gs, ctx, d, scene, cams, spp, crops, seed):
     return out
 
 
-def test_denoise_matches_oracle_forward(gs):
-    print("case: gs_denoise 64x64 vs oracle forward", flush=True)
+@pytest.mark.parametrize("width,height", [(64, 64), (61, 45)])
+def test_denoise_matches_oracle_forward(gs, width, height):
+    """Inference: the last convolution runs the 1x1 head and writes the
+    width x height corner of the padded (multiple-of-8) frame."""
+    print(f"case: gs_denoise {width}x{height} vs oracle forward", flush=True)
     g = gs.generate_scene(0x5EED0D01, 20000, 8, 0.0)
-    cam = scenes.random_view_camera(0x5EED0D01, 64, 64)
+    cam = scenes.random_view_camera(0x5EED0D01, width, height)
     with gs.Context(0) as ctx, gs.Denoiser(ctx, seed=5) as d:
         ctx.upload(g)
         flat = _params(17)
@@ -139,13 +142,15 @@ def test_denoise_matches_oracle_forward(gs):
         acc = ctx.read_image(abi.GS_IMAGE_ACCUM_RGBA32F)
         dep = ctx.read_image(abi.GS_IMAGE_GPS_DEPTH_R32F)
         d.denoise()
-        img = d.read_image(64, 64).astype(np.float64)
-    # the network sees the device's row order (GL: bottom row first); readback is top row first
-    inp = np.zeros((1, 64, 64, 8), np.float32)
-    inp[0, ..., :4] = acc[::-1]
-    inp[0, ..., 4] = dep[::-1]
-    _, rout, _ = unet.forward_backward(flat, inp, np.zeros((1, 64, 64, 4)), want_grad=False)
-    err = np.abs(img - rout[0][::-1]).max()
+        img = d.read_image(width, height).astype(np.float64)
+    # the network sees the device's row order (GL: bottom row first), zero
+    # padded to multiples of 8; readback is top row first
+    hp, wp = -(-height // 8) * 8, -(-width // 8) * 8
+    inp = np.zeros((1, hp, wp, 8), np.float32)
+    inp[0, :height, :width, :4] = acc[::-1]
+    inp[0, :height, :width, 4] = dep[::-1]
+    _, rout, _ = unet.forward_backward(flat, inp, np.zeros((1, hp, wp, 4)), want_grad=False)
+    err = np.abs(img - rout[0][:height, :width][::-1]).max()
     print(f"denoise max abs err vs oracle {err:.3e}", flush=True)
     assert err < 2e-2
 
