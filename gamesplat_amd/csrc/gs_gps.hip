@@ -30,7 +30,7 @@ namespace {
 
 constexpr uint64_t kEmptyCell = 0xFFFFFFFF00000000ull;
 
-// Prepared record of visible position v (gps.glsl:152-169): mean, Cholesky
+// Prepared record of visible position v (gps.glsl:151-169): mean, Cholesky
 // factor of the twice-dilated covariance, opacity and its Li2, depth key,
 // colour (or the splat id for linear colour) and the splat id.
 __device__ void write_prepared(const GpsLaunch& a, uint32_t v, uint32_t splat_id, float4 r0, float4 r1) {
@@ -60,7 +60,7 @@ __device__ void write_prepared(const GpsLaunch& a, uint32_t v, uint32_t splat_id
 // Pass 1: work-item count (gps.glsl:127-142) per visible splat, item count
 // per block.  The prepared record is needed only by splats whose
 // reservation starts inside the budget (the reference writes it for those
-// alone, gps.glsl:145-170), which only the scan decides: a block writes it
+// alone, gps.glsl:148-169), which only the scan decides: a block writes it
 // here, for every splat with items, when its hint says that the block kept
 // work in the frame before, and leaves it to pass 2 (kept splats only)
 // otherwise.  Either way the same records reach the splat pass; the hint
