@@ -194,8 +194,9 @@ struct gs_ctx {
     bool have_cull = false, have_sort = false, have_composite = false, have_gps = false, have_accum = false;
     // the last cull's visible list (keys[0], ids[0], vrects[0]) is built:
     // GPS and the visible-list readbacks build it on demand; the depth sort
-    // otherwise compacts in its first pass
-    bool compacted = false;
+    // otherwise compacts in its first pass.  0: none, 1: ids only (GPS),
+    // 2: keys, ids and tile rectangles
+    int compacted = 0;
     // box of the scene's Gaussian centres (valid: every centre finite), which
     // bounds the depth keys' range and so the depth-sort passes launched
     bool box_valid = false;
@@ -1067,7 +1068,7 @@ SortLaunch make_depth_sort(gs_ctx* c) {
     // without a visible list the first pass reads the cull's dense keys and
     // drops the culled slots (the list it keeps is the one the compaction
     // would build, in the same order)
-    if (!c->compacted) {
+    if (c->compacted != 2) {
         const CullLaunch& cl = c->plan.cull;
         s.dense_keys = cl.dense_keys;
         s.dense_count = c->views * c->n;
@@ -1174,22 +1175,27 @@ int stage_cull(gs_ctx* c, const FramePlan& p) {
                                 p.cull.block_diag, c->dc()->cull.diag, c->stream, p.cull.block_range,
                                 &c->dc()->cull.key_min));
     GS_CHECK(timed_end(c, &t));
-    c->compacted = false;
+    c->compacted = 0;
     return GS_OK;
 }
 
 // The last cull's visible list in index order (keys[0], ids[0], vrects[0]),
-// for the consumers that read it unsorted.
-int ensure_compacted(gs_ctx* c) {
-    if (c->compacted) return GS_OK;
+// for the consumers that read it unsorted; `ids_only` (GPS) writes ids[0]
+// alone, a third of the bytes.
+int ensure_compacted(gs_ctx* c, bool ids_only = false) {
+    if (c->compacted == 2 || (ids_only && c->compacted == 1)) return GS_OK;
     const FramePlan& p = c->plan;
     uint32_t* visible = &c->dc()->cull.visible_count;
     GS_HIP(launch_scan_counts(p.cull.block_counts, p.cull_blocks * p.cull.views, ptr<uint32_t>(c->cull_offsets),
                               visible, nullptr, nullptr, c->stream));
-    GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets),
-                          ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
-                          ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_tile ? p.cull.packed_rects : nullptr));
-    c->compacted = true;
+    if (ids_only)
+        GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets), nullptr,
+                              ptr<uint32_t>(c->ids[0]), nullptr, 0, nullptr, c->stream, nullptr));
+    else
+        GS_HIP(launch_compact(p.cull.dense_keys, p.cull.n, p.cull.views, ptr<uint32_t>(c->cull_offsets),
+                              ptr<uint32_t>(c->keys[0]), ptr<uint32_t>(c->ids[0]), p.cull.rect_px, p.pack_tile,
+                              ptr<uint32_t>(c->vrects[0]), c->stream, p.pack_tile ? p.cull.packed_rects : nullptr));
+    c->compacted = ids_only ? 1 : 2;
     return GS_OK;
 }
 
@@ -1509,7 +1515,7 @@ int gs_render_views(gs_ctx* c, const GsCameraUniforms* cams, uint32_t count, con
     set_cull_rects(c, &p);
     c->rect_tile = p.pack_tile;
     c->rect_pad = 0;
-    c->compacted = false;  // this frame's cull builds no visible list
+    c->compacted = 0;  // this frame's cull builds no visible list
     p.depth = make_depth_sort(c);
     // the depth sort's last needed pass writes the sorted list to buffer 3
     c->sort_slot = 3;
@@ -1675,7 +1681,7 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     c->res_w = W;
     c->res_h = H;
 
-    GS_CHECK(ensure_compacted(c));
+    GS_CHECK(ensure_compacted(c, true));
     GpsLaunch g{};
     g.records = ptr<float4>(c->records);
     g.vis_ids = ptr<uint32_t>(c->ids[0]);

@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(256) compact_kernel(const uint32_t* __restrict
     if (visible) {
         const uint32_t pos = block_base + before +
                              (uint32_t)__popcll(bal & lanemask_lt());
-        vis_keys[pos] = key;
+        if (vis_keys) vis_keys[pos] = key;
         vis_ids[pos] = (uint32_t)slot;
         if (vis_rects) vis_rects[pos] = PACKED ? pre : pack_tile_rect(px, tile_shift);
     }

@@ -336,6 +336,33 @@ def test_gps_budget_sequence(ctx, orc, c1_scene):
         ctx.set_gps_budget(16 * 1024 * 1024)
 
 
+def test_gps_then_sort_and_readbacks(ctx, orc, c1_scene):
+    """One cull serves a GPS sample (which compacts the visible ids alone),
+    the depth sort (which then compacts in its first pass), the visible-list
+    readbacks (which compact keys and ids) and a second GPS sample."""
+    g, cfg, cam = c1_scene
+    W, H = cfg["width"], cfg["height"]
+    print("case: gps, sort, visible readbacks and gps again on one cull", flush=True)
+    ok, oi, op, ov, od = _cull_pair(ctx, orc, g, cam)
+    ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 1)
+    try:
+        ctx.gps_sample(frame_seed=2, supersample=1)
+        first = ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * W,)).copy()
+        ctx.sort()
+        sk, si = orc.radix_sort(ok, oi)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_IDS), si)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_SORTED_KEYS), sk)
+        vis = np.nonzero(oi != 0xFFFFFFFF)[0].astype(np.uint32)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_VISIBLE_IDS), vis)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_VISIBLE_KEYS), ok[vis])
+        ctx.gps_sample(frame_seed=2, supersample=1)
+        np.testing.assert_array_equal(ctx.read_buffer(abi.GS_BUFFER_GPS_PACKED, shape=(H * W,)), first)
+        prepared, work, requested, _ = orc.gps_expand(op, oi, 16 << 20, 1.0, 0.0)
+        np.testing.assert_array_equal(first, orc.gps_splat(prepared, work, min(requested, 16 << 20), W, H, 2, 1.0))
+    finally:
+        ctx.set_option(abi.GS_OPT_GPS_KEEP_CELLS, 0)
+
+
 def test_gps_accumulate(ctx, orc, c1_scene):
     g, cfg, cam = c1_scene
     W, H = cfg["width"], cfg["height"]
