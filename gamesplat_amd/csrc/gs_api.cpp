@@ -1681,6 +1681,9 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     c->res_w = W;
     c->res_h = H;
 
+    // the expand stage includes the compaction of the visible ids
+    TimedLaunch t{};
+    GS_CHECK(timed_begin(c, GS_STAGE_GPS_EXPAND, &t));
     GS_CHECK(ensure_compacted(c, true));
     GpsLaunch g{};
     g.records = ptr<float4>(c->records);
@@ -1709,9 +1712,7 @@ int gs_gps_sample(gs_ctx* c, uint32_t frame_seed, uint32_t supersample, int line
     g.sample = ptr<float4>(c->sample);
     g.depth = ptr<float>(c->depth);
 
-    TimedLaunch t{};
     GS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(&c->dc()->gps), 0u, sizeof(GpsCounters) / 4, c->stream));
-    GS_CHECK(timed_begin(c, GS_STAGE_GPS_EXPAND, &t));
     GS_HIP(launch_gps_expand(g, c->stream));
     GS_CHECK(timed_end(c, &t));
     TimedLaunch t2{};
