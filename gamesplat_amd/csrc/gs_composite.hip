@@ -769,8 +769,9 @@ __device__ __forceinline__ void composite_tile_ref(const CompositeLaunch& a, uin
 // splats in index order with a LESS depth test).  Along a tile list the
 // depth does not decrease, so a pixel is settled once the current splat lies
 // more than a few D24 steps behind its winner.  The fragment arithmetic is
-// the reference's, in source order, with the deterministic exp and log, so
-// keep decisions are those of the CPU restatement.
+// the reference's, in source order, with the deterministic exp and log (the
+// cutoff's log in the binary32 FMA form), so keep decisions are those of the
+// CPU restatement.
 __device__ __forceinline__ float fract_(float x) { return x - __builtin_floorf(x); }
 
 __device__ __forceinline__ float hash13(float x, float y, float z) {
@@ -826,7 +827,7 @@ __global__ void __launch_bounds__(256) stochastic_kernel(StochasticLaunch sl) {
                                                             t.height, false, sl.jitter_x, sl.jitter_y));
             s_rect[tid] = u32x4_t{rect.x, rect.y, 0u, 0u};
             s_geo[tid] = f32x4_t{r0.x + sl.jitter_x, r0.y + sl.jitter_y, k0, 2.0f * k1};
-            s_par[tid] = f32x4_t{k2, r0.z, logf_((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
+            s_par[tid] = f32x4_t{k2, r0.z, logf_fma((1.0f / 255.0f) / maxf(r0.z, 1e-8f)),
                                  __uint_as_float(depth24(r0.w, sl.clip_z_01))};
             s_col[tid] = f32x4_t{__half2float(__float2half_rn(r2.x)), __half2float(__float2half_rn(r2.y)),
                                  __half2float(__float2half_rn(r2.z)), __uint_as_float(g)};

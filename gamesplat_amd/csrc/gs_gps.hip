@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) gps_splat_kernel(GpsLaunch a) {
             float r1 = (float)y * 2.32830643654e-10f;
             float arg = inv_dilog((1.0f - r0) * dla) / alpha;
             arg = clampf(arg, 1.0e-7f, 1.0f);
-            float radius = __builtin_sqrtf(-2.0f * gps_logf(arg));
+            float radius = __builtin_sqrtf(-2.0f * logf_fma(arg));
             float az = 6.28318530718f * r1;
             float sn, cs;
             gps_sincosf(az, &sn, &cs);

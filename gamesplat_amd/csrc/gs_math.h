@@ -6,9 +6,9 @@
 // -ffp-contract=off), then rounds to float.  The same algorithm gives the
 // same bits on the host and on gfx950, so the projected splats and the
 // compositor's discard decisions are reproducible bit for bit on the CPU.
-// The GPS point sampler, which evaluates a log and a sin/cos per point, uses
-// the binary32 fused-multiply-add forms further down instead (equally
-// reproducible).
+// The GPS point sampler (a log and a sin/cos per point) and the
+// StochasticSplats fragment cutoff (a log per staged splat) use the binary32
+// fused-multiply-add forms further down instead (equally reproducible).
 //
 //   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
 //        s = (m-1)/(m+1), series to s^21.
@@ -164,13 +164,14 @@ GS_HD float expf_(float x) { return (float)exp_d((double)x); }
 GS_HD float sinf_(float x) { return (float)sin_d((double)x); }
 GS_HD float cosf_(float x) { return (float)cos_d((double)x); }
 
-// GPS point radius and angle: log, sin and cos in binary32 arithmetic with
-// explicit fused multiply-adds (one rounding each, the same on the host and
-// on gfx950), a third of the instructions of the double-precision forms.
+// Binary32 log, sin and cos with explicit fused multiply-adds (one rounding
+// each, the same on the host and on gfx950), a third of the instructions of
+// the double-precision forms: the GPS point sampler's radius and angle, and
+// the StochasticSplats fragment cutoff log(1 / (255 o)).
 // Coefficients: minimax fits by Lawson's iteratively reweighted least
 // squares on Chebyshev nodes, each coefficient rounded to float in turn and
 // the rest refitted (oracle/fit_gps_math.py).
-//   gps_logf: positive normal x; x = 2^k m, m in [sqrt(1/2), sqrt(2)),
+//   logf_fma: positive normal x; x = 2^k m, m in [sqrt(1/2), sqrt(2)),
 //     log m = f + f^2 Q(f), f = m - 1, Q of degree 8 with Q(0) = -1/2.
 //     Within 1 ulp of the correctly rounded log for every normal float up
 //     to 1e30 (checked exhaustively).
@@ -178,17 +179,17 @@ GS_HD float cosf_(float x) { return (float)cos_d((double)x); }
 //     sin r = r + r^3 S(r^2), cos r = 1 - r^2/2 + r^4 C(r^2), S and C of
 //     degree 2 on |r| <= 0.801.  For every float in [0, 2 pi): within 1 ulp
 //     (sin), 2 ulp (cos) and 9.3e-8 absolute.
-constexpr float kGpsLogQ[9] = {-0.5f, 0.3333333134651184f, -0.2500081956386566f, 0.2000124156475067f, -0.166234090924263f, 0.14201579988002777f, -0.13159610331058502f, 0.12762217223644257f, -0.07636439055204391f};
+constexpr float kLogFmaQ[9] = {-0.5f, 0.3333333134651184f, -0.2500081956386566f, 0.2000124156475067f, -0.166234090924263f, 0.14201579988002777f, -0.13159610331058502f, 0.12762217223644257f, -0.07636439055204391f};
 constexpr float kGpsSinS[3] = {-0.16666653752326965f, 0.008332093246281147f, -0.00019505145610310137f};
 constexpr float kGpsCosC[3] = {0.04166664183139801f, -0.0013887130189687014f, 2.4412080165348016e-05f};
 
-GS_HD float gps_logf(float x) {
+GS_HD float logf_fma(float x) {
     uint32_t ix = fbits(x) + (0x3f800000u - 0x3f3504f3u);
     const int k = (int)(ix >> 23) - 127;
     ix = (ix & 0x007fffffu) + 0x3f3504f3u;
     const float f = bitsf(ix) - 1.0f;
-    float q = kGpsLogQ[8];
-    for (int i = 7; i >= 0; --i) q = __builtin_fmaf(q, f, kGpsLogQ[i]);
+    float q = kLogFmaQ[8];
+    for (int i = 7; i >= 0; --i) q = __builtin_fmaf(q, f, kLogFmaQ[i]);
     const float r = __builtin_fmaf(f * f, q, f);
     const float fk = (float)k;
     return __builtin_fmaf(fk, 0.693147182f, __builtin_fmaf(fk, -1.90465421e-09f, r));

@@ -1,6 +1,7 @@
-"""Derive the polynomial coefficients of the GPS sampler's binary32 log and
-sin/cos (gps_logf / gps_sincosf in gamesplat_amd/csrc/gs_math.h,
-or_gps_logf / or_gps_sincosf in oracle/gs_oracle.c).
+"""Derive the polynomial coefficients of the binary32 FMA log and sin/cos
+(logf_fma / gps_sincosf in gamesplat_amd/csrc/gs_math.h, or_logf_fma /
+or_gps_sincosf in oracle/gs_oracle.c): the GPS point sampler's radius and
+angle and the StochasticSplats fragment cutoff.
 
 python oracle/fit_gps_math.py
 
@@ -62,7 +63,7 @@ def main():
     def show(name, v, e):
         print(f"{name} = {{{', '.join(f'{x!r}f' for x in v)}}};  // fit error {e / 2 ** -24:.3f} ulp")
 
-    show("kGpsLogQ[9]", log_q, e_log)
+    show("kLogFmaQ[9]", log_q, e_log)
     show("kGpsSinS[3]", sin_s, e_sin)
     show("kGpsCosC[3]", cos_c, e_cos)
 

@@ -164,23 +164,24 @@ float or_sinf(float x) { return (float)or_sin_d((double)x); }
 float or_cosf(float x) { return (float)or_cos_d((double)x); }
 
 /* The GPS point sampler's log and sin/cos (the log, cos and sin of
- * gps_splat_cs, shaders/gps.glsl:270-276): binary32 arithmetic with explicit
+ * gps_splat_cs, shaders/gps.glsl:270-276) and the StochasticSplats fragment
+ * cutoff's log (shaders/splat.glsl:107-146): binary32 arithmetic with explicit
  * fmaf (one rounding each; glibc's fmaf is exact), minimax polynomial
  * coefficients from oracle/fit_gps_math.py.  log: x = 2^k m with m in
  * [sqrt(1/2), sqrt(2)), log m = f + f^2 Q(f), f = m - 1.  sin/cos: reduction
  * by pi/2 in three parts, sin r = r + r^3 S(r^2), cos r = 1 - r^2/2 +
  * r^4 C(r^2).  Accuracy against libm is pinned in tests/test_oracle_math.py. */
-static const float kGpsLogQ[9] = {-0.5f, 0.3333333134651184f, -0.2500081956386566f, 0.2000124156475067f, -0.166234090924263f, 0.14201579988002777f, -0.13159610331058502f, 0.12762217223644257f, -0.07636439055204391f};
+static const float kLogFmaQ[9] = {-0.5f, 0.3333333134651184f, -0.2500081956386566f, 0.2000124156475067f, -0.166234090924263f, 0.14201579988002777f, -0.13159610331058502f, 0.12762217223644257f, -0.07636439055204391f};
 static const float kGpsSinS[3] = {-0.16666653752326965f, 0.008332093246281147f, -0.00019505145610310137f};
 static const float kGpsCosC[3] = {0.04166664183139801f, -0.0013887130189687014f, 2.4412080165348016e-05f};
 
-float or_gps_logf(float x) {
+float or_logf_fma(float x) {
     uint32_t ix = fbits(x) + (0x3f800000u - 0x3f3504f3u);
     int k = (int)(ix >> 23) - 127;
     ix = (ix & 0x007fffffu) + 0x3f3504f3u;
     float f = bitsf(ix) - 1.0f;
-    float q = kGpsLogQ[8];
-    for (int i = 7; i >= 0; --i) q = fmaf(q, f, kGpsLogQ[i]);
+    float q = kLogFmaQ[8];
+    for (int i = 7; i >= 0; --i) q = fmaf(q, f, kLogFmaQ[i]);
     float r = fmaf(f * f, q, f);
     float fk = (float)k;
     return fmaf(fk, 0.693147182f, fmaf(fk, -1.90465421e-09f, r));
@@ -205,8 +206,8 @@ void or_gps_sincosf(float x, float* sn, float* cs) {
     }
 }
 
-void or_gps_logf_n(const float* x, float* y, uint32_t n) {
-    for (uint32_t i = 0; i < n; ++i) y[i] = or_gps_logf(x[i]);
+void or_logf_fma_n(const float* x, float* y, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) y[i] = or_logf_fma(x[i]);
 }
 
 void or_gps_sincosf_n(const float* x, float* sn, float* cs, uint32_t n) {
@@ -906,7 +907,7 @@ void or_gps_splat(const uint32_t* prepared, const uint32_t* work, uint32_t work_
             float r1 = (float)v[1] * 2.32830643654e-10f;
             float arg = or_inv_dilog((1.0f - r0) * dla) / alpha;
             arg = or_clampf(arg, 1.0e-7f, 1.0f);
-            float radius = sqrtf(-2.0f * or_gps_logf(arg));
+            float radius = sqrtf(-2.0f * or_logf_fma(arg));
             float az = 6.28318530718f * r1;
             float sn, cs;
             or_gps_sincosf(az, &sn, &cs);
@@ -1027,7 +1028,7 @@ void or_stochastic_splats(const float* projected, const uint32_t* ids, uint32_t 
             if (i_lo < 0) i_lo = 0;
             if (i_hi > width) i_hi = width;
             const float o = rec[3];
-            const float cutoff = or_logf((1.0f / 255.0f) / fmaxf(o, 1e-8f));
+            const float cutoff = or_logf_fma((1.0f / 255.0f) / fmaxf(o, 1e-8f));
             const float cx = rec[4] + jitter_x, cy = rec[5] + jitter_y;
             const float k0 = rec[8], k1 = rec[9], k2 = rec[10];
             const uint32_t dq = or_depth24(rec[11], clip_z_01);

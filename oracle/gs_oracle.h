@@ -39,9 +39,9 @@ float or_sinf(float x);
 float or_cosf(float x);
 float or_dilog(float x);
 float or_inv_dilog(float x);
-float or_gps_logf(float x);
+float or_logf_fma(float x);
 void or_gps_sincosf(float x, float* sn, float* cs);
-void or_gps_logf_n(const float* x, float* y, uint32_t n);
+void or_logf_fma_n(const float* x, float* y, uint32_t n);
 void or_gps_sincosf_n(const float* x, float* sn, float* cs, uint32_t n);
 uint32_t or_hash01_bits(uint32_t x);
 float or_hash01(uint32_t x);

@@ -28,8 +28,8 @@ _SIGS = {
     "or_cosf": (C.c_float, [C.c_float]),
     "or_dilog": (C.c_float, [C.c_float]),
     "or_inv_dilog": (C.c_float, [C.c_float]),
-    "or_gps_logf": (C.c_float, [C.c_float]),
-    "or_gps_logf_n": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "or_logf_fma": (C.c_float, [C.c_float]),
+    "or_logf_fma_n": (None, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "or_gps_sincosf_n": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "or_hash01_bits": (C.c_uint32, [C.c_uint32]),
     "or_hash01": (C.c_float, [C.c_uint32]),
@@ -255,11 +255,12 @@ def accumulate(history, sample, n):
     return out
 
 
-def gps_logf(x: np.ndarray) -> np.ndarray:
-    """The GPS sampler's binary32 log of each element (positive normal floats)."""
+def logf_fma(x: np.ndarray) -> np.ndarray:
+    """The binary32 FMA log of each element (positive normal floats): the GPS
+    sampler's and the StochasticSplats cutoff's."""
     x = np.ascontiguousarray(x, np.float32)
     y = np.empty_like(x)
-    load().or_gps_logf_n(_p(x), _p(y), x.size)
+    load().or_logf_fma_n(_p(x), _p(y), x.size)
     return y
 
 

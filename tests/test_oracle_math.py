@@ -134,8 +134,8 @@ def _ulps(a, b):
     return np.abs(ia - ib)
 
 
-def test_gps_logf_against_libm(orc):
-    """The GPS sampler's binary32 log: every 61st float from the smallest
+def test_logf_fma_against_libm(orc):
+    """The binary32 FMA log (GPS sampler, StochasticSplats cutoff): every 61st float from the smallest
     normal to 1e30 and every 7th of its own domain [1e-7, 1] within 1 ulp of
     libm's log rounded to float, nearly all of them exact."""
     lo, hi = np.float32(1.17549435e-38).view(np.uint32), np.float32(1e30).view(np.uint32)
@@ -143,14 +143,14 @@ def test_gps_logf_against_libm(orc):
                            np.arange(np.float32(1e-7).view(np.uint32), np.float32(1.0).view(np.uint32) + 1, 7,
                                      dtype=np.uint32)])
     x = bits.view(np.float32)
-    got = orc.gps_logf(x)
+    got = orc.logf_fma(x)
     ref = np.log(x.astype(np.float64)).astype(np.float32)
     u = _ulps(got, ref)
     assert u.max() <= 1 and (u == 0).mean() > 0.98
     k = np.arange(-126, 100)
-    np.testing.assert_array_equal(orc.gps_logf(np.ldexp(np.float32(1), k).astype(np.float32)),
+    np.testing.assert_array_equal(orc.logf_fma(np.ldexp(np.float32(1), k).astype(np.float32)),
                                   (k * np.log(2.0)).astype(np.float32))
-    assert orc.scalar("or_gps_logf", 1.0) == 0.0
+    assert orc.scalar("or_logf_fma", 1.0) == 0.0
 
 
 def test_gps_sincosf_against_libm(orc):

@@ -67,7 +67,7 @@ def test_ties_go_to_the_lowest_index():
             k0, k2 = rec[0, 8], rec[0, 10]
             power = np.float32(-0.5) * (k0 * dx * dx + np.float32(0.0) + k2 * dy * dy)
             alpha = min(np.float32(0.6) * np.float32(orc.scalar("or_expf", float(power))), np.float32(0.99))
-            cutoff = orc.scalar("or_logf", (1.0 / 255.0) / 0.6)
+            cutoff = orc.scalar("or_logf_fma", (1.0 / 255.0) / 0.6)
             covered = power >= cutoff
             keep0 = covered and orc.scalar("or_hash13", fx, fy, 0.0 + seed) < alpha
             keep1 = covered and orc.scalar("or_hash13", fx, fy, 1.0 + seed) < alpha
